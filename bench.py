@@ -1,0 +1,211 @@
+"""Log-evidence throughput on one MI355X: mixed-precision batched path vs plain FP64 GPU path.
+
+Workload (the batching configuration): one dataset with M = 5,000 image
+pixels and K = 1e6 visibilities, 64 lens-model samples with S ~ U{300..1500}
+Delaunay source pixels each (own mesh, own barycentric mapping, B = 3) and
+lambda log-uniform in [0.01, 100].  Synthetic data (datasets.config5).
+
+One step = build W-tilde for the dataset + log-evidence of all 64 samples:
+  mixed  : W-tilde at --precision (default int8x3: exact-integer MFMA on
+           3 x 8-bit digits of FP32 cos/sin), then one batched evaluation.
+  plain  : W-tilde in FP64 (FP64 MFMA), then 64 sequential single-sample
+           evaluations.
+Both use FP64 for F, H, Cholesky, solves and log-determinants.
+value = log-evidence evaluations / s of the mixed path (64 per step).
+
+python bench.py [--gpus 1] [--steps 3] [--warmup 1]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+for p in (ROOT, ROOT / "python-autojax_amd"):
+    if str(p) not in sys.path:
+        sys.path.insert(0, str(p))
+
+import numpy as np  # noqa: E402
+
+PEAKS_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0, "int8x3": 5000.0, "int8x4": 5000.0}
+RAW_FACTOR = {"fp64": 1, "fp32": 1, "bf16x3": 3, "int8x3": 9, "int8x4": 16}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(ds, k_sub, n_like, threads):
+    """Oracle timed on the host: C/OpenMP W-tilde on k_sub visibilities (scaled to K) and the
+    numpy likelihood (dense T, two DGEMMs, LU solve, two slogdets) on n_like samples (scaled)."""
+    import ctypes
+
+    from oracle import restated as R
+
+    lib = ctypes.CDLL(str(ROOT / "oracle" / "liboracle_wtilde.so"))
+    lib.oracle_w_tilde.restype = ctypes.c_int
+    lib.oracle_w_tilde.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_int64, ctypes.c_int64, ctypes.c_int, ctypes.c_void_p]
+    M, K = ds.M, ds.K
+    n = np.ascontiguousarray(ds.noise_map_real[:k_sub])
+    uv = np.ascontiguousarray(ds.uv_wavelengths[:k_sub])
+    g = np.ascontiguousarray(ds.grid_radians_slim)
+    W = np.empty((M, M))
+    t0 = time.perf_counter()
+    rc = lib.oracle_w_tilde(n.ctypes.data, uv.ctypes.data, g.ctypes.data, k_sub, M, threads, W.ctypes.data)
+    t_w = (time.perf_counter() - t0) * (K / k_sub)
+    assert rc == 0
+    y = ds.dirty_image
+    t0 = time.perf_counter()
+    for smp in ds.samples[:n_like]:
+        R.log_likelihood_function_via_w_tilde_from(y, ds.data, ds.noise_map, W, smp.pix_indexes_for_sub_slim_index,
+                                                   smp.pix_weights_for_sub_slim_index, smp.neighbors,
+                                                   smp.neighbors_sizes)
+    t_like = (time.perf_counter() - t0) / n_like * len(ds.samples)
+    return len(ds.samples) / (t_w + t_like), t_w, t_like
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--precision", default="int8x3", choices=list(PEAKS_TFLOPS))
+    ap.add_argument("--plain-steps", type=int, default=1)
+    ap.add_argument("--samples", type=int, default=64)
+    ap.add_argument("--K", type=int, default=1_000_000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-k-sub", type=int, default=160)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from autojax_amd import _lib
+    from autojax_amd import datasets as dsm
+    from autojax_amd import device as dv
+    from autojax_amd import evidence as E
+
+    _lib.load()
+    t0 = time.perf_counter()
+    ds = dsm.config5(K=args.K, n_samples=args.samples)
+    log(f"dataset: M={ds.M} K={ds.K} samples={len(ds.samples)} S={[s.pixels for s in ds.samples[:8]]}... "
+        f"({time.perf_counter() - t0:.1f}s)")
+    noise = dv.to_device(ds.noise_map_real)
+    uv = dv.to_device(ds.uv_wavelengths)
+    grid = dv.to_device(ds.grid_radians_slim)
+    y = E.dirty_image(ds.data, ds.noise_map, ds.uv_wavelengths, ds.grid_radians_slim)
+    c2nn = E.chi2_noise_norm(ds.data, ds.noise_map)
+    batch = E.SampleBatch(ds.samples, ds.M)
+    singles = [E.SampleBatch([s], ds.M) for s in ds.samples]
+    M, K = ds.M, ds.K
+    W_mixed = dv.empty((M, M))
+    W_plain = dv.empty((M, M))
+    ws_mixed = dv.workspace(_lib.query("aj_w_tilde_workspace_bytes", K, M, _lib.PRECISIONS[args.precision]))
+    ws_plain = dv.workspace(_lib.query("aj_w_tilde_workspace_bytes", K, M, 0))
+    torch.cuda.synchronize()
+    log("inputs resident")
+
+    def mixed_step():
+        E.w_tilde(noise, uv, grid, args.precision, out=W_mixed, workspace=ws_mixed)
+        return batch.log_evidence(W_mixed, y, c2nn)
+
+    def plain_step():
+        E.w_tilde(noise, uv, grid, "fp64", out=W_plain, workspace=ws_plain)
+        return torch.cat([s.log_evidence(W_plain, y, c2nn).clone() for s in singles])
+
+    def timed(fn, steps, warmup, label):
+        for i in range(warmup):
+            fn()
+            torch.cuda.synchronize()
+            log(f"{label} warmup {i + 1}/{warmup} done")
+        _lib.timing_enable(True)
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        for i in range(steps):
+            out = fn()
+        b.record()
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b) / steps
+        timers = {k: _lib.timing_read(k) for k in _lib.TIMERS}
+        _lib.timing_enable(False)
+        log(f"{label}: {ms:.2f} ms/step; timers " + ", ".join(f"{k}={v[0] / steps:.2f}ms/{v[1]}" for k, v in timers.items()))
+        return ms, out, timers
+
+    ms_mixed, out_mixed, tm = timed(mixed_step, args.steps, args.warmup, "mixed")
+    ms_plain, out_plain, tp = timed(plain_step, args.plain_steps, 1, "plain")
+    mixed_ev = out_mixed.cpu().numpy()
+    plain_ev = out_plain.cpu().numpy()
+    dlogl = float(np.abs(mixed_ev - plain_ev).max())
+    info_bad = int(batch.info.any().item())
+    n = len(ds.samples)
+    value = n / (ms_mixed / 1e3)
+    plain_value = n / (ms_plain / 1e3)
+
+    # roofline of the dominant kernel: the W-tilde SYRK launches of the mixed path
+    syrk_ms, syrk_launches = tm["wt_syrk"]
+    algo_flops_step = 2.0 * M * M * K  # lower-triangle SYRK count, cos and sin terms
+    achieved = algo_flops_step * args.steps / (syrk_ms / 1e3) / 1e12
+    peak = PEAKS_TFLOPS[args.precision]
+    wt_ms = (tm["wt_syrk"][0] + tm["wt_features"][0]) / args.steps
+    batch_ms = ms_mixed - wt_ms
+
+    line = {
+        "metric": "log-evidence evals/sec (mixed+batched vs plain FP64 GPU) at <=1e-3 |dlogL|",
+        "value": round(value, 3),
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_mixed, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": f"{args.precision} W-tilde MFMA + fp64 likelihood",
+        "data": "synthetic",
+        "config": {
+            "workload": "batched log-evidence: M=5000 px, K=1e6 vis, 64 Delaunay samples S~U{300..1500}, B=3, "
+                        "lambda~logU[0.01,100] (datasets.config5)",
+            "M": M, "K": K, "samples": n, "sum_S": int(batch.S_host.sum()), "precision": args.precision,
+        },
+        "plain_fp64": {"value": round(plain_value, 3), "ms_per_step": round(ms_plain, 3),
+                       "w_tilde_ms": round((tp["wt_syrk"][0] + tp["wt_features"][0]) / args.plain_steps, 3)},
+        "speedup_vs_plain": round(value / plain_value, 3),
+        "max_abs_dlogL_vs_plain": dlogl,
+        "nonpd_samples": info_bad,
+        "breakdown_ms": {"w_tilde_syrk": round(tm["wt_syrk"][0] / args.steps, 3),
+                         "w_tilde_features": round(tm["wt_features"][0] / args.steps, 3),
+                         "omega": round(tm["omega"][0] / args.steps, 3),
+                         "fgather": round(tm["fgather"][0] / args.steps, 3),
+                         "cholesky": round(tm["chol"][0] / args.steps, 3),
+                         "solve": round(tm["solve"][0] / args.steps, 3),
+                         "batch_total": round(batch_ms, 3)},
+        "evals_per_s_excluding_w_tilde": round(n / (batch_ms / 1e3), 3),
+        "roofline": {"bound": "mfma", "kernel": f"syrk_kernel<{args.precision}>", "achieved": round(achieved, 3),
+                     "peak": peak, "unit": "TFLOP/s", "frac": round(achieved / peak, 5),
+                     "raw_mfma_frac": round(achieved * RAW_FACTOR[args.precision] / peak, 5),
+                     "avg_launch_ms": round(syrk_ms / max(syrk_launches, 1), 4), "launches_per_step": syrk_launches // args.steps,
+                     "traffic": None},
+    }
+    if not args.no_cpu_baseline:
+        ds.dirty_image = y.cpu().numpy()
+        threads = len(os.sched_getaffinity(0))
+        threads = min(threads, 16)
+        cv, t_w, t_like = cpu_baseline(ds, args.cpu_k_sub, 2, threads)
+        line["cpu_baseline"] = {"value": round(cv, 6), "unit": "evals/s", "cores": threads, "kind": "port",
+                                "sample": f"C/OpenMP W-tilde on {args.cpu_k_sub} of {K} visibilities scaled x{K // args.cpu_k_sub} "
+                                          f"({t_w:.1f}s), numpy likelihood on 2 of {n} samples scaled ({t_like:.1f}s)"}
+        log(f"cpu baseline: {cv:.4g} evals/s")
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

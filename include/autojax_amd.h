@@ -1,0 +1,172 @@
+/* C ABI of libautojax_amd: MI355X (gfx950) interferometer-inversion likelihood.
+ *
+ * Every pointer argument is DEVICE memory owned by the caller unless the
+ * parameter name ends in _host.  Sizes are int64_t.  `stream` is a
+ * hipStream_t (NULL = the default stream).  Functions return AJ_OK (0) or an
+ * error status; aj_last_error() gives the text of the last error of the
+ * calling thread.  Functions that need scratch memory take a caller-provided
+ * `workspace` of at least the size returned by the matching *_workspace_bytes
+ * query; the library allocates nothing.
+ *
+ * Each entry point cites the function of the reference (ickc/python-autojax,
+ * src/autojax/numba.py) whose arguments and result it reproduces.  Array
+ * layouts are the reference's C-contiguous ones: f8[M,2] grid (y, x),
+ * f8[K,2] uv, complex128 arrays as interleaved (re, im) doubles, i8[M,B]
+ * mapping indices with -1 terminating a row, i8[S,P] neighbour lists with -1
+ * terminating a row.
+ */
+#ifndef AUTOJAX_AMD_H
+#define AUTOJAX_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AJ_OK 0
+#define AJ_EINVAL 1  /* bad argument (size, null pointer, precision) */
+#define AJ_ENOTPD 2  /* matrix not positive definite (see info output) */
+#define AJ_EHIP 3    /* HIP runtime error */
+#define AJ_ESPACE 4  /* workspace too small */
+
+/* W-tilde arithmetic modes. */
+#define AJ_PREC_FP64 0   /* FP64 features, FP64 MFMA, FP64 accumulation */
+#define AJ_PREC_FP32 1   /* FP32 features and MFMA, FP32 accumulation per 4096-visibility chunk */
+#define AJ_PREC_BF16X3 2 /* split-BF16 features, 3 products, FP32 accumulation per chunk */
+#define AJ_PREC_INT8X3 3 /* 3 signed 8-bit digits per feature, exact int32 accumulation */
+#define AJ_PREC_INT8X4 4 /* 4 signed 8-bit digits per feature, exact int32 accumulation */
+
+const char* aj_last_error(void);
+int aj_abi_version(void);
+
+/* Kernel timing (instrumentation, no reference counterpart).  When enabled,
+ * HIP events are recorded around the launches named by the timer ids below,
+ * on the caller's stream; aj_timing_read synchronises on them and returns the
+ * summed duration and the launch count since the last aj_timing_enable. */
+#define AJ_TIMER_WT_SYRK 0     /* W-tilde MFMA SYRK launches */
+#define AJ_TIMER_WT_FEATURES 1 /* W-tilde feature launches */
+#define AJ_TIMER_OMEGA 2       /* Omega = W T gather (batched) */
+#define AJ_TIMER_FGATHER 3     /* F = T^T Omega gather (batched) */
+#define AJ_TIMER_CHOL 4        /* batched Cholesky of A and H */
+#define AJ_TIMER_SOLVE 5       /* triangular solves + log-evidence */
+#define AJ_TIMER_COUNT 6
+int aj_timing_enable(int on);
+int aj_timing_read(int timer, double* total_ms, int64_t* count);
+
+/* numba.py:155-207 w_tilde_curvature_interferometer_from(noise_map_real f8[K],
+ * uv_wavelengths f8[K,2], grid_radians_slim f8[M,2]) -> f8[M,M].
+ * Writes the full symmetric matrix w_tilde[M*M]. */
+size_t aj_w_tilde_workspace_bytes(int64_t K, int64_t M, int precision);
+int aj_w_tilde_curvature_interferometer_from(const double* noise_map_real, const double* uv_wavelengths,
+                                             const double* grid_radians_slim, int64_t K, int64_t M, int precision,
+                                             double* w_tilde, void* workspace, size_t workspace_bytes, void* stream);
+
+
+/* numba.py:95-152 w_tilde_data_interferometer_from(visibilities_real f8[K],
+ * noise_map_real f8[K], uv_wavelengths f8[K,2], grid_radians_slim f8[M,2]) -> f8[M].
+ * Same weight as the reference, (n_k^2 / v_k)^2 (numba.py:149). */
+size_t aj_visibility_sum_workspace_bytes(int64_t K, int64_t M);
+int aj_w_tilde_data_interferometer_from(const double* visibilities_real, const double* noise_map_real,
+                                        const double* uv_wavelengths, const double* grid_radians_slim, int64_t K,
+                                        int64_t M, double* w_tilde_data, void* workspace, size_t workspace_bytes,
+                                        void* stream);
+
+/* Dirty image consistent with W-tilde: sum_k (Re d_k cos phi_ik + Im d_k sin phi_ik) / n_k^2,
+ * phi_ik = 2 pi (g_i1 u_k0 + g_i0 u_k1), n_k = Re noise_map_k.  No reference counterpart (the
+ * reference takes the dirty image as an input, numba.py:762); data preparation for its callers.
+ * Uses aj_visibility_sum_workspace_bytes. */
+int aj_dirty_image_from(const double* data, const double* noise_map, const double* uv_wavelengths,
+                        const double* grid_radians_slim, int64_t K, int64_t M, double* dirty_image, void* workspace,
+                        size_t workspace_bytes, void* stream);
+
+/* numba.py:293-376 mapping_matrix_from(pix_indexes_for_sub_slim_index i8[M,B],
+ * pix_weights_for_sub_slim_index f8[M,B], pixels) -> f8[M,pixels] (dense, assignment). */
+int aj_mapping_matrix_from(const int64_t* pix_indexes_for_sub_slim_index, const double* pix_weights_for_sub_slim_index,
+                           int64_t M, int64_t B, int64_t pixels, double* mapping_matrix, void* stream);
+
+/* numba.py:379-406 data_vector_from(mapping_matrix f8[M,S], dirty_image f8[M]) -> f8[S]. */
+int aj_data_vector_from(const double* mapping_matrix, const double* dirty_image, int64_t M, int64_t S,
+                        double* data_vector, void* stream);
+
+/* numba.py:409-435 curvature_matrix_via_w_tilde_from(w_tilde f8[M,M], mapping_matrix f8[M,S]) -> f8[S,S]
+ * (dense T^T W T on FP64 MFMA). */
+size_t aj_curvature_dense_workspace_bytes(int64_t M, int64_t S);
+int aj_curvature_matrix_via_w_tilde_from(const double* w_tilde, const double* mapping_matrix, int64_t M, int64_t S,
+                                         double* curvature_matrix, void* workspace, size_t workspace_bytes,
+                                         void* stream);
+
+/* numba.py:552-587 sparse_mapping_matrix_transpose_matmul(matrix f8[M,S2], pix_indexes i8[M,B],
+ * pix_weights f8[M,B], pixels) -> f8[pixels,S2]. */
+size_t aj_sparse_transpose_matmul_workspace_bytes(int64_t M, int64_t B, int64_t S);
+int aj_sparse_mapping_matrix_transpose_matmul(const double* matrix, const int64_t* pix_indexes_for_sub_slim_index,
+                                              const double* pix_weights_for_sub_slim_index, int64_t M, int64_t S2,
+                                              int64_t B, int64_t pixels, double* out, void* workspace,
+                                              size_t workspace_bytes, void* stream);
+
+/* numba.py:590-614 curvature_matrix_via_w_wilde_sparse_mapping_matrix_from(w_tilde f8[M,M],
+ * pix_indexes i8[M,B], pix_weights f8[M,B], pixels) -> f8[pixels,pixels] (w_tilde symmetric). */
+size_t aj_curvature_sparse_workspace_bytes(int64_t M, int64_t B, int64_t S);
+int aj_curvature_matrix_via_w_wilde_sparse_mapping_matrix_from(const double* w_tilde,
+                                                               const int64_t* pix_indexes_for_sub_slim_index,
+                                                               const double* pix_weights_for_sub_slim_index, int64_t M,
+                                                               int64_t B, int64_t pixels, double* curvature_matrix,
+                                                               void* workspace, size_t workspace_bytes, void* stream);
+
+/* numba.py:656-695 constant_regularization_matrix_from(coefficient, neighbors i8[S,P],
+ * neighbors_sizes i8[S]) -> f8[S,S]. */
+int aj_constant_regularization_matrix_from(double coefficient, const int64_t* neighbors,
+                                           const int64_t* neighbors_sizes, int64_t S, int64_t P,
+                                           double* regularization_matrix, void* stream);
+
+/* numba.py:698-731 reconstruction_positive_negative_from(data_vector f8[S], curvature_reg_matrix f8[S,S])
+ * -> f8[S].  FP64 Cholesky (lower triangle of the input is used) and two triangular solves.
+ * logdet (may be NULL) receives log|A|; info[0] = 0 or the 1-based failing pivot (AJ_ENOTPD is
+ * reported by the caller after reading info). */
+size_t aj_solve_workspace_bytes(int64_t S);
+int aj_reconstruction_positive_negative_from(const double* data_vector, const double* curvature_reg_matrix, int64_t S,
+                                             double* reconstruction, double* logdet, int64_t* info, void* workspace,
+                                             size_t workspace_bytes, void* stream);
+
+/* numba.py:734-752 noise_normalization_complex_from(noise_map c16[K]) -> f8 (device scalar). */
+int aj_noise_normalization_complex_from(const double* noise_map, int64_t K, double* out, void* stream);
+
+/* chi.chi of numba.py:846-847 and the noise normalisation of numba.py:752 in one pass:
+ * out2[0] = sum over 2K of (data/noise)^2, out2[1] = noise normalisation. */
+int aj_chi_squared_noise_normalization(const double* data, const double* noise_map, int64_t K, double* out2,
+                                       void* stream);
+
+/* numba.py:755-857 log_likelihood_function_via_w_tilde_from(dirty_image f8[M], data c16[K],
+ * noise_map c16[K], w_tilde f8[M,M], pix_indexes i8[M,B], pix_weights f8[M,B], neighbors i8[S,P],
+ * neighbors_sizes i8[S]) -> f8, coefficient 1 as at numba.py:819.  log_likelihood is a device
+ * scalar; info[2] = failing pivots of A and H (0 = none). */
+size_t aj_log_likelihood_workspace_bytes(int64_t M, int64_t K, int64_t B, int64_t S, int64_t P);
+int aj_log_likelihood_function_via_w_tilde_from(const double* dirty_image, const double* data, const double* noise_map,
+                                                const double* w_tilde, const int64_t* pix_indexes_for_sub_slim_index,
+                                                const double* pix_weights_for_sub_slim_index, const int64_t* neighbors,
+                                                const int64_t* neighbors_sizes, int64_t M, int64_t K, int64_t B,
+                                                int64_t S, int64_t P, double* log_likelihood, int64_t* info,
+                                                void* workspace, size_t workspace_bytes, void* stream);
+
+/* Batched log-evidence: numba.py:755-857 evaluated for n samples that share one w_tilde, with an
+ * explicit regularisation coefficient per sample (the reference fixes it to 1, numba.py:819).
+ * Sample b has S_host[b] source pixels; mapping blocks are [n,M,B]; neighbour rows of all samples
+ * are concatenated ([sum S, P], [sum S]).  chi2_noise_norm is the device pair from
+ * aj_chi_squared_noise_normalization.  deflation_beta_host (may be NULL) gives per sample the
+ * rank-one shift used for log|H| (0 = plain factorisation of H).  Outputs: log_evidence[n];
+ * reconstruction[sum S] (may be NULL); logdets[n,2] = (log|A|, log|H|) (may be NULL);
+ * info[n,2] failing pivots (may be NULL). */
+size_t aj_log_evidence_batched_workspace_bytes(int64_t M, int64_t B, int64_t n_samples, const int64_t* S_host);
+int aj_log_evidence_batched(const double* w_tilde, int64_t M, const double* dirty_image, const double* chi2_noise_norm,
+                            int64_t n_samples, int64_t B, int64_t P, const int64_t* S_host, const int64_t* pix_indexes,
+                            const double* pix_weights, const int64_t* neighbors, const int64_t* neighbors_sizes,
+                            const double* coefficients_host, const double* deflation_beta_host, double* log_evidence,
+                            double* reconstruction, double* logdets, int64_t* info, void* workspace,
+                            size_t workspace_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,247 @@
+"""FP64 numpy restatement of the reference's likelihood path (test oracle only).
+
+Each function keeps the name, parameter names and parameter order of the
+function in ``src/autojax/numba.py`` it restates, and cites the lines.  The
+arithmetic is plain FP64 numpy/LAPACK; loops that the reference writes as
+explicit loops are vectorised where that cannot change the value beyond FP64
+rounding order.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+# numba.py:6-7
+TWO_PI = 2.0 * np.pi
+LOG_TWO_PI = np.log(TWO_PI)
+
+
+def mask_2d_centres_from(shape_native, pixel_scales, centre):
+    """numba.py:13-44: (y, x) scaled centre of a mask."""
+    return (
+        0.5 * (shape_native[0] - 1) - (centre[0] / pixel_scales[0]),
+        0.5 * (shape_native[1] - 1) + (centre[1] / pixel_scales[1]),
+    )
+
+
+def mask_2d_circular_from(shape_native, pixel_scales, radius, centre):
+    """numba.py:50-86: True = masked; unmasked where r <= radius."""
+    cy, cx = mask_2d_centres_from(shape_native, pixel_scales, centre)
+    ys, xs = np.indices(shape_native)
+    return (radius * radius) < (
+        np.square((ys - cy) * pixel_scales[0]) + np.square((xs - cx) * pixel_scales[1])
+    )
+
+
+def w_tilde_data_interferometer_from(visibilities_real, noise_map_real, uv_wavelengths, grid_radians_slim):
+    """numba.py:95-152.
+
+    res_i = sum_k (n_k^2 / v_k)^2 cos(2 pi (g_i1 u_k0 + g_i0 u_k1)).  The weight
+    is the reference's own expression, kept as written (numba.py:149).
+    """
+    g_2pi = TWO_PI * grid_radians_slim
+    w_k = np.square(np.square(noise_map_real) / visibilities_real)
+    res = np.zeros(grid_radians_slim.shape[0])
+    for k0 in range(0, uv_wavelengths.shape[0], 4096):
+        uv = uv_wavelengths[k0 : k0 + 4096]
+        ph = g_2pi[:, 1:2] * uv[None, :, 0] + g_2pi[:, 0:1] * uv[None, :, 1]
+        res += np.cos(ph) @ w_k[k0 : k0 + 4096]
+    return res
+
+
+def w_tilde_curvature_interferometer_from(noise_map_real, uv_wavelengths, grid_radians_slim, k_block=256):
+    """numba.py:155-207.
+
+    W_ij = sum_k n_k^-2 cos(dg_ij1 u_k0 + dg_ij0 u_k1), dg = 2 pi (g_i - g_j).
+    Same pairing as numba.py:204: grid column 1 with uv column 0, grid column
+    0 with uv column 1.  Streams K in blocks; memory is O(M^2).
+    """
+    M = grid_radians_slim.shape[0]
+    g_2pi = TWO_PI * grid_radians_slim
+    d0 = g_2pi[:, None, 0] - g_2pi[None, :, 0]
+    d1 = g_2pi[:, None, 1] - g_2pi[None, :, 1]
+    w = np.zeros((M, M))
+    inv_n2 = np.reciprocal(np.square(noise_map_real))
+    for k in range(uv_wavelengths.shape[0]):
+        w += np.cos(d1 * uv_wavelengths[k, 0] + d0 * uv_wavelengths[k, 1]) * inv_n2[k]
+    return w
+
+
+def w_compact_curvature_interferometer_from(grid_size, noise_map_real, uv_wavelengths, pixel_scale):
+    """numba.py:210-234: lattice form, N x (2N-1) unique W values."""
+    N = grid_size
+    OFFSET = N - 1
+    N_DIFF = 2 * N - 1
+    TWOPI_D = (np.pi * np.pi * pixel_scale) / 324000.0
+    d0 = (TWOPI_D * np.arange(N, dtype=np.float64)).reshape(-1, 1)
+    d1 = TWOPI_D * (np.arange(N_DIFF, dtype=np.float64) - OFFSET)
+    out = np.zeros((N, N_DIFF))
+    inv_n2 = np.square(np.reciprocal(noise_map_real))
+    for k in range(uv_wavelengths.shape[0]):
+        out += np.cos(d1 * uv_wavelengths[k, 0] - d0 * uv_wavelengths[k, 1]) * inv_n2[k]
+    return out
+
+
+def w_tilde_via_compact_from(w_compact, native_index_for_slim_index):
+    """numba.py:237-255: expand the lattice form to M x M."""
+    OFFSET = w_compact.shape[0] - 1
+    i0 = native_index_for_slim_index[:, 0]
+    i1 = native_index_for_slim_index[:, 1]
+    m = i0[:, None] - i0[None, :]
+    n = np.where(m >= 0, i1[:, None] - i1[None, :], i1[None, :] - i1[:, None]) + OFFSET
+    return w_compact[np.abs(m), n]
+
+
+def mapping_matrix_from(pix_indexes_for_sub_slim_index, pix_weights_for_sub_slim_index, pixels):
+    """numba.py:293-376: dense T[m, s] = t (assignment), rows stop at the first -1."""
+    M, B = pix_indexes_for_sub_slim_index.shape
+    T = np.zeros((M, pixels))
+    for m in range(M):
+        for b in range(B):
+            s = pix_indexes_for_sub_slim_index[m, b]
+            if s == -1:
+                break
+            T[m, s] = pix_weights_for_sub_slim_index[m, b]
+    return T
+
+
+def data_vector_from(mapping_matrix, dirty_image):
+    """numba.py:379-406: D = dirty @ T."""
+    return dirty_image @ mapping_matrix
+
+
+def curvature_matrix_via_w_tilde_from(w_tilde, mapping_matrix):
+    """numba.py:409-435: F = T^T W T (dense)."""
+    return mapping_matrix.T @ w_tilde @ mapping_matrix
+
+
+def sparse_mapping_matrix_transpose_matmul(matrix, pix_indexes_for_sub_slim_index, pix_weights_for_sub_slim_index, pixels):
+    """numba.py:552-587: T^T @ matrix from the sparse mapping (rows stop at -1)."""
+    M, S2 = matrix.shape
+    B = pix_indexes_for_sub_slim_index.shape[1]
+    F = np.zeros((pixels, S2))
+    for m in range(M):
+        for b in range(B):
+            s1 = pix_indexes_for_sub_slim_index[m, b]
+            if s1 == -1:
+                break
+            F[s1, :] += pix_weights_for_sub_slim_index[m, b] * matrix[m, :]
+    return F
+
+
+def curvature_matrix_via_w_wilde_sparse_mapping_matrix_from(w_tilde, pix_indexes_for_sub_slim_index, pix_weights_for_sub_slim_index, pixels):
+    """numba.py:590-614: F = T^T (T^T W)^T with two sparse products."""
+    TW = sparse_mapping_matrix_transpose_matmul(w_tilde, pix_indexes_for_sub_slim_index, pix_weights_for_sub_slim_index, pixels)
+    return sparse_mapping_matrix_transpose_matmul(
+        np.ascontiguousarray(TW.T), pix_indexes_for_sub_slim_index, pix_weights_for_sub_slim_index, pixels
+    )
+
+
+def constant_regularization_matrix_from(coefficient, neighbors, neighbors_sizes):
+    """numba.py:656-695: H = diag(1e-8 + c^2 sizes); H[i, k] -= c^2 per neighbour until -1."""
+    S, P = neighbors.shape
+    c2 = coefficient * coefficient
+    H = np.diag(1e-8 + c2 * neighbors_sizes.astype(np.float64))
+    for i in range(S):
+        for j in range(P):
+            k = neighbors[i, j]
+            if k == -1:
+                break
+            H[i, k] -= c2
+    return H
+
+
+def reconstruction_positive_negative_from(data_vector, curvature_reg_matrix):
+    """numba.py:698-731: s = A^-1 D via LAPACK LU."""
+    return np.linalg.solve(curvature_reg_matrix, data_vector)
+
+
+def noise_normalization_complex_from(noise_map):
+    """numba.py:734-752: 2 (K log 2 pi + sum over 2K entries of log|n|)."""
+    return 2.0 * (noise_map.size * LOG_TWO_PI + np.log(np.abs(noise_map.view(np.float64))).sum())
+
+
+def log_likelihood_function_via_w_tilde_from(
+    dirty_image,
+    data,
+    noise_map,
+    w_tilde,
+    pix_indexes_for_sub_slim_index,
+    pix_weights_for_sub_slim_index,
+    neighbors,
+    neighbors_sizes,
+):
+    """numba.py:755-857 (lambda = 1 hard-coded at numba.py:819).
+
+    logL = -1/2 (chi.chi - s.D + log|A| - log|H| + noise_norm), A = F + H.
+    """
+    S = neighbors_sizes.size
+    coefficient = 1.0
+    noise_normalization = noise_normalization_complex_from(noise_map)
+    T = mapping_matrix_from(pix_indexes_for_sub_slim_index, pix_weights_for_sub_slim_index, S)
+    F = curvature_matrix_via_w_tilde_from(w_tilde, T)
+    H = constant_regularization_matrix_from(coefficient, neighbors, neighbors_sizes)
+    A = F + H
+    D = data_vector_from(T, dirty_image)
+    s = reconstruction_positive_negative_from(D, A)
+    chi = data.view(np.float64) / noise_map.view(np.float64)
+    reg_chi2 = chi @ chi - s @ D
+    return -0.5 * (reg_chi2 + np.linalg.slogdet(A)[1] - np.linalg.slogdet(H)[1] + noise_normalization)
+
+
+def log_likelihood_original_form(dirty_image, data, noise_map, w_tilde, pix_indexes_for_sub_slim_index,
+                                 pix_weights_for_sub_slim_index, neighbors, neighbors_sizes, coefficient=1.0):
+    """original.py:890-923: chi^2 = s^T F s - 2 s^T D + sum chi^2 ; + s^T H s."""
+    S = neighbors_sizes.size
+    T = mapping_matrix_from(pix_indexes_for_sub_slim_index, pix_weights_for_sub_slim_index, S)
+    D = data_vector_from(T, dirty_image)
+    F = curvature_matrix_via_w_tilde_from(w_tilde, T)
+    H = constant_regularization_matrix_from(coefficient, neighbors, neighbors_sizes)
+    A = F + H
+    s = np.linalg.solve(A, D)
+    chi2 = s @ F @ s - 2.0 * s @ D + np.sum(data.real**2 / noise_map.real**2) + np.sum(data.imag**2 / noise_map.imag**2)
+    reg = s @ H @ s
+    nn = np.sum(np.log(2 * np.pi * noise_map.real**2)) + np.sum(np.log(2 * np.pi * noise_map.imag**2))
+    return float(-0.5 * (chi2 + reg + np.linalg.slogdet(A)[1] - np.linalg.slogdet(H)[1] + nn))
+
+
+def log_likelihood_cholesky_form(dirty_image, data, noise_map, w_tilde, pix_indexes_for_sub_slim_index,
+                                 pix_weights_for_sub_slim_index, neighbors, neighbors_sizes, coefficient=1.0):
+    """jax.py:1032-1058: Cholesky half-log-determinants of H and A."""
+    import scipy.linalg as sla
+
+    S = neighbors_sizes.size
+    T = mapping_matrix_from(pix_indexes_for_sub_slim_index, pix_weights_for_sub_slim_index, S)
+    F = curvature_matrix_via_w_tilde_from(w_tilde, T)
+    H = constant_regularization_matrix_from(coefficient, neighbors, neighbors_sizes)
+    A = F + H
+    D = data_vector_from(T, dirty_image)
+    half_log_h = np.log(np.diag(np.linalg.cholesky(H))).sum()
+    L = np.linalg.cholesky(A)
+    s = sla.cho_solve((L, True), D)
+    half_log_a = np.log(np.diag(L)).sum()
+    chi_r = data.real / noise_map.real
+    chi_i = data.imag / noise_map.imag
+    r = chi_r @ chi_r + chi_i @ chi_i - s @ D
+    nn = noise_normalization_complex_from(noise_map)
+    return -0.5 * (r + nn) + half_log_h - half_log_a
+
+
+# ---------------------------------------------------------------------------
+# Explicit-lambda forms used by the batched path (the reference fixes lambda
+# = 1; these are the same formulas with the coefficient as a parameter).
+# ---------------------------------------------------------------------------
+
+
+def log_evidence_from_parts(F, D, chi_squared, noise_normalization, neighbors, neighbors_sizes, coefficient,
+                            logdet_h=None):
+    """numba.py:839-857 with an explicit coefficient.  Returns (logL, s, logdet A, logdet H)."""
+    H = constant_regularization_matrix_from(coefficient, neighbors, neighbors_sizes)
+    A = F + H
+    L = np.linalg.cholesky(A)
+    y = np.linalg.solve(L, D)
+    s = np.linalg.solve(L.T, y)
+    ld_a = 2.0 * np.log(np.diag(L)).sum()
+    if logdet_h is None:
+        logdet_h = np.linalg.slogdet(H)[1]
+    return -0.5 * (chi_squared - s @ D + ld_a - logdet_h + noise_normalization), s, ld_a, logdet_h

@@ -1,0 +1,344 @@
+"""Drop-in for the likelihood path of ``autojax.numba`` on MI355X.
+
+Every public function here has the name, parameter names and parameter order
+of the function of the same name in the reference's ``src/autojax/numba.py``
+(cited per function), so that the reference's name-binding harness
+(``tests.py:564-575``) can call it unchanged.  Arguments may be numpy arrays
+(results come back as numpy, like numba) or torch device tensors (results
+stay on the device, on the current stream).  Extra options are keyword-only
+with defaults, so positional calls are the reference's.
+
+All arithmetic runs in libautojax_amd (HIP, gfx950); this module validates
+arguments, allocates outputs and workspaces, and maps errors to exceptions.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import device as dv
+
+TWO_PI = 2.0 * np.pi
+LOG_TWO_PI = np.log(TWO_PI)
+
+
+# ---------------------------------------------------------------------------
+# argument handling
+# ---------------------------------------------------------------------------
+
+def _is_t(x) -> bool:
+    return isinstance(x, torch.Tensor)
+
+
+def _dev(x, dtype, name, ndim=None):
+    """Validate dtype/rank and return a contiguous device tensor."""
+    if _is_t(x):
+        want = {np.float64: torch.float64, np.int64: torch.int64, np.complex128: torch.complex128}[dtype]
+        if x.dtype != want:
+            raise TypeError(f"{name}: expected {want}, got {x.dtype}")
+        if ndim is not None and x.dim() != ndim:
+            raise ValueError(f"{name}: expected {ndim}-d, got shape {tuple(x.shape)}")
+        return dv.to_device(x)
+    a = np.asarray(x)
+    if a.dtype != np.dtype(dtype):
+        raise TypeError(f"{name}: expected {np.dtype(dtype)}, got {a.dtype}")
+    if ndim is not None and a.ndim != ndim:
+        raise ValueError(f"{name}: expected {ndim}-d, got shape {a.shape}")
+    return dv.to_device(a)
+
+
+def _host(x):
+    return x.detach().cpu().numpy() if _is_t(x) else np.asarray(x)
+
+
+def _out(t, like_torch: bool):
+    return t if like_torch else dv.to_host(t)
+
+
+def _any_t(*xs) -> bool:
+    return any(_is_t(x) for x in xs)
+
+
+def _view_f8(c: torch.Tensor) -> torch.Tensor:
+    """complex128 [K] -> interleaved float64 [2K] view."""
+    return torch.view_as_real(c).reshape(-1)
+
+
+def validate_mapping(pix_indexes_for_sub_slim_index, pixels, unique=True):
+    """Indices must lie in [-1, pixels); with ``unique`` no index repeats within a row.
+
+    The reference writes out of bounds for other values (numba.py:371-375);
+    repeated indices are assigned in the dense form and summed in the sparse
+    forms of the reference, so they are rejected here.
+    """
+    idx = _host(pix_indexes_for_sub_slim_index)
+    if idx.size and (idx.min() < -1 or idx.max() >= pixels):
+        raise ValueError(f"pix_indexes_for_sub_slim_index out of range [-1, {pixels})")
+    if unique and idx.shape[1] > 1:
+        live = np.ones(idx.shape, bool)
+        live[:, 1:] = np.cumprod(idx[:, :-1] != -1, axis=1).astype(bool)
+        s = np.where(live & (idx >= 0), idx, -2 - np.arange(idx.shape[1])[None, :])
+        ss = np.sort(s, axis=1)
+        if np.any((ss[:, 1:] == ss[:, :-1]) & (ss[:, 1:] >= 0)):
+            raise ValueError("pix_indexes_for_sub_slim_index repeats a source pixel within a row")
+
+
+def validate_neighbors(neighbors, neighbors_sizes, symmetric=True):
+    nb = _host(neighbors)
+    sz = _host(neighbors_sizes)
+    S = sz.size
+    if nb.shape[0] != S:
+        raise ValueError(f"neighbors has {nb.shape[0]} rows, neighbors_sizes {S}")
+    if nb.size and (nb.min() < -1 or nb.max() >= S):
+        raise ValueError(f"neighbors out of range [-1, {S})")
+    if symmetric:
+        live = np.ones(nb.shape, bool)
+        live[:, 1:] = np.cumprod(nb[:, :-1] != -1, axis=1).astype(bool)
+        live &= nb >= 0
+        i = np.repeat(np.arange(S), nb.shape[1])[live.ravel()]
+        k = nb.ravel()[live.ravel()]
+        a = set(zip(i.tolist(), k.tolist()))
+        if any((kk, ii) not in a for ii, kk in a):
+            raise ValueError("neighbors is not a symmetric relation: the regularisation matrix would not be symmetric")
+
+
+# ---------------------------------------------------------------------------
+# numba.py geometry helpers (host-side, no GPU work)
+# ---------------------------------------------------------------------------
+
+def mask_2d_centres_from(shape_native, pixel_scales, centre):
+    """numba.py:13-44."""
+    return (
+        0.5 * (shape_native[0] - 1) - (centre[0] / pixel_scales[0]),
+        0.5 * (shape_native[1] - 1) + (centre[1] / pixel_scales[1]),
+    )
+
+
+def mask_2d_circular_from(shape_native, pixel_scales, radius, centre):
+    """numba.py:50-86 (True = masked)."""
+    cy, cx = mask_2d_centres_from(shape_native, pixel_scales, centre)
+    ys, xs = np.indices(shape_native)
+    return (radius * radius) < (np.square((ys - cy) * pixel_scales[0]) + np.square((xs - cx) * pixel_scales[1]))
+
+
+# ---------------------------------------------------------------------------
+# W-tilde and data-side sums
+# ---------------------------------------------------------------------------
+
+def w_tilde_curvature_interferometer_from(noise_map_real, uv_wavelengths, grid_radians_slim, *, precision="fp64"):
+    """numba.py:155-207 -> f8[M, M].
+
+    ``precision``: "fp64" (default, the reference's arithmetic), "int8x3" /
+    "int8x4" (exact integer MFMA on 3/4 8-bit digits of FP32 features),
+    "fp32" or "bf16x3" (FP32-accumulated MFMA).
+    """
+    torch_in = _any_t(noise_map_real, uv_wavelengths, grid_radians_slim)
+    n = _dev(noise_map_real, np.float64, "noise_map_real", 1)
+    uv = _dev(uv_wavelengths, np.float64, "uv_wavelengths", 2)
+    g = _dev(grid_radians_slim, np.float64, "grid_radians_slim", 2)
+    K, M = n.shape[0], g.shape[0]
+    if uv.shape != (K, 2) or g.shape != (M, 2):
+        raise ValueError("shape mismatch: noise_map_real [K], uv_wavelengths [K,2], grid_radians_slim [M,2]")
+    prec = _lib.PRECISIONS[precision]
+    out = dv.empty((M, M))
+    ws = dv.workspace(_lib.query("aj_w_tilde_workspace_bytes", K, M, prec))
+    _lib.call("aj_w_tilde_curvature_interferometer_from", n.data_ptr(), uv.data_ptr(), g.data_ptr(), K, M, prec,
+              out.data_ptr(), ws.data_ptr(), ws.numel(), dv.stream_ptr())
+    return _out(out, torch_in)
+
+
+def w_tilde_data_interferometer_from(visibilities_real, noise_map_real, uv_wavelengths, grid_radians_slim):
+    """numba.py:95-152 -> f8[M]."""
+    torch_in = _any_t(visibilities_real, noise_map_real, uv_wavelengths, grid_radians_slim)
+    v = _dev(visibilities_real, np.float64, "visibilities_real", 1)
+    n = _dev(noise_map_real, np.float64, "noise_map_real", 1)
+    uv = _dev(uv_wavelengths, np.float64, "uv_wavelengths", 2)
+    g = _dev(grid_radians_slim, np.float64, "grid_radians_slim", 2)
+    K, M = v.shape[0], g.shape[0]
+    out = dv.empty((M,))
+    ws = dv.workspace(_lib.query("aj_visibility_sum_workspace_bytes", K, M))
+    _lib.call("aj_w_tilde_data_interferometer_from", v.data_ptr(), n.data_ptr(), uv.data_ptr(), g.data_ptr(), K, M,
+              out.data_ptr(), ws.data_ptr(), ws.numel(), dv.stream_ptr())
+    return _out(out, torch_in)
+
+
+def dirty_image_from(data, noise_map, uv_wavelengths, grid_radians_slim):
+    """Dirty image consistent with W-tilde (no reference counterpart; data preparation)."""
+    torch_in = _any_t(data, noise_map, uv_wavelengths, grid_radians_slim)
+    d = _view_f8(_dev(data, np.complex128, "data", 1))
+    n = _view_f8(_dev(noise_map, np.complex128, "noise_map", 1))
+    uv = _dev(uv_wavelengths, np.float64, "uv_wavelengths", 2)
+    g = _dev(grid_radians_slim, np.float64, "grid_radians_slim", 2)
+    K, M = uv.shape[0], g.shape[0]
+    out = dv.empty((M,))
+    ws = dv.workspace(_lib.query("aj_visibility_sum_workspace_bytes", K, M))
+    _lib.call("aj_dirty_image_from", d.data_ptr(), n.data_ptr(), uv.data_ptr(), g.data_ptr(), K, M, out.data_ptr(),
+              ws.data_ptr(), ws.numel(), dv.stream_ptr())
+    return _out(out, torch_in)
+
+
+# ---------------------------------------------------------------------------
+# mapping, data vector, curvature
+# ---------------------------------------------------------------------------
+
+def mapping_matrix_from(pix_indexes_for_sub_slim_index, pix_weights_for_sub_slim_index, pixels):
+    """numba.py:293-376 -> f8[M, pixels] (assignment; rows stop at the first -1)."""
+    torch_in = _any_t(pix_indexes_for_sub_slim_index, pix_weights_for_sub_slim_index)
+    validate_mapping(pix_indexes_for_sub_slim_index, pixels, unique=False)
+    idx = _dev(pix_indexes_for_sub_slim_index, np.int64, "pix_indexes_for_sub_slim_index", 2)
+    w = _dev(pix_weights_for_sub_slim_index, np.float64, "pix_weights_for_sub_slim_index", 2)
+    M, B = idx.shape
+    out = dv.empty((M, int(pixels)))
+    _lib.call("aj_mapping_matrix_from", idx.data_ptr(), w.data_ptr(), M, B, int(pixels), out.data_ptr(),
+              dv.stream_ptr())
+    return _out(out, torch_in)
+
+
+def data_vector_from(mapping_matrix, dirty_image):
+    """numba.py:379-406 -> f8[S] = dirty_image @ mapping_matrix."""
+    torch_in = _any_t(mapping_matrix, dirty_image)
+    T = _dev(mapping_matrix, np.float64, "mapping_matrix", 2)
+    y = _dev(dirty_image, np.float64, "dirty_image", 1)
+    M, S = T.shape
+    if y.shape[0] != M:
+        raise ValueError("dirty_image length must equal mapping_matrix rows")
+    out = dv.empty((S,))
+    _lib.call("aj_data_vector_from", T.data_ptr(), y.data_ptr(), M, S, out.data_ptr(), dv.stream_ptr())
+    return _out(out, torch_in)
+
+
+def curvature_matrix_via_w_tilde_from(w_tilde, mapping_matrix):
+    """numba.py:409-435 -> f8[S, S] = T^T W T (dense, FP64 MFMA)."""
+    torch_in = _any_t(w_tilde, mapping_matrix)
+    W = _dev(w_tilde, np.float64, "w_tilde", 2)
+    T = _dev(mapping_matrix, np.float64, "mapping_matrix", 2)
+    M, S = T.shape
+    if W.shape != (M, M):
+        raise ValueError("w_tilde must be [M, M] with M = mapping_matrix rows")
+    out = dv.empty((S, S))
+    ws = dv.workspace(_lib.query("aj_curvature_dense_workspace_bytes", M, S))
+    _lib.call("aj_curvature_matrix_via_w_tilde_from", W.data_ptr(), T.data_ptr(), M, S, out.data_ptr(), ws.data_ptr(),
+              ws.numel(), dv.stream_ptr())
+    return _out(out, torch_in)
+
+
+def sparse_mapping_matrix_transpose_matmul(matrix, pix_indexes_for_sub_slim_index, pix_weights_for_sub_slim_index,
+                                           pixels):
+    """numba.py:552-587 -> f8[pixels, S2] = T^T @ matrix."""
+    torch_in = _any_t(matrix, pix_indexes_for_sub_slim_index, pix_weights_for_sub_slim_index)
+    validate_mapping(pix_indexes_for_sub_slim_index, pixels)
+    X = _dev(matrix, np.float64, "matrix", 2)
+    idx = _dev(pix_indexes_for_sub_slim_index, np.int64, "pix_indexes_for_sub_slim_index", 2)
+    w = _dev(pix_weights_for_sub_slim_index, np.float64, "pix_weights_for_sub_slim_index", 2)
+    M, S2 = X.shape
+    B = idx.shape[1]
+    if idx.shape[0] != M:
+        raise ValueError("matrix rows must equal mapping rows")
+    out = dv.empty((int(pixels), S2))
+    ws = dv.workspace(_lib.query("aj_sparse_transpose_matmul_workspace_bytes", M, B, int(pixels)))
+    _lib.call("aj_sparse_mapping_matrix_transpose_matmul", X.data_ptr(), idx.data_ptr(), w.data_ptr(), M, S2, B,
+              int(pixels), out.data_ptr(), ws.data_ptr(), ws.numel(), dv.stream_ptr())
+    return _out(out, torch_in)
+
+
+def curvature_matrix_via_w_wilde_sparse_mapping_matrix_from(w_tilde, pix_indexes_for_sub_slim_index,
+                                                            pix_weights_for_sub_slim_index, pixels):
+    """numba.py:590-614 -> f8[pixels, pixels] (sparse mapping; w_tilde symmetric)."""
+    torch_in = _any_t(w_tilde, pix_indexes_for_sub_slim_index, pix_weights_for_sub_slim_index)
+    validate_mapping(pix_indexes_for_sub_slim_index, pixels)
+    W = _dev(w_tilde, np.float64, "w_tilde", 2)
+    idx = _dev(pix_indexes_for_sub_slim_index, np.int64, "pix_indexes_for_sub_slim_index", 2)
+    w = _dev(pix_weights_for_sub_slim_index, np.float64, "pix_weights_for_sub_slim_index", 2)
+    M, B = idx.shape
+    if W.shape != (M, M):
+        raise ValueError("w_tilde must be [M, M]")
+    out = dv.empty((int(pixels), int(pixels)))
+    ws = dv.workspace(_lib.query("aj_curvature_sparse_workspace_bytes", M, B, int(pixels)))
+    _lib.call("aj_curvature_matrix_via_w_wilde_sparse_mapping_matrix_from", W.data_ptr(), idx.data_ptr(),
+              w.data_ptr(), M, B, int(pixels), out.data_ptr(), ws.data_ptr(), ws.numel(), dv.stream_ptr())
+    return _out(out, torch_in)
+
+
+# ---------------------------------------------------------------------------
+# regularisation, solve, normalisation, likelihood
+# ---------------------------------------------------------------------------
+
+def constant_regularization_matrix_from(coefficient, neighbors, neighbors_sizes):
+    """numba.py:656-695 -> f8[S, S]."""
+    torch_in = _any_t(neighbors, neighbors_sizes)
+    validate_neighbors(neighbors, neighbors_sizes, symmetric=False)
+    nb = _dev(neighbors, np.int64, "neighbors", 2)
+    sz = _dev(neighbors_sizes, np.int64, "neighbors_sizes", 1)
+    S, P = nb.shape
+    out = dv.empty((S, S))
+    _lib.call("aj_constant_regularization_matrix_from", float(coefficient), nb.data_ptr(), sz.data_ptr(), S, P,
+              out.data_ptr(), dv.stream_ptr())
+    return _out(out, torch_in)
+
+
+def reconstruction_positive_negative_from(data_vector, curvature_reg_matrix):
+    """numba.py:698-731 -> f8[S] solving A s = D (FP64 Cholesky; A symmetric positive definite).
+
+    Raises numpy.linalg.LinAlgError when A is not positive definite.
+    """
+    torch_in = _any_t(data_vector, curvature_reg_matrix)
+    D = _dev(data_vector, np.float64, "data_vector", 1)
+    A = _dev(curvature_reg_matrix, np.float64, "curvature_reg_matrix", 2)
+    S = D.shape[0]
+    if A.shape != (S, S):
+        raise ValueError("curvature_reg_matrix must be [S, S]")
+    out = dv.empty((S,))
+    info = torch.zeros(1, dtype=torch.int64, device=out.device)
+    ws = dv.workspace(_lib.query("aj_solve_workspace_bytes", S))
+    _lib.call("aj_reconstruction_positive_negative_from", D.data_ptr(), A.data_ptr(), S, out.data_ptr(), None,
+              info.data_ptr(), ws.data_ptr(), ws.numel(), dv.stream_ptr())
+    bad = int(info.item())
+    if bad:
+        raise np.linalg.LinAlgError(f"curvature_reg_matrix is not positive definite (pivot {bad})")
+    return _out(out, torch_in)
+
+
+def noise_normalization_complex_from(noise_map):
+    """numba.py:734-752 -> float."""
+    n = _view_f8(_dev(noise_map, np.complex128, "noise_map", 1))
+    out = dv.empty((1,))
+    _lib.call("aj_noise_normalization_complex_from", n.data_ptr(), n.numel() // 2, out.data_ptr(), dv.stream_ptr())
+    return float(out.item())
+
+
+def log_likelihood_function_via_w_tilde_from(dirty_image, data, noise_map, w_tilde, pix_indexes_for_sub_slim_index,
+                                             pix_weights_for_sub_slim_index, neighbors, neighbors_sizes):
+    """numba.py:755-857 -> float (coefficient 1, numba.py:819).
+
+    Sparse mapping path: F = T^T W T by source-pixel gathers, FP64 Cholesky of
+    A = F + H and of H (the reference's two LU slogdets and solve).
+    """
+    S = int(_host(neighbors_sizes).size)
+    validate_mapping(pix_indexes_for_sub_slim_index, S)
+    validate_neighbors(neighbors, neighbors_sizes)
+    y = _dev(dirty_image, np.float64, "dirty_image", 1)
+    d = _view_f8(_dev(data, np.complex128, "data", 1))
+    n = _view_f8(_dev(noise_map, np.complex128, "noise_map", 1))
+    W = _dev(w_tilde, np.float64, "w_tilde", 2)
+    idx = _dev(pix_indexes_for_sub_slim_index, np.int64, "pix_indexes_for_sub_slim_index", 2)
+    w = _dev(pix_weights_for_sub_slim_index, np.float64, "pix_weights_for_sub_slim_index", 2)
+    nb = _dev(neighbors, np.int64, "neighbors", 2)
+    sz = _dev(neighbors_sizes, np.int64, "neighbors_sizes", 1)
+    M, B = idx.shape
+    K = d.numel() // 2
+    P = nb.shape[1]
+    if W.shape != (M, M) or y.shape[0] != M or n.numel() != 2 * K:
+        raise ValueError("shape mismatch between dirty_image, w_tilde, mapping, data and noise_map")
+    out = dv.empty((1,))
+    info = torch.zeros(2, dtype=torch.int64, device=out.device)
+    ws = dv.workspace(_lib.query("aj_log_likelihood_workspace_bytes", M, K, B, S, P))
+    _lib.call("aj_log_likelihood_function_via_w_tilde_from", y.data_ptr(), d.data_ptr(), n.data_ptr(), W.data_ptr(),
+              idx.data_ptr(), w.data_ptr(), nb.data_ptr(), sz.data_ptr(), M, K, B, S, P, out.data_ptr(),
+              info.data_ptr(), ws.data_ptr(), ws.numel(), dv.stream_ptr())
+    inf = info.tolist()
+    if inf[0] or inf[1]:
+        which = "curvature_reg_matrix" if inf[0] else "regularization_matrix"
+        raise np.linalg.LinAlgError(f"{which} is not positive definite (pivot {inf[0] or inf[1]})")
+    return float(out.item())

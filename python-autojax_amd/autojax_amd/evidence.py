@@ -1,0 +1,169 @@
+"""Batched log-evidence on one shared W-tilde (device API).
+
+A *sample* is one lens model: a mapping (pix_indexes / pix_weights, [M, B]),
+a source neighbour graph and a regularisation coefficient.  Many samples
+share the dataset (grid, visibilities, W-tilde, dirty image), as in a
+sampler that proposes lens models against fixed data.  ``SampleBatch``
+uploads a list of samples once; ``log_evidence`` evaluates all of them with
+one pass over W-tilde (numba.py:755-857 per sample, explicit coefficient).
+
+log|H| uses a rank-one deflation when the neighbour graph is a connected,
+symmetric graph with sizes equal to the listed neighbour counts: then
+H 1 = 1e-8 * 1 exactly and
+    log|H| = log|H + (beta/S) 1 1^T| - log(1e-8 + beta) + log(1e-8),
+with beta = c^2 * mean degree, which removes the 1e-8 pivot that makes a
+plain factorisation of H lose digits at large coefficients.
+"""
+
+from __future__ import annotations
+
+from collections import deque
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import device as dv
+from .api import validate_mapping, validate_neighbors
+
+
+def deflation_beta(neighbors, neighbors_sizes, coefficient) -> float:
+    """beta for the rank-one deflation of log|H|, or 0 if the graph does not allow it."""
+    nb = np.asarray(neighbors)
+    sz = np.asarray(neighbors_sizes)
+    S = sz.size
+    live = np.ones(nb.shape, bool)
+    if nb.shape[1] > 1:
+        live[:, 1:] = np.cumprod(nb[:, :-1] != -1, axis=1).astype(bool)
+    live &= nb >= 0
+    if np.any(live.sum(axis=1) != sz):
+        return 0.0
+    if np.any((nb == np.arange(S)[:, None]) & live):
+        return 0.0
+    adj = [nb[i][live[i]] for i in range(S)]
+    seen = np.zeros(S, bool)
+    seen[0] = True
+    q = deque([0])
+    while q:
+        i = q.popleft()
+        for k in adj[i]:
+            if not seen[k]:
+                seen[k] = True
+                q.append(k)
+    if not seen.all():
+        return 0.0
+    return float(coefficient) ** 2 * float(sz.mean())
+
+
+class SampleBatch:
+    """Samples uploaded to the device, ready for ``log_evidence``."""
+
+    def __init__(self, samples, M, deflate=True, validate=True):
+        if len(samples) == 0:
+            raise ValueError("empty batch")
+        B = samples[0].pix_indexes_for_sub_slim_index.shape[1]
+        P = max(s.neighbors.shape[1] for s in samples)
+        self.n = len(samples)
+        self.M = int(M)
+        self.B = int(B)
+        self.P = int(P)
+        self.S_host = np.array([s.pixels for s in samples], dtype=np.int64)
+        idx = np.empty((self.n, M, B), np.int64)
+        wts = np.empty((self.n, M, B), np.float64)
+        nbr = np.full((int(self.S_host.sum()), P), -1, np.int64)
+        nsz = np.empty(int(self.S_host.sum()), np.int64)
+        off = 0
+        for b, s in enumerate(samples):
+            if s.pix_indexes_for_sub_slim_index.shape != (M, B):
+                raise ValueError(f"sample {b}: mapping shape {s.pix_indexes_for_sub_slim_index.shape} != {(M, B)}")
+            if validate:
+                validate_mapping(s.pix_indexes_for_sub_slim_index, s.pixels)
+                validate_neighbors(s.neighbors, s.neighbors_sizes)
+            idx[b] = s.pix_indexes_for_sub_slim_index
+            wts[b] = s.pix_weights_for_sub_slim_index
+            S = s.pixels
+            nbr[off : off + S, : s.neighbors.shape[1]] = s.neighbors
+            nsz[off : off + S] = s.neighbors_sizes
+            off += S
+        self.coef_host = np.array([float(s.coefficient) for s in samples], np.float64)
+        # mean degree of deflatable graphs (0 = plain factorisation of H)
+        self.deg_host = np.array(
+            [deflation_beta(s.neighbors, s.neighbors_sizes, 1.0) if deflate else 0.0 for s in samples], np.float64
+        )
+        self.beta_host = self.coef_host**2 * self.deg_host
+        self.idx = dv.to_device(idx)
+        self.wts = dv.to_device(wts)
+        self.nbr = dv.to_device(nbr)
+        self.nsz = dv.to_device(nsz)
+        self.ws = dv.workspace(
+            _lib.query("aj_log_evidence_batched_workspace_bytes", self.M, self.B, self.n,
+                       self.S_host.ctypes.data_as(_lib.C.c_void_p))
+        )
+        dev = self.idx.device
+        self.out = torch.empty(self.n, dtype=torch.float64, device=dev)
+        self.logdets = torch.empty((self.n, 2), dtype=torch.float64, device=dev)
+        self.info = torch.zeros((self.n, 2), dtype=torch.int64, device=dev)
+        self.recon = torch.empty(int(self.S_host.sum()), dtype=torch.float64, device=dev)
+
+    def set_coefficients(self, coefficients):
+        c = np.asarray(coefficients, np.float64).reshape(self.n).copy()
+        self.coef_host = c
+        self.beta_host = c**2 * self.deg_host
+
+    def log_evidence(self, w_tilde, dirty_image, chi2_noise_norm, check=False):
+        """Evaluate all samples; returns the device tensor [n] (valid after the stream runs).
+
+        ``check`` synchronises and raises LinAlgError for a non-positive-definite matrix.
+        """
+        if w_tilde.shape != (self.M, self.M) or w_tilde.dtype != torch.float64:
+            raise ValueError("w_tilde must be float64 [M, M] on the device")
+        _lib.call(
+            "aj_log_evidence_batched",
+            w_tilde.data_ptr(), self.M, dirty_image.data_ptr(), chi2_noise_norm.data_ptr(), self.n, self.B, self.P,
+            self.S_host.ctypes.data_as(_lib.C.c_void_p), self.idx.data_ptr(), self.wts.data_ptr(),
+            self.nbr.data_ptr(), self.nsz.data_ptr(), self.coef_host.ctypes.data_as(_lib.C.c_void_p),
+            self.beta_host.ctypes.data_as(_lib.C.c_void_p), self.out.data_ptr(), self.recon.data_ptr(),
+            self.logdets.data_ptr(), self.info.data_ptr(), self.ws.data_ptr(), self.ws.numel(), dv.stream_ptr(),
+        )
+        if check:
+            bad = torch.nonzero(self.info.any(dim=1)).flatten().tolist()
+            if bad:
+                raise np.linalg.LinAlgError(f"samples {bad}: matrix not positive definite, info={self.info[bad].tolist()}")
+        return self.out
+
+    def reconstructions(self):
+        off = np.concatenate([[0], np.cumsum(self.S_host)])
+        return [self.recon[off[b] : off[b + 1]] for b in range(self.n)]
+
+
+def chi2_noise_norm(data, noise_map):
+    """Device pair (chi.chi, noise normalisation) of numba.py:846-847 and :752."""
+    d = torch.view_as_real(dv.to_device(data)).reshape(-1)
+    n = torch.view_as_real(dv.to_device(noise_map)).reshape(-1)
+    out = dv.empty((2,))
+    _lib.call("aj_chi_squared_noise_normalization", d.data_ptr(), n.data_ptr(), d.numel() // 2, out.data_ptr(),
+              dv.stream_ptr())
+    return out
+
+
+def w_tilde(noise_map_real, uv_wavelengths, grid_radians_slim, precision="fp64", out=None, workspace=None):
+    """Device W-tilde [M, M] float64 at the requested arithmetic precision."""
+    n = dv.to_device(noise_map_real)
+    uv = dv.to_device(uv_wavelengths)
+    g = dv.to_device(grid_radians_slim)
+    K, M = n.shape[0], g.shape[0]
+    prec = _lib.PRECISIONS[precision]
+    if out is None:
+        out = dv.empty((M, M))
+    if workspace is None:
+        workspace = dv.workspace(_lib.query("aj_w_tilde_workspace_bytes", K, M, prec))
+    _lib.call("aj_w_tilde_curvature_interferometer_from", n.data_ptr(), uv.data_ptr(), g.data_ptr(), K, M, prec,
+              out.data_ptr(), workspace.data_ptr(), workspace.numel(), dv.stream_ptr())
+    return out
+
+
+def dirty_image(data, noise_map, uv_wavelengths, grid_radians_slim):
+    from .api import dirty_image_from
+
+    return dirty_image_from(dv.to_device(data), dv.to_device(noise_map), dv.to_device(uv_wavelengths),
+                            dv.to_device(grid_radians_slim))

@@ -1,0 +1,80 @@
+// Shared definitions for the gfx950 kernels of libautojax_amd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <algorithm>
+#include <type_traits>
+
+#include "../../include/autojax_amd.h"
+
+namespace aj {
+
+// Thread-local error text returned by aj_last_error().
+void set_error(const char* fmt, ...);
+
+// Optional HIP-event timing of selected launches (aj_timing_enable / aj_timing_read).
+void timer_begin(int id, hipStream_t st);
+void timer_end(int id, hipStream_t st);
+
+#define AJ_CHECK_HIP(expr)                                                              \
+  do {                                                                                  \
+    hipError_t _e = (expr);                                                             \
+    if (_e != hipSuccess) {                                                             \
+      ::aj::set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), __FILE__,  \
+                      __LINE__);                                                        \
+      return AJ_EHIP;                                                                   \
+    }                                                                                   \
+  } while (0)
+
+#define AJ_REQUIRE(cond, ...)          \
+  do {                                 \
+    if (!(cond)) {                     \
+      ::aj::set_error(__VA_ARGS__);    \
+      return AJ_EINVAL;                \
+    }                                  \
+  } while (0)
+
+#define AJ_LAUNCH_CHECK()                                                               \
+  do {                                                                                  \
+    hipError_t _e = hipGetLastError();                                                  \
+    if (_e != hipSuccess) {                                                             \
+      ::aj::set_error("kernel launch failed: %s (%s:%d)", hipGetErrorString(_e),        \
+                      __FILE__, __LINE__);                                              \
+      return AJ_EHIP;                                                                   \
+    }                                                                                   \
+  } while (0)
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// Bump allocator over a caller-provided workspace.
+struct Arena {
+  char* base;
+  size_t cap;
+  size_t off = 0;
+  Arena(void* p, size_t n) : base(static_cast<char*>(p)), cap(n) {}
+  template <class T>
+  T* take(size_t count) {
+    off = align_up(off, 256);
+    T* p = reinterpret_cast<T*>(base + off);
+    off += count * sizeof(T);
+    return p;
+  }
+  bool ok() const { return off <= cap; }
+};
+
+// Wave-level sum of a double over the 64 lanes of a wavefront.
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+}  // namespace aj

@@ -1,0 +1,1134 @@
+// Likelihood path after W-tilde: sparse mapping, data vector, curvature,
+// regularisation, FP64 Cholesky, solves and the log-evidence, for one sample
+// or a batch of samples sharing one W-tilde.
+//
+// Reference: src/autojax/numba.py
+//   mapping_matrix_from                   :293-376  (rows stop at the first -1)
+//   data_vector_from                      :379-406  (D = dirty @ T)
+//   curvature via sparse mapping          :552-614  (F = T^T W T)
+//   constant_regularization_matrix_from   :656-695
+//   reconstruction_positive_negative_from :698-731  (here: Cholesky, as jax.py:1032-1044)
+//   noise_normalization_complex_from      :734-752
+//   log_likelihood_function_via_w_tilde_from :755-857
+//
+// Design: the mapping is inverted once per sample into source-pixel buckets
+// (CSC, sorted by image pixel), so every gather below is a deterministic
+// ordered sum with no floating-point atomics.
+#include "aj_common.h"
+
+#include <vector>
+
+namespace aj {
+namespace lk {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// Per-sample table, device resident.
+struct SampleDesc {
+  int64_t S;          // source pixels
+  int64_t s_off;      // offset of this sample's rows in neighbor / D / reconstruction arrays
+  int64_t map_off;    // offset of this sample's [M,B] mapping block
+  int64_t csc_ptr;    // offset of the S+1 bucket pointers
+  int64_t csc_ent;    // offset of the M*B bucket entries
+  int64_t omega_off;  // offset of the M*S omega block (doubles)
+  int64_t mat_off;    // offset of the S*S matrices A and H (doubles)
+  double coef2;       // regularisation coefficient squared
+  double beta;        // rank-one deflation shift for log|H| (0 = none)
+};
+
+// ---------------------------------------------------------------------------
+// Mapping inversion (CSC): bucket s holds the entries (m, t) with idx[m,b] = s
+// for b before the first -1, ordered by (m, b).
+// ---------------------------------------------------------------------------
+__global__ void csc_count_kernel(const SampleDesc* __restrict__ sd, const int64_t* __restrict__ idx, int64_t M,
+                                 int64_t B, int* __restrict__ ptrs) {
+  const int smp = blockIdx.y;
+  const SampleDesc d = sd[smp];
+  int* cnt = ptrs + d.csc_ptr;
+  const int64_t* ix = idx + d.map_off;
+  for (int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; m < M; m += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t b = 0; b < B; ++b) {
+      int64_t s = ix[m * B + b];
+      if (s == -1) break;
+      if (s >= 0 && s < d.S) atomicAdd(&cnt[s + 1], 1);
+    }
+  }
+}
+
+// Inclusive scan of counts in place (ptr[0] = 0): one block per sample.
+__global__ void csc_scan_kernel(const SampleDesc* __restrict__ sd, int* __restrict__ ptrs, int* __restrict__ cursor) {
+  const SampleDesc d = sd[blockIdx.x];
+  int* p = ptrs + d.csc_ptr;
+  int* cur = cursor + d.csc_ptr;
+  __shared__ int carry;
+  __shared__ int buf[1024];
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int64_t base = 1; base <= d.S; base += 1024) {
+    int64_t i = base + threadIdx.x;
+    int v = (i <= d.S) ? p[i] : 0;
+    buf[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+      int t = (threadIdx.x >= o) ? buf[threadIdx.x - o] : 0;
+      __syncthreads();
+      buf[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i <= d.S) p[i] = buf[threadIdx.x] + carry;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += buf[1023];
+    __syncthreads();
+  }
+  for (int64_t i = threadIdx.x; i <= d.S; i += 1024) cur[i] = p[i];
+}
+
+__global__ void csc_fill_kernel(const SampleDesc* __restrict__ sd, const int64_t* __restrict__ idx,
+                                const double* __restrict__ wts, int64_t M, int64_t B, int* __restrict__ cursor,
+                                int* __restrict__ ent_key, double* __restrict__ ent_t) {
+  const int smp = blockIdx.y;
+  const SampleDesc d = sd[smp];
+  int* cur = cursor + d.csc_ptr;
+  const int64_t* ix = idx + d.map_off;
+  const double* wt = wts + d.map_off;
+  for (int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; m < M; m += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t b = 0; b < B; ++b) {
+      int64_t s = ix[m * B + b];
+      if (s == -1) break;
+      if (s >= 0 && s < d.S) {
+        int pos = atomicAdd(&cur[s], 1);
+        ent_key[d.csc_ent + pos] = (int)(m * B + b);
+        ent_t[d.csc_ent + pos] = wt[m * B + b];
+      }
+    }
+  }
+}
+
+// Sort each bucket by key (m*B + b): insertion sort, buckets are short.
+__global__ void csc_sort_kernel(const SampleDesc* __restrict__ sd, const int* __restrict__ ptrs, int64_t B,
+                                int* __restrict__ ent_key, double* __restrict__ ent_t) {
+  const SampleDesc d = sd[blockIdx.y];
+  const int* p = ptrs + d.csc_ptr;
+  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < d.S; s += (int64_t)gridDim.x * blockDim.x) {
+    const int lo = p[s], hi = p[s + 1];
+    int* k = ent_key + d.csc_ent;
+    double* t = ent_t + d.csc_ent;
+    for (int i = lo + 1; i < hi; ++i) {
+      int kk = k[i];
+      double tt = t[i];
+      int j = i - 1;
+      while (j >= lo && k[j] > kk) {
+        k[j + 1] = k[j];
+        t[j + 1] = t[j];
+        --j;
+      }
+      k[j + 1] = kk;
+      t[j + 1] = tt;
+    }
+    // keys now hold m*B+b; convert to m for the gathers
+    for (int i = lo; i < hi; ++i) k[i] = (int)(k[i] / B);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// D[s] = sum_{(m,t) in bucket s} t * dirty[m]     (numba.py:406)
+// ---------------------------------------------------------------------------
+__global__ void data_vector_kernel(const SampleDesc* __restrict__ sd, const int* __restrict__ ptrs,
+                                   const int* __restrict__ ent_m, const double* __restrict__ ent_t,
+                                   const double* __restrict__ dirty, double* __restrict__ D) {
+  const SampleDesc d = sd[blockIdx.y];
+  const int* p = ptrs + d.csc_ptr;
+  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < d.S; s += (int64_t)gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    for (int e = p[s]; e < p[s + 1]; ++e) acc = fma(ent_t[d.csc_ent + e], dirty[ent_m[d.csc_ent + e]], acc);
+    D[d.s_off + s] = acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Omega = W T for every sample: a workgroup stages R rows of W in LDS (read
+// once from HBM for the whole batch), then each thread forms Omega[m, s] for
+// one source pixel s and the R rows by gathering its bucket from LDS.
+// ---------------------------------------------------------------------------
+constexpr int OMEGA_THREADS = 256;
+constexpr size_t OMEGA_LDS_BUDGET = 150 * 1024;
+
+__global__ void __launch_bounds__(OMEGA_THREADS) omega_kernel(const double* __restrict__ W, int64_t M, int R,
+                                                               int nsmp, const SampleDesc* __restrict__ sd,
+                                                               const int* __restrict__ ptrs,
+                                                               const int* __restrict__ ent_m,
+                                                               const double* __restrict__ ent_t,
+                                                               double* __restrict__ omega) {
+  extern __shared__ double wrow[];  // R x M
+  const int64_t m0 = (int64_t)blockIdx.x * R;
+  const int rows = (int)std::min<int64_t>(R, M - m0);
+  for (int64_t i = threadIdx.x; i < (int64_t)rows * M; i += OMEGA_THREADS) wrow[i] = W[m0 * M + i];
+  __syncthreads();
+  for (int smp = 0; smp < nsmp; ++smp) {
+    const SampleDesc d = sd[smp];
+    const int* p = ptrs + d.csc_ptr;
+    const int* em = ent_m + d.csc_ent;
+    const double* et = ent_t + d.csc_ent;
+    double* om = omega + d.omega_off;
+    for (int64_t s = threadIdx.x; s < d.S; s += OMEGA_THREADS) {
+      double acc[8];
+#pragma unroll
+      for (int r = 0; r < 8; ++r) acc[r] = 0.0;
+      for (int e = p[s]; e < p[s + 1]; ++e) {
+        const int m = em[e];
+        const double t = et[e];
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+          if (r < rows) acc[r] = fma(t, wrow[r * M + m], acc[r]);
+      }
+#pragma unroll
+      for (int r = 0; r < 8; ++r)
+        if (r < rows) om[(m0 + r) * d.S + s] = acc[r];
+    }
+  }
+}
+
+// Same product when a row of W does not fit in LDS: gather from global memory.
+__global__ void omega_global_kernel(const double* __restrict__ W, int64_t M, int nsmp,
+                                    const SampleDesc* __restrict__ sd, const int* __restrict__ ptrs,
+                                    const int* __restrict__ ent_m, const double* __restrict__ ent_t,
+                                    double* __restrict__ omega) {
+  const int64_t m = blockIdx.x;
+  const double* w = W + m * M;
+  for (int smp = 0; smp < nsmp; ++smp) {
+    const SampleDesc d = sd[smp];
+    const int* p = ptrs + d.csc_ptr;
+    for (int64_t s = threadIdx.x; s < d.S; s += blockDim.x) {
+      double acc = 0.0;
+      for (int e = p[s]; e < p[s + 1]; ++e) acc = fma(ent_t[d.csc_ent + e], w[ent_m[d.csc_ent + e]], acc);
+      omega[d.omega_off + m * d.S + s] = acc;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// F[s1, s2] = sum_{(m,t) in bucket s1} t * Omega[m, s2], written into the
+// matrix block (lower triangle only when lower_only).  One workgroup per s1.
+// ---------------------------------------------------------------------------
+__global__ void fgather_kernel(const SampleDesc* __restrict__ sd, const int* __restrict__ ptrs,
+                               const int* __restrict__ ent_m, const double* __restrict__ ent_t,
+                               const double* __restrict__ omega, double* __restrict__ mats, int lower_only) {
+  const SampleDesc d = sd[blockIdx.y];
+  const int64_t s1 = blockIdx.x;
+  if (s1 >= d.S) return;
+  const int* p = ptrs + d.csc_ptr;
+  const int lo = p[s1], hi = p[s1 + 1];
+  const int64_t ncol = lower_only ? s1 + 1 : d.S;
+  double* out = mats + d.mat_off + s1 * d.S;
+  const double* om = omega + d.omega_off;
+  for (int64_t s2 = threadIdx.x; s2 < ncol; s2 += blockDim.x) {
+    double acc = 0.0;
+    for (int e = lo; e < hi; ++e) acc = fma(ent_t[d.csc_ent + e], om[(int64_t)ent_m[d.csc_ent + e] * d.S + s2], acc);
+    out[s2] = acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Regularisation (numba.py:686-694), added into A and written into H:
+//   H = diag(1e-8 + c^2 sizes);  H[i, k] -= c^2 for k in neighbours[i] until -1.
+// With beta > 0, H receives + beta/S everywhere (rank-one deflation of the
+// constant null vector for log|H|).  lower_only restricts both to k <= i.
+// ---------------------------------------------------------------------------
+__global__ void fill_kernel(const SampleDesc* __restrict__ sd, double* __restrict__ mats, int which_h) {
+  const SampleDesc d = sd[blockIdx.y];
+  double* H = mats + d.mat_off + (which_h ? d.S * d.S : 0);
+  const double v = d.beta > 0.0 ? d.beta / (double)d.S : 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < d.S * d.S; i += (int64_t)gridDim.x * blockDim.x)
+    H[i] = v;
+}
+
+__global__ void regularize_kernel(const SampleDesc* __restrict__ sd, const int64_t* __restrict__ nbr,
+                                  const int64_t* __restrict__ nsz, int64_t P, double* __restrict__ mats,
+                                  int add_to_a, int write_h, int lower_only) {
+  const SampleDesc d = sd[blockIdx.y];
+  double* A = mats + d.mat_off;
+  double* H = A + d.S * d.S;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < d.S; i += (int64_t)gridDim.x * blockDim.x) {
+    const double diag = 1e-8 + d.coef2 * (double)nsz[d.s_off + i];
+    if (add_to_a) A[i * d.S + i] += diag;
+    if (write_h) H[i * d.S + i] += diag;
+    const int64_t* row = nbr + (d.s_off + i) * P;
+    for (int64_t j = 0; j < P; ++j) {
+      const int64_t k = row[j];
+      if (k == -1) break;
+      if (k < 0 || k >= d.S) continue;
+      if (lower_only && k > i) continue;
+      if (add_to_a) A[i * d.S + k] -= d.coef2;
+      if (write_h) H[i * d.S + k] -= d.coef2;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Batched FP64 Cholesky, one workgroup (256 threads) per matrix, in place on
+// the lower triangle.  Right-looking, 32-wide panels; the trailing update of
+// each 64x64 tile runs on v_mfma_f64_16x16x4 with the panel staged in LDS.
+// info[b] = 0 or the 1-based column of the first non-positive pivot.
+// logdet[b] = 2 sum log L_jj (+ deflation correction when beta > 0).
+// ---------------------------------------------------------------------------
+constexpr int NB = 32;
+
+__device__ void chol_one(double* __restrict__ A, int64_t n, double* logdet_out, int64_t* info_out) {
+  __shared__ double Lkk[NB][NB + 1];
+  __shared__ double Pi[64][NB + 1];
+  __shared__ double Pj[64][NB + 1];
+  __shared__ double Xs[256][NB + 1];
+  __shared__ int fail;
+  __shared__ double ld_acc;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  if (tid == 0) {
+    fail = 0;
+    ld_acc = 0.0;
+  }
+  __syncthreads();
+  for (int64_t k0 = 0; k0 < n; k0 += NB) {
+    const int nb = (int)std::min<int64_t>(NB, n - k0);
+    // (a) factor the diagonal block in LDS
+    for (int e = tid; e < NB * NB; e += 256) {
+      int i = e / NB, j = e % NB;
+      Lkk[i][j] = (i < nb && j <= i) ? A[(k0 + i) * n + k0 + j] : 0.0;
+    }
+    __syncthreads();
+    for (int j = 0; j < nb; ++j) {
+      if (tid == 0) {
+        double dj = Lkk[j][j];
+        if (!(dj > 0.0)) {
+          if (!fail) fail = (int)(k0 + j + 1);
+          dj = 1.0;  // keep the arithmetic finite; the result is flagged
+        }
+        dj = sqrt(dj);
+        Lkk[j][j] = dj;
+        ld_acc += log(dj);
+      }
+      __syncthreads();
+      if (tid > j && tid < nb) Lkk[tid][j] /= Lkk[j][j];
+      __syncthreads();
+      for (int e = tid; e < NB * NB; e += 256) {
+        int i = e / NB, l = e % NB;
+        if (i > j && l > j && l <= i && i < nb) Lkk[i][l] -= Lkk[i][j] * Lkk[l][j];
+      }
+      __syncthreads();
+    }
+    for (int e = tid; e < NB * NB; e += 256) {
+      int i = e / NB, j = e % NB;
+      if (i < nb && j <= i) A[(k0 + i) * n + k0 + j] = Lkk[i][j];
+    }
+    // (b) panel rows below: x L_kk^T = a; each thread solves its own row,
+    // kept in its own LDS row (no cross-thread traffic, no barriers).
+    const int64_t r0 = k0 + nb;
+    for (int64_t base = r0; base < n; base += 256) {
+      const int64_t i = base + tid;
+      if (i < n) {
+        double* row = A + i * n + k0;
+        for (int j = 0; j < nb; ++j) Xs[tid][j] = row[j];
+        for (int j = 0; j < nb; ++j) {
+          double v = Xs[tid][j];
+          for (int l = 0; l < j; ++l) v -= Xs[tid][l] * Lkk[j][l];
+          Xs[tid][j] = v / Lkk[j][j];
+        }
+        for (int j = 0; j < nb; ++j) row[j] = Xs[tid][j];
+      }
+    }
+    __syncthreads();
+    // (c) trailing update A[r0:, r0:] -= P P^T, lower 64x64 tiles, MFMA f64
+    const int64_t nt = (n - r0 + 63) / 64;
+    for (int64_t ti = 0; ti < nt; ++ti) {
+      for (int64_t tj = 0; tj <= ti; ++tj) {
+        const int64_t R0 = r0 + ti * 64, C0 = r0 + tj * 64;
+        for (int e = tid; e < 64 * NB; e += 256) {
+          int r = e / NB, p = e % NB;
+          int64_t gi = R0 + r, gj = C0 + r;
+          Pi[r][p] = (gi < n && p < nb) ? A[gi * n + k0 + p] : 0.0;
+          Pj[r][p] = (gj < n && p < nb) ? A[gj * n + k0 + p] : 0.0;
+        }
+        __syncthreads();
+        const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+        const int lr = lane & 15, lk = lane >> 4;
+        d4 acc[2][2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              int64_t gi = R0 + wr + a * 16 + lk * 4 + r, gj = C0 + wc + b * 16 + lr;
+              acc[a][b][r] = (gi < n && gj < n && gj <= gi) ? A[gi * n + gj] : 0.0;
+            }
+#pragma unroll
+        for (int ks = 0; ks < NB / 4; ++ks) {
+          double fa[2], fb[2];
+#pragma unroll
+          for (int a = 0; a < 2; ++a) {
+            fa[a] = -Pi[wr + a * 16 + lr][ks * 4 + lk];
+            fb[a] = Pj[wc + a * 16 + lr][ks * 4 + lk];
+          }
+#pragma unroll
+          for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[a], fb[b], acc[a][b], 0, 0, 0);
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              int64_t gi = R0 + wr + a * 16 + lk * 4 + r, gj = C0 + wc + b * 16 + lr;
+              if (gi < n && gj < n && gj <= gi) A[gi * n + gj] = acc[a][b][r];
+            }
+        __syncthreads();
+      }
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    *logdet_out = 2.0 * ld_acc;
+    *info_out = fail;
+  }
+}
+
+__global__ void __launch_bounds__(256) chol_batched_kernel(const SampleDesc* __restrict__ sd, double* __restrict__ mats,
+                                                           int which_h, double* __restrict__ logdet,
+                                                           int64_t* __restrict__ info) {
+  const SampleDesc d = sd[blockIdx.x];
+  double* A = mats + d.mat_off + (which_h ? d.S * d.S : 0);
+  double ld;
+  int64_t inf;
+  chol_one(A, d.S, &ld, &inf);
+  if (threadIdx.x == 0) {
+    if (which_h && d.beta > 0.0) ld += log(1e-8) - log(1e-8 + d.beta);
+    logdet[2 * blockIdx.x + which_h] = ld;
+    info[2 * blockIdx.x + which_h] = inf;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Solves L y = D, L^T s = y and the log-evidence, one workgroup per sample.
+//   logL = -1/2 (chi.chi - s.D + log|A| - log|H| + noise_norm)   (numba.py:852-857)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) solve_loglike_kernel(const SampleDesc* __restrict__ sd,
+                                                            const double* __restrict__ mats,
+                                                            const double* __restrict__ D,
+                                                            const double* __restrict__ chi2_nn,
+                                                            const double* __restrict__ logdet,
+                                                            double* __restrict__ recon, double* __restrict__ logl) {
+  extern __shared__ double y[];
+  __shared__ double red[256];
+  const SampleDesc d = sd[blockIdx.x];
+  const double* L = mats + d.mat_off;
+  const int64_t n = d.S;
+  const int tid = threadIdx.x;
+  for (int64_t i = tid; i < n; i += 256) y[i] = D[d.s_off + i];
+  __syncthreads();
+  // forward: column-oriented
+  for (int64_t j = 0; j < n; ++j) {
+    const double yj = y[j] / L[j * n + j];
+    __syncthreads();
+    if (tid == 0) y[j] = yj;
+    for (int64_t i = j + 1 + tid; i < n; i += 256) y[i] -= L[i * n + j] * yj;
+    __syncthreads();
+  }
+  // backward: row j of L is column j of L^T
+  for (int64_t j = n - 1; j >= 0; --j) {
+    const double sj = y[j] / L[j * n + j];
+    __syncthreads();
+    if (tid == 0) y[j] = sj;
+    for (int64_t i = tid; i < j; i += 256) y[i] -= L[j * n + i] * sj;
+    __syncthreads();
+  }
+  double acc = 0.0;
+  for (int64_t i = tid; i < n; i += 256) {
+    acc += y[i] * D[d.s_off + i];
+    if (recon) recon[d.s_off + i] = y[i];
+  }
+  red[tid] = acc;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (tid < o) red[tid] += red[tid + o];
+    __syncthreads();
+  }
+  if (tid == 0) {
+    const double sD = red[0];
+    logl[blockIdx.x] =
+        -0.5 * (chi2_nn[0] - sD + logdet[2 * blockIdx.x] - logdet[2 * blockIdx.x + 1] + chi2_nn[1]);
+  }
+}
+
+// chi.chi over the 2K real entries and the noise normalisation (numba.py:752, :846-847).
+__global__ void chi2_noise_kernel(const double* __restrict__ data, const double* __restrict__ noise, int64_t K2,
+                                  double* __restrict__ out) {
+  __shared__ double r0[256], r1[256];
+  double a = 0.0, b = 0.0;
+  for (int64_t i = threadIdx.x; i < K2; i += 256) {
+    const double c = data[i] / noise[i];
+    a = fma(c, c, a);
+    b += log(fabs(noise[i]));
+  }
+  r0[threadIdx.x] = a;
+  r1[threadIdx.x] = b;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      r0[threadIdx.x] += r0[threadIdx.x + o];
+      r1[threadIdx.x] += r1[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = r0[0];
+    out[1] = 2.0 * ((double)(K2 / 2) * log(2.0 * M_PI) + r1[0]);
+  }
+}
+
+__global__ void noise_norm_kernel(const double* __restrict__ noise, int64_t K2, double* __restrict__ out) {
+  __shared__ double r1[256];
+  double b = 0.0;
+  for (int64_t i = threadIdx.x; i < K2; i += 256) b += log(fabs(noise[i]));
+  r1[threadIdx.x] = b;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) r1[threadIdx.x] += r1[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = 2.0 * ((double)(K2 / 2) * log(2.0 * M_PI) + r1[0]);
+}
+
+// ---------------------------------------------------------------------------
+// Host-side planning of the batch workspace.
+// ---------------------------------------------------------------------------
+struct BatchPlan {
+  int64_t n, M, B, sumS, sumS2, omega_elems, csc_ptr_elems, csc_ent_elems;
+  size_t off_desc, off_ptr, off_cur, off_key, off_t, off_omega, off_mats, off_D, off_logdet, off_info, bytes;
+};
+
+inline BatchPlan plan_batch(int64_t M, int64_t B, int64_t n, const int64_t* S_host) {
+  BatchPlan p{};
+  p.n = n;
+  p.M = M;
+  p.B = B;
+  for (int64_t b = 0; b < n; ++b) {
+    p.sumS += S_host[b];
+    p.sumS2 += S_host[b] * S_host[b];
+  }
+  p.omega_elems = M * p.sumS;
+  p.csc_ptr_elems = p.sumS + n;
+  p.csc_ent_elems = n * M * B;
+  size_t o = 0;
+  auto put = [&](size_t bytes) {
+    o = align_up(o, 256);
+    size_t r = o;
+    o += bytes;
+    return r;
+  };
+  p.off_desc = put(sizeof(SampleDesc) * n);
+  p.off_ptr = put(sizeof(int) * p.csc_ptr_elems);
+  p.off_cur = put(sizeof(int) * p.csc_ptr_elems);
+  p.off_key = put(sizeof(int) * p.csc_ent_elems);
+  p.off_t = put(sizeof(double) * p.csc_ent_elems);
+  p.off_omega = put(sizeof(double) * p.omega_elems);
+  p.off_mats = put(sizeof(double) * 2 * p.sumS2);
+  p.off_D = put(sizeof(double) * p.sumS);
+  p.off_logdet = put(sizeof(double) * 2 * n);
+  p.off_info = put(sizeof(int64_t) * 2 * n);
+  p.bytes = align_up(o, 256);
+  return p;
+}
+
+}  // namespace lk
+}  // namespace aj
+
+using namespace aj;
+using namespace aj::lk;
+
+namespace {
+
+// Runs the whole per-sample pipeline for a batch.  Returns the workspace
+// views through `p`; outputs per-sample log-evidence into logl.
+int batch_pipeline(const double* w_tilde, int64_t M, const double* dirty_image, const double* chi2_nn, int64_t n,
+                   int64_t B, int64_t P, const int64_t* S_host, const int64_t* pix_indexes, const double* pix_weights,
+                   const int64_t* neighbors, const int64_t* neighbors_sizes, const double* coef_host,
+                   const double* beta_host, double* logl, double* recon, double* logdets_out, int64_t* info_out,
+                   void* workspace, size_t workspace_bytes, hipStream_t st) {
+  BatchPlan p = plan_batch(M, B, n, S_host);
+  AJ_REQUIRE(workspace && workspace_bytes >= p.bytes, "workspace too small: need %zu bytes, got %zu", p.bytes,
+             workspace_bytes);
+  char* ws = static_cast<char*>(workspace);
+  SampleDesc* desc = reinterpret_cast<SampleDesc*>(ws + p.off_desc);
+  int* ptrs = reinterpret_cast<int*>(ws + p.off_ptr);
+  int* cur = reinterpret_cast<int*>(ws + p.off_cur);
+  int* ekey = reinterpret_cast<int*>(ws + p.off_key);
+  double* et = reinterpret_cast<double*>(ws + p.off_t);
+  double* omega = reinterpret_cast<double*>(ws + p.off_omega);
+  double* mats = reinterpret_cast<double*>(ws + p.off_mats);
+  double* D = reinterpret_cast<double*>(ws + p.off_D);
+  double* logdet = logdets_out ? logdets_out : reinterpret_cast<double*>(ws + p.off_logdet);
+  int64_t* info = info_out ? info_out : reinterpret_cast<int64_t*>(ws + p.off_info);
+
+  // Per-sample table (host -> device).
+  static thread_local std::vector<SampleDesc> hd;
+  hd.resize(n);
+  int64_t s_off = 0, mat_off = 0, om_off = 0, ptr_off = 0, maxS = 0;
+  for (int64_t b = 0; b < n; ++b) {
+    const int64_t S = S_host[b];
+    AJ_REQUIRE(S > 0 && S < (1ll << 31), "sample %lld: bad pixel count %lld", (long long)b, (long long)S);
+    hd[b] = SampleDesc{S, s_off, b * M * B, ptr_off, b * M * B, om_off, mat_off, coef_host[b] * coef_host[b],
+                       beta_host ? beta_host[b] : 0.0};
+    s_off += S;
+    mat_off += 2 * S * S;
+    om_off += M * S;
+    ptr_off += S + 1;
+    maxS = std::max(maxS, S);
+  }
+  AJ_CHECK_HIP(hipMemcpyAsync(desc, hd.data(), sizeof(SampleDesc) * n, hipMemcpyHostToDevice, st));
+  AJ_CHECK_HIP(hipMemsetAsync(ptrs, 0, sizeof(int) * p.csc_ptr_elems, st));
+
+  const unsigned gm = (unsigned)std::min<int64_t>(ceil_div(M, 256), 1024);
+  hipLaunchKernelGGL(csc_count_kernel, dim3(gm, (unsigned)n), dim3(256), 0, st, desc, pix_indexes, M, B, ptrs);
+  AJ_LAUNCH_CHECK();
+  hipLaunchKernelGGL(csc_scan_kernel, dim3((unsigned)n), dim3(1024), 0, st, desc, ptrs, cur);
+  AJ_LAUNCH_CHECK();
+  hipLaunchKernelGGL(csc_fill_kernel, dim3(gm, (unsigned)n), dim3(256), 0, st, desc, pix_indexes, pix_weights, M, B,
+                     cur, ekey, et);
+  AJ_LAUNCH_CHECK();
+  const unsigned gs = (unsigned)std::min<int64_t>(ceil_div(maxS, 256), 1024);
+  hipLaunchKernelGGL(csc_sort_kernel, dim3(gs, (unsigned)n), dim3(256), 0, st, desc, ptrs, B, ekey, et);
+  AJ_LAUNCH_CHECK();
+  hipLaunchKernelGGL(data_vector_kernel, dim3(gs, (unsigned)n), dim3(256), 0, st, desc, ptrs, ekey, et, dirty_image,
+                     D);
+  AJ_LAUNCH_CHECK();
+  const int R = (int)std::min<int64_t>(8, (int64_t)(OMEGA_LDS_BUDGET / (sizeof(double) * M)));
+  timer_begin(AJ_TIMER_OMEGA, st);
+  if (R >= 1) {
+    AJ_CHECK_HIP(hipFuncSetAttribute((const void*)omega_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)(R * M * sizeof(double))));
+    hipLaunchKernelGGL(omega_kernel, dim3((unsigned)ceil_div(M, R)), dim3(OMEGA_THREADS),
+                       (size_t)R * M * sizeof(double), st, w_tilde, M, R, (int)n, desc, ptrs, ekey, et, omega);
+  } else {
+    hipLaunchKernelGGL(omega_global_kernel, dim3((unsigned)M), dim3(256), 0, st, w_tilde, M, (int)n, desc, ptrs,
+                       ekey, et, omega);
+  }
+  timer_end(AJ_TIMER_OMEGA, st);
+  AJ_LAUNCH_CHECK();
+  timer_begin(AJ_TIMER_FGATHER, st);
+  hipLaunchKernelGGL(fgather_kernel, dim3((unsigned)maxS, (unsigned)n), dim3(256), 0, st, desc, ptrs, ekey, et, omega,
+                     mats, 1);
+  timer_end(AJ_TIMER_FGATHER, st);
+  AJ_LAUNCH_CHECK();
+  // H block: constant fill (0 or beta/S), then the regulariser into A and H.
+  hipLaunchKernelGGL(fill_kernel, dim3(256, (unsigned)n), dim3(256), 0, st, desc, mats, 1);
+  AJ_LAUNCH_CHECK();
+  hipLaunchKernelGGL(regularize_kernel, dim3(gs, (unsigned)n), dim3(256), 0, st, desc, neighbors, neighbors_sizes, P,
+                     mats, 1, 1, 1);
+  AJ_LAUNCH_CHECK();
+  timer_begin(AJ_TIMER_CHOL, st);
+  hipLaunchKernelGGL(chol_batched_kernel, dim3((unsigned)n), dim3(256), 0, st, desc, mats, 0, logdet, info);
+  AJ_LAUNCH_CHECK();
+  hipLaunchKernelGGL(chol_batched_kernel, dim3((unsigned)n), dim3(256), 0, st, desc, mats, 1, logdet, info);
+  timer_end(AJ_TIMER_CHOL, st);
+  AJ_LAUNCH_CHECK();
+  timer_begin(AJ_TIMER_SOLVE, st);
+  hipLaunchKernelGGL(solve_loglike_kernel, dim3((unsigned)n), dim3(256), sizeof(double) * maxS, st, desc, mats, D,
+                     chi2_nn, logdet, recon, logl);
+  timer_end(AJ_TIMER_SOLVE, st);
+  AJ_LAUNCH_CHECK();
+  return AJ_OK;
+}
+
+}  // namespace
+
+extern "C" size_t aj_log_evidence_batched_workspace_bytes(int64_t M, int64_t B, int64_t n_samples,
+                                                          const int64_t* S_host) {
+  if (M <= 0 || B <= 0 || n_samples <= 0 || !S_host) return 0;
+  return plan_batch(M, B, n_samples, S_host).bytes;
+}
+
+extern "C" int aj_chi_squared_noise_normalization(const double* data, const double* noise_map, int64_t K,
+                                                  double* out2, void* stream) {
+  AJ_REQUIRE(K >= 0 && data && noise_map && out2, "bad arguments");
+  hipLaunchKernelGGL(chi2_noise_kernel, dim3(1), dim3(256), 0, as_stream(stream), data, noise_map, 2 * K, out2);
+  AJ_LAUNCH_CHECK();
+  return AJ_OK;
+}
+
+extern "C" int aj_noise_normalization_complex_from(const double* noise_map, int64_t K, double* out, void* stream) {
+  AJ_REQUIRE(K >= 0 && noise_map && out, "bad arguments");
+  hipLaunchKernelGGL(noise_norm_kernel, dim3(1), dim3(256), 0, as_stream(stream), noise_map, 2 * K, out);
+  AJ_LAUNCH_CHECK();
+  return AJ_OK;
+}
+
+extern "C" int aj_log_evidence_batched(const double* w_tilde, int64_t M, const double* dirty_image,
+                                       const double* chi2_noise_norm, int64_t n_samples, int64_t B, int64_t P,
+                                       const int64_t* S_host, const int64_t* pix_indexes, const double* pix_weights,
+                                       const int64_t* neighbors, const int64_t* neighbors_sizes,
+                                       const double* coefficients_host, const double* deflation_beta_host,
+                                       double* log_evidence, double* reconstruction, double* logdets,
+                                       int64_t* info, void* workspace, size_t workspace_bytes, void* stream) {
+  AJ_REQUIRE(M > 0 && B > 0 && P > 0 && n_samples > 0, "bad sizes");
+  AJ_REQUIRE(w_tilde && dirty_image && chi2_noise_norm && S_host && pix_indexes && pix_weights && neighbors &&
+                 neighbors_sizes && coefficients_host && log_evidence,
+             "null pointer");
+  return batch_pipeline(w_tilde, M, dirty_image, chi2_noise_norm, n_samples, B, P, S_host, pix_indexes, pix_weights,
+                        neighbors, neighbors_sizes, coefficients_host, deflation_beta_host, log_evidence,
+                        reconstruction, logdets, info, workspace, workspace_bytes, as_stream(stream));
+}
+
+extern "C" size_t aj_log_likelihood_workspace_bytes(int64_t M, int64_t K, int64_t B, int64_t S, int64_t P) {
+  (void)K;
+  (void)P;
+  if (M <= 0 || B <= 0 || S <= 0) return 0;
+  return align_up(plan_batch(M, B, 1, &S).bytes, 256) + 256;
+}
+
+extern "C" int aj_log_likelihood_function_via_w_tilde_from(
+    const double* dirty_image, const double* data, const double* noise_map, const double* w_tilde,
+    const int64_t* pix_indexes_for_sub_slim_index, const double* pix_weights_for_sub_slim_index,
+    const int64_t* neighbors, const int64_t* neighbors_sizes, int64_t M, int64_t K, int64_t B, int64_t S, int64_t P,
+    double* log_likelihood, int64_t* info, void* workspace, size_t workspace_bytes, void* stream) {
+  AJ_REQUIRE(M > 0 && K >= 0 && B > 0 && S > 0 && P > 0, "bad sizes");
+  AJ_REQUIRE(dirty_image && data && noise_map && w_tilde && pix_indexes_for_sub_slim_index &&
+                 pix_weights_for_sub_slim_index && neighbors && neighbors_sizes && log_likelihood && info,
+             "null pointer");
+  const size_t need = aj_log_likelihood_workspace_bytes(M, K, B, S, P);
+  AJ_REQUIRE(workspace && workspace_bytes >= need, "workspace too small: need %zu bytes, got %zu", need,
+             workspace_bytes);
+  hipStream_t st = as_stream(stream);
+  double* chi2nn = reinterpret_cast<double*>(static_cast<char*>(workspace) + need - 256);
+  hipLaunchKernelGGL(chi2_noise_kernel, dim3(1), dim3(256), 0, st, data, noise_map, 2 * K, chi2nn);
+  AJ_LAUNCH_CHECK();
+  const double coef = 1.0;  // numba.py:819
+  return batch_pipeline(w_tilde, M, dirty_image, chi2nn, 1, B, P, &S, pix_indexes_for_sub_slim_index,
+                        pix_weights_for_sub_slim_index, neighbors, neighbors_sizes, &coef, nullptr, log_likelihood,
+                        nullptr, nullptr, info, workspace, need - 256, st);
+}
+
+// ===========================================================================
+// Single-function entry points mirroring the individual reference functions.
+// ===========================================================================
+namespace aj {
+namespace lk {
+
+__global__ void mapping_dense_kernel(const int64_t* __restrict__ idx, const double* __restrict__ w, int64_t M,
+                                     int64_t B, int64_t S, double* __restrict__ T) {
+  for (int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; m < M; m += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t b = 0; b < B; ++b) {
+      const int64_t s = idx[m * B + b];
+      if (s == -1) break;
+      if (s >= 0 && s < S) T[m * S + s] = w[m * B + b];  // assignment, as numba.py:375
+    }
+  }
+}
+
+__global__ void dense_data_vector_kernel(const double* __restrict__ T, const double* __restrict__ dirty, int64_t M,
+                                         int64_t S, double* __restrict__ D) {
+  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < S; s += (int64_t)gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    for (int64_t m = 0; m < M; ++m) acc = fma(dirty[m], T[m * S + s], acc);
+    D[s] = acc;
+  }
+}
+
+// C[i][j] = sum_k opA(i,k) B[k][j]; opA(i,k) = A[i*lda+k] (trans=0) or A[k*lda+i] (trans=1).
+// B row-major with ldb, C row-major with ldc.  64x64 tiles, FP64 MFMA.
+__global__ void __launch_bounds__(256) gemm_f64_kernel(const double* __restrict__ A, int64_t lda, int transA,
+                                                       const double* __restrict__ Bm, int64_t ldb,
+                                                       double* __restrict__ C, int64_t ldc, int64_t Mr, int64_t N,
+                                                       int64_t Kd) {
+  __shared__ double As[64][17];
+  __shared__ double Bs[16][65];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int64_t i0 = (int64_t)blockIdx.y * 64, j0 = (int64_t)blockIdx.x * 64;
+  d4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = d4{0, 0, 0, 0};
+  for (int64_t k0 = 0; k0 < Kd; k0 += 16) {
+    for (int e = tid; e < 64 * 16; e += 256) {
+      int r, c;
+      if (transA) {
+        c = e / 64;
+        r = e % 64;
+      } else {
+        r = e / 16;
+        c = e % 16;
+      }
+      const int64_t gi = i0 + r, gk = k0 + c;
+      double v = 0.0;
+      if (gi < Mr && gk < Kd) v = transA ? A[gk * lda + gi] : A[gi * lda + gk];
+      As[r][c] = v;
+      const int kr = e / 64, jc = e % 64;
+      const int64_t bk = k0 + kr, bj = j0 + jc;
+      Bs[kr][jc] = (bk < Kd && bj < N) ? Bm[bk * ldb + bj] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      double fa[2], fb[2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        fa[a] = As[wr + a * 16 + lr][ks * 4 + lk];
+        fb[a] = Bs[ks * 4 + lk][wc + a * 16 + lr];
+      }
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[a], fb[b], acc[a][b], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t gi = i0 + wr + a * 16 + lk * 4 + r, gj = j0 + wc + b * 16 + lr;
+        if (gi < Mr && gj < N) C[gi * ldc + gj] = acc[a][b][r];
+      }
+}
+
+// out[s1, :] = sum_{(m,t) in bucket s1} t * X[m, :]   (numba.py:578-586), X is [M, W].
+__global__ void tgather_kernel(const SampleDesc* __restrict__ sd, const int* __restrict__ ptrs,
+                               const int* __restrict__ ent_m, const double* __restrict__ ent_t,
+                               const double* __restrict__ X, int64_t Wd, double* __restrict__ out) {
+  const SampleDesc d = sd[0];
+  const int64_t s1 = blockIdx.x;
+  const int* p = ptrs + d.csc_ptr;
+  const int lo = p[s1], hi = p[s1 + 1];
+  for (int64_t c = threadIdx.x; c < Wd; c += blockDim.x) {
+    double acc = 0.0;
+    for (int e = lo; e < hi; ++e) acc = fma(ent_t[d.csc_ent + e], X[(int64_t)ent_m[d.csc_ent + e] * Wd + c], acc);
+    out[s1 * Wd + c] = acc;
+  }
+}
+
+__global__ void reg_standalone_kernel(double coef2, const int64_t* __restrict__ nbr, const int64_t* __restrict__ nsz,
+                                      int64_t S, int64_t P, double* __restrict__ H) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < S; i += (int64_t)gridDim.x * blockDim.x) {
+    H[i * S + i] += 1e-8 + coef2 * (double)nsz[i];
+    for (int64_t j = 0; j < P; ++j) {
+      const int64_t k = nbr[i * P + j];
+      if (k == -1) break;
+      if (k >= 0 && k < S) H[i * S + k] -= coef2;
+    }
+  }
+}
+
+__global__ void copy_lower_kernel(const double* __restrict__ src, int64_t n, double* __restrict__ dst) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n * n; e += (int64_t)gridDim.x * blockDim.x)
+    dst[e] = src[e];
+}
+
+__global__ void __launch_bounds__(256) chol_solve_kernel(double* __restrict__ A, int64_t n,
+                                                         const double* __restrict__ D, double* __restrict__ s,
+                                                         double* __restrict__ logdet, int64_t* __restrict__ info) {
+  extern __shared__ double y[];
+  double ld;
+  int64_t inf;
+  chol_one(A, n, &ld, &inf);
+  const int tid = threadIdx.x;
+  for (int64_t i = tid; i < n; i += 256) y[i] = D[i];
+  __syncthreads();
+  for (int64_t j = 0; j < n; ++j) {
+    const double yj = y[j] / A[j * n + j];
+    __syncthreads();
+    if (tid == 0) y[j] = yj;
+    for (int64_t i = j + 1 + tid; i < n; i += 256) y[i] -= A[i * n + j] * yj;
+    __syncthreads();
+  }
+  for (int64_t j = n - 1; j >= 0; --j) {
+    const double sj = y[j] / A[j * n + j];
+    __syncthreads();
+    if (tid == 0) y[j] = sj;
+    for (int64_t i = tid; i < j; i += 256) y[i] -= A[j * n + i] * sj;
+    __syncthreads();
+  }
+  for (int64_t i = tid; i < n; i += 256) s[i] = y[i];
+  if (tid == 0) {
+    if (logdet) *logdet = ld;
+    *info = inf;
+  }
+}
+
+// Data-side sums over visibilities, per image pixel, absolute phases:
+//   out_i = sum_k (a_k cos 2pi r_ik + b_k sin 2pi r_ik),  r_ik = g_i1 u_k0 + g_i0 u_k1.
+// Partial sums per K-split, reduced in a fixed order (deterministic).
+constexpr int DSPLIT = 8192;
+__global__ void __launch_bounds__(256) vis_sum_partial_kernel(const double* __restrict__ a, int64_t a_stride,
+                                                              const double* __restrict__ b, int64_t b_stride,
+                                                              const double* __restrict__ wdiv, int mode,
+                                                              const double* __restrict__ uv,
+                                                              const double* __restrict__ grid, int64_t K, int64_t M,
+                                                              double* __restrict__ partial) {
+  __shared__ double su0[256], su1[256], sa[256], sb[256];
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t kb = (int64_t)blockIdx.y * DSPLIT;
+  const int64_t ke = std::min<int64_t>(K, kb + DSPLIT);
+  double gx = 0.0, gy = 0.0;
+  if (i < M) {
+    gy = grid[2 * i];
+    gx = grid[2 * i + 1];
+  }
+  double acc = 0.0;
+  for (int64_t k0 = kb; k0 < ke; k0 += 256) {
+    const int64_t k = k0 + threadIdx.x;
+    __syncthreads();
+    if (k < ke) {
+      su0[threadIdx.x] = uv[2 * k];
+      su1[threadIdx.x] = uv[2 * k + 1];
+      double av, bv;
+      if (mode == 0) {
+        // dirty image: (Re d, Im d) / sigma^2 with sigma = Re noise
+        const double n = wdiv[2 * k];
+        av = a[a_stride * k] / (n * n);
+        bv = b[b_stride * k] / (n * n);
+      } else {
+        // numba.py:149: w_k = (n_k^2 / v_k)^2 on the cos term only
+        const double n = wdiv[k];
+        const double q = n * n / a[k];
+        av = q * q;
+        bv = 0.0;
+      }
+      sa[threadIdx.x] = av;
+      sb[threadIdx.x] = bv;
+    } else {
+      su0[threadIdx.x] = su1[threadIdx.x] = sa[threadIdx.x] = sb[threadIdx.x] = 0.0;
+    }
+    __syncthreads();
+    const int nk = (int)std::min<int64_t>(256, ke - k0);
+    if (i < M) {
+      for (int j = 0; j < nk; ++j) {
+        double r = fma(gx, su0[j], gy * su1[j]);
+        r -= rint(r);
+        double sv, cv;
+        sincospi(2.0 * r, &sv, &cv);
+        acc = fma(sa[j], cv, acc);
+        if (mode == 0) acc = fma(sb[j], sv, acc);
+      }
+    }
+  }
+  if (i < M) partial[(int64_t)blockIdx.y * M + i] = acc;
+}
+
+__global__ void vis_sum_reduce_kernel(const double* __restrict__ partial, int64_t nsplit, int64_t M,
+                                      double* __restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < M; i += (int64_t)gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    for (int64_t s = 0; s < nsplit; ++s) acc += partial[s * M + i];
+    out[i] = acc;
+  }
+}
+
+}  // namespace lk
+}  // namespace aj
+
+namespace {
+
+// CSC for one sample (mapping [M,B]) in a workspace; returns the views.
+struct OneCsc {
+  SampleDesc* desc;
+  int* ptrs;
+  int* ekey;
+  double* et;
+};
+
+int build_one_csc(const int64_t* idx, const double* w, int64_t M, int64_t B, int64_t S, Arena& ar, OneCsc& o,
+                  hipStream_t st) {
+  o.desc = ar.take<SampleDesc>(1);
+  o.ptrs = ar.take<int>(S + 1);
+  int* cur = ar.take<int>(S + 1);
+  o.ekey = ar.take<int>(M * B);
+  o.et = ar.take<double>(M * B);
+  if (!ar.ok()) {
+    set_error("workspace too small");
+    return AJ_ESPACE;
+  }
+  SampleDesc h{S, 0, 0, 0, 0, 0, 0, 0.0, 0.0};
+  AJ_CHECK_HIP(hipMemcpyAsync(o.desc, &h, sizeof(h), hipMemcpyHostToDevice, st));
+  AJ_CHECK_HIP(hipMemsetAsync(o.ptrs, 0, sizeof(int) * (S + 1), st));
+  const unsigned gm = (unsigned)std::min<int64_t>(ceil_div(M, 256), 1024);
+  const unsigned gs = (unsigned)std::min<int64_t>(ceil_div(S, 256), 1024);
+  hipLaunchKernelGGL(csc_count_kernel, dim3(gm, 1), dim3(256), 0, st, o.desc, idx, M, B, o.ptrs);
+  hipLaunchKernelGGL(csc_scan_kernel, dim3(1), dim3(1024), 0, st, o.desc, o.ptrs, cur);
+  hipLaunchKernelGGL(csc_fill_kernel, dim3(gm, 1), dim3(256), 0, st, o.desc, idx, w, M, B, cur, o.ekey, o.et);
+  hipLaunchKernelGGL(csc_sort_kernel, dim3(gs, 1), dim3(256), 0, st, o.desc, o.ptrs, B, o.ekey, o.et);
+  AJ_LAUNCH_CHECK();
+  return AJ_OK;
+}
+
+size_t csc_bytes(int64_t M, int64_t B, int64_t S) {
+  return 4 * 256 + sizeof(SampleDesc) + 2 * sizeof(int) * (S + 1) + (sizeof(int) + sizeof(double)) * M * B + 1024;
+}
+
+}  // namespace
+
+extern "C" int aj_mapping_matrix_from(const int64_t* pix_indexes_for_sub_slim_index,
+                                      const double* pix_weights_for_sub_slim_index, int64_t M, int64_t B,
+                                      int64_t pixels, double* mapping_matrix, void* stream) {
+  AJ_REQUIRE(M > 0 && B > 0 && pixels > 0, "bad sizes");
+  AJ_REQUIRE(pix_indexes_for_sub_slim_index && pix_weights_for_sub_slim_index && mapping_matrix, "null pointer");
+  hipStream_t st = as_stream(stream);
+  AJ_CHECK_HIP(hipMemsetAsync(mapping_matrix, 0, sizeof(double) * M * pixels, st));
+  hipLaunchKernelGGL(mapping_dense_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(M, 256), 1024)), dim3(256), 0, st,
+                     pix_indexes_for_sub_slim_index, pix_weights_for_sub_slim_index, M, B, pixels, mapping_matrix);
+  AJ_LAUNCH_CHECK();
+  return AJ_OK;
+}
+
+extern "C" int aj_data_vector_from(const double* mapping_matrix, const double* dirty_image, int64_t M, int64_t S,
+                                   double* data_vector, void* stream) {
+  AJ_REQUIRE(M > 0 && S > 0 && mapping_matrix && dirty_image && data_vector, "bad arguments");
+  hipLaunchKernelGGL(dense_data_vector_kernel, dim3((unsigned)ceil_div(S, 256)), dim3(256), 0, as_stream(stream),
+                     mapping_matrix, dirty_image, M, S, data_vector);
+  AJ_LAUNCH_CHECK();
+  return AJ_OK;
+}
+
+extern "C" size_t aj_curvature_dense_workspace_bytes(int64_t M, int64_t S) {
+  return (M > 0 && S > 0) ? sizeof(double) * M * S : 0;
+}
+
+extern "C" int aj_curvature_matrix_via_w_tilde_from(const double* w_tilde, const double* mapping_matrix, int64_t M,
+                                                    int64_t S, double* curvature_matrix, void* workspace,
+                                                    size_t workspace_bytes, void* stream) {
+  AJ_REQUIRE(M > 0 && S > 0 && w_tilde && mapping_matrix && curvature_matrix, "bad arguments");
+  AJ_REQUIRE(workspace && workspace_bytes >= aj_curvature_dense_workspace_bytes(M, S), "workspace too small");
+  hipStream_t st = as_stream(stream);
+  double* omega = static_cast<double*>(workspace);
+  // Omega = W T  (M x S), then F = T^T Omega (S x S): the two products of numba.py:435.
+  hipLaunchKernelGGL(gemm_f64_kernel, dim3((unsigned)ceil_div(S, 64), (unsigned)ceil_div(M, 64)), dim3(256), 0, st,
+                     w_tilde, M, 0, mapping_matrix, S, omega, S, M, S, M);
+  hipLaunchKernelGGL(gemm_f64_kernel, dim3((unsigned)ceil_div(S, 64), (unsigned)ceil_div(S, 64)), dim3(256), 0, st,
+                     mapping_matrix, S, 1, omega, S, curvature_matrix, S, S, S, M);
+  AJ_LAUNCH_CHECK();
+  return AJ_OK;
+}
+
+extern "C" size_t aj_sparse_transpose_matmul_workspace_bytes(int64_t M, int64_t B, int64_t S) {
+  return (M > 0 && B > 0 && S > 0) ? csc_bytes(M, B, S) : 0;
+}
+
+extern "C" int aj_sparse_mapping_matrix_transpose_matmul(const double* matrix, const int64_t* pix_indexes_for_sub_slim_index,
+                                                         const double* pix_weights_for_sub_slim_index, int64_t M,
+                                                         int64_t S2, int64_t B, int64_t pixels, double* out,
+                                                         void* workspace, size_t workspace_bytes, void* stream) {
+  AJ_REQUIRE(M > 0 && S2 > 0 && B > 0 && pixels > 0, "bad sizes");
+  AJ_REQUIRE(matrix && pix_indexes_for_sub_slim_index && pix_weights_for_sub_slim_index && out && workspace,
+             "null pointer");
+  hipStream_t st = as_stream(stream);
+  Arena ar(workspace, workspace_bytes);
+  OneCsc c;
+  int rc = build_one_csc(pix_indexes_for_sub_slim_index, pix_weights_for_sub_slim_index, M, B, pixels, ar, c, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(tgather_kernel, dim3((unsigned)pixels), dim3(256), 0, st, c.desc, c.ptrs, c.ekey, c.et, matrix,
+                     S2, out);
+  AJ_LAUNCH_CHECK();
+  return AJ_OK;
+}
+
+extern "C" size_t aj_curvature_sparse_workspace_bytes(int64_t M, int64_t B, int64_t S) {
+  return (M > 0 && B > 0 && S > 0) ? csc_bytes(M, B, S) + sizeof(double) * M * S + 256 : 0;
+}
+
+extern "C" int aj_curvature_matrix_via_w_wilde_sparse_mapping_matrix_from(
+    const double* w_tilde, const int64_t* pix_indexes_for_sub_slim_index, const double* pix_weights_for_sub_slim_index,
+    int64_t M, int64_t B, int64_t pixels, double* curvature_matrix, void* workspace, size_t workspace_bytes,
+    void* stream) {
+  AJ_REQUIRE(M > 0 && B > 0 && pixels > 0, "bad sizes");
+  AJ_REQUIRE(w_tilde && pix_indexes_for_sub_slim_index && pix_weights_for_sub_slim_index && curvature_matrix &&
+                 workspace,
+             "null pointer");
+  hipStream_t st = as_stream(stream);
+  Arena ar(workspace, workspace_bytes);
+  OneCsc c;
+  int rc = build_one_csc(pix_indexes_for_sub_slim_index, pix_weights_for_sub_slim_index, M, B, pixels, ar, c, st);
+  if (rc) return rc;
+  double* omega = ar.take<double>(M * pixels);
+  AJ_REQUIRE(ar.ok(), "workspace too small");
+  const int R = (int)std::min<int64_t>(8, (int64_t)(OMEGA_LDS_BUDGET / (sizeof(double) * M)));
+  if (R >= 1) {
+    AJ_CHECK_HIP(hipFuncSetAttribute((const void*)omega_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)(R * M * sizeof(double))));
+    hipLaunchKernelGGL(omega_kernel, dim3((unsigned)ceil_div(M, R)), dim3(OMEGA_THREADS),
+                       (size_t)R * M * sizeof(double), st, w_tilde, M, R, 1, c.desc, c.ptrs, c.ekey, c.et, omega);
+  } else {
+    hipLaunchKernelGGL(omega_global_kernel, dim3((unsigned)M), dim3(256), 0, st, w_tilde, M, 1, c.desc, c.ptrs,
+                       c.ekey, c.et, omega);
+  }
+  hipLaunchKernelGGL(tgather_kernel, dim3((unsigned)pixels), dim3(256), 0, st, c.desc, c.ptrs, c.ekey, c.et, omega,
+                     pixels, curvature_matrix);
+  AJ_LAUNCH_CHECK();
+  return AJ_OK;
+}
+
+extern "C" int aj_constant_regularization_matrix_from(double coefficient, const int64_t* neighbors,
+                                                      const int64_t* neighbors_sizes, int64_t S, int64_t P,
+                                                      double* regularization_matrix, void* stream) {
+  AJ_REQUIRE(S > 0 && P > 0 && neighbors && neighbors_sizes && regularization_matrix, "bad arguments");
+  hipStream_t st = as_stream(stream);
+  AJ_CHECK_HIP(hipMemsetAsync(regularization_matrix, 0, sizeof(double) * S * S, st));
+  hipLaunchKernelGGL(reg_standalone_kernel, dim3((unsigned)ceil_div(S, 256)), dim3(256), 0, st,
+                     coefficient * coefficient, neighbors, neighbors_sizes, S, P, regularization_matrix);
+  AJ_LAUNCH_CHECK();
+  return AJ_OK;
+}
+
+extern "C" size_t aj_solve_workspace_bytes(int64_t S) { return S > 0 ? sizeof(double) * S * S : 0; }
+
+extern "C" int aj_reconstruction_positive_negative_from(const double* data_vector, const double* curvature_reg_matrix,
+                                                        int64_t S, double* reconstruction, double* logdet,
+                                                        int64_t* info, void* workspace, size_t workspace_bytes,
+                                                        void* stream) {
+  AJ_REQUIRE(S > 0 && data_vector && curvature_reg_matrix && reconstruction && info, "bad arguments");
+  AJ_REQUIRE(workspace && workspace_bytes >= aj_solve_workspace_bytes(S), "workspace too small");
+  hipStream_t st = as_stream(stream);
+  double* L = static_cast<double*>(workspace);
+  hipLaunchKernelGGL(copy_lower_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(S * S, 256), 4096)), dim3(256), 0,
+                     st, curvature_reg_matrix, S, L);
+  hipLaunchKernelGGL(chol_solve_kernel, dim3(1), dim3(256), sizeof(double) * S, st, L, S, data_vector,
+                     reconstruction, logdet, info);
+  AJ_LAUNCH_CHECK();
+  return AJ_OK;
+}
+
+extern "C" size_t aj_visibility_sum_workspace_bytes(int64_t K, int64_t M) {
+  return (M > 0 && K >= 0) ? sizeof(double) * M * std::max<int64_t>(1, ceil_div(K, DSPLIT)) : 0;
+}
+
+static int vis_sum(const double* a, int64_t as, const double* b, int64_t bs, const double* wdiv, int mode,
+                   const double* uv, const double* grid, int64_t K, int64_t M, double* out, void* ws, size_t wsb,
+                   hipStream_t st) {
+  AJ_REQUIRE(M > 0 && K >= 0 && grid && out && (K == 0 || (a && wdiv && uv)), "bad arguments");
+  AJ_REQUIRE(ws && wsb >= aj_visibility_sum_workspace_bytes(K, M), "workspace too small");
+  const int64_t ns = std::max<int64_t>(1, ceil_div(K, DSPLIT));
+  double* part = static_cast<double*>(ws);
+  hipLaunchKernelGGL(vis_sum_partial_kernel, dim3((unsigned)ceil_div(M, 256), (unsigned)ns), dim3(256), 0, st, a, as,
+                     b, bs, wdiv, mode, uv, grid, K, M, part);
+  hipLaunchKernelGGL(vis_sum_reduce_kernel, dim3((unsigned)ceil_div(M, 256)), dim3(256), 0, st, part, ns, M, out);
+  AJ_LAUNCH_CHECK();
+  return AJ_OK;
+}
+
+extern "C" int aj_dirty_image_from(const double* data, const double* noise_map, const double* uv_wavelengths,
+                                   const double* grid_radians_slim, int64_t K, int64_t M, double* dirty_image,
+                                   void* workspace, size_t workspace_bytes, void* stream) {
+  return vis_sum(data, 2, data ? data + 1 : nullptr, 2, noise_map, 0, uv_wavelengths, grid_radians_slim, K, M,
+                 dirty_image, workspace, workspace_bytes, as_stream(stream));
+}
+
+extern "C" int aj_w_tilde_data_interferometer_from(const double* visibilities_real, const double* noise_map_real,
+                                                   const double* uv_wavelengths, const double* grid_radians_slim,
+                                                   int64_t K, int64_t M, double* w_tilde_data, void* workspace,
+                                                   size_t workspace_bytes, void* stream) {
+  return vis_sum(visibilities_real, 1, nullptr, 0, noise_map_real, 1, uv_wavelengths, grid_radians_slim, K, M,
+                 w_tilde_data, workspace, workspace_bytes, as_stream(stream));
+}

@@ -1,0 +1,511 @@
+// W-tilde curvature build (the NUFFT Gram matrix) for gfx950.
+//
+// Reference: src/autojax/numba.py:155-207 (w_tilde_curvature_interferometer_from)
+//   W_ij = sum_k n_k^-2 cos(2 pi [(g_i1 - g_j1) u_k0 + (g_i0 - g_j0) u_k1])
+//
+// Rewritten as a Gram matrix of cos/sin features so that it runs on MFMA:
+//   r_ik = g_i1 u_k0 + g_i0 u_k1            (revolutions, FP64, reduced to [-1/2, 1/2])
+//   C_ik = sqrt(w_k) cos(2 pi r_ik),  S_ik = sqrt(w_k) sin(2 pi r_ik)
+//   W    = [C S] [C S]^T                    (lower-triangle SYRK, inner dim 2K)
+// The grid is centred first (W depends on differences only), which keeps the
+// phases small.  K is processed in chunks of KC visibilities; a launch covers
+// a group of chunks whose features sit in the caller's workspace.
+//
+// Precision modes (feature element / MFMA / accumulation):
+//   FP64    : double features (sincospi), v_mfma_f64_16x16x4, FP64 accumulate.
+//   FP32    : float features, v_mfma_f32_16x16x4, FP32 within a chunk, FP64 across.
+//   BF16X3  : float features split hi+lo bf16, hi*hi + hi*lo + lo*hi on
+//             v_mfma_f32_16x16x32_bf16, FP32 within a chunk, FP64 across.
+//   INT8XN  : float features scaled per chunk to N signed 8-bit digits,
+//             all N^2 digit products on v_mfma_i32_16x16x64_i8 with exact int32
+//             accumulation per digit-order group, FP64 across chunks.  The
+//             result is the exact Gram matrix of the quantised features.
+#include "aj_common.h"
+
+namespace aj {
+namespace wt {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef int i4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+constexpr int KC = 4096;        // visibilities per chunk
+constexpr int FC = 2 * KC;      // features per chunk (cos block then sin block)
+constexpr int TILE = 64;        // output tile edge per workgroup
+constexpr int ROWB = 128;       // bytes per row per slice per LDS stage
+constexpr int PITCH = ROWB + 16;  // padded LDS row pitch (bytes)
+
+template <int MODE> struct Mode;
+template <> struct Mode<AJ_PREC_FP64> {
+  using E = double; static constexpr int NS = 1, MK = 4, NG = 1;
+};
+template <> struct Mode<AJ_PREC_FP32> {
+  using E = float; static constexpr int NS = 1, MK = 4, NG = 1;
+};
+template <> struct Mode<AJ_PREC_BF16X3> {
+  using E = uint16_t; static constexpr int NS = 2, MK = 32, NG = 1;
+};
+template <> struct Mode<AJ_PREC_INT8X3> {
+  using E = int8_t; static constexpr int NS = 3, MK = 64, NG = 5;
+};
+template <> struct Mode<AJ_PREC_INT8X4> {
+  using E = int8_t; static constexpr int NS = 4, MK = 64, NG = 7;
+};
+
+inline int slices_of(int mode) {
+  switch (mode) {
+    case AJ_PREC_BF16X3: return 2;
+    case AJ_PREC_INT8X3: return 3;
+    case AJ_PREC_INT8X4: return 4;
+    default: return 1;
+  }
+}
+inline int esize_of(int mode) {
+  switch (mode) {
+    case AJ_PREC_FP64: return 8;
+    case AJ_PREC_FP32: return 4;
+    case AJ_PREC_BF16X3: return 2;
+    default: return 1;
+  }
+}
+
+// Largest integer digit pattern for N signed 8-bit digits: 127 * 256^(N-1).
+__host__ __device__ inline double qmax_of(int ns) {
+  double q = 127.0;
+  for (int i = 1; i < ns; ++i) q *= 256.0;
+  return q;
+}
+
+// ---------------------------------------------------------------------------
+// Grid centring: mean of the grid, FP64, one block.
+// ---------------------------------------------------------------------------
+__global__ void grid_mean_kernel(const double* __restrict__ grid, int64_t M, double* __restrict__ out) {
+  __shared__ double s0[256], s1[256];
+  double a = 0.0, b = 0.0;
+  for (int64_t i = threadIdx.x; i < M; i += 256) {
+    a += grid[2 * i];
+    b += grid[2 * i + 1];
+  }
+  s0[threadIdx.x] = a;
+  s1[threadIdx.x] = b;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      s0[threadIdx.x] += s0[threadIdx.x + o];
+      s1[threadIdx.x] += s1[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = s0[0] / (double)M;
+    out[1] = s1[0] / (double)M;
+  }
+}
+
+// Per-chunk scale for the integer modes: max_k 1/|n_k| over the chunk.
+__global__ void chunk_scale_kernel(const double* __restrict__ noise, int64_t K, int64_t k_begin, int nchunks,
+                                   double* __restrict__ sc) {
+  __shared__ double red[256];
+  const int c = blockIdx.x;
+  double m = 0.0;
+  for (int k = threadIdx.x; k < KC; k += 256) {
+    int64_t kg = k_begin + (int64_t)c * KC + k;
+    if (kg < K) m = fmax(m, 1.0 / fabs(noise[kg]));
+  }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sc[c] = red[0] > 0.0 ? red[0] : 1.0;
+}
+
+// ---------------------------------------------------------------------------
+// Feature kernels.  Thread = one visibility of the group, loops over ROWS
+// image pixels.  Layout: X[slice][i][c*FC + k] = cos part, [.. + KC + k] = sin.
+// Rows are padded to a multiple of TILE; padded rows hold zeros.
+// ---------------------------------------------------------------------------
+constexpr int FEAT_ROWS = 16;
+
+__device__ inline double reduced_revs(double gx, double gy, double u0, double u1) {
+  double r = fma(gx, u0, gy * u1);
+  return r - rint(r);
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(256) features_kernel(
+    const double* __restrict__ noise, const double* __restrict__ uv, const double* __restrict__ grid,
+    const double* __restrict__ centre, int64_t K, int64_t M, int64_t Mpad, int64_t k_begin, int nchunks,
+    const double* __restrict__ chunk_sc, void* __restrict__ Xv, int64_t ldx) {
+  using E = typename Mode<MODE>::E;
+  constexpr int NS = Mode<MODE>::NS;
+  E* X = reinterpret_cast<E*>(Xv);
+  const int64_t kl = (int64_t)blockIdx.x * 256 + threadIdx.x;  // local visibility index in group
+  if (kl >= (int64_t)nchunks * KC) return;
+  const int c = (int)(kl / KC);
+  const int kc = (int)(kl % KC);
+  const int64_t kg = k_begin + kl;
+  double u0 = 0.0, u1 = 0.0, sw = 0.0;
+  if (kg < K) {
+    u0 = uv[2 * kg];
+    u1 = uv[2 * kg + 1];
+    sw = 1.0 / fabs(noise[kg]);
+  }
+  const double cy = centre[0], cx = centre[1];
+  const int64_t col_c = (int64_t)c * FC + kc;
+  const int64_t col_s = col_c + KC;
+  const int64_t sstride = Mpad * ldx;
+  double qscale = 0.0;
+  if constexpr (MODE == AJ_PREC_INT8X3 || MODE == AJ_PREC_INT8X4) qscale = sw / chunk_sc[c] * qmax_of(NS);
+  const int64_t i0 = (int64_t)blockIdx.y * FEAT_ROWS;
+  for (int r = 0; r < FEAT_ROWS; ++r) {
+    const int64_t i = i0 + r;
+    if (i >= Mpad) break;
+    double cv = 0.0, sv = 0.0;
+    float cf = 0.f, sf = 0.f;
+    const bool live = (i < M) && (kg < K);
+    if constexpr (MODE == AJ_PREC_FP64) {
+      if (live) {
+        double rv = reduced_revs(grid[2 * i + 1] - cx, grid[2 * i] - cy, u0, u1);
+        sincospi(2.0 * rv, &sv, &cv);
+        cv *= sw;
+        sv *= sw;
+      }
+      X[i * ldx + col_c] = cv;
+      X[i * ldx + col_s] = sv;
+    } else {
+      if (live) {
+        double rv = reduced_revs(grid[2 * i + 1] - cx, grid[2 * i] - cy, u0, u1);
+        sincospif(2.0f * (float)rv, &sf, &cf);
+      }
+      if constexpr (MODE == AJ_PREC_FP32) {
+        X[i * ldx + col_c] = cf * (float)sw;
+        X[i * ldx + col_s] = sf * (float)sw;
+      } else if constexpr (MODE == AJ_PREC_BF16X3) {
+        float a = cf * (float)sw, b = sf * (float)sw;
+        __bf16 ah = (__bf16)a, bh = (__bf16)b;
+        __bf16 al = (__bf16)(a - (float)ah), bl = (__bf16)(b - (float)bh);
+        X[i * ldx + col_c] = __builtin_bit_cast(uint16_t, ah);
+        X[i * ldx + col_s] = __builtin_bit_cast(uint16_t, bh);
+        X[sstride + i * ldx + col_c] = __builtin_bit_cast(uint16_t, al);
+        X[sstride + i * ldx + col_s] = __builtin_bit_cast(uint16_t, bl);
+      } else {
+        long long qc = live ? llrint((double)cf * qscale) : 0;
+        long long qs = live ? llrint((double)sf * qscale) : 0;
+        // Balanced base-256 digits, least significant last: q = sum_p a_p 256^(NS-1-p).
+#pragma unroll
+        for (int p = NS - 1; p >= 0; --p) {
+          int8_t dc = (int8_t)(qc & 0xFF);
+          int8_t ds = (int8_t)(qs & 0xFF);
+          qc = (qc - dc) >> 8;
+          qs = (qs - ds) >> 8;
+          X[p * sstride + i * ldx + col_c] = dc;
+          X[p * sstride + i * ldx + col_s] = ds;
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// SYRK over the lower triangle of 64x64 tiles.  4 waves as 2x2, each wave a
+// 32x32 block = 2x2 MFMA 16x16 blocks.  Per LDS stage each slice contributes
+// 64 rows x 128 bytes of features for both operands.
+// ---------------------------------------------------------------------------
+__device__ inline void tile_of(int64_t t, int64_t& I, int64_t& J) {
+  int64_t i = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+  while (i * (i + 1) / 2 > t) --i;
+  while ((i + 1) * (i + 2) / 2 <= t) ++i;
+  I = i;
+  J = t - i * (i + 1) / 2;
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(256) syrk_kernel(const void* __restrict__ Xv, int64_t M, int64_t Mpad,
+                                                   int64_t ldx, int nchunks, const double* __restrict__ chunk_sc,
+                                                   double* __restrict__ W, int first) {
+  using E = typename Mode<MODE>::E;
+  constexpr int NS = Mode<MODE>::NS;
+  constexpr int MK = Mode<MODE>::MK;
+  constexpr int NG = Mode<MODE>::NG;
+  constexpr int ES = (int)sizeof(E);
+  constexpr int KT = ROWB / ES;          // features per stage
+  constexpr int NSTEP = KT / MK;         // MFMA k-steps per stage
+  constexpr bool INT8 = (MODE == AJ_PREC_INT8X3 || MODE == AJ_PREC_INT8X4);
+
+  __shared__ __attribute__((aligned(16))) char lds[2][NS][TILE * PITCH];
+
+  int64_t TI, TJ;
+  tile_of(blockIdx.x, TI, TJ);
+  const int64_t I0 = TI * TILE, J0 = TJ * TILE;
+  const bool diag = (TI == TJ);
+  const char* X = reinterpret_cast<const char*>(Xv);
+  const int64_t sstride = Mpad * ldx * ES;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wr = (wave >> 1) * 32;  // wave row offset in tile
+  const int wc = (wave & 1) * 32;   // wave col offset in tile
+  const int lr = lane & 15;         // fragment row within 16-block
+  const int lk = lane >> 4;         // k-group of the lane
+
+  double accd[2][2][4];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) accd[a][b][r] = 0.0;
+
+  using AccT = typename std::conditional<MODE == AJ_PREC_FP64, d4,
+               typename std::conditional<INT8, i4, f4>::type>::type;
+  AccT acc[NG][2][2];
+
+  const int64_t chunk_cols = FC;
+  for (int c = 0; c < nchunks; ++c) {
+#pragma unroll
+    for (int g = 0; g < NG; ++g)
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[g][a][b] = AccT{0, 0, 0, 0};
+
+    for (int64_t k0 = (int64_t)c * chunk_cols; k0 < (int64_t)(c + 1) * chunk_cols; k0 += KT) {
+      // Global -> LDS: 64 rows x 128 B per slice per operand; 8 threads per row.
+      __syncthreads();
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int idx = tid + h * 256;
+          const int row = idx >> 3, seg = idx & 7;
+          const int4* srcA = reinterpret_cast<const int4*>(X + s * sstride + ((I0 + row) * ldx + k0) * ES) + seg;
+          *reinterpret_cast<int4*>(&lds[0][s][row * PITCH + seg * 16]) = *srcA;
+          if (!diag) {
+            const int4* srcB = reinterpret_cast<const int4*>(X + s * sstride + ((J0 + row) * ldx + k0) * ES) + seg;
+            *reinterpret_cast<int4*>(&lds[1][s][row * PITCH + seg * 16]) = *srcB;
+          }
+        }
+      }
+      __syncthreads();
+      const int bsel = diag ? 0 : 1;
+#pragma unroll
+      for (int st = 0; st < NSTEP; ++st) {
+        if constexpr (MODE == AJ_PREC_FP64 || MODE == AJ_PREC_FP32) {
+          E fa[2], fb[2];
+#pragma unroll
+          for (int a = 0; a < 2; ++a) {
+            fa[a] = *reinterpret_cast<const E*>(&lds[0][0][(wr + a * 16 + lr) * PITCH + (st * MK + lk) * ES]);
+            fb[a] = *reinterpret_cast<const E*>(&lds[bsel][0][(wc + a * 16 + lr) * PITCH + (st * MK + lk) * ES]);
+          }
+#pragma unroll
+          for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+              if constexpr (MODE == AJ_PREC_FP64)
+                acc[0][a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[a], fb[b], acc[0][a][b], 0, 0, 0);
+              else
+                acc[0][a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[a], fb[b], acc[0][a][b], 0, 0, 0);
+            }
+        } else if constexpr (MODE == AJ_PREC_BF16X3) {
+          bf8 fa[2][2], fb[2][2];  // [slice][block]
+#pragma unroll
+          for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+              fa[s][a] = *reinterpret_cast<const bf8*>(&lds[0][s][(wr + a * 16 + lr) * PITCH + (st * MK + lk * 8) * ES]);
+              fb[s][a] = *reinterpret_cast<const bf8*>(&lds[bsel][s][(wc + a * 16 + lr) * PITCH + (st * MK + lk * 8) * ES]);
+            }
+#pragma unroll
+          for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+              acc[0][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[1][a], fb[0][b], acc[0][a][b], 0, 0, 0);
+              acc[0][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[0][a], fb[1][b], acc[0][a][b], 0, 0, 0);
+              acc[0][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[0][a], fb[0][b], acc[0][a][b], 0, 0, 0);
+            }
+        } else {
+          i4 fa[NS][2], fb[NS][2];
+#pragma unroll
+          for (int s = 0; s < NS; ++s)
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+              fa[s][a] = *reinterpret_cast<const i4*>(&lds[0][s][(wr + a * 16 + lr) * PITCH + st * MK + lk * 16]);
+              fb[s][a] = *reinterpret_cast<const i4*>(&lds[bsel][s][(wc + a * 16 + lr) * PITCH + st * MK + lk * 16]);
+            }
+#pragma unroll
+          for (int p = 0; p < NS; ++p)
+#pragma unroll
+            for (int q = 0; q < NS; ++q)
+#pragma unroll
+              for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+                  acc[p + q][a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[p][a], fb[q][b], acc[p + q][a][b], 0, 0, 0);
+        }
+      }
+    }
+    // Chunk flush into FP64.
+    if constexpr (MODE == AJ_PREC_FP64) {
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) accd[a][b][r] += acc[0][a][b][r];
+    } else if constexpr (INT8) {
+      const double q = qmax_of(NS);
+      const double s2 = chunk_sc[c] * chunk_sc[c] / (q * q);
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            double v = 0.0;
+#pragma unroll
+            for (int g = 0; g < NG; ++g) v = v * 256.0 + (double)acc[g][a][b][r];
+            accd[a][b][r] += v * s2;
+          }
+    } else {
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) accd[a][b][r] += (double)acc[0][a][b][r];
+    }
+  }
+
+  // Write / accumulate the tile: lane holds rows 4*(lane/16)+r, column lane%16.
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t row = I0 + wr + a * 16 + lk * 4 + r;
+        const int64_t col = J0 + wc + b * 16 + lr;
+        if (row < M && col < M) {
+          double* p = W + row * M + col;
+          *p = first ? accd[a][b][r] : *p + accd[a][b][r];
+        }
+      }
+}
+
+// Copy the lower triangle to the upper one (32x32 tiles through LDS).
+__global__ void mirror_kernel(double* __restrict__ W, int64_t M) {
+  __shared__ double t[32][33];
+  const int64_t bi = blockIdx.y, bj = blockIdx.x;
+  if (bj > bi) return;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 256 threads: 32 x 8
+  for (int yy = ty; yy < 32; yy += 8) {
+    int64_t r = bi * 32 + yy, c = bj * 32 + tx;
+    t[yy][tx] = (r < M && c < M) ? W[r * M + c] : 0.0;
+  }
+  __syncthreads();
+  for (int yy = ty; yy < 32; yy += 8) {
+    int64_t r = bj * 32 + yy, c = bi * 32 + tx;  // upper element (r < c)
+    if (r < M && c < M && c > r) W[r * M + c] = t[tx][yy];
+  }
+}
+
+struct Plan {
+  int mode, ns, es;
+  int64_t Mpad, ldx;
+  int group;  // chunks per launch
+  size_t feat_bytes, bytes;
+};
+
+inline Plan make_plan(int64_t K, int64_t M, int mode) {
+  Plan p{};
+  p.mode = mode;
+  p.ns = slices_of(mode);
+  p.es = esize_of(mode);
+  p.Mpad = ceil_div(M, TILE) * TILE;
+  const int64_t nch = std::max<int64_t>(1, ceil_div(K, KC));
+  const double per_chunk = (double)p.ns * p.Mpad * FC * p.es;
+  int64_t g = (int64_t)((1ll << 30) / per_chunk);
+  g = std::max<int64_t>(1, std::min<int64_t>(g, nch));
+  p.group = (int)g;
+  p.ldx = (int64_t)p.group * FC;
+  p.feat_bytes = (size_t)p.ns * p.Mpad * p.ldx * p.es;
+  p.bytes = align_up(p.feat_bytes, 256) + 256 /*centre*/ + align_up(sizeof(double) * p.group, 256) + 256;
+  return p;
+}
+
+template <int MODE>
+int run(const double* noise, const double* uv, const double* grid, int64_t K, int64_t M, double* W,
+        const Plan& p, Arena& ar, hipStream_t st) {
+  void* X = ar.take<char>(p.feat_bytes);
+  double* centre = ar.take<double>(2);
+  double* sc = ar.take<double>(p.group);
+  if (!ar.ok()) {
+    set_error("workspace too small: need %zu bytes, got %zu", p.bytes, ar.cap);
+    return AJ_ESPACE;
+  }
+  hipLaunchKernelGGL(grid_mean_kernel, dim3(1), dim3(256), 0, st, grid, M, centre);
+  AJ_LAUNCH_CHECK();
+  const int64_t nch = std::max<int64_t>(1, ceil_div(K, KC));
+  const int64_t nt = p.Mpad / TILE;
+  const int64_t ntiles = nt * (nt + 1) / 2;
+  int first = 1;
+  for (int64_t c0 = 0; c0 < nch; c0 += p.group) {
+    const int ng = (int)std::min<int64_t>(p.group, nch - c0);
+    const int64_t k_begin = c0 * KC;
+    if (MODE == AJ_PREC_INT8X3 || MODE == AJ_PREC_INT8X4) {
+      hipLaunchKernelGGL(chunk_scale_kernel, dim3(ng), dim3(256), 0, st, noise, K, k_begin, ng, sc);
+      AJ_LAUNCH_CHECK();
+    }
+    dim3 fg((unsigned)ceil_div((int64_t)ng * KC, 256), (unsigned)ceil_div(p.Mpad, FEAT_ROWS));
+    timer_begin(AJ_TIMER_WT_FEATURES, st);
+    hipLaunchKernelGGL(features_kernel<MODE>, fg, dim3(256), 0, st, noise, uv, grid, centre, K, M, p.Mpad, k_begin,
+                       ng, sc, X, p.ldx);
+    timer_end(AJ_TIMER_WT_FEATURES, st);
+    AJ_LAUNCH_CHECK();
+    timer_begin(AJ_TIMER_WT_SYRK, st);
+    hipLaunchKernelGGL(syrk_kernel<MODE>, dim3((unsigned)ntiles), dim3(256), 0, st, X, M, p.Mpad, p.ldx, ng, sc, W,
+                       first);
+    timer_end(AJ_TIMER_WT_SYRK, st);
+    AJ_LAUNCH_CHECK();
+    first = 0;
+  }
+  dim3 mg((unsigned)ceil_div(M, 32), (unsigned)ceil_div(M, 32));
+  hipLaunchKernelGGL(mirror_kernel, mg, dim3(256), 0, st, W, M);
+  AJ_LAUNCH_CHECK();
+  return AJ_OK;
+}
+
+}  // namespace wt
+}  // namespace aj
+
+using namespace aj;
+
+extern "C" size_t aj_w_tilde_workspace_bytes(int64_t K, int64_t M, int precision) {
+  if (K < 0 || M <= 0 || precision < AJ_PREC_FP64 || precision > AJ_PREC_INT8X4) return 0;
+  return wt::make_plan(K, M, precision).bytes;
+}
+
+extern "C" int aj_w_tilde_curvature_interferometer_from(const double* noise_map_real, const double* uv_wavelengths,
+                                                        const double* grid_radians_slim, int64_t K, int64_t M,
+                                                        int precision, double* w_tilde, void* workspace,
+                                                        size_t workspace_bytes, void* stream) {
+  AJ_REQUIRE(M > 0 && K >= 0, "bad sizes M=%lld K=%lld", (long long)M, (long long)K);
+  AJ_REQUIRE(precision >= AJ_PREC_FP64 && precision <= AJ_PREC_INT8X4, "bad precision %d", precision);
+  AJ_REQUIRE(w_tilde && grid_radians_slim && (K == 0 || (noise_map_real && uv_wavelengths)), "null pointer");
+  wt::Plan p = wt::make_plan(K, M, precision);
+  AJ_REQUIRE(workspace_bytes >= p.bytes && workspace, "workspace too small: need %zu bytes, got %zu", p.bytes,
+             workspace_bytes);
+  Arena ar(workspace, workspace_bytes);
+  hipStream_t st = as_stream(stream);
+  switch (precision) {
+    case AJ_PREC_FP64: return wt::run<AJ_PREC_FP64>(noise_map_real, uv_wavelengths, grid_radians_slim, K, M, w_tilde, p, ar, st);
+    case AJ_PREC_FP32: return wt::run<AJ_PREC_FP32>(noise_map_real, uv_wavelengths, grid_radians_slim, K, M, w_tilde, p, ar, st);
+    case AJ_PREC_BF16X3: return wt::run<AJ_PREC_BF16X3>(noise_map_real, uv_wavelengths, grid_radians_slim, K, M, w_tilde, p, ar, st);
+    case AJ_PREC_INT8X3: return wt::run<AJ_PREC_INT8X3>(noise_map_real, uv_wavelengths, grid_radians_slim, K, M, w_tilde, p, ar, st);
+    default: return wt::run<AJ_PREC_INT8X4>(noise_map_real, uv_wavelengths, grid_radians_slim, K, M, w_tilde, p, ar, st);
+  }
+}

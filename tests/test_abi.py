@@ -1,0 +1,92 @@
+"""The C-ABI library loads and exports every symbol include/autojax_amd.h declares (CPU only)."""
+
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+HEADER = ROOT / "include" / "autojax_amd.h"
+
+
+def declared():
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r"^(?:int|size_t|const char\*)\s+(aj_\w+)\(", text, re.M)))
+
+
+def test_header_declares_entry_points():
+    names = declared()
+    assert "aj_w_tilde_curvature_interferometer_from" in names
+    assert "aj_log_likelihood_function_via_w_tilde_from" in names
+    assert len(names) >= 20
+
+
+def test_library_exports_all_declared_symbols():
+    from autojax_amd import _lib
+
+    lib = _lib.load()
+    missing = [n for n in declared() if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(declared()) == set(_lib.SIGNATURES), set(declared()) ^ set(_lib.SIGNATURES)
+
+
+def test_workspace_queries_without_gpu():
+    from autojax_amd import _lib
+
+    assert _lib.query("aj_w_tilde_workspace_bytes", 2000, 716, 0) > 716 * 8192 * 8
+    assert _lib.query("aj_w_tilde_workspace_bytes", 2000, 716, 99) == 0
+    assert _lib.query("aj_solve_workspace_bytes", 100) == 800 * 100
+    assert _lib.load().aj_abi_version() == 1
+
+
+def test_bad_arguments_rejected_before_any_gpu_work():
+    from autojax_amd import _lib
+
+    with pytest.raises(ValueError):
+        _lib.call("aj_w_tilde_curvature_interferometer_from", None, None, None, 10, 0, 0, None, None, 0, None)
+    with pytest.raises(ValueError):
+        _lib.call("aj_w_tilde_curvature_interferometer_from", None, None, None, 10, 5, 7, None, None, 0, None)
+
+
+def test_drop_in_signatures_match_reference_names():
+    """Parameter names and order of the drop-in equal those of src/autojax/numba.py (listed here,
+    since that module cannot be imported without numba)."""
+    import inspect
+
+    from autojax_amd import api
+
+    ref = {
+        "w_tilde_curvature_interferometer_from": ["noise_map_real", "uv_wavelengths", "grid_radians_slim"],
+        "w_tilde_data_interferometer_from": ["visibilities_real", "noise_map_real", "uv_wavelengths", "grid_radians_slim"],
+        "mapping_matrix_from": ["pix_indexes_for_sub_slim_index", "pix_weights_for_sub_slim_index", "pixels"],
+        "data_vector_from": ["mapping_matrix", "dirty_image"],
+        "curvature_matrix_via_w_tilde_from": ["w_tilde", "mapping_matrix"],
+        "sparse_mapping_matrix_transpose_matmul": ["matrix", "pix_indexes_for_sub_slim_index", "pix_weights_for_sub_slim_index", "pixels"],
+        "curvature_matrix_via_w_wilde_sparse_mapping_matrix_from": ["w_tilde", "pix_indexes_for_sub_slim_index", "pix_weights_for_sub_slim_index", "pixels"],
+        "constant_regularization_matrix_from": ["coefficient", "neighbors", "neighbors_sizes"],
+        "reconstruction_positive_negative_from": ["data_vector", "curvature_reg_matrix"],
+        "noise_normalization_complex_from": ["noise_map"],
+        "log_likelihood_function_via_w_tilde_from": ["dirty_image", "data", "noise_map", "w_tilde", "pix_indexes_for_sub_slim_index", "pix_weights_for_sub_slim_index", "neighbors", "neighbors_sizes"],
+        "mask_2d_centres_from": ["shape_native", "pixel_scales", "centre"],
+        "mask_2d_circular_from": ["shape_native", "pixel_scales", "radius", "centre"],
+    }
+    for name, params in ref.items():
+        sig = inspect.signature(getattr(api, name))
+        pos = [p.name for p in sig.parameters.values() if p.kind == p.POSITIONAL_OR_KEYWORD]
+        assert pos == params, (name, pos)
+
+
+def test_host_validation():
+    import numpy as np
+
+    from autojax_amd import api
+
+    with pytest.raises(ValueError):
+        api.validate_mapping(np.array([[0, 5]]), 5)
+    with pytest.raises(ValueError):
+        api.validate_mapping(np.array([[1, 1]]), 5)
+    api.validate_mapping(np.array([[1, -1, 1]]), 5)  # repeat after the -1 is ignored
+    with pytest.raises(ValueError):
+        api.validate_neighbors(np.array([[1], [-1]]), np.array([1, 0]))
+    api.validate_neighbors(np.array([[1], [0]]), np.array([1, 1]))

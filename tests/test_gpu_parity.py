@@ -1,0 +1,197 @@
+"""GPU parity: the HIP path (through the C-ABI) against the CPU oracle and the golden fixtures."""
+
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from autojax_amd import datasets as ds
+from oracle import evidence as ev
+from oracle import restated as R
+
+pytestmark = pytest.mark.gpu
+GOLD = Path(__file__).resolve().parent / "golden"
+
+# normwise tolerances of W-tilde against the FP64 oracle, relative to max|W| (= the diagonal)
+WT_TOL = {"fp64": 1e-12, "int8x4": 1e-9, "int8x3": 3e-7, "fp32": 3e-6, "bf16x3": 3e-4}
+
+
+@pytest.fixture(scope="module")
+def cfg1_small():
+    """The 716-pixel workload with K cut to 600 visibilities (oracle W-tilde in seconds)."""
+    d = ds.config1(K=600)
+    W = R.w_tilde_curvature_interferometer_from(d.noise_map_real, d.uv_wavelengths, d.grid_radians_slim)
+    d.ensure_dirty_image()
+    return d, W
+
+
+@pytest.fixture(scope="module")
+def delaunay_small():
+    d = ds.small_delaunay(31, 700, 1500, (40, 120), n_samples=6, lam_range=(0.01, 100.0))
+    W = R.w_tilde_curvature_interferometer_from(d.noise_map_real, d.uv_wavelengths, d.grid_radians_slim)
+    d.ensure_dirty_image()
+    return d, W
+
+
+def _rel(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+def test_w_tilde_fp64_matches_fsum_golden(gpu):
+    from autojax_amd import api
+
+    g = np.load(GOLD / "tiny_wtilde.npz")
+    W = api.w_tilde_curvature_interferometer_from(g["noise_map_real"], g["uv_wavelengths"], g["grid_radians_slim"])
+    print("tiny golden rel err", _rel(W, g["w_tilde"]), flush=True)
+    assert _rel(W, g["w_tilde"]) < 1e-13
+
+
+@pytest.mark.parametrize("precision", ["fp64", "int8x4", "int8x3", "fp32", "bf16x3"])
+def test_w_tilde_modes_vs_oracle(gpu, cfg1_small, precision):
+    from autojax_amd import api
+
+    d, Wo = cfg1_small
+    W = api.w_tilde_curvature_interferometer_from(d.noise_map_real, d.uv_wavelengths, d.grid_radians_slim,
+                                                  precision=precision)
+    err = _rel(W, Wo)
+    print(f"{precision}: normwise rel err {err:.3e}", flush=True)
+    assert err < WT_TOL[precision]
+    np.testing.assert_array_equal(W, W.T)
+
+
+def test_w_tilde_multichunk_rows(gpu):
+    """K spanning several 4096-visibility chunks; oracle on a row subset at full K."""
+    from autojax_amd import api
+
+    d = ds.config1(K=9000)
+    W = api.w_tilde_curvature_interferometer_from(d.noise_map_real, d.uv_wavelengths, d.grid_radians_slim)
+    rows = np.array([0, 1, 57, 300, 715])
+    g = d.grid_radians_slim
+    n = d.noise_map_real
+    for i in rows:
+        dg = 2 * np.pi * (g[i] - g)
+        ph = dg[:, 1:2] * d.uv_wavelengths[None, :, 0] + dg[:, 0:1] * d.uv_wavelengths[None, :, 1]
+        want = np.cos(ph) @ (1 / n**2)
+        assert np.abs(W[i] - want).max() < 1e-12 * want.max()
+
+
+def test_w_tilde_run_to_run_bitwise(gpu, cfg1_small):
+    from autojax_amd import api
+
+    d, _ = cfg1_small
+    a = api.w_tilde_curvature_interferometer_from(d.noise_map_real, d.uv_wavelengths, d.grid_radians_slim, precision="int8x3")
+    b = api.w_tilde_curvature_interferometer_from(d.noise_map_real, d.uv_wavelengths, d.grid_radians_slim, precision="int8x3")
+    np.testing.assert_array_equal(a, b)
+
+
+def test_data_side_sums(gpu, cfg1_small):
+    from autojax_amd import api
+
+    d, _ = cfg1_small
+    y = api.dirty_image_from(d.data, d.noise_map, d.uv_wavelengths, d.grid_radians_slim)
+    np.testing.assert_allclose(y, d.dirty_image, rtol=0, atol=1e-11 * np.abs(d.dirty_image).max())
+    v = np.ascontiguousarray(d.data.real) + 3.0
+    got = api.w_tilde_data_interferometer_from(v, d.noise_map_real, d.uv_wavelengths, d.grid_radians_slim)
+    want = R.w_tilde_data_interferometer_from(v, d.noise_map_real, d.uv_wavelengths, d.grid_radians_slim)
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-11 * np.abs(want).max())
+
+
+def test_mapping_data_vector_curvature(gpu, delaunay_small):
+    from autojax_amd import api
+
+    d, W = delaunay_small
+    smp = d.samples[0]
+    S = smp.pixels
+    T = api.mapping_matrix_from(smp.pix_indexes_for_sub_slim_index, smp.pix_weights_for_sub_slim_index, S)
+    To = R.mapping_matrix_from(smp.pix_indexes_for_sub_slim_index, smp.pix_weights_for_sub_slim_index, S)
+    np.testing.assert_array_equal(T, To)
+    D = api.data_vector_from(T, d.dirty_image)
+    np.testing.assert_allclose(D, R.data_vector_from(To, d.dirty_image), rtol=1e-12, atol=1e-12 * np.abs(D).max())
+    Fo = R.curvature_matrix_via_w_tilde_from(W, To)
+    F1 = api.curvature_matrix_via_w_tilde_from(W, T)
+    F2 = api.curvature_matrix_via_w_wilde_sparse_mapping_matrix_from(W, smp.pix_indexes_for_sub_slim_index,
+                                                                      smp.pix_weights_for_sub_slim_index, S)
+    print("F dense", _rel(F1, Fo), "F sparse", _rel(F2, Fo), flush=True)
+    assert _rel(F1, Fo) < 1e-12 and _rel(F2, Fo) < 1e-12
+    X = np.random.default_rng(0).standard_normal((W.shape[0], 17))
+    G = api.sparse_mapping_matrix_transpose_matmul(X, smp.pix_indexes_for_sub_slim_index,
+                                                   smp.pix_weights_for_sub_slim_index, S)
+    Go = R.sparse_mapping_matrix_transpose_matmul(X, smp.pix_indexes_for_sub_slim_index,
+                                                  smp.pix_weights_for_sub_slim_index, S)
+    assert _rel(G, Go) < 1e-13
+
+
+def test_regularization_solve_noise(gpu, delaunay_small):
+    from autojax_amd import api
+
+    d, W = delaunay_small
+    smp = d.samples[1]
+    H = api.constant_regularization_matrix_from(1.7, smp.neighbors, smp.neighbors_sizes)
+    np.testing.assert_array_equal(H, R.constant_regularization_matrix_from(1.7, smp.neighbors, smp.neighbors_sizes))
+    To = R.mapping_matrix_from(smp.pix_indexes_for_sub_slim_index, smp.pix_weights_for_sub_slim_index, smp.pixels)
+    A = R.curvature_matrix_via_w_tilde_from(W, To) + H
+    Dv = R.data_vector_from(To, d.dirty_image)
+    s = api.reconstruction_positive_negative_from(Dv, A)
+    so = R.reconstruction_positive_negative_from(Dv, A)
+    print("solve rel", _rel(s, so), flush=True)
+    assert _rel(s, so) < 1e-9
+    nn = api.noise_normalization_complex_from(d.noise_map)
+    assert abs(nn - R.noise_normalization_complex_from(d.noise_map)) < 1e-12 * abs(nn)
+    with pytest.raises(np.linalg.LinAlgError):
+        api.reconstruction_positive_negative_from(Dv, -A)
+
+
+def test_log_likelihood_golden_evidence(gpu):
+    from autojax_amd import api
+
+    g = np.load(GOLD / "tiny_evidence.npz")
+    W = api.w_tilde_curvature_interferometer_from(g["noise_map"].real.copy(), g["uv_wavelengths"], g["grid_radians_slim"])
+    got = api.log_likelihood_function_via_w_tilde_from(g["dirty_image"], g["data"], g["noise_map"], W,
+                                                       g["pix_indexes_for_sub_slim_index"],
+                                                       g["pix_weights_for_sub_slim_index"], g["neighbors"],
+                                                       g["neighbors_sizes"])
+    want = g["log_evidence"][1]
+    print("golden evidence", got, want, flush=True)
+    assert abs(got - want) < 3e-8 * abs(want)
+
+
+def test_log_likelihood_cfg1_vs_oracle(gpu, cfg1_small):
+    from autojax_amd import api
+
+    d, W = cfg1_small
+    smp = d.samples[0]
+    args = (d.dirty_image, d.data, d.noise_map, W, smp.pix_indexes_for_sub_slim_index,
+            smp.pix_weights_for_sub_slim_index, smp.neighbors, smp.neighbors_sizes)
+    got = api.log_likelihood_function_via_w_tilde_from(*args)
+    want = R.log_likelihood_function_via_w_tilde_from(*args)
+    print("cfg1 logL", got, want, got - want, flush=True)
+    assert abs(got - want) < 1e-10 * abs(want)
+
+
+def test_batched_log_evidence_vs_oracle(gpu, delaunay_small):
+    import torch
+
+    from autojax_amd import evidence as E
+
+    d, W = delaunay_small
+    Wd = torch.from_numpy(W).cuda()
+    batch = E.SampleBatch(d.samples, d.M)
+    c2nn = E.chi2_noise_norm(d.data, d.noise_map)
+    y = torch.from_numpy(d.dirty_image).cuda()
+    out = batch.log_evidence(Wd, y, c2nn, check=True).cpu().numpy()
+    out2 = batch.log_evidence(Wd, y, c2nn, check=True).cpu().numpy()
+    np.testing.assert_array_equal(out, out2)
+    chi = d.data.view(np.float64) / d.noise_map.view(np.float64)
+    nn = R.noise_normalization_complex_from(d.noise_map)
+    logdets = batch.logdets.cpu().numpy()
+    for b, smp in enumerate(d.samples):
+        To = R.mapping_matrix_from(smp.pix_indexes_for_sub_slim_index, smp.pix_weights_for_sub_slim_index, smp.pixels)
+        F = R.curvature_matrix_via_w_tilde_from(W, To)
+        D = R.data_vector_from(To, d.dirty_image)
+        H = R.constant_regularization_matrix_from(smp.coefficient, smp.neighbors, smp.neighbors_sizes)
+        ldh = ev.logdet_deflated(H, smp.coefficient**2 * smp.neighbors_sizes.mean())
+        want, s, lda, _ = R.log_evidence_from_parts(F, D, chi @ chi, nn, smp.neighbors, smp.neighbors_sizes,
+                                                     smp.coefficient, logdet_h=ldh)
+        print(f"sample {b} S={smp.pixels} lam={smp.coefficient:.3g}: {out[b]:.12g} vs {want:.12g} "
+              f"d={out[b]-want:.2e} ldA {logdets[b,0]-lda:.2e} ldH {logdets[b,1]-ldh:.2e}", flush=True)
+        assert abs(out[b] - want) < 1e-9 * max(1.0, abs(want))
