@@ -165,6 +165,31 @@ int aj_log_evidence_batched(const double* w_tilde, int64_t M, const double* dirt
                             double* reconstruction, double* logdets, int64_t* info, void* workspace,
                             size_t workspace_bytes, void* stream);
 
+/* Backward of aj_log_evidence_batched (no reference counterpart: the reference differentiates its
+ * JAX form, jax.py:1032-1058, by autodiff).  Uses the Cholesky factors and Omega = W T left in
+ * forward_workspace by the immediately preceding aj_log_evidence_batched call with the same batch,
+ * and that call's reconstruction.  coefficients_dev is the device copy of the coefficients.
+ * Outputs dcoefficients[n] = dlogL_b/dc_b and dweights[n,M,B] = dlogL_b/dt_b[m,k] (0 after a -1). */
+size_t aj_log_evidence_backward_workspace_bytes(int64_t M, int64_t n_samples, const int64_t* S_host);
+int aj_log_evidence_batched_backward(const double* w_tilde, int64_t M, const double* dirty_image, int64_t n_samples,
+                                     int64_t B, int64_t P, const int64_t* S_host, const int64_t* pix_indexes,
+                                     const double* pix_weights, const int64_t* neighbors,
+                                     const int64_t* neighbors_sizes, const double* coefficients_dev,
+                                     const double* reconstruction, const void* forward_workspace,
+                                     size_t forward_workspace_bytes, double* dcoefficients, double* dweights,
+                                     void* workspace, size_t workspace_bytes, void* stream);
+
+/* Barycentric weights of fixed points positions[M,2] (y, x) in the triangles triangles[M,3] of
+ * vertices[S,2]: weights[M,3] (each image pixel's source-mesh interpolation, done upstream of the
+ * reference by its mapper).  aj_barycentric_vjp maps dweights[M,3] to dvertices[S,2] with the
+ * points held fixed. */
+int aj_barycentric_weights(const double* vertices, const double* positions, const int64_t* triangles, int64_t M,
+                           double* weights, void* stream);
+size_t aj_barycentric_vjp_workspace_bytes(int64_t M, int64_t S);
+int aj_barycentric_vjp(const double* vertices, const double* positions, const int64_t* triangles, int64_t M, int64_t S,
+                       const double* dweights, double* dvertices, void* workspace, size_t workspace_bytes,
+                       void* stream);
+
 #ifdef __cplusplus
 }
 #endif

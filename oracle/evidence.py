@@ -94,3 +94,46 @@ def longdouble_cholesky_logdet(A):
         L[j, j] = np.sqrt(d)
         L[j + 1 :, j] = (A[j + 1 :, j] - L[j + 1 :, :j] @ L[j, :j]) / L[j, j]
     return float(2.0 * np.sum(np.log(np.diag(L))))
+
+
+def torch_log_evidence(coefficient, weights, idx, W, dirty, chi2, nn, neighbors, neighbors_sizes):
+    """Dense FP64 torch restatement of numba.py:818-857 with an explicit coefficient, for autograd
+    cross-checks on the CPU (coefficient and weights may require grad).  Tiny sizes only."""
+    import torch
+
+    M, B = idx.shape
+    S = neighbors_sizes.shape[0]
+    live = np.ones(idx.shape, bool)
+    if B > 1:
+        live[:, 1:] = np.cumprod(idx[:, :-1] != -1, axis=1).astype(bool)
+    live &= idx >= 0
+    rows, cols = np.nonzero(live)
+    T = torch.zeros((M, S), dtype=torch.float64)
+    T = T.index_put((torch.as_tensor(rows), torch.as_tensor(idx[rows, cols])), weights[rows, cols])
+    Wt = torch.as_tensor(W)
+    F = T.T @ Wt @ T
+    D = T.T @ torch.as_tensor(dirty)
+    Lap = torch.diag(torch.as_tensor(neighbors_sizes, dtype=torch.float64))
+    for i in range(S):
+        for k in neighbors[i]:
+            if k == -1:
+                break
+            Lap[i, k] -= 1.0
+    H = coefficient**2 * Lap + 1e-8 * torch.eye(S, dtype=torch.float64)
+    A = F + H
+    s = torch.linalg.solve(A, D)
+    return -0.5 * (chi2 - s @ D + torch.linalg.slogdet(A)[1] - torch.linalg.slogdet(H)[1] + nn)
+
+
+def torch_barycentric(vertices, positions, triangles):
+    """Barycentric weights by areas, torch (autograd w.r.t. vertices)."""
+    import torch
+
+    V = vertices[torch.as_tensor(triangles)]  # [M,3,2] (y, x)
+    P = torch.as_tensor(positions)
+
+    def a2(p0, p1, p2):
+        return (p1[:, 1] - p0[:, 1]) * (p2[:, 0] - p0[:, 0]) - (p2[:, 1] - p0[:, 1]) * (p1[:, 0] - p0[:, 0])
+
+    A = a2(V[:, 0], V[:, 1], V[:, 2])
+    return torch.stack([a2(P, V[:, 1], V[:, 2]) / A, a2(V[:, 0], P, V[:, 2]) / A, a2(V[:, 0], V[:, 1], P) / A], 1)

@@ -54,6 +54,12 @@ SIGNATURES = {
     "aj_log_evidence_batched_workspace_bytes": (_S, [_I, _I, _I, _P]),
     "aj_log_evidence_batched": (
         _INT, [_P, _I, _P, _P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _S, _P]),
+    "aj_log_evidence_backward_workspace_bytes": (_S, [_I, _I, _P]),
+    "aj_log_evidence_batched_backward": (
+        _INT, [_P, _I, _P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _S, _P, _P, _P, _S, _P]),
+    "aj_barycentric_weights": (_INT, [_P, _P, _P, _I, _P, _P]),
+    "aj_barycentric_vjp_workspace_bytes": (_S, [_I, _I]),
+    "aj_barycentric_vjp": (_INT, [_P, _P, _P, _I, _I, _P, _P, _P, _S, _P]),
 }
 
 _lib = None

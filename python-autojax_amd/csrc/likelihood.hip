@@ -1132,3 +1132,386 @@ extern "C" int aj_w_tilde_data_interferometer_from(const double* visibilities_re
   return vis_sum(visibilities_real, 1, nullptr, 0, noise_map_real, 1, uv_wavelengths, grid_radians_slim, K, M,
                  w_tilde_data, workspace, workspace_bytes, as_stream(stream));
 }
+
+// ===========================================================================
+// Backward of the batched log-evidence (no reference counterpart; the
+// reference differentiates its JAX form, jax.py:1032-1058, with autodiff).
+// With A = F + H, s = A^-1 D, F = T^T W T, D = T^T y, H = c^2 L + eps I:
+//   dlogL/dc       = -c [ s^T L s + tr(A^-1 L) - tr(H^-1 L) ],
+//                    tr(H^-1 L) = (S - eps tr H^-1) / c^2
+//   dlogL/dT[m,s]  = s_s (y_m - (W T s)_m) - (W T A^-1)[m, s]
+// evaluated at the nonzeros (m, b) of the mapping.  Uses the Cholesky
+// factors left in the forward workspace.
+// ===========================================================================
+namespace aj {
+namespace lk {
+
+// Column j of the inverse from the lower factor L (one wave per column):
+//   inv != nullptr: inv[:, j] = L^-T L^-1 e_j;   colnorm2[j] = ||L^-1 e_j||^2 always.
+__global__ void __launch_bounds__(64) factor_inverse_kernel(const SampleDesc* __restrict__ sd,
+                                                            const double* __restrict__ mats, int which_h,
+                                                            double* __restrict__ inv_all,
+                                                            double* __restrict__ colnorm2_all) {
+  extern __shared__ double y[];
+  const SampleDesc d = sd[blockIdx.y];
+  const int64_t n = d.S;
+  const int64_t j = blockIdx.x;
+  if (j >= n) return;
+  const double* L = mats + d.mat_off + (which_h ? n * n : 0);
+  const int lane = threadIdx.x;
+  for (int64_t i = lane; i < n; i += 64) y[i] = 0.0;
+  __syncthreads();
+  // forward: L y = e_j, y_i = 0 for i < j
+  for (int64_t i = j; i < n; ++i) {
+    double acc = 0.0;
+    for (int64_t k = j + lane; k < i; k += 64) acc = fma(L[i * n + k], y[k], acc);
+    acc = wave_sum(acc);
+    if (lane == 0) y[i] = ((i == j ? 1.0 : 0.0) - acc) / L[i * n + i];
+    __syncthreads();
+  }
+  double nrm = 0.0;
+  for (int64_t i = j + lane; i < n; i += 64) nrm = fma(y[i], y[i], nrm);
+  nrm = wave_sum(nrm);
+  if (lane == 0) colnorm2_all[d.s_off + j] = nrm;
+  if (!inv_all) return;
+  // backward: L^T x = y (x overwrites y), row i of L^T is column i of L
+  for (int64_t i = n - 1; i >= 0; --i) {
+    double acc = 0.0;
+    for (int64_t k = i + 1 + lane; k < n; k += 64) acc = fma(L[k * n + i], y[k], acc);
+    acc = wave_sum(acc);
+    if (lane == 0) y[i] = (y[i] - acc) / L[i * n + i];
+    __syncthreads();
+  }
+  double* inv = inv_all + d.mat_off / 2;  // one S*S block per sample
+  for (int64_t i = lane; i < n; i += 64) inv[i * n + j] = y[i];
+}
+
+// Per-sample scalar gradient: one workgroup per sample.
+__global__ void __launch_bounds__(256) coef_grad_kernel(const SampleDesc* __restrict__ sd,
+                                                        const int64_t* __restrict__ nbr,
+                                                        const int64_t* __restrict__ nsz, int64_t P,
+                                                        const double* __restrict__ recon,
+                                                        const double* __restrict__ inv_all,
+                                                        const double* __restrict__ hnorm2,
+                                                        const double* __restrict__ coef, double* __restrict__ dcoef) {
+  __shared__ double r0[256], r1[256], r2[256];
+  const SampleDesc d = sd[blockIdx.x];
+  const int64_t n = d.S;
+  const double* s = recon + d.s_off;
+  const double* Ai = inv_all + d.mat_off / 2;
+  double sls = 0.0, tra = 0.0, trh = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += 256) {
+    const double sz = (double)nsz[d.s_off + i];
+    sls += sz * s[i] * s[i];
+    tra += sz * Ai[i * n + i];
+    trh += hnorm2[d.s_off + i];
+    const int64_t* row = nbr + (d.s_off + i) * P;
+    for (int64_t p = 0; p < P; ++p) {
+      const int64_t k = row[p];
+      if (k == -1) break;
+      if (k < 0 || k >= n) continue;
+      sls -= s[i] * s[k];
+      tra -= Ai[i * n + k];
+    }
+  }
+  r0[threadIdx.x] = sls;
+  r1[threadIdx.x] = tra;
+  r2[threadIdx.x] = trh;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      r0[threadIdx.x] += r0[threadIdx.x + o];
+      r1[threadIdx.x] += r1[threadIdx.x + o];
+      r2[threadIdx.x] += r2[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double eps = 1e-8;
+    const double c = coef[blockIdx.x];
+    double tr_hinv = r2[0];
+    if (d.beta > 0.0) tr_hinv += 1.0 / eps - 1.0 / (eps + d.beta);  // undo the rank-one deflation
+    const double tr_hl = ((double)n - eps * tr_hinv) / (c * c);
+    dcoef[blockIdx.x] = -c * (r0[0] + r1[0] - tr_hl);
+  }
+}
+
+// u = T s per sample (M-vector), u_all[b*M + m].
+__global__ void map_recon_kernel(const SampleDesc* __restrict__ sd, const int64_t* __restrict__ idx,
+                                 const double* __restrict__ wts, int64_t M, int64_t B,
+                                 const double* __restrict__ recon, double* __restrict__ u_all) {
+  const int smp = blockIdx.y;
+  const SampleDesc d = sd[smp];
+  for (int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; m < M; m += (int64_t)gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    for (int64_t b = 0; b < B; ++b) {
+      const int64_t s = idx[d.map_off + m * B + b];
+      if (s == -1) break;
+      if (s >= 0 && s < d.S) acc = fma(wts[d.map_off + m * B + b], recon[d.s_off + s], acc);
+    }
+    u_all[(int64_t)smp * M + m] = acc;
+  }
+}
+
+// v_b = W u_b for all samples; one workgroup per row of W (read once for the batch).
+__global__ void __launch_bounds__(256) batched_gemv_kernel(const double* __restrict__ W, int64_t M, int nsmp,
+                                                           const double* __restrict__ u_all,
+                                                           double* __restrict__ v_all) {
+  __shared__ double red[4];
+  const int64_t m = blockIdx.x;
+  const double* w = W + m * M;
+  for (int b = 0; b < nsmp; ++b) {
+    const double* u = u_all + (int64_t)b * M;
+    double acc = 0.0;
+    for (int64_t j = threadIdx.x; j < M; j += 256) acc = fma(w[j], u[j], acc);
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) v_all[(int64_t)b * M + m] = (red[0] + red[1]) + (red[2] + red[3]);
+    __syncthreads();
+  }
+}
+
+// dT[m,b] = s_s (y_m - v_m) - sum_j Omega[m, j] Ainv[s, j]; one wave per image pixel.
+__global__ void __launch_bounds__(64) weight_grad_kernel(const SampleDesc* __restrict__ sd,
+                                                         const int64_t* __restrict__ idx, int64_t M, int64_t B,
+                                                         const double* __restrict__ dirty,
+                                                         const double* __restrict__ recon,
+                                                         const double* __restrict__ v_all,
+                                                         const double* __restrict__ omega,
+                                                         const double* __restrict__ inv_all,
+                                                         double* __restrict__ dw) {
+  const int smp = blockIdx.y;
+  const SampleDesc d = sd[smp];
+  const int64_t m = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int64_t n = d.S;
+  const double* om = omega + d.omega_off + m * n;
+  const double* Ai = inv_all + d.mat_off / 2;
+  bool live = true;
+  for (int64_t b = 0; b < B; ++b) {
+    const int64_t s = idx[d.map_off + m * B + b];
+    if (s == -1) live = false;
+    double g = 0.0;
+    if (live && s >= 0 && s < n) {
+      double acc = 0.0;
+      for (int64_t j = lane; j < n; j += 64) acc = fma(om[j], Ai[s * n + j], acc);
+      acc = wave_sum(acc);
+      g = recon[d.s_off + s] * (dirty[m] - v_all[(int64_t)smp * M + m]) - acc;
+    }
+    if (lane == 0) dw[d.map_off + m * B + b] = g;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Barycentric weights of fixed points p_m in triangles of moving vertices,
+// and the vertex gradient of a loss through them.
+// ---------------------------------------------------------------------------
+__device__ inline double area2(double y0, double x0, double y1, double x1, double y2, double x2) {
+  return (x1 - x0) * (y2 - y0) - (x2 - x0) * (y1 - y0);
+}
+
+__global__ void barycentric_kernel(const double* __restrict__ V, const double* __restrict__ Ppos,
+                                   const int64_t* __restrict__ tri, int64_t M, double* __restrict__ t) {
+  for (int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; m < M; m += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t a = tri[3 * m], b = tri[3 * m + 1], c = tri[3 * m + 2];
+    const double py = Ppos[2 * m], px = Ppos[2 * m + 1];
+    const double ya = V[2 * a], xa = V[2 * a + 1], yb = V[2 * b], xb = V[2 * b + 1], yc = V[2 * c], xc = V[2 * c + 1];
+    const double A = area2(ya, xa, yb, xb, yc, xc);
+    t[3 * m] = area2(py, px, yb, xb, yc, xc) / A;
+    t[3 * m + 1] = area2(ya, xa, py, px, yc, xc) / A;
+    t[3 * m + 2] = area2(ya, xa, yb, xb, py, px) / A;
+  }
+}
+
+// contrib[m, k, 2] = sum_i dt_i d t_i / d V_{tri[m,k]} for the three corners k.
+__global__ void barycentric_vjp_kernel(const double* __restrict__ V, const double* __restrict__ Ppos,
+                                       const int64_t* __restrict__ tri, const double* __restrict__ dt, int64_t M,
+                                       double* __restrict__ contrib) {
+  for (int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; m < M; m += (int64_t)gridDim.x * blockDim.x) {
+    double y[3], x[3];
+    for (int k = 0; k < 3; ++k) {
+      const int64_t v = tri[3 * m + k];
+      y[k] = V[2 * v];
+      x[k] = V[2 * v + 1];
+    }
+    const double py = Ppos[2 * m], px = Ppos[2 * m + 1];
+    const double A = area2(y[0], x[0], y[1], x[1], y[2], x[2]);
+    double Ai[3], t[3];
+    Ai[0] = area2(py, px, y[1], x[1], y[2], x[2]);
+    Ai[1] = area2(y[0], x[0], py, px, y[2], x[2]);
+    Ai[2] = area2(y[0], x[0], y[1], x[1], py, px);
+    double st = 0.0;
+    for (int k = 0; k < 3; ++k) {
+      t[k] = Ai[k] / A;
+      st += dt[3 * m + k] * t[k];
+    }
+    // d area2(P0,P1,P2)/d(y_k, x_k): k=0: (x2-x1, y1-y2), k=1: (x0-x2, y2-y0), k=2: (x1-x0, y0-y1)
+    for (int k = 0; k < 3; ++k) {
+      const int k1 = (k + 1) % 3, k2 = (k + 2) % 3;
+      // full triangle
+      const double dAy = x[k2] - x[k1], dAx = y[k1] - y[k2];
+      // sub-triangles i != k contain corner k with p replacing corner i
+      double gy = 0.0, gx = 0.0;
+      for (int i = 0; i < 3; ++i) {
+        if (i == k) continue;
+        double yy[3] = {y[0], y[1], y[2]}, xx[3] = {x[0], x[1], x[2]};
+        yy[i] = py;
+        xx[i] = px;
+        gy += dt[3 * m + i] * (xx[k2] - xx[k1]);
+        gx += dt[3 * m + i] * (yy[k1] - yy[k2]);
+      }
+      contrib[6 * m + 2 * k] = (gy - st * dAy) / A;
+      contrib[6 * m + 2 * k + 1] = (gx - st * dAx) / A;
+    }
+  }
+}
+
+// dV[s] = sum over the bucket of s (entries (m, b) with tri[m, b] = s) of contrib[m, b].
+__global__ void vertex_reduce_kernel(const int* __restrict__ ptrs, const int* __restrict__ ekey,
+                                     const double* __restrict__ contrib, int64_t S, double* __restrict__ dV) {
+  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < S; s += (int64_t)gridDim.x * blockDim.x) {
+    double gy = 0.0, gx = 0.0;
+    for (int e = ptrs[s]; e < ptrs[s + 1]; ++e) {
+      const int key = ekey[e];  // m*3 + b before conversion; here raw keys are kept
+      gy += contrib[2 * key];
+      gx += contrib[2 * key + 1];
+    }
+    dV[2 * s] = gy;
+    dV[2 * s + 1] = gx;
+  }
+}
+
+__global__ void csc_keys_kernel(const int64_t* __restrict__ tri, int64_t M, int64_t S, int* __restrict__ cnt) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < 3 * M; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = tri[e];
+    if (s >= 0 && s < S) atomicAdd(&cnt[s + 1], 1);
+  }
+}
+
+__global__ void csc_keys_fill_kernel(const int64_t* __restrict__ tri, int64_t M, int64_t S, int* __restrict__ cur,
+                                     int* __restrict__ ekey) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < 3 * M; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = tri[e];
+    if (s >= 0 && s < S) ekey[atomicAdd(&cur[s], 1)] = (int)e;
+  }
+}
+
+__global__ void csc_keys_sort_kernel(const int* __restrict__ ptrs, int64_t S, int* __restrict__ ekey) {
+  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < S; s += (int64_t)gridDim.x * blockDim.x) {
+    for (int i = ptrs[s] + 1; i < ptrs[s + 1]; ++i) {
+      int kk = ekey[i], j = i - 1;
+      while (j >= ptrs[s] && ekey[j] > kk) {
+        ekey[j + 1] = ekey[j];
+        --j;
+      }
+      ekey[j + 1] = kk;
+    }
+  }
+}
+
+__global__ void scan_small_kernel(int* __restrict__ p, int64_t S, int* __restrict__ cur) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    for (int64_t i = 1; i <= S; ++i) p[i] += p[i - 1];
+    for (int64_t i = 0; i <= S; ++i) cur[i] = p[i];
+  }
+}
+
+}  // namespace lk
+}  // namespace aj
+
+extern "C" size_t aj_log_evidence_backward_workspace_bytes(int64_t M, int64_t n_samples, const int64_t* S_host) {
+  if (M <= 0 || n_samples <= 0 || !S_host) return 0;
+  int64_t sumS = 0, sumS2 = 0;
+  for (int64_t b = 0; b < n_samples; ++b) {
+    sumS += S_host[b];
+    sumS2 += S_host[b] * S_host[b];
+  }
+  return align_up(sizeof(double) * sumS2, 256) + align_up(sizeof(double) * sumS, 256) +
+         2 * align_up(sizeof(double) * M * n_samples, 256) + 1024;
+}
+
+extern "C" int aj_log_evidence_batched_backward(const double* w_tilde, int64_t M, const double* dirty_image,
+                                                int64_t n_samples, int64_t B, int64_t P, const int64_t* S_host,
+                                                const int64_t* pix_indexes, const double* pix_weights,
+                                                const int64_t* neighbors, const int64_t* neighbors_sizes,
+                                                const double* coefficients_dev, const double* reconstruction,
+                                                const void* forward_workspace, size_t forward_workspace_bytes,
+                                                double* dcoefficients, double* dweights, void* workspace,
+                                                size_t workspace_bytes, void* stream) {
+  AJ_REQUIRE(M > 0 && B > 0 && P > 0 && n_samples > 0 && S_host, "bad sizes");
+  AJ_REQUIRE(w_tilde && dirty_image && pix_indexes && pix_weights && neighbors && neighbors_sizes &&
+                 coefficients_dev && reconstruction && forward_workspace && dcoefficients && dweights,
+             "null pointer");
+  BatchPlan p = plan_batch(M, B, n_samples, S_host);
+  AJ_REQUIRE(forward_workspace_bytes >= p.bytes, "forward workspace does not match this batch");
+  const size_t need = aj_log_evidence_backward_workspace_bytes(M, n_samples, S_host);
+  AJ_REQUIRE(workspace && workspace_bytes >= need, "workspace too small: need %zu bytes, got %zu", need,
+             workspace_bytes);
+  hipStream_t st = as_stream(stream);
+  const char* fw = static_cast<const char*>(forward_workspace);
+  const SampleDesc* desc = reinterpret_cast<const SampleDesc*>(fw + p.off_desc);
+  const double* omega = reinterpret_cast<const double*>(fw + p.off_omega);
+  const double* mats = reinterpret_cast<const double*>(fw + p.off_mats);
+  Arena ar(workspace, workspace_bytes);
+  double* inv = ar.take<double>(p.sumS2);
+  double* hn2 = ar.take<double>(p.sumS);
+  double* u = ar.take<double>(M * n_samples);
+  double* v = ar.take<double>(M * n_samples);
+  double* an2 = v;  // column norms of the A factor are not needed; reuse as scratch before v is written
+  int64_t maxS = 0;
+  for (int64_t b = 0; b < n_samples; ++b) maxS = std::max(maxS, S_host[b]);
+  // inverse of A (mat_off counts both A and H blocks, so inv uses mat_off/2 per sample)
+  hipLaunchKernelGGL(factor_inverse_kernel, dim3((unsigned)maxS, (unsigned)n_samples), dim3(64),
+                     sizeof(double) * maxS, st, desc, mats, 0, inv, an2);
+  hipLaunchKernelGGL(factor_inverse_kernel, dim3((unsigned)maxS, (unsigned)n_samples), dim3(64),
+                     sizeof(double) * maxS, st, desc, mats, 1, nullptr, hn2);
+  hipLaunchKernelGGL(coef_grad_kernel, dim3((unsigned)n_samples), dim3(256), 0, st, desc, neighbors, neighbors_sizes,
+                     P, reconstruction, inv, hn2, coefficients_dev, dcoefficients);
+  const unsigned gm = (unsigned)std::min<int64_t>(ceil_div(M, 256), 1024);
+  hipLaunchKernelGGL(map_recon_kernel, dim3(gm, (unsigned)n_samples), dim3(256), 0, st, desc, pix_indexes,
+                     pix_weights, M, B, reconstruction, u);
+  hipLaunchKernelGGL(batched_gemv_kernel, dim3((unsigned)M), dim3(256), 0, st, w_tilde, M, (int)n_samples, u, v);
+  hipLaunchKernelGGL(weight_grad_kernel, dim3((unsigned)M, (unsigned)n_samples), dim3(64), 0, st, desc, pix_indexes,
+                     M, B, dirty_image, reconstruction, v, omega, inv, dweights);
+  AJ_LAUNCH_CHECK();
+  return AJ_OK;
+}
+
+extern "C" int aj_barycentric_weights(const double* vertices, const double* positions, const int64_t* triangles,
+                                      int64_t M, double* weights, void* stream) {
+  AJ_REQUIRE(M > 0 && vertices && positions && triangles && weights, "bad arguments");
+  hipLaunchKernelGGL(barycentric_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(M, 256), 1024)), dim3(256), 0,
+                     as_stream(stream), vertices, positions, triangles, M, weights);
+  AJ_LAUNCH_CHECK();
+  return AJ_OK;
+}
+
+extern "C" size_t aj_barycentric_vjp_workspace_bytes(int64_t M, int64_t S) {
+  return (M > 0 && S > 0) ? sizeof(double) * 6 * M + sizeof(int) * (2 * (S + 1) + 3 * M) + 1024 : 0;
+}
+
+extern "C" int aj_barycentric_vjp(const double* vertices, const double* positions, const int64_t* triangles,
+                                  int64_t M, int64_t S, const double* dweights, double* dvertices, void* workspace,
+                                  size_t workspace_bytes, void* stream) {
+  AJ_REQUIRE(M > 0 && S > 0 && vertices && positions && triangles && dweights && dvertices, "bad arguments");
+  AJ_REQUIRE(workspace && workspace_bytes >= aj_barycentric_vjp_workspace_bytes(M, S), "workspace too small");
+  hipStream_t st = as_stream(stream);
+  Arena ar(workspace, workspace_bytes);
+  double* contrib = ar.take<double>(6 * M);
+  int* ptrs = ar.take<int>(S + 1);
+  int* cur = ar.take<int>(S + 1);
+  int* ekey = ar.take<int>(3 * M);
+  AJ_CHECK_HIP(hipMemsetAsync(ptrs, 0, sizeof(int) * (S + 1), st));
+  const unsigned g = (unsigned)std::min<int64_t>(ceil_div(3 * M, 256), 1024);
+  hipLaunchKernelGGL(barycentric_vjp_kernel, dim3(g), dim3(256), 0, st, vertices, positions, triangles, dweights, M,
+                     contrib);
+  hipLaunchKernelGGL(csc_keys_kernel, dim3(g), dim3(256), 0, st, triangles, M, S, ptrs);
+  hipLaunchKernelGGL(scan_small_kernel, dim3(1), dim3(64), 0, st, ptrs, S, cur);
+  hipLaunchKernelGGL(csc_keys_fill_kernel, dim3(g), dim3(256), 0, st, triangles, M, S, cur, ekey);
+  hipLaunchKernelGGL(csc_keys_sort_kernel, dim3((unsigned)ceil_div(S, 256)), dim3(256), 0, st, ptrs, S, ekey);
+  hipLaunchKernelGGL(vertex_reduce_kernel, dim3((unsigned)ceil_div(S, 256)), dim3(256), 0, st, ptrs, ekey, contrib, S,
+                     dvertices);
+  AJ_LAUNCH_CHECK();
+  return AJ_OK;
+}
