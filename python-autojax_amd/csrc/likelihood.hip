@@ -22,6 +22,8 @@ namespace aj {
 namespace lk {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+// v_mfma_f64_16x16x4: A lane l holds A[l%16][l/16], B lane l holds B[l/16][l%16], and the
+// accumulator register r of lane l holds D[4r + l/16][l%16].
 
 // Per-sample table, device resident.
 struct SampleDesc {
@@ -357,7 +359,7 @@ __device__ void chol_one(double* __restrict__ A, int64_t n, double* logdet_out, 
           for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-              int64_t gi = R0 + wr + a * 16 + lk * 4 + r, gj = C0 + wc + b * 16 + lr;
+              int64_t gi = R0 + wr + a * 16 + lk + 4 * r, gj = C0 + wc + b * 16 + lr;
               acc[a][b][r] = (gi < n && gj < n && gj <= gi) ? A[gi * n + gj] : 0.0;
             }
 #pragma unroll
@@ -379,7 +381,7 @@ __device__ void chol_one(double* __restrict__ A, int64_t n, double* logdet_out, 
           for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-              int64_t gi = R0 + wr + a * 16 + lk * 4 + r, gj = C0 + wc + b * 16 + lr;
+              int64_t gi = R0 + wr + a * 16 + lk + 4 * r, gj = C0 + wc + b * 16 + lr;
               if (gi < n && gj < n && gj <= gi) A[gi * n + gj] = acc[a][b][r];
             }
         __syncthreads();
@@ -791,7 +793,7 @@ __global__ void __launch_bounds__(256) gemm_f64_kernel(const double* __restrict_
     for (int b = 0; b < 2; ++b)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int64_t gi = i0 + wr + a * 16 + lk * 4 + r, gj = j0 + wc + b * 16 + lr;
+        const int64_t gi = i0 + wr + a * 16 + lk + 4 * r, gj = j0 + wc + b * 16 + lr;
         if (gi < Mr && gj < N) C[gi * ldc + gj] = acc[a][b][r];
       }
 }

@@ -175,6 +175,23 @@ __global__ void __launch_bounds__(256) features_kernel(
       }
       X[i * ldx + col_c] = cv;
       X[i * ldx + col_s] = sv;
+    } else if constexpr (MODE == AJ_PREC_INT8X3 || MODE == AJ_PREC_INT8X4) {
+      // FP64 cos/sin, quantised to NS balanced 8-bit digits of a fixed-point value
+      if (live) {
+        double rv = reduced_revs(grid[2 * i + 1] - cx, grid[2 * i] - cy, u0, u1);
+        sincospi(2.0 * rv, &sv, &cv);
+      }
+      long long qc = live ? llrint(cv * qscale) : 0;
+      long long qs = live ? llrint(sv * qscale) : 0;
+#pragma unroll
+      for (int p = NS - 1; p >= 0; --p) {
+        int8_t dc = (int8_t)(qc & 0xFF);
+        int8_t ds = (int8_t)(qs & 0xFF);
+        qc = (qc - dc) >> 8;
+        qs = (qs - ds) >> 8;
+        X[p * sstride + i * ldx + col_c] = dc;
+        X[p * sstride + i * ldx + col_s] = ds;
+      }
     } else {
       if (live) {
         double rv = reduced_revs(grid[2 * i + 1] - cx, grid[2 * i] - cy, u0, u1);
@@ -191,19 +208,6 @@ __global__ void __launch_bounds__(256) features_kernel(
         X[i * ldx + col_s] = __builtin_bit_cast(uint16_t, bh);
         X[sstride + i * ldx + col_c] = __builtin_bit_cast(uint16_t, al);
         X[sstride + i * ldx + col_s] = __builtin_bit_cast(uint16_t, bl);
-      } else {
-        long long qc = live ? llrint((double)cf * qscale) : 0;
-        long long qs = live ? llrint((double)sf * qscale) : 0;
-        // Balanced base-256 digits, least significant last: q = sum_p a_p 256^(NS-1-p).
-#pragma unroll
-        for (int p = NS - 1; p >= 0; --p) {
-          int8_t dc = (int8_t)(qc & 0xFF);
-          int8_t ds = (int8_t)(qs & 0xFF);
-          qc = (qc - dc) >> 8;
-          qs = (qs - ds) >> 8;
-          X[p * sstride + i * ldx + col_c] = dc;
-          X[p * sstride + i * ldx + col_s] = ds;
-        }
       }
     }
   }
@@ -387,13 +391,160 @@ __global__ void __launch_bounds__(256) syrk_kernel(const void* __restrict__ Xv, 
     for (int b = 0; b < 2; ++b)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int64_t row = I0 + wr + a * 16 + lk * 4 + r;
+        // v_mfma_f64_16x16x4 holds row 4r + lane/16; the f32/bf16/i8 forms hold row 4*(lane/16) + r
+        const int64_t row = I0 + wr + a * 16 + (MODE == AJ_PREC_FP64 ? lk + 4 * r : lk * 4 + r);
         const int64_t col = J0 + wc + b * 16 + lr;
         if (row < M && col < M) {
           double* p = W + row * M + col;
           *p = first ? accd[a][b][r] : *p + accd[a][b][r];
         }
       }
+}
+
+// ---------------------------------------------------------------------------
+// Integer-digit SYRK: W += sum_c scale_c^2/Q^2 * sum_{p,q} 256^(2NS-2-p-q) A_p A_q^T.
+// Workgroup tile 64 rows x 128 columns (4 waves as 2x2, each 32x64 = 2x4
+// MFMA 16x16 blocks); one int32 accumulator set per digit order p+q, exact
+// within a 4096-visibility chunk, flushed in FP64 straight into W (zeroed by
+// the caller).  LDS double-buffered by 64-feature stages with a register
+// prefetch of the next stage.  Tiles: row blocks I of 64, column blocks
+// J <= I/2 of 128 (the lower triangle), placed so that consecutive tiles,
+// which share row panels, land on the same XCD.
+// ---------------------------------------------------------------------------
+constexpr int I8_TM = 64, I8_TN = 128, I8_KS = 64, I8_PITCH = 80;
+
+__device__ inline void tile_i8(int64_t t, int64_t& I, int64_t& J) {
+  // rows 2q and 2q+1 each hold q+1 tiles; rows < 2q hold q(q+1) tiles in total
+  int64_t q = (int64_t)((sqrt(4.0 * (double)t + 1.0) - 1.0) * 0.5);
+  while (q * (q + 1) > t) --q;
+  while ((q + 1) * (q + 2) <= t) ++q;
+  const int64_t r = t - q * (q + 1);
+  I = 2 * q + (r >= q + 1 ? 1 : 0);
+  J = r % (q + 1);
+}
+
+inline int64_t i8_tiles(int64_t Mpad) {
+  const int64_t nI = Mpad / I8_TM;  // even (Mpad is a multiple of 128)
+  return (nI / 2) * (nI / 2 + 1);
+}
+
+template <int NS>
+__global__ void __launch_bounds__(256, 1) syrk_i8_kernel(const int8_t* __restrict__ X, int64_t M, int64_t Mpad,
+                                                         int64_t ldx, int nchunks, const double* __restrict__ chunk_sc,
+                                                         double* __restrict__ W, int64_t ntiles) {
+  constexpr int NG = 2 * NS - 1;
+  constexpr int ROWS = I8_TM + I8_TN;
+  __shared__ __attribute__((aligned(16))) int8_t lds[2][NS][ROWS * I8_PITCH];
+
+  // XCD-aware placement: block b runs on XCD b % 8; give each XCD a contiguous run of tiles.
+  const int64_t per = (ntiles + 7) / 8;
+  const int64_t t = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (t >= ntiles) return;
+  int64_t TI, TJ;
+  tile_i8(t, TI, TJ);
+  const int64_t I0 = TI * I8_TM, J0 = TJ * I8_TN;
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 64;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int64_t sstride = Mpad * ldx;
+
+  // global -> register prefetch: thread loads 16 B of row (tid/4) [A], rows (tid/4) and 64+(tid/4) [B]
+  const int grow = tid >> 2, gseg = (tid & 3) * 16;
+  const int8_t* srcA = X + (I0 + grow) * ldx + gseg;
+  const int8_t* srcB0 = X + (J0 + grow) * ldx + gseg;
+  const int8_t* srcB1 = X + (J0 + 64 + grow) * ldx + gseg;
+  // the next stage is prefetched in two halves of the digit slices, each overlapped with half of
+  // the MFMA work of the current stage, so only half of the slices sit in registers at a time
+  constexpr int H0 = (NS + 1) / 2;
+  int4 pf[H0][3];
+  auto gload = [&](int64_t k0, int s0, int s1) {
+#pragma unroll
+    for (int s = s0; s < s1; ++s) {
+      pf[s - s0][0] = *reinterpret_cast<const int4*>(srcA + s * sstride + k0);
+      pf[s - s0][1] = *reinterpret_cast<const int4*>(srcB0 + s * sstride + k0);
+      pf[s - s0][2] = *reinterpret_cast<const int4*>(srcB1 + s * sstride + k0);
+    }
+  };
+  auto lstore = [&](int buf, int s0, int s1) {
+#pragma unroll
+    for (int s = s0; s < s1; ++s) {
+      *reinterpret_cast<int4*>(&lds[buf][s][grow * I8_PITCH + gseg]) = pf[s - s0][0];
+      *reinterpret_cast<int4*>(&lds[buf][s][(I8_TM + grow) * I8_PITCH + gseg]) = pf[s - s0][1];
+      *reinterpret_cast<int4*>(&lds[buf][s][(I8_TM + 64 + grow) * I8_PITCH + gseg]) = pf[s - s0][2];
+    }
+  };
+
+  i4 acc[NG][2][4];
+  const int64_t nstage = (int64_t)nchunks * (FC / I8_KS);
+  constexpr int SPC = FC / I8_KS;  // stages per chunk
+  gload(0, 0, H0);
+  lstore(0, 0, H0);
+  gload(0, H0, NS);
+  lstore(0, H0, NS);
+  __syncthreads();
+  for (int64_t st = 0; st < nstage; ++st) {
+    if (st % SPC == 0) {
+#pragma unroll
+      for (int g = 0; g < NG; ++g)
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 4; ++b) acc[g][a][b] = i4{0, 0, 0, 0};
+    }
+    const int buf = (int)(st & 1);
+    const bool more = st + 1 < nstage;
+    i4 fa[NS][2];
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+        fa[s][a] = *reinterpret_cast<const i4*>(&lds[buf][s][(wr + a * 16 + lr) * I8_PITCH + lk * 16]);
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+      if (more) gload((st + 1) * I8_KS, half ? H0 : 0, half ? NS : H0);
+#pragma unroll
+      for (int b = 2 * half; b < 2 * half + 2; ++b) {
+        i4 fb[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+          fb[s] = *reinterpret_cast<const i4*>(&lds[buf][s][(I8_TM + wc + b * 16 + lr) * I8_PITCH + lk * 16]);
+#pragma unroll
+        for (int p = 0; p < NS; ++p)
+#pragma unroll
+          for (int q = 0; q < NS; ++q)
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+              acc[p + q][a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[p][a], fb[q], acc[p + q][a][b], 0, 0, 0);
+      }
+      if (more) lstore(buf ^ 1, half ? H0 : 0, half ? NS : H0);
+    }
+    if (st % SPC == SPC - 1) {
+      const double q = qmax_of(NS);
+      const int c = (int)(st / SPC);
+      const double s2 = chunk_sc[c] * chunk_sc[c] / (q * q);
+      // an opaque copy of M keeps the 32 output addresses from being hoisted out of the
+      // stage loop (where they would occupy registers the prefetch needs)
+      int64_t Mo = M;
+      asm volatile("" : "+s"(Mo));
+      double* Wt = W + (I0 + wr + lk * 4) * Mo + (J0 + wc + lr);
+      const int64_t rlim = M - (I0 + wr + lk * 4), clim = M - (J0 + wc + lr);
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            double v = 0.0;
+#pragma unroll
+            for (int g = 0; g < NG; ++g) v = v * 256.0 + (double)acc[g][a][b][r];
+            if (a * 16 + r < rlim && b * 16 < clim) Wt[(a * 16 + r) * Mo + b * 16] += v * s2;
+            // one output at a time: keeps the scheduler from reading all accumulators at once
+            __builtin_amdgcn_sched_barrier(0);
+          }
+    }
+    __syncthreads();
+  }
 }
 
 // Copy the lower triangle to the upper one (32x32 tiles through LDS).
@@ -425,7 +576,8 @@ inline Plan make_plan(int64_t K, int64_t M, int mode) {
   p.mode = mode;
   p.ns = slices_of(mode);
   p.es = esize_of(mode);
-  p.Mpad = ceil_div(M, TILE) * TILE;
+  const int64_t pad = (mode == AJ_PREC_INT8X3 || mode == AJ_PREC_INT8X4) ? 2 * TILE : TILE;
+  p.Mpad = ceil_div(M, pad) * pad;
   const int64_t nch = std::max<int64_t>(1, ceil_div(K, KC));
   const double per_chunk = (double)p.ns * p.Mpad * FC * p.es;
   int64_t g = (int64_t)((1ll << 30) / per_chunk);
@@ -453,6 +605,7 @@ int run(const double* noise, const double* uv, const double* grid, int64_t K, in
   const int64_t nt = p.Mpad / TILE;
   const int64_t ntiles = nt * (nt + 1) / 2;
   int first = 1;
+  if (MODE == AJ_PREC_INT8X3 || MODE == AJ_PREC_INT8X4) AJ_CHECK_HIP(hipMemsetAsync(W, 0, sizeof(double) * M * M, st));
   for (int64_t c0 = 0; c0 < nch; c0 += p.group) {
     const int ng = (int)std::min<int64_t>(p.group, nch - c0);
     const int64_t k_begin = c0 * KC;
@@ -467,8 +620,15 @@ int run(const double* noise, const double* uv, const double* grid, int64_t K, in
     timer_end(AJ_TIMER_WT_FEATURES, st);
     AJ_LAUNCH_CHECK();
     timer_begin(AJ_TIMER_WT_SYRK, st);
-    hipLaunchKernelGGL(syrk_kernel<MODE>, dim3((unsigned)ntiles), dim3(256), 0, st, X, M, p.Mpad, p.ldx, ng, sc, W,
-                       first);
+    if constexpr (MODE == AJ_PREC_INT8X3 || MODE == AJ_PREC_INT8X4) {
+      constexpr int NSL = Mode<MODE>::NS;
+      const int64_t nt8 = i8_tiles(p.Mpad);
+      hipLaunchKernelGGL(syrk_i8_kernel<NSL>, dim3((unsigned)(8 * ((nt8 + 7) / 8))), dim3(256), 0, st,
+                         reinterpret_cast<const int8_t*>(X), M, p.Mpad, p.ldx, ng, sc, W, nt8);
+    } else {
+      hipLaunchKernelGGL(syrk_kernel<MODE>, dim3((unsigned)ntiles), dim3(256), 0, st, X, M, p.Mpad, p.ldx, ng, sc, W,
+                         first);
+    }
     timer_end(AJ_TIMER_WT_SYRK, st);
     AJ_LAUNCH_CHECK();
     first = 0;

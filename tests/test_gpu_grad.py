@@ -71,6 +71,9 @@ def test_grad_matches_central_fd(gpu, prob):
         fd = (_gpu_eval(d, W, cp)[0][b].item() - _gpu_eval(d, W, cm)[0][b].item()) / (2 * h * coefs[b])
         print(f"sample {b}: dc {c.grad[b].item():.10e} fd {fd:.10e}", flush=True)
         assert abs(c.grad[b].item() - fd) < 1e-5 * abs(fd)
+    # weights use h = 1e-5: at 1e-6 the FP64 rounding of logL (~1e-13 relative, |logL| ~ 2e3) would
+    # put ~1e-4 relative noise into the difference quotient
+    h = 1e-5
     w0 = np.stack([s.pix_weights_for_sub_slim_index for s in d.samples])
     rng = np.random.default_rng(0)
     for _ in range(4):

@@ -13,7 +13,7 @@ pytestmark = pytest.mark.gpu
 GOLD = Path(__file__).resolve().parent / "golden"
 
 # normwise tolerances of W-tilde against the FP64 oracle, relative to max|W| (= the diagonal)
-WT_TOL = {"fp64": 1e-12, "int8x4": 1e-9, "int8x3": 3e-7, "fp32": 3e-6, "bf16x3": 3e-4}
+WT_TOL = {"fp64": 1e-12, "int8x4": 5e-8, "int8x3": 3e-7, "fp32": 3e-6, "bf16x3": 3e-4}
 
 
 @pytest.fixture(scope="module")
@@ -127,7 +127,9 @@ def test_regularization_solve_noise(gpu, delaunay_small):
     d, W = delaunay_small
     smp = d.samples[1]
     H = api.constant_regularization_matrix_from(1.7, smp.neighbors, smp.neighbors_sizes)
-    np.testing.assert_array_equal(H, R.constant_regularization_matrix_from(1.7, smp.neighbors, smp.neighbors_sizes))
+    # equal up to the rounding of a fused multiply-add in 1e-8 + c^2 * size
+    np.testing.assert_allclose(H, R.constant_regularization_matrix_from(1.7, smp.neighbors, smp.neighbors_sizes),
+                               rtol=2e-16, atol=0)
     To = R.mapping_matrix_from(smp.pix_indexes_for_sub_slim_index, smp.pix_weights_for_sub_slim_index, smp.pixels)
     A = R.curvature_matrix_via_w_tilde_from(W, To) + H
     Dv = R.data_vector_from(To, d.dirty_image)
