@@ -1,0 +1,92 @@
+"""Single-kernel drivers for timing and hardware-counter runs (rocprofv3).
+
+    python -m autojax_amd.kernel_bench wtilde --M 5000 --K 131072 --precision int8x4 --reps 3
+    python -m autojax_amd.kernel_bench batch --M 5000 --samples 64 --K 8192
+    python -m autojax_amd.kernel_bench copy --mib 1024
+
+``wtilde`` builds W-tilde on synthetic data of the batching workload's
+generator; ``batch`` evaluates the batched log-evidence of a sample batch on
+one W-tilde; ``copy`` runs a plain device copy of known size, the reference
+for scaling the FETCH_SIZE / WRITE_SIZE counters.  Each prints one JSON line
+with the per-call time measured with HIP events.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+
+import numpy as np
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("what", choices=["wtilde", "batch", "copy"])
+    ap.add_argument("--M", type=int, default=5000)
+    ap.add_argument("--K", type=int, default=131072)
+    ap.add_argument("--precision", default="int8x4")
+    ap.add_argument("--samples", type=int, default=64)
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--mib", type=int, default=1024)
+    a = ap.parse_args(argv)
+
+    import torch
+
+    from . import _lib
+    from . import datasets as dsm
+    from . import device as dv
+    from . import evidence as E
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        for _ in range(reps):
+            fn()
+        e.record()
+        torch.cuda.synchronize()
+        return s.elapsed_time(e) / reps
+
+    if a.what == "copy":
+        n = a.mib * (1 << 20) // 8
+        x = torch.empty(n, dtype=torch.float64, device="cuda").uniform_()
+        y = torch.empty_like(x)
+        ms = timed(lambda: y.copy_(x), a.reps)
+        print(json.dumps({"kernel": "copy", "bytes_read": n * 8, "bytes_written": n * 8, "ms": ms,
+                          "GBps": 2 * n * 8 / ms / 1e6}), flush=True)
+        return
+    rng = np.random.default_rng(5)
+    grid, _ = dsm.grid_for_pixel_count(a.M, 0.05)
+    uv = dsm.uv_loguniform(rng, a.K, 1e3, 5e5)
+    data, noise = dsm.visibilities(rng, a.K, 0.5, 2.0)
+    n_d = dv.to_device(np.ascontiguousarray(noise.real))
+    uv_d = dv.to_device(uv)
+    g_d = dv.to_device(grid)
+    M, K = grid.shape[0], a.K
+    W = dv.empty((M, M))
+    if a.what == "wtilde":
+        ws = dv.workspace(_lib.query("aj_w_tilde_workspace_bytes", K, M, _lib.PRECISIONS[a.precision]))
+        ms = timed(lambda: E.w_tilde(n_d, uv_d, g_d, a.precision, out=W, workspace=ws), a.reps)
+        print(json.dumps({"kernel": "w_tilde", "precision": a.precision, "M": M, "K": K, "ms": ms,
+                          "algorithmic_tflops": 2.0 * M * M * K / ms / 1e9}), flush=True)
+        return
+    E.w_tilde(n_d, uv_d, g_d, "fp64", out=W)
+    samples = []
+    for _ in range(a.samples):
+        S = int(rng.integers(300, 1501))
+        mesh = dsm.delaunay_source(rng, S)
+        idx, w = dsm.barycentric_mapping(rng, M, mesh.triangles)
+        lam = float(np.exp(rng.uniform(np.log(0.01), np.log(100.0))))
+        samples.append(dsm.Sample(idx, w, mesh.neighbors, mesh.neighbors_sizes, lam, mesh))
+    batch = E.SampleBatch(samples, M)
+    y = E.dirty_image(data, noise, uv, grid)
+    c2 = E.chi2_noise_norm(data, noise)
+    ms = timed(lambda: batch.log_evidence(W, y, c2), a.reps)
+    print(json.dumps({"kernel": "log_evidence_batched", "M": M, "samples": a.samples,
+                      "sum_S": int(batch.S_host.sum()), "ms": ms}), flush=True)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

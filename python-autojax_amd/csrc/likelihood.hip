@@ -277,8 +277,6 @@ constexpr int NB = 32;
 
 __device__ void chol_one(double* __restrict__ A, int64_t n, double* logdet_out, int64_t* info_out) {
   __shared__ double Lkk[NB][NB + 1];
-  __shared__ double Pi[64][NB + 1];
-  __shared__ double Pj[64][NB + 1];
   __shared__ double Xs[256][NB + 1];
   __shared__ int fail;
   __shared__ double ld_acc;
@@ -338,20 +336,20 @@ __device__ void chol_one(double* __restrict__ A, int64_t n, double* logdet_out, 
       }
     }
     __syncthreads();
-    // (c) trailing update A[r0:, r0:] -= P P^T, lower 64x64 tiles, MFMA f64
-    const int64_t nt = (n - r0 + 63) / 64;
-    for (int64_t ti = 0; ti < nt; ++ti) {
-      for (int64_t tj = 0; tj <= ti; ++tj) {
-        const int64_t R0 = r0 + ti * 64, C0 = r0 + tj * 64;
-        for (int e = tid; e < 64 * NB; e += 256) {
-          int r = e / NB, p = e % NB;
-          int64_t gi = R0 + r, gj = C0 + r;
-          Pi[r][p] = (gi < n && p < nb) ? A[gi * n + k0 + p] : 0.0;
-          Pj[r][p] = (gj < n && p < nb) ? A[gj * n + k0 + p] : 0.0;
-        }
-        __syncthreads();
-        const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-        const int lr = lane & 15, lk = lane >> 4;
+    // (c) trailing update A[r0:, r0:] -= P P^T on the lower triangle, P = A[r0:, k0:k0+nb].
+    // Each wave owns whole 32x32 blocks (2x2 MFMA 16x16 blocks) and reads the panel rows
+    // straight from global memory, so the four waves work on different blocks without barriers.
+    {
+      const int64_t m = n - r0;
+      const int64_t nbk = (m + 31) / 32;
+      const int64_t nblk = nbk * (nbk + 1) / 2;
+      const int lr = lane & 15, lk = lane >> 4;
+      for (int64_t blk = wave; blk < nblk; blk += 4) {
+        int64_t bi = (int64_t)((sqrt(8.0 * (double)blk + 1.0) - 1.0) * 0.5);
+        while (bi * (bi + 1) / 2 > blk) --bi;
+        while ((bi + 1) * (bi + 2) / 2 <= blk) ++bi;
+        const int64_t bj = blk - bi * (bi + 1) / 2;
+        const int64_t R0 = r0 + bi * 32, C0 = r0 + bj * 32;
         d4 acc[2][2];
 #pragma unroll
         for (int a = 0; a < 2; ++a)
@@ -359,21 +357,23 @@ __device__ void chol_one(double* __restrict__ A, int64_t n, double* logdet_out, 
           for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-              int64_t gi = R0 + wr + a * 16 + lk + 4 * r, gj = C0 + wc + b * 16 + lr;
-              acc[a][b][r] = (gi < n && gj < n && gj <= gi) ? A[gi * n + gj] : 0.0;
+              const int64_t gi = R0 + a * 16 + lk + 4 * r, gj = C0 + b * 16 + lr;
+              acc[a][b][r] = (gi < n && gj <= gi) ? A[gi * n + gj] : 0.0;
             }
+        const int64_t ia0 = R0 + lr, ia1 = R0 + 16 + lr, jb0 = C0 + lr, jb1 = C0 + 16 + lr;
 #pragma unroll
         for (int ks = 0; ks < NB / 4; ++ks) {
-          double fa[2], fb[2];
-#pragma unroll
-          for (int a = 0; a < 2; ++a) {
-            fa[a] = -Pi[wr + a * 16 + lr][ks * 4 + lk];
-            fb[a] = Pj[wc + a * 16 + lr][ks * 4 + lk];
-          }
-#pragma unroll
-          for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[a], fb[b], acc[a][b], 0, 0, 0);
+          const int p = ks * 4 + lk;
+          if (ks * 4 >= nb) break;
+          const bool pv = p < nb;
+          const double fa0 = (pv && ia0 < n) ? -A[ia0 * n + k0 + p] : 0.0;
+          const double fa1 = (pv && ia1 < n) ? -A[ia1 * n + k0 + p] : 0.0;
+          const double fb0 = (pv && jb0 < n) ? A[jb0 * n + k0 + p] : 0.0;
+          const double fb1 = (pv && jb1 < n) ? A[jb1 * n + k0 + p] : 0.0;
+          acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa0, fb0, acc[0][0], 0, 0, 0);
+          acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa0, fb1, acc[0][1], 0, 0, 0);
+          acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa1, fb0, acc[1][0], 0, 0, 0);
+          acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa1, fb1, acc[1][1], 0, 0, 0);
         }
 #pragma unroll
         for (int a = 0; a < 2; ++a)
@@ -381,11 +381,11 @@ __device__ void chol_one(double* __restrict__ A, int64_t n, double* logdet_out, 
           for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-              int64_t gi = R0 + wr + a * 16 + lk + 4 * r, gj = C0 + wc + b * 16 + lr;
-              if (gi < n && gj < n && gj <= gi) A[gi * n + gj] = acc[a][b][r];
+              const int64_t gi = R0 + a * 16 + lk + 4 * r, gj = C0 + b * 16 + lr;
+              if (gi < n && gj <= gi) A[gi * n + gj] = acc[a][b][r];
             }
-        __syncthreads();
       }
+      __syncthreads();
     }
   }
   __syncthreads();
@@ -395,18 +395,22 @@ __device__ void chol_one(double* __restrict__ A, int64_t n, double* logdet_out, 
   }
 }
 
-__global__ void __launch_bounds__(256) chol_batched_kernel(const SampleDesc* __restrict__ sd, double* __restrict__ mats,
-                                                           int which_h, double* __restrict__ logdet,
-                                                           int64_t* __restrict__ info) {
-  const SampleDesc d = sd[blockIdx.x];
+// grid: 2 * n blocks, block b factorises matrix (b & 1) (0 = A, 1 = H) of sample b / 2,
+// ordered so that the largest samples start first.
+__global__ void __launch_bounds__(256) chol_batched_kernel(const SampleDesc* __restrict__ sd,
+                                                           const int* __restrict__ order, double* __restrict__ mats,
+                                                           double* __restrict__ logdet, int64_t* __restrict__ info) {
+  const int smp = order[blockIdx.x >> 1];
+  const int which_h = blockIdx.x & 1;
+  const SampleDesc d = sd[smp];
   double* A = mats + d.mat_off + (which_h ? d.S * d.S : 0);
   double ld;
   int64_t inf;
   chol_one(A, d.S, &ld, &inf);
   if (threadIdx.x == 0) {
     if (which_h && d.beta > 0.0) ld += log(1e-8) - log(1e-8 + d.beta);
-    logdet[2 * blockIdx.x + which_h] = ld;
-    info[2 * blockIdx.x + which_h] = inf;
+    logdet[2 * smp + which_h] = ld;
+    info[2 * smp + which_h] = inf;
   }
 }
 
@@ -506,7 +510,7 @@ __global__ void noise_norm_kernel(const double* __restrict__ noise, int64_t K2, 
 // ---------------------------------------------------------------------------
 struct BatchPlan {
   int64_t n, M, B, sumS, sumS2, omega_elems, csc_ptr_elems, csc_ent_elems;
-  size_t off_desc, off_ptr, off_cur, off_key, off_t, off_omega, off_mats, off_D, off_logdet, off_info, bytes;
+  size_t off_desc, off_order, off_ptr, off_cur, off_key, off_t, off_omega, off_mats, off_D, off_logdet, off_info, bytes;
 };
 
 inline BatchPlan plan_batch(int64_t M, int64_t B, int64_t n, const int64_t* S_host) {
@@ -529,6 +533,7 @@ inline BatchPlan plan_batch(int64_t M, int64_t B, int64_t n, const int64_t* S_ho
     return r;
   };
   p.off_desc = put(sizeof(SampleDesc) * n);
+  p.off_order = put(sizeof(int) * n);
   p.off_ptr = put(sizeof(int) * p.csc_ptr_elems);
   p.off_cur = put(sizeof(int) * p.csc_ptr_elems);
   p.off_key = put(sizeof(int) * p.csc_ent_elems);
@@ -588,6 +593,12 @@ int batch_pipeline(const double* w_tilde, int64_t M, const double* dirty_image, 
     maxS = std::max(maxS, S);
   }
   AJ_CHECK_HIP(hipMemcpyAsync(desc, hd.data(), sizeof(SampleDesc) * n, hipMemcpyHostToDevice, st));
+  static thread_local std::vector<int> ho;
+  ho.resize(n);
+  for (int64_t b = 0; b < n; ++b) ho[b] = (int)b;
+  std::stable_sort(ho.begin(), ho.end(), [&](int x, int y) { return S_host[x] > S_host[y]; });
+  int* order = reinterpret_cast<int*>(ws + p.off_order);
+  AJ_CHECK_HIP(hipMemcpyAsync(order, ho.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
   AJ_CHECK_HIP(hipMemsetAsync(ptrs, 0, sizeof(int) * p.csc_ptr_elems, st));
 
   const unsigned gm = (unsigned)std::min<int64_t>(ceil_div(M, 256), 1024);
@@ -629,9 +640,7 @@ int batch_pipeline(const double* w_tilde, int64_t M, const double* dirty_image, 
                      mats, 1, 1, 1);
   AJ_LAUNCH_CHECK();
   timer_begin(AJ_TIMER_CHOL, st);
-  hipLaunchKernelGGL(chol_batched_kernel, dim3((unsigned)n), dim3(256), 0, st, desc, mats, 0, logdet, info);
-  AJ_LAUNCH_CHECK();
-  hipLaunchKernelGGL(chol_batched_kernel, dim3((unsigned)n), dim3(256), 0, st, desc, mats, 1, logdet, info);
+  hipLaunchKernelGGL(chol_batched_kernel, dim3((unsigned)(2 * n)), dim3(256), 0, st, desc, order, mats, logdet, info);
   timer_end(AJ_TIMER_CHOL, st);
   AJ_LAUNCH_CHECK();
   timer_begin(AJ_TIMER_SOLVE, st);

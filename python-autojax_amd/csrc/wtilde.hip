@@ -22,6 +22,8 @@
 //             result is the exact Gram matrix of the quantised features.
 #include "aj_common.h"
 
+#include <vector>
+
 namespace aj {
 namespace wt {
 
@@ -413,14 +415,22 @@ __global__ void __launch_bounds__(256) syrk_kernel(const void* __restrict__ Xv, 
 // ---------------------------------------------------------------------------
 constexpr int I8_TM = 64, I8_TN = 128, I8_KS = 64, I8_PITCH = 80;
 
-__device__ inline void tile_i8(int64_t t, int64_t& I, int64_t& J) {
-  // rows 2q and 2q+1 each hold q+1 tiles; rows < 2q hold q(q+1) tiles in total
-  int64_t q = (int64_t)((sqrt(4.0 * (double)t + 1.0) - 1.0) * 0.5);
-  while (q * (q + 1) > t) --q;
-  while ((q + 1) * (q + 2) <= t) ++q;
-  const int64_t r = t - q * (q + 1);
-  I = 2 * q + (r >= q + 1 ? 1 : 0);
-  J = r % (q + 1);
+// Tile order: 512 x 512 super-tiles (8 row blocks x 4 column blocks), lower triangle, so that the
+// 32 workgroups an XCD runs at a time share their row and column panels in its L2.
+inline void build_i8_tiles(int64_t Mpad, std::vector<int>& out) {
+  const int64_t nI = Mpad / I8_TM, nJ = Mpad / I8_TN;
+  const int64_t nS = (nI + 7) / 8;
+  out.clear();
+  for (int64_t SI = 0; SI < nS; ++SI)
+    for (int64_t SJ = 0; SJ <= SI; ++SJ)
+      for (int64_t i = 0; i < 8; ++i)
+        for (int64_t j = 0; j < 4; ++j) {
+          const int64_t I = 8 * SI + i, J = 4 * SJ + j;
+          if (I < nI && J < nJ && J <= I / 2) {
+            out.push_back((int)I);
+            out.push_back((int)J);
+          }
+        }
 }
 
 inline int64_t i8_tiles(int64_t Mpad) {
@@ -431,7 +441,8 @@ inline int64_t i8_tiles(int64_t Mpad) {
 template <int NS>
 __global__ void __launch_bounds__(256, 1) syrk_i8_kernel(const int8_t* __restrict__ X, int64_t M, int64_t Mpad,
                                                          int64_t ldx, int nchunks, const double* __restrict__ chunk_sc,
-                                                         double* __restrict__ W, int64_t ntiles) {
+                                                         double* __restrict__ W, int64_t ntiles,
+                                                         const int* __restrict__ tiles) {
   constexpr int NG = 2 * NS - 1;
   constexpr int ROWS = I8_TM + I8_TN;
   __shared__ __attribute__((aligned(16))) int8_t lds[2][NS][ROWS * I8_PITCH];
@@ -440,9 +451,7 @@ __global__ void __launch_bounds__(256, 1) syrk_i8_kernel(const int8_t* __restric
   const int64_t per = (ntiles + 7) / 8;
   const int64_t t = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
   if (t >= ntiles) return;
-  int64_t TI, TJ;
-  tile_i8(t, TI, TJ);
-  const int64_t I0 = TI * I8_TM, J0 = TJ * I8_TN;
+  const int64_t I0 = (int64_t)tiles[2 * t] * I8_TM, J0 = (int64_t)tiles[2 * t + 1] * I8_TN;
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 64;
@@ -455,33 +464,41 @@ __global__ void __launch_bounds__(256, 1) syrk_i8_kernel(const int8_t* __restric
   const int8_t* srcB0 = X + (J0 + grow) * ldx + gseg;
   const int8_t* srcB1 = X + (J0 + 64 + grow) * ldx + gseg;
   // the next stage is prefetched in two halves of the digit slices, each overlapped with half of
-  // the MFMA work of the current stage, so only half of the slices sit in registers at a time
+  // the MFMA work of the current stage, so only half of the slices sit in registers at a time.
+  // (All indices below are compile-time constants after unrolling, so pf stays in registers.)
   constexpr int H0 = (NS + 1) / 2;
-  int4 pf[H0][3];
-  auto gload = [&](int64_t k0, int s0, int s1) {
-#pragma unroll
-    for (int s = s0; s < s1; ++s) {
-      pf[s - s0][0] = *reinterpret_cast<const int4*>(srcA + s * sstride + k0);
-      pf[s - s0][1] = *reinterpret_cast<const int4*>(srcB0 + s * sstride + k0);
-      pf[s - s0][2] = *reinterpret_cast<const int4*>(srcB1 + s * sstride + k0);
-    }
-  };
-  auto lstore = [&](int buf, int s0, int s1) {
-#pragma unroll
-    for (int s = s0; s < s1; ++s) {
-      *reinterpret_cast<int4*>(&lds[buf][s][grow * I8_PITCH + gseg]) = pf[s - s0][0];
-      *reinterpret_cast<int4*>(&lds[buf][s][(I8_TM + grow) * I8_PITCH + gseg]) = pf[s - s0][1];
-      *reinterpret_cast<int4*>(&lds[buf][s][(I8_TM + 64 + grow) * I8_PITCH + gseg]) = pf[s - s0][2];
-    }
-  };
+  // six named registers (no array: an array here is turned into LDS storage by the compiler)
+  int4 pA0, pB0, pC0, pA1, pB1, pC1;
+#define AJ_I8_LD1(V0, V1, V2, SS, K0)                                              \
+  if ((SS) < NS) {                                                                 \
+    V0 = *reinterpret_cast<const int4*>(srcA + (int64_t)(SS) * sstride + (K0));    \
+    V1 = *reinterpret_cast<const int4*>(srcB0 + (int64_t)(SS) * sstride + (K0));   \
+    V2 = *reinterpret_cast<const int4*>(srcB1 + (int64_t)(SS) * sstride + (K0));   \
+  }
+#define AJ_I8_ST1(V0, V1, V2, SS, BUF)                                                             \
+  if ((SS) < NS) {                                                                                 \
+    *reinterpret_cast<int4*>(&lds[BUF][SS][grow * I8_PITCH + gseg]) = V0;                          \
+    *reinterpret_cast<int4*>(&lds[BUF][SS][(I8_TM + grow) * I8_PITCH + gseg]) = V1;                \
+    *reinterpret_cast<int4*>(&lds[BUF][SS][(I8_TM + 64 + grow) * I8_PITCH + gseg]) = V2;           \
+  }
+#define AJ_I8_GLOAD(HALF, K0)                                 \
+  {                                                           \
+    AJ_I8_LD1(pA0, pB0, pC0, (HALF) * H0 + 0, K0)             \
+    if (H0 > 1) { AJ_I8_LD1(pA1, pB1, pC1, (HALF) * H0 + 1, K0) } \
+  }
+#define AJ_I8_LSTORE(HALF, BUF)                               \
+  {                                                           \
+    AJ_I8_ST1(pA0, pB0, pC0, (HALF) * H0 + 0, BUF)            \
+    if (H0 > 1) { AJ_I8_ST1(pA1, pB1, pC1, (HALF) * H0 + 1, BUF) } \
+  }
 
   i4 acc[NG][2][4];
   const int64_t nstage = (int64_t)nchunks * (FC / I8_KS);
   constexpr int SPC = FC / I8_KS;  // stages per chunk
-  gload(0, 0, H0);
-  lstore(0, 0, H0);
-  gload(0, H0, NS);
-  lstore(0, H0, NS);
+  AJ_I8_GLOAD(0, 0)
+  AJ_I8_LSTORE(0, 0)
+  AJ_I8_GLOAD(1, 0)
+  AJ_I8_LSTORE(1, 0)
   __syncthreads();
   for (int64_t st = 0; st < nstage; ++st) {
     if (st % SPC == 0) {
@@ -494,6 +511,7 @@ __global__ void __launch_bounds__(256, 1) syrk_i8_kernel(const int8_t* __restric
     }
     const int buf = (int)(st & 1);
     const bool more = st + 1 < nstage;
+    const int64_t knext = (st + 1) * I8_KS;
     i4 fa[NS][2];
 #pragma unroll
     for (int s = 0; s < NS; ++s)
@@ -502,7 +520,9 @@ __global__ void __launch_bounds__(256, 1) syrk_i8_kernel(const int8_t* __restric
         fa[s][a] = *reinterpret_cast<const i4*>(&lds[buf][s][(wr + a * 16 + lr) * I8_PITCH + lk * 16]);
 #pragma unroll
     for (int half = 0; half < 2; ++half) {
-      if (more) gload((st + 1) * I8_KS, half ? H0 : 0, half ? NS : H0);
+      if (more) { AJ_I8_GLOAD(half, knext) }
+      // keep the order: loads issued, MFMA work, then the LDS stores that wait for the loads
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int b = 2 * half; b < 2 * half + 2; ++b) {
         i4 fb[NS];
@@ -517,8 +537,16 @@ __global__ void __launch_bounds__(256, 1) syrk_i8_kernel(const int8_t* __restric
             for (int a = 0; a < 2; ++a)
               acc[p + q][a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[p][a], fb[q], acc[p + q][a][b], 0, 0, 0);
       }
-      if (more) lstore(buf ^ 1, half ? H0 : 0, half ? NS : H0);
+      __builtin_amdgcn_sched_barrier(0);
+      if (more) {
+        if (buf) { AJ_I8_LSTORE(half, 0) } else { AJ_I8_LSTORE(half, 1) }
+      }
+      __builtin_amdgcn_sched_barrier(0);
     }
+#undef AJ_I8_GLOAD
+#undef AJ_I8_LSTORE
+#undef AJ_I8_LD1
+#undef AJ_I8_ST1
     if (st % SPC == SPC - 1) {
       const double q = qmax_of(NS);
       const int c = (int)(st / SPC);
@@ -586,6 +614,7 @@ inline Plan make_plan(int64_t K, int64_t M, int mode) {
   p.ldx = (int64_t)p.group * FC;
   p.feat_bytes = (size_t)p.ns * p.Mpad * p.ldx * p.es;
   p.bytes = align_up(p.feat_bytes, 256) + 256 /*centre*/ + align_up(sizeof(double) * p.group, 256) + 256;
+  p.bytes += align_up(sizeof(int) * 2 * (size_t)i8_tiles(ceil_div(M, 2 * TILE) * 2 * TILE), 256) + 256;
   return p;
 }
 
@@ -595,6 +624,19 @@ int run(const double* noise, const double* uv, const double* grid, int64_t K, in
   void* X = ar.take<char>(p.feat_bytes);
   double* centre = ar.take<double>(2);
   double* sc = ar.take<double>(p.group);
+  int* tiles = nullptr;
+  int64_t nt8 = 0;
+  if (MODE == AJ_PREC_INT8X3 || MODE == AJ_PREC_INT8X4) {
+    static thread_local std::vector<int> tl;
+    build_i8_tiles(p.Mpad, tl);
+    nt8 = (int64_t)tl.size() / 2;
+    tiles = ar.take<int>(tl.size());
+    if (!ar.ok()) {
+      set_error("workspace too small: need %zu bytes, got %zu", p.bytes, ar.cap);
+      return AJ_ESPACE;
+    }
+    AJ_CHECK_HIP(hipMemcpyAsync(tiles, tl.data(), sizeof(int) * tl.size(), hipMemcpyHostToDevice, st));
+  }
   if (!ar.ok()) {
     set_error("workspace too small: need %zu bytes, got %zu", p.bytes, ar.cap);
     return AJ_ESPACE;
@@ -622,9 +664,8 @@ int run(const double* noise, const double* uv, const double* grid, int64_t K, in
     timer_begin(AJ_TIMER_WT_SYRK, st);
     if constexpr (MODE == AJ_PREC_INT8X3 || MODE == AJ_PREC_INT8X4) {
       constexpr int NSL = Mode<MODE>::NS;
-      const int64_t nt8 = i8_tiles(p.Mpad);
       hipLaunchKernelGGL(syrk_i8_kernel<NSL>, dim3((unsigned)(8 * ((nt8 + 7) / 8))), dim3(256), 0, st,
-                         reinterpret_cast<const int8_t*>(X), M, p.Mpad, p.ldx, ng, sc, W, nt8);
+                         reinterpret_cast<const int8_t*>(X), M, p.Mpad, p.ldx, ng, sc, W, nt8, tiles);
     } else {
       hipLaunchKernelGGL(syrk_kernel<MODE>, dim3((unsigned)ntiles), dim3(256), 0, st, X, M, p.Mpad, p.ldx, ng, sc, W,
                          first);
