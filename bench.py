@@ -33,7 +33,7 @@ for p in (ROOT, ROOT / "python-autojax_amd"):
 import numpy as np  # noqa: E402
 
 PEAKS_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0, "int8x3": 5000.0, "int8x4": 5000.0}
-RAW_FACTOR = {"fp64": 1, "fp32": 1, "bf16x3": 3, "int8x3": 9, "int8x4": 16}
+RAW_FACTOR = {"fp64": 1, "fp32": 1, "bf16x3": 3, "int8x3": 9, "int8x4": 15}
 
 
 def log(msg):

@@ -64,6 +64,14 @@ int aj_w_tilde_curvature_interferometer_from(const double* noise_map_real, const
                                              double* w_tilde, void* workspace, size_t workspace_bytes, void* stream);
 
 
+/* Tuning of the W-tilde build, process-wide, no reference counterpart.
+ * aj_w_tilde_set_variant: kernel variant of the integer-mode SYRK (default 0).
+ * aj_w_tilde_set_group_bytes: cap on the feature matrix of one SYRK launch (0 = default 256 MiB);
+ * changes aj_w_tilde_workspace_bytes, so query the workspace after setting it. */
+int aj_w_tilde_set_variant(int variant);
+int aj_w_tilde_variant_count(void);
+int aj_w_tilde_set_group_bytes(size_t bytes);
+
 /* numba.py:95-152 w_tilde_data_interferometer_from(visibilities_real f8[K],
  * noise_map_real f8[K], uv_wavelengths f8[K,2], grid_radians_slim f8[M,2]) -> f8[M].
  * Same weight as the reference, (n_k^2 / v_k)^2 (numba.py:149). */

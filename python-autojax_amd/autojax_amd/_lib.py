@@ -30,6 +30,9 @@ SIGNATURES = {
     "aj_abi_version": (_INT, []),
     "aj_timing_enable": (_INT, [_INT]),
     "aj_timing_read": (_INT, [_INT, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    "aj_w_tilde_set_variant": (_INT, [_INT]),
+    "aj_w_tilde_variant_count": (_INT, []),
+    "aj_w_tilde_set_group_bytes": (_INT, [_S]),
     "aj_w_tilde_workspace_bytes": (_S, [_I, _I, _INT]),
     "aj_w_tilde_curvature_interferometer_from": (_INT, [_P, _P, _P, _I, _I, _INT, _P, _P, _S, _P]),
     "aj_visibility_sum_workspace_bytes": (_S, [_I, _I]),

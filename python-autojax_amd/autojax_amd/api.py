@@ -130,9 +130,10 @@ def mask_2d_circular_from(shape_native, pixel_scales, radius, centre):
 def w_tilde_curvature_interferometer_from(noise_map_real, uv_wavelengths, grid_radians_slim, *, precision="fp64"):
     """numba.py:155-207 -> f8[M, M].
 
-    ``precision``: "fp64" (default, the reference's arithmetic), "int8x3" /
-    "int8x4" (exact integer MFMA on 3/4 8-bit digits of FP32 features),
-    "fp32" or "bf16x3" (FP32-accumulated MFMA).
+    ``precision``: "fp64" (default, the reference's arithmetic); "int8x3",
+    "int8x4" (exact integer MFMA on 3 or 4 signed 8-bit digits of FP64
+    cos/sin features, all digit products that reach FP64); "fp32" or
+    "bf16x3" (FP32 cos/sin, FP32-accumulated MFMA).
     """
     torch_in = _any_t(noise_map_real, uv_wavelengths, grid_radians_slim)
     n = _dev(noise_map_real, np.float64, "noise_map_real", 1)

@@ -29,6 +29,9 @@ def main(argv=None):
     ap.add_argument("--samples", type=int, default=64)
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--mib", type=int, default=1024)
+    ap.add_argument("--variant", type=int, default=None, help="integer SYRK kernel variant (LDS layout)")
+    ap.add_argument("--check", action="store_true", help="compare W-tilde with kernel variant 0 (bitwise)")
+    ap.add_argument("--group-mib", type=int, default=0, help="cap on the features of one SYRK launch (0 = default)")
     a = ap.parse_args(argv)
 
     import torch
@@ -49,6 +52,9 @@ def main(argv=None):
         torch.cuda.synchronize()
         return s.elapsed_time(e) / reps
 
+    if a.variant is not None:
+        _lib.call("aj_w_tilde_set_variant", a.variant)
+    _lib.call("aj_w_tilde_set_group_bytes", a.group_mib << 20)
     if a.what == "copy":
         n = a.mib * (1 << 20) // 8
         x = torch.empty(n, dtype=torch.float64, device="cuda").uniform_()
@@ -69,8 +75,14 @@ def main(argv=None):
     if a.what == "wtilde":
         ws = dv.workspace(_lib.query("aj_w_tilde_workspace_bytes", K, M, _lib.PRECISIONS[a.precision]))
         ms = timed(lambda: E.w_tilde(n_d, uv_d, g_d, a.precision, out=W, workspace=ws), a.reps)
-        print(json.dumps({"kernel": "w_tilde", "precision": a.precision, "M": M, "K": K, "ms": ms,
-                          "algorithmic_tflops": 2.0 * M * M * K / ms / 1e9}), flush=True)
+        rec = {"kernel": "w_tilde", "precision": a.precision, "variant": a.variant, "group_mib": a.group_mib,
+               "M": M, "K": K, "ms": ms,
+               "algorithmic_tflops": 2.0 * M * M * K / ms / 1e9}
+        if a.check:
+            _lib.call("aj_w_tilde_set_variant", 0)
+            W0 = E.w_tilde(n_d, uv_d, g_d, a.precision, workspace=ws)
+            rec["max_abs_diff_vs_variant0"] = float((W - W0).abs().max().item())
+        print(json.dumps(rec), flush=True)
         return
     E.w_tilde(n_d, uv_d, g_d, "fp64", out=W)
     samples = []

@@ -31,6 +31,7 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef int i4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef int i16v __attribute__((ext_vector_type(16)));
 
 constexpr int KC = 4096;        // visibilities per chunk
 constexpr int FC = 2 * KC;      // features per chunk (cos block then sin block)
@@ -52,8 +53,19 @@ template <> struct Mode<AJ_PREC_INT8X3> {
   using E = int8_t; static constexpr int NS = 3, MK = 64, NG = 5;
 };
 template <> struct Mode<AJ_PREC_INT8X4> {
-  using E = int8_t; static constexpr int NS = 4, MK = 64, NG = 7;
+  using E = int8_t; static constexpr int NS = 4, MK = 64, NG = 6;
 };
+__host__ __device__ constexpr bool is_i8(int mode) {
+  return mode == AJ_PREC_INT8X3 || mode == AJ_PREC_INT8X4;
+}
+
+// Digit-order groups kept by the integer SYRK kernels (Mode<>::NG: products a_p*b_q with p+q < NG).
+//   int8x3  : all orders 0..4 of 3 digits (9 products).
+//   int8x4  : orders 0..5 of 4 digits (15 products); the single order-6 product a3*b3 (weight 256^-6
+//             of the leading term, about 1e-17 relative) is below the FP64 rounding of the sum.
+// Dropping whole orders above the FP64 rounding (e.g. keeping orders 0..3 of 4 digits) leaves W-tilde
+// accurate to ~1e-9 but no longer the Gram matrix of any feature matrix: it can lose positive
+// semi-definiteness, and F + H then fails to factor at small regularization coefficients.
 
 inline int slices_of(int mode) {
   switch (mode) {
@@ -160,7 +172,7 @@ __global__ void __launch_bounds__(256) features_kernel(
   const int64_t col_s = col_c + KC;
   const int64_t sstride = Mpad * ldx;
   double qscale = 0.0;
-  if constexpr (MODE == AJ_PREC_INT8X3 || MODE == AJ_PREC_INT8X4) qscale = sw / chunk_sc[c] * qmax_of(NS);
+  if constexpr (is_i8(MODE)) qscale = sw / chunk_sc[c] * qmax_of(NS);
   const int64_t i0 = (int64_t)blockIdx.y * FEAT_ROWS;
   for (int r = 0; r < FEAT_ROWS; ++r) {
     const int64_t i = i0 + r;
@@ -177,7 +189,7 @@ __global__ void __launch_bounds__(256) features_kernel(
       }
       X[i * ldx + col_c] = cv;
       X[i * ldx + col_s] = sv;
-    } else if constexpr (MODE == AJ_PREC_INT8X3 || MODE == AJ_PREC_INT8X4) {
+    } else if constexpr (is_i8(MODE)) {
       // FP64 cos/sin, quantised to NS balanced 8-bit digits of a fixed-point value
       if (live) {
         double rv = reduced_revs(grid[2 * i + 1] - cx, grid[2 * i] - cy, u0, u1);
@@ -239,7 +251,7 @@ __global__ void __launch_bounds__(256) syrk_kernel(const void* __restrict__ Xv, 
   constexpr int ES = (int)sizeof(E);
   constexpr int KT = ROWB / ES;          // features per stage
   constexpr int NSTEP = KT / MK;         // MFMA k-steps per stage
-  constexpr bool INT8 = (MODE == AJ_PREC_INT8X3 || MODE == AJ_PREC_INT8X4);
+  constexpr bool INT8 = is_i8(MODE);
 
   __shared__ __attribute__((aligned(16))) char lds[2][NS][TILE * PITCH];
 
@@ -413,7 +425,21 @@ __global__ void __launch_bounds__(256) syrk_kernel(const void* __restrict__ Xv, 
 // J <= I/2 of 128 (the lower triangle), placed so that consecutive tiles,
 // which share row panels, land on the same XCD.
 // ---------------------------------------------------------------------------
-constexpr int I8_TM = 64, I8_TN = 128, I8_KS = 64, I8_PITCH = 80;
+constexpr int I8_TM = 64, I8_TN = 128, I8_KS = 64;
+constexpr int I8_NLAYOUT = 8;  // 0-4: register prefetch with the LDS layouts below; 5: direct-to-LDS; 6: 32x32x32 MFMA; 7: 8 waves
+
+// LDS placement of 16-byte unit u (0..3) of tile row r within one 64-byte stage row.
+//   0: rows padded to 80 bytes;  1: 64-byte rows, unit ^= (r/2)%4;  2: rows padded to 72 bytes;
+//   3: 64-byte rows, unit ^= ((r/2) ^ (r/8))%4;  4: rows padded to 96 bytes.
+template <int L>
+struct I8Layout {
+  static constexpr int pitch = L == 0 ? 80 : L == 2 ? 72 : L == 4 ? 96 : 64;
+  __device__ static inline int off(int r, int u) {
+    if constexpr (L == 1) return r * 64 + 16 * (u ^ ((r >> 1) & 3));
+    else if constexpr (L == 3) return r * 64 + 16 * (u ^ (((r >> 1) ^ (r >> 3)) & 3));
+    else return r * pitch + 16 * u;
+  }
+};
 
 // Tile order: 512 x 512 super-tiles (8 row blocks x 4 column blocks), lower triangle, so that the
 // 32 workgroups an XCD runs at a time share their row and column panels in its L2.
@@ -438,14 +464,15 @@ inline int64_t i8_tiles(int64_t Mpad) {
   return (nI / 2) * (nI / 2 + 1);
 }
 
-template <int NS>
+template <int NS, int NG, int LAYOUT>
 __global__ void __launch_bounds__(256, 1) syrk_i8_kernel(const int8_t* __restrict__ X, int64_t M, int64_t Mpad,
                                                          int64_t ldx, int nchunks, const double* __restrict__ chunk_sc,
                                                          double* __restrict__ W, int64_t ntiles,
                                                          const int* __restrict__ tiles) {
-  constexpr int NG = 2 * NS - 1;
+  constexpr int NFULL = 2 * NS - 1;  // orders 0 .. 2NS-2
   constexpr int ROWS = I8_TM + I8_TN;
-  __shared__ __attribute__((aligned(16))) int8_t lds[2][NS][ROWS * I8_PITCH];
+  using LY = I8Layout<LAYOUT>;
+  __shared__ __attribute__((aligned(16))) int8_t lds[2][NS][ROWS * LY::pitch];
 
   // XCD-aware placement: block b runs on XCD b % 8; give each XCD a contiguous run of tiles.
   const int64_t per = (ntiles + 7) / 8;
@@ -459,7 +486,7 @@ __global__ void __launch_bounds__(256, 1) syrk_i8_kernel(const int8_t* __restric
   const int64_t sstride = Mpad * ldx;
 
   // global -> register prefetch: thread loads 16 B of row (tid/4) [A], rows (tid/4) and 64+(tid/4) [B]
-  const int grow = tid >> 2, gseg = (tid & 3) * 16;
+  const int grow = tid >> 2, gunit = tid & 3, gseg = gunit * 16;
   const int8_t* srcA = X + (I0 + grow) * ldx + gseg;
   const int8_t* srcB0 = X + (J0 + grow) * ldx + gseg;
   const int8_t* srcB1 = X + (J0 + 64 + grow) * ldx + gseg;
@@ -477,9 +504,9 @@ __global__ void __launch_bounds__(256, 1) syrk_i8_kernel(const int8_t* __restric
   }
 #define AJ_I8_ST1(V0, V1, V2, SS, BUF)                                                             \
   if ((SS) < NS) {                                                                                 \
-    *reinterpret_cast<int4*>(&lds[BUF][SS][grow * I8_PITCH + gseg]) = V0;                          \
-    *reinterpret_cast<int4*>(&lds[BUF][SS][(I8_TM + grow) * I8_PITCH + gseg]) = V1;                \
-    *reinterpret_cast<int4*>(&lds[BUF][SS][(I8_TM + 64 + grow) * I8_PITCH + gseg]) = V2;           \
+    *reinterpret_cast<int4*>(&lds[BUF][SS][LY::off(grow, gunit)]) = V0;                            \
+    *reinterpret_cast<int4*>(&lds[BUF][SS][LY::off(I8_TM + grow, gunit)]) = V1;                    \
+    *reinterpret_cast<int4*>(&lds[BUF][SS][LY::off(I8_TM + 64 + grow, gunit)]) = V2;               \
   }
 #define AJ_I8_GLOAD(HALF, K0)                                 \
   {                                                           \
@@ -517,7 +544,7 @@ __global__ void __launch_bounds__(256, 1) syrk_i8_kernel(const int8_t* __restric
     for (int s = 0; s < NS; ++s)
 #pragma unroll
       for (int a = 0; a < 2; ++a)
-        fa[s][a] = *reinterpret_cast<const i4*>(&lds[buf][s][(wr + a * 16 + lr) * I8_PITCH + lk * 16]);
+        fa[s][a] = *reinterpret_cast<const i4*>(&lds[buf][s][LY::off(wr + a * 16 + lr, lk)]);
 #pragma unroll
     for (int half = 0; half < 2; ++half) {
       if (more) { AJ_I8_GLOAD(half, knext) }
@@ -528,14 +555,15 @@ __global__ void __launch_bounds__(256, 1) syrk_i8_kernel(const int8_t* __restric
         i4 fb[NS];
 #pragma unroll
         for (int s = 0; s < NS; ++s)
-          fb[s] = *reinterpret_cast<const i4*>(&lds[buf][s][(I8_TM + wc + b * 16 + lr) * I8_PITCH + lk * 16]);
+          fb[s] = *reinterpret_cast<const i4*>(&lds[buf][s][LY::off(I8_TM + wc + b * 16 + lr, lk)]);
 #pragma unroll
         for (int p = 0; p < NS; ++p)
 #pragma unroll
           for (int q = 0; q < NS; ++q)
+            if (p + q < NG)
 #pragma unroll
-            for (int a = 0; a < 2; ++a)
-              acc[p + q][a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[p][a], fb[q], acc[p + q][a][b], 0, 0, 0);
+              for (int a = 0; a < 2; ++a)
+                acc[p + q][a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[p][a], fb[q], acc[p + q][a][b], 0, 0, 0);
       }
       __builtin_amdgcn_sched_barrier(0);
       if (more) {
@@ -566,6 +594,8 @@ __global__ void __launch_bounds__(256, 1) syrk_i8_kernel(const int8_t* __restric
             double v = 0.0;
 #pragma unroll
             for (int g = 0; g < NG; ++g) v = v * 256.0 + (double)acc[g][a][b][r];
+#pragma unroll
+            for (int g = NG; g < NFULL; ++g) v *= 256.0;
             if (a * 16 + r < rlim && b * 16 < clim) Wt[(a * 16 + r) * Mo + b * 16] += v * s2;
             // one output at a time: keeps the scheduler from reading all accumulators at once
             __builtin_amdgcn_sched_barrier(0);
@@ -573,6 +603,404 @@ __global__ void __launch_bounds__(256, 1) syrk_i8_kernel(const int8_t* __restric
     }
     __syncthreads();
   }
+}
+
+template <int NS, int NG>
+__global__ void __launch_bounds__(256, 1) syrk_i8_k32_kernel(const int8_t* __restrict__ X, int64_t M, int64_t Mpad,
+                                                         int64_t ldx, int nchunks, const double* __restrict__ chunk_sc,
+                                                         double* __restrict__ W, int64_t ntiles,
+                                                         const int* __restrict__ tiles) {
+  constexpr int NFULL = 2 * NS - 1;  // orders 0 .. 2NS-2
+  constexpr int ROWS = I8_TM + I8_TN;
+  using LY = I8Layout<1>;
+  __shared__ __attribute__((aligned(16))) int8_t lds[2][NS][ROWS * LY::pitch];
+
+  // XCD-aware placement: block b runs on XCD b % 8; give each XCD a contiguous run of tiles.
+  const int64_t per = (ntiles + 7) / 8;
+  const int64_t t = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (t >= ntiles) return;
+  const int64_t I0 = (int64_t)tiles[2 * t] * I8_TM, J0 = (int64_t)tiles[2 * t + 1] * I8_TN;
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 64;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int64_t sstride = Mpad * ldx;
+
+  // global -> register prefetch: thread loads 16 B of row (tid/4) [A], rows (tid/4) and 64+(tid/4) [B]
+  const int grow = tid >> 2, gunit = tid & 3, gseg = gunit * 16;
+  const int8_t* srcA = X + (I0 + grow) * ldx + gseg;
+  const int8_t* srcB0 = X + (J0 + grow) * ldx + gseg;
+  const int8_t* srcB1 = X + (J0 + 64 + grow) * ldx + gseg;
+  // the next stage is prefetched in two halves of the digit slices, each overlapped with half of
+  // the MFMA work of the current stage, so only half of the slices sit in registers at a time.
+  // (All indices below are compile-time constants after unrolling, so pf stays in registers.)
+  constexpr int H0 = (NS + 1) / 2;
+  // six named registers (no array: an array here is turned into LDS storage by the compiler)
+  int4 pA0, pB0, pC0, pA1, pB1, pC1;
+#define AJ_I8_LD1(V0, V1, V2, SS, K0)                                              \
+  if ((SS) < NS) {                                                                 \
+    V0 = *reinterpret_cast<const int4*>(srcA + (int64_t)(SS) * sstride + (K0));    \
+    V1 = *reinterpret_cast<const int4*>(srcB0 + (int64_t)(SS) * sstride + (K0));   \
+    V2 = *reinterpret_cast<const int4*>(srcB1 + (int64_t)(SS) * sstride + (K0));   \
+  }
+#define AJ_I8_ST1(V0, V1, V2, SS, BUF)                                                             \
+  if ((SS) < NS) {                                                                                 \
+    *reinterpret_cast<int4*>(&lds[BUF][SS][LY::off(grow, gunit)]) = V0;                            \
+    *reinterpret_cast<int4*>(&lds[BUF][SS][LY::off(I8_TM + grow, gunit)]) = V1;                    \
+    *reinterpret_cast<int4*>(&lds[BUF][SS][LY::off(I8_TM + 64 + grow, gunit)]) = V2;               \
+  }
+#define AJ_I8_GLOAD(HALF, K0)                                 \
+  {                                                           \
+    AJ_I8_LD1(pA0, pB0, pC0, (HALF) * H0 + 0, K0)             \
+    if (H0 > 1) { AJ_I8_LD1(pA1, pB1, pC1, (HALF) * H0 + 1, K0) } \
+  }
+#define AJ_I8_LSTORE(HALF, BUF)                               \
+  {                                                           \
+    AJ_I8_ST1(pA0, pB0, pC0, (HALF) * H0 + 0, BUF)            \
+    if (H0 > 1) { AJ_I8_ST1(pA1, pB1, pC1, (HALF) * H0 + 1, BUF) } \
+  }
+
+  i16v acc[NG][2];
+  const int64_t nstage = (int64_t)nchunks * (FC / I8_KS);
+  constexpr int SPC = FC / I8_KS;  // stages per chunk
+  AJ_I8_GLOAD(0, 0)
+  AJ_I8_LSTORE(0, 0)
+  AJ_I8_GLOAD(1, 0)
+  AJ_I8_LSTORE(1, 0)
+  __syncthreads();
+  for (int64_t st = 0; st < nstage; ++st) {
+    if (st % SPC == 0) {
+#pragma unroll
+      for (int g = 0; g < NG; ++g)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[g][b] = i16v{};
+    }
+    const int buf = (int)(st & 1);
+    const bool more = st + 1 < nstage;
+    const int64_t knext = (st + 1) * I8_KS;
+    // v_mfma_i32_32x32x32_i8: lane l holds A[row l%32][k 16*(l/32) .. +16) and B likewise; the
+    // wave's 32x64 block is two 32x32 blocks; a 64-byte stage row is two k-steps of 32.
+    const int lr32 = lane & 31, lh = lane >> 5;
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+      if (more) { AJ_I8_GLOAD(half, knext) }
+      __builtin_amdgcn_sched_barrier(0);
+      const int ks = half;  // k-step of this half
+      i4 fa[NS], fb[2][NS];
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        fa[s] = *reinterpret_cast<const i4*>(&lds[buf][s][LY::off(wr + lr32, 2 * ks + lh)]);
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+          fb[b][s] = *reinterpret_cast<const i4*>(&lds[buf][s][LY::off(I8_TM + wc + 32 * b + lr32, 2 * ks + lh)]);
+      }
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int p = 0; p < NS; ++p)
+#pragma unroll
+          for (int q = 0; q < NS; ++q)
+            if (p + q < NG) acc[p + q][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[p], fb[b][q], acc[p + q][b], 0, 0, 0);
+      __builtin_amdgcn_sched_barrier(0);
+      if (more) {
+        if (buf) { AJ_I8_LSTORE(half, 0) } else { AJ_I8_LSTORE(half, 1) }
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+#undef AJ_I8_GLOAD
+#undef AJ_I8_LSTORE
+#undef AJ_I8_LD1
+#undef AJ_I8_ST1
+    if (st % SPC == SPC - 1) {
+      const double q = qmax_of(NS);
+      const int c = (int)(st / SPC);
+      const double s2 = chunk_sc[c] * chunk_sc[c] / (q * q);
+      // an opaque copy of M keeps the 32 output addresses from being hoisted out of the
+      // stage loop (where they would occupy registers the prefetch needs)
+      int64_t Mo = M;
+      asm volatile("" : "+s"(Mo));
+      // accumulator register r of lane l holds row 8(r/4) + 4(l/32) + r%4, column l%32
+      double* Wt = W + (I0 + wr + 4 * lh) * Mo + (J0 + wc + lr32);
+      const int64_t rlim = M - (I0 + wr + 4 * lh), clim = M - (J0 + wc + lr32);
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int ro = 8 * (r >> 2) + (r & 3);
+          double v = 0.0;
+#pragma unroll
+          for (int g = 0; g < NG; ++g) v = v * 256.0 + (double)acc[g][b][r];
+#pragma unroll
+          for (int g = NG; g < NFULL; ++g) v *= 256.0;
+          if (ro < rlim && 32 * b < clim) Wt[ro * Mo + 32 * b] += v * s2;
+          __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Integer SYRK, two waves per SIMD: the same 64x128 workgroup tile split over
+// 8 waves of 32x32 (2x2 MFMA 16x16 blocks), so each wave holds half the
+// accumulators of syrk_i8_kernel and a second wave on the SIMD covers the
+// LDS-read latency of the first.  Register prefetch of the next stage in two
+// halves, LDS rows of 64 bytes with 16-byte units XOR-swizzled by (row/2)%4.
+// ---------------------------------------------------------------------------
+template <int NS, int NG>
+__global__ void __launch_bounds__(512, 1) syrk_i8_w8_kernel(const int8_t* __restrict__ X, int64_t M, int64_t Mpad,
+                                                            int64_t ldx, int nchunks,
+                                                            const double* __restrict__ chunk_sc,
+                                                            double* __restrict__ W, int64_t ntiles,
+                                                            const int* __restrict__ tiles) {
+  constexpr int NFULL = 2 * NS - 1;
+  constexpr int ROWS = I8_TM + I8_TN;
+  using LY = I8Layout<1>;
+  __shared__ __attribute__((aligned(16))) int8_t lds[2][NS][ROWS * LY::pitch];
+
+  const int64_t per = (ntiles + 7) / 8;
+  const int64_t t = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (t >= ntiles) return;
+  const int64_t I0 = (int64_t)tiles[2 * t] * I8_TM, J0 = (int64_t)tiles[2 * t + 1] * I8_TN;
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = (wave >> 2) * 32, wc = (wave & 3) * 32;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int64_t sstride = Mpad * ldx;
+
+  // thread t moves 16 bytes of B row t/4 and, for t < 256, of A row t/4
+  const int grow = tid >> 2, gunit = tid & 3, gseg = gunit * 16;
+  const bool hasA = tid < 256;
+  const int8_t* srcA = X + (I0 + (grow & 63)) * ldx + gseg;
+  const int8_t* srcB = X + (J0 + grow) * ldx + gseg;
+  constexpr int H0 = (NS + 1) / 2;
+  int4 pa0, pb0, pa1, pb1;
+#define AJ_W8_LD1(VA, VB, SS, K0)                                                   \
+  if ((SS) < NS) {                                                                  \
+    if (hasA) VA = *reinterpret_cast<const int4*>(srcA + (int64_t)(SS) * sstride + (K0)); \
+    VB = *reinterpret_cast<const int4*>(srcB + (int64_t)(SS) * sstride + (K0));    \
+  }
+#define AJ_W8_ST1(VA, VB, SS, BUF)                                                          \
+  if ((SS) < NS) {                                                                          \
+    if (hasA) *reinterpret_cast<int4*>(&lds[BUF][SS][LY::off(grow, gunit)]) = VA;           \
+    *reinterpret_cast<int4*>(&lds[BUF][SS][LY::off(I8_TM + grow, gunit)]) = VB;             \
+  }
+#define AJ_W8_GLOAD(HALF, K0)                                       \
+  {                                                                 \
+    AJ_W8_LD1(pa0, pb0, (HALF) * H0 + 0, K0)                        \
+    if (H0 > 1) { AJ_W8_LD1(pa1, pb1, (HALF) * H0 + 1, K0) }        \
+  }
+#define AJ_W8_LSTORE(HALF, BUF)                                     \
+  {                                                                 \
+    AJ_W8_ST1(pa0, pb0, (HALF) * H0 + 0, BUF)                       \
+    if (H0 > 1) { AJ_W8_ST1(pa1, pb1, (HALF) * H0 + 1, BUF) }       \
+  }
+
+  i4 acc[NG][2][2];
+  const int64_t nstage = (int64_t)nchunks * (FC / I8_KS);
+  constexpr int SPC = FC / I8_KS;
+  AJ_W8_GLOAD(0, 0)
+  AJ_W8_LSTORE(0, 0)
+  AJ_W8_GLOAD(1, 0)
+  AJ_W8_LSTORE(1, 0)
+  __syncthreads();
+  for (int64_t st = 0; st < nstage; ++st) {
+    if (st % SPC == 0) {
+#pragma unroll
+      for (int g = 0; g < NG; ++g)
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b) acc[g][a][b] = i4{0, 0, 0, 0};
+    }
+    const int buf = (int)(st & 1);
+    const bool more = st + 1 < nstage;
+    const int64_t knext = (st + 1) * I8_KS;
+    i4 fa[NS][2];
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+        fa[s][a] = *reinterpret_cast<const i4*>(&lds[buf][s][LY::off(wr + a * 16 + lr, lk)]);
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+      if (more) { AJ_W8_GLOAD(half, knext) }
+      __builtin_amdgcn_sched_barrier(0);
+      const int b = half;
+      i4 fb[NS];
+#pragma unroll
+      for (int s = 0; s < NS; ++s)
+        fb[s] = *reinterpret_cast<const i4*>(&lds[buf][s][LY::off(I8_TM + wc + b * 16 + lr, lk)]);
+#pragma unroll
+      for (int p = 0; p < NS; ++p)
+#pragma unroll
+        for (int q = 0; q < NS; ++q)
+          if (p + q < NG)
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+              acc[p + q][a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[p][a], fb[q], acc[p + q][a][b], 0, 0, 0);
+      __builtin_amdgcn_sched_barrier(0);
+      if (more) {
+        if (buf) { AJ_W8_LSTORE(half, 0) } else { AJ_W8_LSTORE(half, 1) }
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    if (st % SPC == SPC - 1) {
+      const double qm = qmax_of(NS);
+      const int c = (int)(st / SPC);
+      const double s2 = chunk_sc[c] * chunk_sc[c] / (qm * qm);
+      int64_t Mo = M;
+      asm volatile("" : "+s"(Mo));
+      double* Wt = W + (I0 + wr + lk * 4) * Mo + (J0 + wc + lr);
+      const int64_t rlim = M - (I0 + wr + lk * 4), clim = M - (J0 + wc + lr);
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            double v = 0.0;
+#pragma unroll
+            for (int g = 0; g < NG; ++g) v = v * 256.0 + (double)acc[g][a][b][r];
+#pragma unroll
+            for (int g = NG; g < NFULL; ++g) v *= 256.0;
+            if (a * 16 + r < rlim && b * 16 < clim) Wt[(a * 16 + r) * Mo + b * 16] += v * s2;
+            __builtin_amdgcn_sched_barrier(0);
+          }
+    }
+    __syncthreads();
+  }
+#undef AJ_W8_GLOAD
+#undef AJ_W8_LSTORE
+#undef AJ_W8_LD1
+#undef AJ_W8_ST1
+}
+
+// ---------------------------------------------------------------------------
+// Integer SYRK, pipelined variant: the same tiles and arithmetic as
+// syrk_i8_kernel, with the feature tiles moved by direct global->LDS loads
+// (global_load_lds, 16 bytes per lane, lane i of a wave writes base + 16 i)
+// into two LDS stages, the next stage issued under the MFMA work of the
+// current one, so no registers hold prefetched data.  64-byte LDS rows with the 16-byte
+// units XOR-swizzled by (row/2)%4 (free of bank conflicts for the fragment
+// reads); the swizzle is applied by choosing each lane's source address.
+// ---------------------------------------------------------------------------
+__device__ inline void dma16(const int8_t* g, int8_t* lds_wave_base) {
+  __builtin_amdgcn_global_load_lds((const void*)g,
+                                   (void __attribute__((address_space(3)))*)(
+                                       (__attribute__((address_space(3))) int8_t*)lds_wave_base),
+                                   16, 0, 0);
+}
+
+template <int NS, int NG>
+__global__ void __launch_bounds__(256, 1) syrk_i8_dma_kernel(const int8_t* __restrict__ X, int64_t M, int64_t Mpad,
+                                                             int64_t ldx, int nchunks,
+                                                             const double* __restrict__ chunk_sc,
+                                                             double* __restrict__ W, int64_t ntiles,
+                                                             const int* __restrict__ tiles) {
+  constexpr int NFULL = 2 * NS - 1;  // orders 0 .. 2NS-2
+  constexpr int SLICE = (I8_TM + I8_TN) * 64;  // bytes of one slice of one stage
+  __shared__ __attribute__((aligned(1024))) int8_t L0[NS * SLICE];
+  __shared__ __attribute__((aligned(1024))) int8_t L1[NS * SLICE];
+
+  const int64_t per = (ntiles + 7) / 8;
+  const int64_t t = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (t >= ntiles) return;
+  const int64_t I0 = (int64_t)tiles[2 * t] * I8_TM, J0 = (int64_t)tiles[2 * t + 1] * I8_TN;
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 64;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int64_t sstride = Mpad * ldx;
+
+  // DMA assignment: the 192 stage rows form 12 blocks of 16 rows (1 KB per slice); wave w moves
+  // blocks w, w+4, w+8.  Lane L fills LDS slot L of its block: row (L/4), stored unit L%4, which
+  // holds global unit (L%4) ^ ((row/2)%4).
+  const int8_t* src[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const int rb = wave + 4 * j;
+    const int row = rb * 16 + (lane >> 2);
+    const int64_t grow = rb < 4 ? I0 + row : J0 + (row - I8_TM);
+    const int u = (lane & 3) ^ ((row >> 1) & 3);
+    src[j] = X + grow * ldx + 16 * u;
+  }
+#define AJ_DMA_ISSUE(BUF, K0)                                                                  \
+  {                                                                                           \
+    const int64_t dk_ = (K0); /* evaluated once, outside the loops below */                   \
+    _Pragma("unroll") for (int dsl_ = 0; dsl_ < NS; ++dsl_)                                   \
+    _Pragma("unroll") for (int dj_ = 0; dj_ < 3; ++dj_)                                       \
+      dma16(src[dj_] + dsl_ * sstride + dk_, BUF + dsl_ * SLICE + (wave + 4 * dj_) * 1024);   \
+  }
+
+  i4 acc[NG][2][4];
+  const int64_t nstage = (int64_t)nchunks * (FC / I8_KS);
+  constexpr int SPC = FC / I8_KS;
+
+  AJ_DMA_ISSUE(L0, 0)
+  __builtin_amdgcn_s_waitcnt(0);  // all counters: own DMA landed in LDS
+
+  // Stage S from CUR; the DMA of stage S+1 into NXT is issued after all fragment reads of CUR and
+  // runs under the MFMA work.  Each wave waits for its own DMA before the next barrier, so after the
+  // barrier every wave's part of NXT is in LDS.
+#define AJ_DMA_STEP(CUR, NXT, S)                                                                           \
+  if ((S) < nstage) {                                                                                      \
+    const int64_t s_ = (S);                                                                                \
+    /* nothing is in flight here (each wave waited for its own DMA), so the fence costs nothing */         \
+    __syncthreads();                                                                                       \
+    if (s_ % SPC == 0) {                                                                                   \
+      _Pragma("unroll") for (int g = 0; g < NG; ++g)                                                       \
+      _Pragma("unroll") for (int a = 0; a < 2; ++a)                                                        \
+      _Pragma("unroll") for (int b = 0; b < 4; ++b) acc[g][a][b] = i4{0, 0, 0, 0};                         \
+    }                                                                                                      \
+    i4 fa[NS][2], fb[4][NS];                                                                               \
+    _Pragma("unroll") for (int s = 0; s < NS; ++s)                                                         \
+    _Pragma("unroll") for (int a = 0; a < 2; ++a) {                                                        \
+      const int r = wr + a * 16 + lr;                                                                      \
+      fa[s][a] = *reinterpret_cast<const i4*>(&CUR[s * SLICE + r * 64 + 16 * (lk ^ ((r >> 1) & 3))]);     \
+    }                                                                                                      \
+    _Pragma("unroll") for (int b = 0; b < 4; ++b)                                                          \
+    _Pragma("unroll") for (int s = 0; s < NS; ++s) {                                                       \
+      const int r = I8_TM + wc + b * 16 + lr;                                                              \
+      fb[b][s] = *reinterpret_cast<const i4*>(&CUR[s * SLICE + r * 64 + 16 * (lk ^ ((r >> 1) & 3))]);     \
+    }                                                                                                      \
+    __builtin_amdgcn_sched_barrier(0);                                                                     \
+    if (s_ + 1 < nstage) { AJ_DMA_ISSUE(NXT, (s_ + 1) * I8_KS) }                                           \
+    __builtin_amdgcn_sched_barrier(0);                                                                     \
+    _Pragma("unroll") for (int b = 0; b < 4; ++b)                                                          \
+    _Pragma("unroll") for (int p = 0; p < NS; ++p)                                                         \
+    _Pragma("unroll") for (int q = 0; q < NS; ++q)                                                         \
+    if (p + q < NG)                                                                                        \
+    _Pragma("unroll") for (int a = 0; a < 2; ++a)                                                          \
+      acc[p + q][a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[p][a], fb[b][q], acc[p + q][a][b], 0, 0, 0); \
+    if (s_ % SPC == SPC - 1) {                                                                             \
+      const double qm = qmax_of(NS);                                                                       \
+      const int c = (int)(s_ / SPC);                                                                       \
+      const double s2 = chunk_sc[c] * chunk_sc[c] / (qm * qm);                                             \
+      int64_t Mo = M;                                                                                      \
+      asm volatile("" : "+s"(Mo));                                                                         \
+      double* Wt = W + (I0 + wr + lk * 4) * Mo + (J0 + wc + lr);                                           \
+      const int64_t rlim = M - (I0 + wr + lk * 4), clim = M - (J0 + wc + lr);                              \
+      _Pragma("unroll") for (int a = 0; a < 2; ++a)                                                        \
+      _Pragma("unroll") for (int b = 0; b < 4; ++b)                                                        \
+      _Pragma("unroll") for (int r = 0; r < 4; ++r) {                                                      \
+        double v = 0.0;                                                                                    \
+        _Pragma("unroll") for (int g = 0; g < NG; ++g) v = v * 256.0 + (double)acc[g][a][b][r];            \
+        _Pragma("unroll") for (int g = NG; g < NFULL; ++g) v *= 256.0;                                     \
+        if (a * 16 + r < rlim && b * 16 < clim) Wt[(a * 16 + r) * Mo + b * 16] += v * s2;                  \
+        __builtin_amdgcn_sched_barrier(0);                                                                 \
+      }                                                                                                    \
+    }                                                                                                      \
+    __builtin_amdgcn_sched_barrier(0);                                                                     \
+    __builtin_amdgcn_s_waitcnt(0); /* all counters: own DMA of stage S+1 landed in LDS */                   \
+  }
+
+  for (int64_t st = 0; st < nstage; st += 2) {
+    AJ_DMA_STEP(L0, L1, st)
+    AJ_DMA_STEP(L1, L0, st + 1)
+  }
+#undef AJ_DMA_STEP
+#undef AJ_DMA_ISSUE
 }
 
 // Copy the lower triangle to the upper one (32x32 tiles through LDS).
@@ -592,6 +1020,9 @@ __global__ void mirror_kernel(double* __restrict__ W, int64_t M) {
   }
 }
 
+// LDS layout variant of the integer SYRK (aj_w_tilde_set_variant; chosen by the autotuner).
+static int g_i8_layout = 0;
+
 struct Plan {
   int mode, ns, es;
   int64_t Mpad, ldx;
@@ -599,16 +1030,23 @@ struct Plan {
   size_t feat_bytes, bytes;
 };
 
+// Upper bound on the feature matrix of one SYRK launch (bytes).  Every tile row and column streams
+// the launch's features, so this is the re-read working set: at 256 MiB it stays within the 256 MB
+// memory-side (infinity) cache and the re-reads do not go to HBM (1 GiB groups were 11% slower for
+// the 4-digit integer SYRK at M = 5000).
+constexpr size_t GROUP_BYTES_DEFAULT = size_t(256) << 20;
+static size_t g_group_bytes = GROUP_BYTES_DEFAULT;
+
 inline Plan make_plan(int64_t K, int64_t M, int mode) {
   Plan p{};
   p.mode = mode;
   p.ns = slices_of(mode);
   p.es = esize_of(mode);
-  const int64_t pad = (mode == AJ_PREC_INT8X3 || mode == AJ_PREC_INT8X4) ? 2 * TILE : TILE;
+  const int64_t pad = is_i8(mode) ? 2 * TILE : TILE;
   p.Mpad = ceil_div(M, pad) * pad;
   const int64_t nch = std::max<int64_t>(1, ceil_div(K, KC));
   const double per_chunk = (double)p.ns * p.Mpad * FC * p.es;
-  int64_t g = (int64_t)((1ll << 30) / per_chunk);
+  int64_t g = (int64_t)((double)g_group_bytes / per_chunk);
   g = std::max<int64_t>(1, std::min<int64_t>(g, nch));
   p.group = (int)g;
   p.ldx = (int64_t)p.group * FC;
@@ -626,7 +1064,7 @@ int run(const double* noise, const double* uv, const double* grid, int64_t K, in
   double* sc = ar.take<double>(p.group);
   int* tiles = nullptr;
   int64_t nt8 = 0;
-  if (MODE == AJ_PREC_INT8X3 || MODE == AJ_PREC_INT8X4) {
+  if (is_i8(MODE)) {
     static thread_local std::vector<int> tl;
     build_i8_tiles(p.Mpad, tl);
     nt8 = (int64_t)tl.size() / 2;
@@ -647,11 +1085,11 @@ int run(const double* noise, const double* uv, const double* grid, int64_t K, in
   const int64_t nt = p.Mpad / TILE;
   const int64_t ntiles = nt * (nt + 1) / 2;
   int first = 1;
-  if (MODE == AJ_PREC_INT8X3 || MODE == AJ_PREC_INT8X4) AJ_CHECK_HIP(hipMemsetAsync(W, 0, sizeof(double) * M * M, st));
+  if (is_i8(MODE)) AJ_CHECK_HIP(hipMemsetAsync(W, 0, sizeof(double) * M * M, st));
   for (int64_t c0 = 0; c0 < nch; c0 += p.group) {
     const int ng = (int)std::min<int64_t>(p.group, nch - c0);
     const int64_t k_begin = c0 * KC;
-    if (MODE == AJ_PREC_INT8X3 || MODE == AJ_PREC_INT8X4) {
+    if (is_i8(MODE)) {
       hipLaunchKernelGGL(chunk_scale_kernel, dim3(ng), dim3(256), 0, st, noise, K, k_begin, ng, sc);
       AJ_LAUNCH_CHECK();
     }
@@ -662,10 +1100,20 @@ int run(const double* noise, const double* uv, const double* grid, int64_t K, in
     timer_end(AJ_TIMER_WT_FEATURES, st);
     AJ_LAUNCH_CHECK();
     timer_begin(AJ_TIMER_WT_SYRK, st);
-    if constexpr (MODE == AJ_PREC_INT8X3 || MODE == AJ_PREC_INT8X4) {
-      constexpr int NSL = Mode<MODE>::NS;
-      hipLaunchKernelGGL(syrk_i8_kernel<NSL>, dim3((unsigned)(8 * ((nt8 + 7) / 8))), dim3(256), 0, st,
-                         reinterpret_cast<const int8_t*>(X), M, p.Mpad, p.ldx, ng, sc, W, nt8, tiles);
+    if constexpr (is_i8(MODE)) {
+      constexpr int NSL = Mode<MODE>::NS, NGL = Mode<MODE>::NG;
+      const dim3 g8((unsigned)(8 * ((nt8 + 7) / 8)));
+      const int8_t* X8 = reinterpret_cast<const int8_t*>(X);
+      switch (g_i8_layout) {
+        case 1: hipLaunchKernelGGL((syrk_i8_kernel<NSL, NGL, 1>), g8, dim3(256), 0, st, X8, M, p.Mpad, p.ldx, ng, sc, W, nt8, tiles); break;
+        case 2: hipLaunchKernelGGL((syrk_i8_kernel<NSL, NGL, 2>), g8, dim3(256), 0, st, X8, M, p.Mpad, p.ldx, ng, sc, W, nt8, tiles); break;
+        case 3: hipLaunchKernelGGL((syrk_i8_kernel<NSL, NGL, 3>), g8, dim3(256), 0, st, X8, M, p.Mpad, p.ldx, ng, sc, W, nt8, tiles); break;
+        case 4: hipLaunchKernelGGL((syrk_i8_kernel<NSL, NGL, 4>), g8, dim3(256), 0, st, X8, M, p.Mpad, p.ldx, ng, sc, W, nt8, tiles); break;
+        case 7: hipLaunchKernelGGL((syrk_i8_w8_kernel<NSL, NGL>), g8, dim3(512), 0, st, X8, M, p.Mpad, p.ldx, ng, sc, W, nt8, tiles); break;
+        case 6: hipLaunchKernelGGL((syrk_i8_k32_kernel<NSL, NGL>), g8, dim3(256), 0, st, X8, M, p.Mpad, p.ldx, ng, sc, W, nt8, tiles); break;
+        case 5: hipLaunchKernelGGL((syrk_i8_dma_kernel<NSL, NGL>), g8, dim3(256), 0, st, X8, M, p.Mpad, p.ldx, ng, sc, W, nt8, tiles); break;
+        default: hipLaunchKernelGGL((syrk_i8_kernel<NSL, NGL, 0>), g8, dim3(256), 0, st, X8, M, p.Mpad, p.ldx, ng, sc, W, nt8, tiles); break;
+      }
     } else {
       hipLaunchKernelGGL(syrk_kernel<MODE>, dim3((unsigned)ntiles), dim3(256), 0, st, X, M, p.Mpad, p.ldx, ng, sc, W,
                          first);
@@ -709,4 +1157,18 @@ extern "C" int aj_w_tilde_curvature_interferometer_from(const double* noise_map_
     case AJ_PREC_INT8X3: return wt::run<AJ_PREC_INT8X3>(noise_map_real, uv_wavelengths, grid_radians_slim, K, M, w_tilde, p, ar, st);
     default: return wt::run<AJ_PREC_INT8X4>(noise_map_real, uv_wavelengths, grid_radians_slim, K, M, w_tilde, p, ar, st);
   }
+}
+
+extern "C" int aj_w_tilde_set_variant(int variant) {
+  AJ_REQUIRE(variant >= 0 && variant < wt::I8_NLAYOUT, "variant %d outside [0, %d)", variant, wt::I8_NLAYOUT);
+  wt::g_i8_layout = variant;
+  return AJ_OK;
+}
+
+extern "C" int aj_w_tilde_variant_count(void) { return wt::I8_NLAYOUT; }
+
+extern "C" int aj_w_tilde_set_group_bytes(size_t bytes) {
+  AJ_REQUIRE(bytes == 0 || bytes >= (size_t(1) << 20), "group bytes %zu below 1 MiB", bytes);
+  wt::g_group_bytes = bytes ? bytes : wt::GROUP_BYTES_DEFAULT;
+  return AJ_OK;
 }
