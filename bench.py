@@ -6,12 +6,20 @@ Delaunay source pixels each (own mesh, own barycentric mapping, B = 3) and
 lambda log-uniform in [0.01, 100].  Synthetic data (datasets.config5).
 
 One step = build W-tilde for the dataset + log-evidence of all 64 samples:
-  mixed  : W-tilde at --precision (default int8x3: exact-integer MFMA on
-           3 x 8-bit digits of FP32 cos/sin), then one batched evaluation.
+  mixed  : W-tilde at --precision (default int8x5: exact int32 MFMA on five
+           signed 8-bit digits of FP64 cos/sin features, FP64 flush per
+           4096-visibility chunk; kernel variant picked by the autotuner before
+           the timed region), then one batched evaluation of all samples.
   plain  : W-tilde in FP64 (FP64 MFMA), then 64 sequential single-sample
            evaluations.
 Both use FP64 for F, H, Cholesky, solves and log-determinants.
 value = log-evidence evaluations / s of the mixed path (64 per step).
+
+roofline: the dominant kernel is the W-tilde SYRK; its algorithmic work is
+2 M^2 K flops per W-tilde (lower triangle, cos and sin terms), timed with HIP
+events on the library's stream around every SYRK launch.  "traffic" (HBM
+bytes per launch) comes from the FETCH_SIZE / WRITE_SIZE counter summary in
+profiles/ when it is present for this kernel and workload.
 
 python bench.py [--gpus 1] [--steps 3] [--warmup 1]
 """
@@ -32,12 +40,44 @@ for p in (ROOT, ROOT / "python-autojax_amd"):
 
 import numpy as np  # noqa: E402
 
-PEAKS_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0, "int8x3": 5000.0, "int8x4": 5000.0}
-RAW_FACTOR = {"fp64": 1, "fp32": 1, "bf16x3": 3, "int8x3": 9, "int8x4": 15}
+PEAKS_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0, "int8x3": 5000.0, "int8x4": 5000.0, "int8x5": 5000.0}
+RAW_FACTOR = {"fp64": 1, "fp32": 1, "bf16x3": 3, "int8x3": 9, "int8x4": 15, "int8x5": 22}
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+I8_KERNELS = {0: "syrk_i8_kernel<{ns},{ng},0>", 1: "syrk_i8_kernel<{ns},{ng},1>", 2: "syrk_i8_kernel<{ns},{ng},2>",
+              3: "syrk_i8_kernel<{ns},{ng},3>", 4: "syrk_i8_kernel<{ns},{ng},4>", 5: "syrk_i8_dma_kernel<{ns},{ng}>",
+              6: "syrk_i8_k32_kernel<{ns},{ng}>", 7: "syrk_i8_w8_kernel<{ns},{ng}>"}
+I8_DIGITS = {"int8x3": (3, 5), "int8x4": (4, 6), "int8x5": (5, 7)}
+
+
+def syrk_kernel_name(precision, variant):
+    if precision not in I8_DIGITS:
+        return f"syrk_kernel<{precision}>"
+    ns, ng = I8_DIGITS[precision]
+    if variant is None:
+        variant = 7 if ns == 5 else 0
+    return I8_KERNELS[variant].format(ns=ns, ng=ng)
+
+
+def counter_traffic(precision, M, kernel):
+    """HBM bytes per SYRK launch from the committed counter summary for this precision and size
+    (profiles/counters_w_tilde_syrk_<precision>_M<M>.md, lines 'kernel: <name>' and
+    'traffic_bytes_per_launch: <float>'), if it was measured on the same kernel; else None."""
+    p = ROOT / "profiles" / f"counters_w_tilde_syrk_{precision}_M{M}.md"
+    if not p.exists():
+        return None
+    fields = {}
+    for line in p.read_text().splitlines():
+        key, sep, val = line.strip().strip("`").partition(":")
+        if sep and key in ("kernel", "traffic_bytes_per_launch"):
+            fields[key] = val.strip().strip("`")
+    if fields.get("kernel") != kernel or "traffic_bytes_per_launch" not in fields:
+        return None
+    return float(fields["traffic_bytes_per_launch"])
 
 
 def cpu_baseline(ds, k_sub, n_like, threads):
@@ -74,7 +114,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", default="int8x3", choices=list(PEAKS_TFLOPS))
+    ap.add_argument("--precision", default="int8x5", choices=list(PEAKS_TFLOPS))
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default SYRK kernel variant")
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--samples", type=int, default=64)
     ap.add_argument("--K", type=int, default=1_000_000)
@@ -111,6 +152,13 @@ def main():
     ws_plain = dv.workspace(_lib.query("aj_w_tilde_workspace_bytes", K, M, 0))
     torch.cuda.synchronize()
     log("inputs resident")
+    tuned = {}
+    if not args.no_autotune:
+        from autojax_amd import autotune
+
+        tuned = autotune.tune_w_tilde(ds.grid_radians_slim, args.precision)
+        if tuned:
+            log("autotune: " + ", ".join(f"v{v}={t:.2f}ms" for v, t in sorted(tuned.items())))
 
     def mixed_step():
         E.w_tilde(noise, uv, grid, args.precision, out=W_mixed, workspace=ws_mixed)
@@ -145,7 +193,7 @@ def main():
     mixed_ev = out_mixed.cpu().numpy()
     plain_ev = out_plain.cpu().numpy()
     dlogl = float(np.abs(mixed_ev - plain_ev).max())
-    info_bad = int(batch.info.any().item())
+    info_bad = int(batch.info.any(dim=1).sum().item())
     n = len(ds.samples)
     value = n / (ms_mixed / 1e3)
     plain_value = n / (ms_plain / 1e3)
@@ -155,6 +203,7 @@ def main():
     algo_flops_step = 2.0 * M * M * K  # lower-triangle SYRK count, cos and sin terms
     achieved = algo_flops_step * args.steps / (syrk_ms / 1e3) / 1e12
     peak = PEAKS_TFLOPS[args.precision]
+    kernel = syrk_kernel_name(args.precision, min(tuned, key=tuned.get) if tuned else None)
     wt_ms = (tm["wt_syrk"][0] + tm["wt_features"][0]) / args.steps
     batch_ms = ms_mixed - wt_ms
 
@@ -180,6 +229,7 @@ def main():
                        "w_tilde_ms": round((tp["wt_syrk"][0] + tp["wt_features"][0]) / args.plain_steps, 3)},
         "speedup_vs_plain": round(value / plain_value, 3),
         "max_abs_dlogL_vs_plain": dlogl,
+        "within_1e-3": bool(dlogl <= 1e-3),
         "nonpd_samples": info_bad,
         "breakdown_ms": {"w_tilde_syrk": round(tm["wt_syrk"][0] / args.steps, 3),
                          "w_tilde_features": round(tm["wt_features"][0] / args.steps, 3),
@@ -189,11 +239,12 @@ def main():
                          "solve": round(tm["solve"][0] / args.steps, 3),
                          "batch_total": round(batch_ms, 3)},
         "evals_per_s_excluding_w_tilde": round(n / (batch_ms / 1e3), 3),
-        "roofline": {"bound": "mfma", "kernel": f"syrk_kernel<{args.precision}>", "achieved": round(achieved, 3),
+        "roofline": {"bound": "mfma", "kernel": kernel, "achieved": round(achieved, 3),
                      "peak": peak, "unit": "TFLOP/s", "frac": round(achieved / peak, 5),
                      "raw_mfma_frac": round(achieved * RAW_FACTOR[args.precision] / peak, 5),
                      "avg_launch_ms": round(syrk_ms / max(syrk_launches, 1), 4), "launches_per_step": syrk_launches // args.steps,
-                     "traffic": None},
+                     "traffic": counter_traffic(args.precision, M, kernel)},
+        "autotune_ms": {str(v): round(t, 3) for v, t in sorted(tuned.items())},
     }
     if not args.no_cpu_baseline:
         ds.dirty_image = y.cpu().numpy()

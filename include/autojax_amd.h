@@ -37,6 +37,7 @@ extern "C" {
 #define AJ_PREC_BF16X3 2 /* split-BF16 features, 3 products, FP32 accumulation per chunk */
 #define AJ_PREC_INT8X3 3 /* 3 signed 8-bit digits per feature, exact int32 accumulation */
 #define AJ_PREC_INT8X4 4 /* 4 signed 8-bit digits per feature, exact int32 accumulation */
+#define AJ_PREC_INT8X5 5 /* 5 signed 8-bit digits per feature, exact int32 accumulation */
 
 const char* aj_last_error(void);
 int aj_abi_version(void);
@@ -65,7 +66,8 @@ int aj_w_tilde_curvature_interferometer_from(const double* noise_map_real, const
 
 
 /* Tuning of the W-tilde build, process-wide, no reference counterpart.
- * aj_w_tilde_set_variant: kernel variant of the integer-mode SYRK (default 0).
+ * aj_w_tilde_set_variant: kernel variant of the integer-mode SYRK, 0 .. count-1, or -1 (default)
+ *   for the fastest measured variant of each digit count.
  * aj_w_tilde_set_group_bytes: cap on the feature matrix of one SYRK launch (0 = default 256 MiB);
  * changes aj_w_tilde_workspace_bytes, so query the workspace after setting it. */
 int aj_w_tilde_set_variant(int variant);

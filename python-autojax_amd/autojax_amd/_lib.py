@@ -16,7 +16,7 @@ import numpy as np
 LIB_PATH = Path(__file__).resolve().parent / "lib" / "libautojax_amd.so"
 
 AJ_OK, AJ_EINVAL, AJ_ENOTPD, AJ_EHIP, AJ_ESPACE = 0, 1, 2, 3, 4
-PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x3": 2, "int8x3": 3, "int8x4": 4}
+PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x3": 2, "int8x3": 3, "int8x4": 4, "int8x5": 5}
 
 _P = C.c_void_p
 _I = C.c_int64

@@ -131,7 +131,7 @@ def w_tilde_curvature_interferometer_from(noise_map_real, uv_wavelengths, grid_r
     """numba.py:155-207 -> f8[M, M].
 
     ``precision``: "fp64" (default, the reference's arithmetic); "int8x3",
-    "int8x4" (exact integer MFMA on 3 or 4 signed 8-bit digits of FP64
+    "int8x4", "int8x5" (exact integer MFMA on 3, 4 or 5 signed 8-bit digits of FP64
     cos/sin features, all digit products that reach FP64); "fp32" or
     "bf16x3" (FP32 cos/sin, FP32-accumulated MFMA).
     """

@@ -55,14 +55,19 @@ template <> struct Mode<AJ_PREC_INT8X3> {
 template <> struct Mode<AJ_PREC_INT8X4> {
   using E = int8_t; static constexpr int NS = 4, MK = 64, NG = 6;
 };
+template <> struct Mode<AJ_PREC_INT8X5> {
+  using E = int8_t; static constexpr int NS = 5, MK = 64, NG = 7;
+};
 __host__ __device__ constexpr bool is_i8(int mode) {
-  return mode == AJ_PREC_INT8X3 || mode == AJ_PREC_INT8X4;
+  return mode == AJ_PREC_INT8X3 || mode == AJ_PREC_INT8X4 || mode == AJ_PREC_INT8X5;
 }
 
 // Digit-order groups kept by the integer SYRK kernels (Mode<>::NG: products a_p*b_q with p+q < NG).
 //   int8x3  : all orders 0..4 of 3 digits (9 products).
 //   int8x4  : orders 0..5 of 4 digits (15 products); the single order-6 product a3*b3 (weight 256^-6
 //             of the leading term, about 1e-17 relative) is below the FP64 rounding of the sum.
+//   int8x5  : orders 0..6 of 5 digits (22 products); orders 7 and 8 (weight 256^-7 and less) are
+//             below the FP64 rounding.
 // Dropping whole orders above the FP64 rounding (e.g. keeping orders 0..3 of 4 digits) leaves W-tilde
 // accurate to ~1e-9 but no longer the Gram matrix of any feature matrix: it can lose positive
 // semi-definiteness, and F + H then fails to factor at small regularization coefficients.
@@ -72,6 +77,7 @@ inline int slices_of(int mode) {
     case AJ_PREC_BF16X3: return 2;
     case AJ_PREC_INT8X3: return 3;
     case AJ_PREC_INT8X4: return 4;
+    case AJ_PREC_INT8X5: return 5;
     default: return 1;
   }
 }
@@ -494,8 +500,10 @@ __global__ void __launch_bounds__(256, 1) syrk_i8_kernel(const int8_t* __restric
   // the MFMA work of the current stage, so only half of the slices sit in registers at a time.
   // (All indices below are compile-time constants after unrolling, so pf stays in registers.)
   constexpr int H0 = (NS + 1) / 2;
-  // six named registers (no array: an array here is turned into LDS storage by the compiler)
-  int4 pA0, pB0, pC0, pA1, pB1, pC1;
+  // named registers, up to three slices per half (no array: an array here is turned into LDS
+  // storage by the compiler)
+  static_assert(H0 <= 3, "prefetch holds at most three digit slices per half stage");
+  int4 pA0, pB0, pC0, pA1, pB1, pC1, pA2, pB2, pC2;
 #define AJ_I8_LD1(V0, V1, V2, SS, K0)                                              \
   if ((SS) < NS) {                                                                 \
     V0 = *reinterpret_cast<const int4*>(srcA + (int64_t)(SS) * sstride + (K0));    \
@@ -512,11 +520,13 @@ __global__ void __launch_bounds__(256, 1) syrk_i8_kernel(const int8_t* __restric
   {                                                           \
     AJ_I8_LD1(pA0, pB0, pC0, (HALF) * H0 + 0, K0)             \
     if (H0 > 1) { AJ_I8_LD1(pA1, pB1, pC1, (HALF) * H0 + 1, K0) } \
+    if (H0 > 2) { AJ_I8_LD1(pA2, pB2, pC2, (HALF) * H0 + 2, K0) } \
   }
 #define AJ_I8_LSTORE(HALF, BUF)                               \
   {                                                           \
     AJ_I8_ST1(pA0, pB0, pC0, (HALF) * H0 + 0, BUF)            \
     if (H0 > 1) { AJ_I8_ST1(pA1, pB1, pC1, (HALF) * H0 + 1, BUF) } \
+    if (H0 > 2) { AJ_I8_ST1(pA2, pB2, pC2, (HALF) * H0 + 2, BUF) } \
   }
 
   i4 acc[NG][2][4];
@@ -635,8 +645,10 @@ __global__ void __launch_bounds__(256, 1) syrk_i8_k32_kernel(const int8_t* __res
   // the MFMA work of the current stage, so only half of the slices sit in registers at a time.
   // (All indices below are compile-time constants after unrolling, so pf stays in registers.)
   constexpr int H0 = (NS + 1) / 2;
-  // six named registers (no array: an array here is turned into LDS storage by the compiler)
-  int4 pA0, pB0, pC0, pA1, pB1, pC1;
+  // named registers, up to three slices per half (no array: an array here is turned into LDS
+  // storage by the compiler)
+  static_assert(H0 <= 3, "prefetch holds at most three digit slices per half stage");
+  int4 pA0, pB0, pC0, pA1, pB1, pC1, pA2, pB2, pC2;
 #define AJ_I8_LD1(V0, V1, V2, SS, K0)                                              \
   if ((SS) < NS) {                                                                 \
     V0 = *reinterpret_cast<const int4*>(srcA + (int64_t)(SS) * sstride + (K0));    \
@@ -653,11 +665,13 @@ __global__ void __launch_bounds__(256, 1) syrk_i8_k32_kernel(const int8_t* __res
   {                                                           \
     AJ_I8_LD1(pA0, pB0, pC0, (HALF) * H0 + 0, K0)             \
     if (H0 > 1) { AJ_I8_LD1(pA1, pB1, pC1, (HALF) * H0 + 1, K0) } \
+    if (H0 > 2) { AJ_I8_LD1(pA2, pB2, pC2, (HALF) * H0 + 2, K0) } \
   }
 #define AJ_I8_LSTORE(HALF, BUF)                               \
   {                                                           \
     AJ_I8_ST1(pA0, pB0, pC0, (HALF) * H0 + 0, BUF)            \
     if (H0 > 1) { AJ_I8_ST1(pA1, pB1, pC1, (HALF) * H0 + 1, BUF) } \
+    if (H0 > 2) { AJ_I8_ST1(pA2, pB2, pC2, (HALF) * H0 + 2, BUF) } \
   }
 
   i16v acc[NG][2];
@@ -774,7 +788,8 @@ __global__ void __launch_bounds__(512, 1) syrk_i8_w8_kernel(const int8_t* __rest
   const int8_t* srcA = X + (I0 + (grow & 63)) * ldx + gseg;
   const int8_t* srcB = X + (J0 + grow) * ldx + gseg;
   constexpr int H0 = (NS + 1) / 2;
-  int4 pa0, pb0, pa1, pb1;
+  static_assert(H0 <= 3, "prefetch holds at most three digit slices per half stage");
+  int4 pa0, pb0, pa1, pb1, pa2, pb2;
 #define AJ_W8_LD1(VA, VB, SS, K0)                                                   \
   if ((SS) < NS) {                                                                  \
     if (hasA) VA = *reinterpret_cast<const int4*>(srcA + (int64_t)(SS) * sstride + (K0)); \
@@ -789,11 +804,13 @@ __global__ void __launch_bounds__(512, 1) syrk_i8_w8_kernel(const int8_t* __rest
   {                                                                 \
     AJ_W8_LD1(pa0, pb0, (HALF) * H0 + 0, K0)                        \
     if (H0 > 1) { AJ_W8_LD1(pa1, pb1, (HALF) * H0 + 1, K0) }        \
+    if (H0 > 2) { AJ_W8_LD1(pa2, pb2, (HALF) * H0 + 2, K0) }        \
   }
 #define AJ_W8_LSTORE(HALF, BUF)                                     \
   {                                                                 \
     AJ_W8_ST1(pa0, pb0, (HALF) * H0 + 0, BUF)                       \
     if (H0 > 1) { AJ_W8_ST1(pa1, pb1, (HALF) * H0 + 1, BUF) }       \
+    if (H0 > 2) { AJ_W8_ST1(pa2, pb2, (HALF) * H0 + 2, BUF) }       \
   }
 
   i4 acc[NG][2][2];
@@ -1021,7 +1038,10 @@ __global__ void mirror_kernel(double* __restrict__ W, int64_t M) {
 }
 
 // LDS layout variant of the integer SYRK (aj_w_tilde_set_variant; chosen by the autotuner).
-static int g_i8_layout = 0;
+// Integer SYRK kernel variant; -1 selects the default per digit count: the 8-wave kernel (7) for 5
+// digits, whose 4-wave forms need more than 512 registers per lane, else variant 0.
+static int g_i8_layout = -1;
+constexpr int default_i8_variant(int ns) { return ns == 5 ? 7 : 0; }
 
 struct Plan {
   int mode, ns, es;
@@ -1104,11 +1124,18 @@ int run(const double* noise, const double* uv, const double* grid, int64_t K, in
       constexpr int NSL = Mode<MODE>::NS, NGL = Mode<MODE>::NG;
       const dim3 g8((unsigned)(8 * ((nt8 + 7) / 8)));
       const int8_t* X8 = reinterpret_cast<const int8_t*>(X);
-      switch (g_i8_layout) {
+      switch (g_i8_layout >= 0 ? g_i8_layout : default_i8_variant(NSL)) {
         case 1: hipLaunchKernelGGL((syrk_i8_kernel<NSL, NGL, 1>), g8, dim3(256), 0, st, X8, M, p.Mpad, p.ldx, ng, sc, W, nt8, tiles); break;
         case 2: hipLaunchKernelGGL((syrk_i8_kernel<NSL, NGL, 2>), g8, dim3(256), 0, st, X8, M, p.Mpad, p.ldx, ng, sc, W, nt8, tiles); break;
         case 3: hipLaunchKernelGGL((syrk_i8_kernel<NSL, NGL, 3>), g8, dim3(256), 0, st, X8, M, p.Mpad, p.ldx, ng, sc, W, nt8, tiles); break;
-        case 4: hipLaunchKernelGGL((syrk_i8_kernel<NSL, NGL, 4>), g8, dim3(256), 0, st, X8, M, p.Mpad, p.ldx, ng, sc, W, nt8, tiles); break;
+        case 4:
+          if constexpr (2 * NSL * (I8_TM + I8_TN) * I8Layout<4>::pitch <= 160 * 1024) {
+            hipLaunchKernelGGL((syrk_i8_kernel<NSL, NGL, 4>), g8, dim3(256), 0, st, X8, M, p.Mpad, p.ldx, ng, sc, W, nt8, tiles);
+          } else {
+            set_error("SYRK variant 4 does not fit in LDS with %d digit slices", NSL);
+            return AJ_EINVAL;
+          }
+          break;
         case 7: hipLaunchKernelGGL((syrk_i8_w8_kernel<NSL, NGL>), g8, dim3(512), 0, st, X8, M, p.Mpad, p.ldx, ng, sc, W, nt8, tiles); break;
         case 6: hipLaunchKernelGGL((syrk_i8_k32_kernel<NSL, NGL>), g8, dim3(256), 0, st, X8, M, p.Mpad, p.ldx, ng, sc, W, nt8, tiles); break;
         case 5: hipLaunchKernelGGL((syrk_i8_dma_kernel<NSL, NGL>), g8, dim3(256), 0, st, X8, M, p.Mpad, p.ldx, ng, sc, W, nt8, tiles); break;
@@ -1134,7 +1161,7 @@ int run(const double* noise, const double* uv, const double* grid, int64_t K, in
 using namespace aj;
 
 extern "C" size_t aj_w_tilde_workspace_bytes(int64_t K, int64_t M, int precision) {
-  if (K < 0 || M <= 0 || precision < AJ_PREC_FP64 || precision > AJ_PREC_INT8X4) return 0;
+  if (K < 0 || M <= 0 || precision < AJ_PREC_FP64 || precision > AJ_PREC_INT8X5) return 0;
   return wt::make_plan(K, M, precision).bytes;
 }
 
@@ -1143,7 +1170,7 @@ extern "C" int aj_w_tilde_curvature_interferometer_from(const double* noise_map_
                                                         int precision, double* w_tilde, void* workspace,
                                                         size_t workspace_bytes, void* stream) {
   AJ_REQUIRE(M > 0 && K >= 0, "bad sizes M=%lld K=%lld", (long long)M, (long long)K);
-  AJ_REQUIRE(precision >= AJ_PREC_FP64 && precision <= AJ_PREC_INT8X4, "bad precision %d", precision);
+  AJ_REQUIRE(precision >= AJ_PREC_FP64 && precision <= AJ_PREC_INT8X5, "bad precision %d", precision);
   AJ_REQUIRE(w_tilde && grid_radians_slim && (K == 0 || (noise_map_real && uv_wavelengths)), "null pointer");
   wt::Plan p = wt::make_plan(K, M, precision);
   AJ_REQUIRE(workspace_bytes >= p.bytes && workspace, "workspace too small: need %zu bytes, got %zu", p.bytes,
@@ -1155,12 +1182,13 @@ extern "C" int aj_w_tilde_curvature_interferometer_from(const double* noise_map_
     case AJ_PREC_FP32: return wt::run<AJ_PREC_FP32>(noise_map_real, uv_wavelengths, grid_radians_slim, K, M, w_tilde, p, ar, st);
     case AJ_PREC_BF16X3: return wt::run<AJ_PREC_BF16X3>(noise_map_real, uv_wavelengths, grid_radians_slim, K, M, w_tilde, p, ar, st);
     case AJ_PREC_INT8X3: return wt::run<AJ_PREC_INT8X3>(noise_map_real, uv_wavelengths, grid_radians_slim, K, M, w_tilde, p, ar, st);
-    default: return wt::run<AJ_PREC_INT8X4>(noise_map_real, uv_wavelengths, grid_radians_slim, K, M, w_tilde, p, ar, st);
+    case AJ_PREC_INT8X4: return wt::run<AJ_PREC_INT8X4>(noise_map_real, uv_wavelengths, grid_radians_slim, K, M, w_tilde, p, ar, st);
+    default: return wt::run<AJ_PREC_INT8X5>(noise_map_real, uv_wavelengths, grid_radians_slim, K, M, w_tilde, p, ar, st);
   }
 }
 
 extern "C" int aj_w_tilde_set_variant(int variant) {
-  AJ_REQUIRE(variant >= 0 && variant < wt::I8_NLAYOUT, "variant %d outside [0, %d)", variant, wt::I8_NLAYOUT);
+  AJ_REQUIRE(variant >= -1 && variant < wt::I8_NLAYOUT, "variant %d outside [-1, %d)", variant, wt::I8_NLAYOUT);
   wt::g_i8_layout = variant;
   return AJ_OK;
 }
