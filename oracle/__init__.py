@@ -21,7 +21,7 @@ Contents
     C/OpenMP restatement of ``numba.py:197-207`` (the W-tilde loop), used as
     the timed CPU baseline and for sub-sampled checks at full grid sizes.
 
-Parity status (what pins each function; see tests/test_oracle_*.py)
+Parity status (what pins each function; tests/test_oracle.py)
 -------------------------------------------------------------------
 w_tilde_curvature_interferometer_from   brute-force fsum triple loop, closed
                                         forms (diag = sum w, K=1 rank-2),
