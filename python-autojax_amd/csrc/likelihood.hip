@@ -267,11 +267,11 @@ __global__ void regularize_kernel(const SampleDesc* __restrict__ sd, const int64
 }
 
 // ---------------------------------------------------------------------------
-// Batched FP64 Cholesky, one workgroup (256 threads) per matrix, in place on
-// the lower triangle.  Right-looking, 32-wide panels; the trailing update of
-// each 64x64 tile runs on v_mfma_f64_16x16x4 with the panel staged in LDS.
-// info[b] = 0 or the 1-based column of the first non-positive pivot.
-// logdet[b] = 2 sum log L_jj (+ deflation correction when beta > 0).
+// FP64 Cholesky of one matrix by one workgroup (256 threads), in place on the
+// lower triangle (the single-matrix solve entry point).  Right-looking, 32-wide
+// panels; the trailing update runs on v_mfma_f64_16x16x4, each wave on its own
+// 32x32 blocks.  *info_out = 0 or the 1-based column of the first non-positive
+// pivot; *logdet_out = 2 sum log L_jj.
 // ---------------------------------------------------------------------------
 constexpr int NB = 32;
 
@@ -395,23 +395,202 @@ __device__ void chol_one(double* __restrict__ A, int64_t n, double* logdet_out, 
   }
 }
 
-// grid: 2 * n blocks, block b factorises matrix (b & 1) (0 = A, 1 = H) of sample b / 2,
-// ordered so that the largest samples start first.
-__global__ void __launch_bounds__(256) chol_batched_kernel(const SampleDesc* __restrict__ sd,
-                                                           const int* __restrict__ order, double* __restrict__ mats,
-                                                           double* __restrict__ logdet, int64_t* __restrict__ info) {
-  const int smp = order[blockIdx.x >> 1];
-  const int which_h = blockIdx.x & 1;
-  const SampleDesc d = sd[smp];
-  double* A = mats + d.mat_off + (which_h ? d.S * d.S : 0);
-  double ld;
-  int64_t inf;
-  chol_one(A, d.S, &ld, &inf);
-  if (threadIdx.x == 0) {
-    if (which_h && d.beta > 0.0) ld += log(1e-8) - log(1e-8 + d.beta);
-    logdet[2 * smp + which_h] = ld;
-    info[2 * smp + which_h] = inf;
+
+
+// ---------------------------------------------------------------------------
+// Step-synchronous blocked Cholesky over a set of matrices of different sizes
+// (every A and H of a batch, or one matrix).  Step k handles columns
+// k0 = 32k .. k0+31 of every matrix with n > k0, in three launches:
+//   chol_diag_kernel   one workgroup per matrix: factor the 32x32 diagonal block in LDS,
+//                      record sum log L_jj and the first non-positive pivot of the block;
+//   chol_panel_kernel  256-row blocks of every matrix: X L_kk^T = A[r0:, k0:k0+32];
+//   chol_trail_kernel  64x64 lower-triangle tiles of every matrix: A22 -= P P^T on
+//                      v_mfma_f64_16x16x4 with the two panel row blocks staged in LDS.
+// One grid spans all matrices, so the device is filled by the many tiles of a batch
+// or of one large matrix.  chol_finish_kernel sums the per-step partials in step order
+// (deterministic).  The factor is left in place in the lower triangle.
+// ---------------------------------------------------------------------------
+struct StepEntry {
+  int mat;     // matrix j: sample j >> 1, A (j & 1 == 0) or H
+  int pfx_p;   // panel workgroups before this matrix in the step's grid
+  int pfx_t;   // trailing tiles before this matrix in the step's grid
+};
+
+__device__ inline int find_entry(const StepEntry* __restrict__ e, int ne, int blk, bool trail) {
+  int lo = 0, hi = ne - 1;  // largest i with prefix(i) <= blk
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if ((trail ? e[mid].pfx_t : e[mid].pfx_p) <= blk) lo = mid; else hi = mid - 1;
   }
+  return lo;
+}
+
+__device__ inline double* mat_ptr(const SampleDesc* __restrict__ sd, double* __restrict__ mats, int j, int64_t& n) {
+  const SampleDesc d = sd[j >> 1];
+  n = d.S;
+  return mats + d.mat_off + ((j & 1) ? d.S * d.S : 0);
+}
+
+__global__ void __launch_bounds__(256) chol_diag_kernel(const SampleDesc* __restrict__ sd,
+                                                        const StepEntry* __restrict__ e, int64_t k0, int kstep,
+                                                        int maxsteps, double* __restrict__ mats,
+                                                        double* __restrict__ ldp, int64_t* __restrict__ failp) {
+  __shared__ double Lkk[NB][NB + 1];
+  __shared__ int fail;
+  __shared__ double ld_acc;
+  const int j = e[blockIdx.x].mat;
+  int64_t n;
+  double* A = mat_ptr(sd, mats, j, n);
+  const int nb = (int)std::min<int64_t>(NB, n - k0);
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    fail = 0;
+    ld_acc = 0.0;
+  }
+  for (int x = tid; x < NB * NB; x += 256) {
+    const int i = x / NB, c = x % NB;
+    Lkk[i][c] = (i < nb && c <= i) ? A[(k0 + i) * n + k0 + c] : 0.0;
+  }
+  __syncthreads();
+  for (int c = 0; c < nb; ++c) {
+    if (tid == 0) {
+      double dj = Lkk[c][c];
+      if (!(dj > 0.0)) {
+        if (!fail) fail = (int)(k0 + c + 1);
+        dj = 1.0;  // keep the arithmetic finite; the result is flagged
+      }
+      dj = sqrt(dj);
+      Lkk[c][c] = dj;
+      ld_acc += log(dj);
+    }
+    __syncthreads();
+    if (tid > c && tid < nb) Lkk[tid][c] /= Lkk[c][c];
+    __syncthreads();
+    for (int x = tid; x < NB * NB; x += 256) {
+      const int i = x / NB, l = x % NB;
+      if (i > c && l > c && l <= i && i < nb) Lkk[i][l] -= Lkk[i][c] * Lkk[l][c];
+    }
+    __syncthreads();
+  }
+  for (int x = tid; x < NB * NB; x += 256) {
+    const int i = x / NB, c = x % NB;
+    if (i < nb && c <= i) A[(k0 + i) * n + k0 + c] = Lkk[i][c];
+  }
+  if (tid == 0) {
+    ldp[(int64_t)j * maxsteps + kstep] = ld_acc;
+    failp[(int64_t)j * maxsteps + kstep] = fail;
+  }
+}
+
+__global__ void __launch_bounds__(256) chol_panel_kernel(const SampleDesc* __restrict__ sd,
+                                                         const StepEntry* __restrict__ e, int ne, int64_t k0,
+                                                         double* __restrict__ mats) {
+  __shared__ double Lkk[NB][NB + 1];
+  __shared__ double Xs[256][NB + 1];
+  const int ei = find_entry(e, ne, (int)blockIdx.x, false);
+  const int j = e[ei].mat;
+  const int rb = (int)blockIdx.x - e[ei].pfx_p;
+  int64_t n;
+  double* A = mat_ptr(sd, mats, j, n);
+  const int nb = (int)std::min<int64_t>(NB, n - k0);
+  const int tid = threadIdx.x;
+  for (int x = tid; x < NB * NB; x += 256) {
+    const int i = x / NB, c = x % NB;
+    Lkk[i][c] = (i < nb && c <= i) ? A[(k0 + i) * n + k0 + c] : 0.0;
+  }
+  __syncthreads();
+  const int64_t i = k0 + nb + (int64_t)rb * 256 + tid;
+  if (i < n) {
+    double* row = A + i * n + k0;
+    for (int c = 0; c < nb; ++c) Xs[tid][c] = row[c];
+    for (int c = 0; c < nb; ++c) {
+      double v = Xs[tid][c];
+      for (int l = 0; l < c; ++l) v -= Xs[tid][l] * Lkk[c][l];
+      Xs[tid][c] = v / Lkk[c][c];
+    }
+    for (int c = 0; c < nb; ++c) row[c] = Xs[tid][c];
+  }
+}
+
+constexpr int TT = 64;  // trailing tile
+
+__global__ void __launch_bounds__(256) chol_trail_kernel(const SampleDesc* __restrict__ sd,
+                                                         const StepEntry* __restrict__ e, int ne, int64_t k0,
+                                                         double* __restrict__ mats) {
+  __shared__ double PI[TT][NB + 1];
+  __shared__ double PJ[TT][NB + 1];
+  const int ei = find_entry(e, ne, (int)blockIdx.x, true);
+  const int j = e[ei].mat;
+  const int64_t t = (int64_t)blockIdx.x - e[ei].pfx_t;
+  int64_t n;
+  double* A = mat_ptr(sd, mats, j, n);
+  const int nb = (int)std::min<int64_t>(NB, n - k0);
+  const int64_t r0 = k0 + nb;
+  int64_t bi = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+  while (bi * (bi + 1) / 2 > t) --bi;
+  while ((bi + 1) * (bi + 2) / 2 <= t) ++bi;
+  const int64_t bj = t - bi * (bi + 1) / 2;
+  const int64_t R0 = r0 + bi * TT, C0 = r0 + bj * TT;
+  const int tid = threadIdx.x;
+  for (int x = tid; x < TT * NB; x += 256) {
+    const int i = x / NB, p = x % NB;
+    PI[i][p] = (R0 + i < n && p < nb) ? -A[(R0 + i) * n + k0 + p] : 0.0;
+    PJ[i][p] = (C0 + i < n && p < nb) ? A[(C0 + i) * n + k0 + p] : 0.0;
+  }
+  __syncthreads();
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wi = wave >> 1, wj = wave & 1;
+  if (bi == bj && wj > wi) return;  // above the diagonal of a diagonal tile
+  const int lr = lane & 15, lk = lane >> 4;
+  d4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t gi = R0 + wi * 32 + a * 16 + lk + 4 * r, gj = C0 + wj * 32 + b * 16 + lr;
+        acc[a][b][r] = (gi < n && gj <= gi) ? A[gi * n + gj] : 0.0;
+      }
+#pragma unroll
+  for (int ks = 0; ks < NB / 4; ++ks) {
+    const int p = ks * 4 + lk;
+    const double fa0 = PI[wi * 32 + lr][p], fa1 = PI[wi * 32 + 16 + lr][p];
+    const double fb0 = PJ[wj * 32 + lr][p], fb1 = PJ[wj * 32 + 16 + lr][p];
+    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa0, fb0, acc[0][0], 0, 0, 0);
+    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa0, fb1, acc[0][1], 0, 0, 0);
+    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa1, fb0, acc[1][0], 0, 0, 0);
+    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa1, fb1, acc[1][1], 0, 0, 0);
+  }
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t gi = R0 + wi * 32 + a * 16 + lk + 4 * r, gj = C0 + wj * 32 + b * 16 + lr;
+        if (gi < n && gj <= gi) A[gi * n + gj] = acc[a][b][r];
+      }
+}
+
+// logdet[j] = 2 sum_k ldp[j][k] (+ the deflation correction of H), info[j] = first failing column.
+__global__ void chol_finish_kernel(const SampleDesc* __restrict__ sd, int nmat, int maxsteps,
+                                   const double* __restrict__ ldp, const int64_t* __restrict__ failp,
+                                   double* __restrict__ logdet, int64_t* __restrict__ info) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nmat) return;
+  const SampleDesc d = sd[j >> 1];
+  const int steps = (int)((d.S + NB - 1) / NB);
+  double s = 0.0;
+  int64_t f = 0;
+  for (int k = 0; k < steps; ++k) {
+    s += ldp[(int64_t)j * maxsteps + k];
+    if (!f) f = failp[(int64_t)j * maxsteps + k];
+  }
+  s *= 2.0;
+  if ((j & 1) && d.beta > 0.0) s += log(1e-8) - log(1e-8 + d.beta);
+  logdet[j] = s;
+  info[j] = f;
 }
 
 // ---------------------------------------------------------------------------
@@ -509,8 +688,9 @@ __global__ void noise_norm_kernel(const double* __restrict__ noise, int64_t K2, 
 // Host-side planning of the batch workspace.
 // ---------------------------------------------------------------------------
 struct BatchPlan {
-  int64_t n, M, B, sumS, sumS2, omega_elems, csc_ptr_elems, csc_ent_elems;
-  size_t off_desc, off_order, off_ptr, off_cur, off_key, off_t, off_omega, off_mats, off_D, off_logdet, off_info, bytes;
+  int64_t n, M, B, sumS, sumS2, omega_elems, csc_ptr_elems, csc_ent_elems, maxsteps, n_entries;
+  size_t off_desc, off_ptr, off_cur, off_key, off_t, off_omega, off_mats, off_D, off_logdet, off_info,
+      off_steps, off_ldp, off_failp, bytes;
 };
 
 inline BatchPlan plan_batch(int64_t M, int64_t B, int64_t n, const int64_t* S_host) {
@@ -521,6 +701,8 @@ inline BatchPlan plan_batch(int64_t M, int64_t B, int64_t n, const int64_t* S_ho
   for (int64_t b = 0; b < n; ++b) {
     p.sumS += S_host[b];
     p.sumS2 += S_host[b] * S_host[b];
+    p.maxsteps = std::max<int64_t>(p.maxsteps, ceil_div(S_host[b], (int64_t)NB));
+    p.n_entries += 2 * ceil_div(S_host[b], (int64_t)NB);
   }
   p.omega_elems = M * p.sumS;
   p.csc_ptr_elems = p.sumS + n;
@@ -533,7 +715,6 @@ inline BatchPlan plan_batch(int64_t M, int64_t B, int64_t n, const int64_t* S_ho
     return r;
   };
   p.off_desc = put(sizeof(SampleDesc) * n);
-  p.off_order = put(sizeof(int) * n);
   p.off_ptr = put(sizeof(int) * p.csc_ptr_elems);
   p.off_cur = put(sizeof(int) * p.csc_ptr_elems);
   p.off_key = put(sizeof(int) * p.csc_ent_elems);
@@ -543,6 +724,9 @@ inline BatchPlan plan_batch(int64_t M, int64_t B, int64_t n, const int64_t* S_ho
   p.off_D = put(sizeof(double) * p.sumS);
   p.off_logdet = put(sizeof(double) * 2 * n);
   p.off_info = put(sizeof(int64_t) * 2 * n);
+  p.off_steps = put(sizeof(StepEntry) * p.n_entries);
+  p.off_ldp = put(sizeof(double) * 2 * n * p.maxsteps);
+  p.off_failp = put(sizeof(int64_t) * 2 * n * p.maxsteps);
   p.bytes = align_up(o, 256);
   return p;
 }
@@ -597,8 +781,6 @@ int batch_pipeline(const double* w_tilde, int64_t M, const double* dirty_image, 
   ho.resize(n);
   for (int64_t b = 0; b < n; ++b) ho[b] = (int)b;
   std::stable_sort(ho.begin(), ho.end(), [&](int x, int y) { return S_host[x] > S_host[y]; });
-  int* order = reinterpret_cast<int*>(ws + p.off_order);
-  AJ_CHECK_HIP(hipMemcpyAsync(order, ho.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
   AJ_CHECK_HIP(hipMemsetAsync(ptrs, 0, sizeof(int) * p.csc_ptr_elems, st));
 
   const unsigned gm = (unsigned)std::min<int64_t>(ceil_div(M, 256), 1024);
@@ -639,8 +821,56 @@ int batch_pipeline(const double* w_tilde, int64_t M, const double* dirty_image, 
   hipLaunchKernelGGL(regularize_kernel, dim3(gs, (unsigned)n), dim3(256), 0, st, desc, neighbors, neighbors_sizes, P,
                      mats, 1, 1, 1);
   AJ_LAUNCH_CHECK();
+  // Step tables of the blocked Cholesky: matrices in decreasing size, per step the ones with
+  // columns left, with prefix counts of panel workgroups and trailing tiles.
+  static thread_local std::vector<StepEntry> he;
+  static thread_local std::vector<int64_t> step_off, step_cnt, grid_p, grid_t;
+  he.clear();
+  step_off.assign(p.maxsteps, 0);
+  step_cnt.assign(p.maxsteps, 0);
+  grid_p.assign(p.maxsteps, 0);
+  grid_t.assign(p.maxsteps, 0);
+  for (int64_t k = 0; k < p.maxsteps; ++k) {
+    const int64_t k0 = k * NB;
+    step_off[k] = (int64_t)he.size();
+    for (int64_t q = 0; q < 2 * n; ++q) {
+      const int j = 2 * ho[q >> 1] + (int)(q & 1);
+      const int64_t S = S_host[j >> 1];
+      if (S <= k0) continue;
+      const int64_t m = S - std::min<int64_t>(S, k0 + NB);  // rows below the diagonal block
+      const int64_t tb = ceil_div(m, (int64_t)TT);
+      AJ_REQUIRE(grid_p[k] + ceil_div(m, (int64_t)256) < (1ll << 31) && grid_t[k] + tb * (tb + 1) / 2 < (1ll << 31),
+                 "batch too large for one Cholesky step grid");
+      he.push_back(StepEntry{j, (int)grid_p[k], (int)grid_t[k]});
+      grid_p[k] += ceil_div(m, (int64_t)256);
+      grid_t[k] += tb * (tb + 1) / 2;
+    }
+    step_cnt[k] = (int64_t)he.size() - step_off[k];
+  }
+  AJ_REQUIRE((int64_t)he.size() == p.n_entries, "internal: step table size %zu != %lld", he.size(),
+             (long long)p.n_entries);
+  StepEntry* steps = reinterpret_cast<StepEntry*>(ws + p.off_steps);
+  double* ldp = reinterpret_cast<double*>(ws + p.off_ldp);
+  int64_t* failp = reinterpret_cast<int64_t*>(ws + p.off_failp);
+  AJ_CHECK_HIP(hipMemcpyAsync(steps, he.data(), sizeof(StepEntry) * he.size(), hipMemcpyHostToDevice, st));
   timer_begin(AJ_TIMER_CHOL, st);
-  hipLaunchKernelGGL(chol_batched_kernel, dim3((unsigned)(2 * n)), dim3(256), 0, st, desc, order, mats, logdet, info);
+  for (int64_t k = 0; k < p.maxsteps; ++k) {
+    const StepEntry* e = steps + step_off[k];
+    const int ne = (int)step_cnt[k];
+    hipLaunchKernelGGL(chol_diag_kernel, dim3((unsigned)ne), dim3(256), 0, st, desc, e, k * NB, (int)k,
+                       (int)p.maxsteps, mats, ldp, failp);
+    AJ_LAUNCH_CHECK();
+    if (grid_p[k] > 0) {
+      hipLaunchKernelGGL(chol_panel_kernel, dim3((unsigned)grid_p[k]), dim3(256), 0, st, desc, e, ne, k * NB, mats);
+      AJ_LAUNCH_CHECK();
+    }
+    if (grid_t[k] > 0) {
+      hipLaunchKernelGGL(chol_trail_kernel, dim3((unsigned)grid_t[k]), dim3(256), 0, st, desc, e, ne, k * NB, mats);
+      AJ_LAUNCH_CHECK();
+    }
+  }
+  hipLaunchKernelGGL(chol_finish_kernel, dim3((unsigned)ceil_div(2 * n, (int64_t)256)), dim3(256), 0, st, desc,
+                     (int)(2 * n), (int)p.maxsteps, ldp, failp, logdet, info);
   timer_end(AJ_TIMER_CHOL, st);
   AJ_LAUNCH_CHECK();
   timer_begin(AJ_TIMER_SOLVE, st);
