@@ -90,6 +90,12 @@ inline int esize_of(int mode) {
   }
 }
 
+// s_waitcnt immediate that waits for vmcnt <= n and leaves expcnt and lgkmcnt unconstrained
+// (gfx9 encoding: vmcnt[3:0] and [15:14], expcnt [6:4], lgkmcnt [11:8]).
+constexpr unsigned aj_vmcnt_imm(unsigned n) { return (n & 0xF) | ((n >> 4) << 14) | (0x7u << 4) | (0xFu << 8); }
+// s_waitcnt immediate that waits for lgkmcnt == 0 only (vmcnt 63 and expcnt 7 unconstrained).
+constexpr unsigned aj_lgkmcnt0_imm() { return 0xFu | (0x3u << 14) | (0x7u << 4); }
+
 // Largest integer digit pattern for N signed 8-bit digits: 127 * 256^(N-1).
 __host__ __device__ inline double qmax_of(int ns) {
   double q = 127.0;
@@ -432,7 +438,7 @@ __global__ void __launch_bounds__(256) syrk_kernel(const void* __restrict__ Xv, 
 // which share row panels, land on the same XCD.
 // ---------------------------------------------------------------------------
 constexpr int I8_TM = 64, I8_TN = 128, I8_KS = 64;
-constexpr int I8_NLAYOUT = 8;  // 0-4: register prefetch with the LDS layouts below; 5: direct-to-LDS; 6: 32x32x32 MFMA; 7: 8 waves
+constexpr int I8_NLAYOUT = 9;  // 0-4: register prefetch with the LDS layouts below; 5: direct-to-LDS; 6: 32x32x32 MFMA; 7: 8 waves; 8: 8 waves, LDS ring
 
 // LDS placement of 16-byte unit u (0..3) of tile row r within one 64-byte stage row.
 //   0: rows padded to 80 bytes;  1: 64-byte rows, unit ^= (r/2)%4;  2: rows padded to 72 bytes;
@@ -1020,6 +1026,175 @@ __global__ void __launch_bounds__(256, 1) syrk_i8_dma_kernel(const int8_t* __res
 #undef AJ_DMA_ISSUE
 }
 
+// ---------------------------------------------------------------------------
+// Integer SYRK, software-pipelined: the 64x128 workgroup tile as 8 waves of
+// 32x32 (one v_mfma_i32_32x32x32_i8 block each, two waves per SIMD), K in
+// stages of 32 bytes, a ring of four LDS stages filled by direct global->LDS
+// loads (global_load_lds, 16 bytes per lane), and two register sets of
+// fragments: while the MFMAs of stage S run on one set, the LDS reads of
+// stage S+1 fill the other, so the LDS read burst that follows each barrier
+// overlaps MFMA work instead of idling the matrix units.  Stage S+3 is loaded
+// during stage S into the buffer of stage S-1, whose reads completed before
+// the MFMAs of stage S-1.  Each LDS stage holds, per digit slice, 192 rows x
+// 32 bytes as 16-byte units at position 2 r + (u ^ ((r >> 3) & 1)); the
+// swizzle is applied through each lane's source address.  The four buffers
+// are distinct LDS objects so that the compiler can tell the loads in flight
+// from the fragment reads.
+// ---------------------------------------------------------------------------
+constexpr int RG_KS = 32;                          // bytes of K per stage
+constexpr int RG_UNITS = (I8_TM + I8_TN) * 2;      // 16-byte units per slice per stage
+constexpr int RG_BLOCKS = RG_UNITS / 64;           // 1 KiB blocks per slice per stage (6)
+constexpr int RG_SLICE = RG_UNITS * 16;            // bytes per slice per stage
+
+__device__ inline int rg_pos(int r, int u) { return 2 * r + (u ^ ((r >> 3) & 1)); }
+
+template <int NS, int NG>
+__global__ void __launch_bounds__(512, 1) syrk_i8_ring_kernel(const int8_t* __restrict__ X, int64_t M, int64_t Mpad,
+                                                              int64_t ldx, int nchunks,
+                                                              const double* __restrict__ chunk_sc,
+                                                              double* __restrict__ W, int64_t ntiles,
+                                                              const int* __restrict__ tiles) {
+  constexpr int NFULL = 2 * NS - 1;
+  constexpr int NBLK = RG_BLOCKS * NS;            // 1 KiB loads per stage
+  constexpr int BPW = (NBLK + 7) / 8;             // loads per wave per stage (padded to equal counts)
+  constexpr int SPC = FC / RG_KS;                 // stages per chunk
+  static_assert(SPC % 4 == 0, "the stage loop is unrolled by the four ring buffers");
+  __shared__ __attribute__((aligned(1024))) int8_t R0[NS * RG_SLICE];
+  __shared__ __attribute__((aligned(1024))) int8_t R1[NS * RG_SLICE];
+  __shared__ __attribute__((aligned(1024))) int8_t R2[NS * RG_SLICE];
+  __shared__ __attribute__((aligned(1024))) int8_t R3[NS * RG_SLICE];
+  __shared__ __attribute__((aligned(1024))) int8_t RD[1024];  // target of the padding loads, never read
+
+  const int64_t per = (ntiles + 7) / 8;
+  const int64_t t = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (t >= ntiles) return;
+  const int64_t I0 = (int64_t)tiles[2 * t] * I8_TM, J0 = (int64_t)tiles[2 * t + 1] * I8_TN;
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = (wave >> 2) * 32, wc = (wave & 3) * 32;
+  const int lr32 = lane & 31, lh = lane >> 5;
+  const int64_t sstride = Mpad * ldx;
+  const int offa = 16 * rg_pos(wr + lr32, lh), offb = 16 * rg_pos(I8_TM + wc + lr32, lh);
+
+  // Load q of wave w moves block b = w + 8q (slice b / 6, 1 KiB block b % 6 of the slice); lane i
+  // fills LDS unit 64 (b % 6) + i of the slice, i.e. row r, unit u with rg_pos(r, u) = that unit.
+  // A wave with fewer real blocks repeats its first source into RD, so every wave issues BPW loads.
+  const int8_t* src[BPW];
+  int ldo[BPW];
+  bool real[BPW];
+#pragma unroll
+  for (int q = 0; q < BPW; ++q) {
+    const int b = wave + 8 * q;
+    real[q] = b < NBLK;
+    const int bb = real[q] ? b : wave;
+    const int sl = bb / RG_BLOCKS, pos = (bb % RG_BLOCKS) * 64 + lane;
+    const int r = pos >> 1;
+    const int u = (pos & 1) ^ ((r >> 3) & 1);
+    const int64_t grow = r < I8_TM ? I0 + r : J0 + (r - I8_TM);
+    src[q] = X + (int64_t)sl * sstride + grow * ldx + 16 * u;
+    ldo[q] = sl * RG_SLICE + (bb % RG_BLOCKS) * 1024;
+  }
+  const int64_t nstage = (int64_t)nchunks * SPC;
+  const int64_t klast = (nstage - 1) * RG_KS;
+
+#define AJ_RG_ISSUE(BUF, KOFF)                                                                            \
+  {                                                                                                       \
+    const int64_t ko_ = (KOFF) < klast ? (KOFF) : klast; /* past the end: a harmless in-range load */     \
+    _Pragma("unroll") for (int q_ = 0; q_ < BPW; ++q_) {                                                  \
+      int8_t* dst_ = real[q_] ? (BUF + ldo[q_]) : RD;                                                     \
+      __builtin_amdgcn_global_load_lds((const void*)(src[q_] + ko_),                                      \
+                                       (void __attribute__((address_space(3)))*)(                         \
+                                           (__attribute__((address_space(3))) int8_t*)dst_),              \
+                                       16, 0, 0);                                                         \
+    }                                                                                                     \
+  }
+#define AJ_RG_READ(BUF, FA, FB)                                                                           \
+  {                                                                                                       \
+    _Pragma("unroll") for (int s = 0; s < NS; ++s) {                                                      \
+      FA[s] = *reinterpret_cast<const i4*>(&BUF[s * RG_SLICE + offa]);                                    \
+      FB[s] = *reinterpret_cast<const i4*>(&BUF[s * RG_SLICE + offb]);                                    \
+    }                                                                                                     \
+  }
+  // Barrier without a memory fence (a fence would also wait for the loads still in flight).  Ordering
+  // comes from the explicit waits: a wave's loads of the stage about to be read have landed (vmcnt)
+  // before it arrives, and its reads of the buffer refilled after the barrier completed (lgkmcnt
+  // before its MFMAs) a stage earlier; the empty asm statements keep the compiler from moving
+  // memory accesses across the barrier.
+#define AJ_RG_BARRIER()                \
+  {                                    \
+    asm volatile("" ::: "memory");     \
+    __builtin_amdgcn_s_barrier();      \
+    asm volatile("" ::: "memory");     \
+  }
+
+  i16v acc[NG];
+  i4 fa0[NS], fb0[NS], fa1[NS], fb1[NS];
+  // prologue: loads of stages 0, 1, 2 into R0, R1, R2; fragments of stage 0 into set 0
+  AJ_RG_ISSUE(R0, 0)
+  AJ_RG_ISSUE(R1, RG_KS)
+  AJ_RG_ISSUE(R2, 2 * RG_KS)
+  __builtin_amdgcn_s_waitcnt(aj_vmcnt_imm(2 * BPW));
+  AJ_RG_BARRIER()
+  AJ_RG_READ(R0, fa0, fb0)
+
+  // Step S: own loads of stage S+1 landed (only stage S+2 may stay in flight), barrier, load stage
+  // S+3 into DMA_BUF (the buffer of stage S-1), read stage S+1 from NXT_BUF into set NXT, MFMA on
+  // set CUR (stage S).
+#define AJ_RG_STEP(S, NXT_BUF, DMA_BUF, FAC, FBC, FAN, FBN)                                             \
+  {                                                                                                    \
+    const int64_t s_ = (S);                                                                            \
+    __builtin_amdgcn_s_waitcnt(aj_vmcnt_imm(BPW));                                                     \
+    AJ_RG_BARRIER()                                                                                    \
+    /* set CUR was read a stage ago: wait for it now, so that only the reads of set NXT are in    */  \
+    /* flight at the MFMAs (two sets at once exceed what the LDS counter can track)                */  \
+    __builtin_amdgcn_s_waitcnt(aj_lgkmcnt0_imm());                                                     \
+    AJ_RG_ISSUE(DMA_BUF, (s_ + 3) * RG_KS)                                                             \
+    AJ_RG_READ(NXT_BUF, FAN, FBN)                                                                      \
+    __builtin_amdgcn_sched_barrier(0);                                                                 \
+    if (s_ % SPC == 0) {                                                                               \
+      _Pragma("unroll") for (int g = 0; g < NG; ++g) acc[g] = i16v{};                                  \
+    }                                                                                                  \
+    _Pragma("unroll") for (int p = 0; p < NS; ++p)                                                     \
+    _Pragma("unroll") for (int q = 0; q < NS; ++q)                                                     \
+      if (p + q < NG) acc[p + q] = __builtin_amdgcn_mfma_i32_32x32x32_i8(FAC[p], FBC[q], acc[p + q], 0, 0, 0); \
+    __builtin_amdgcn_sched_barrier(0);                                                                 \
+  }
+
+  for (int64_t st = 0; st < nstage; st += 4) {
+    AJ_RG_STEP(st, R1, R3, fa0, fb0, fa1, fb1)
+    AJ_RG_STEP(st + 1, R2, R0, fa1, fb1, fa0, fb0)
+    AJ_RG_STEP(st + 2, R3, R1, fa0, fb0, fa1, fb1)
+    AJ_RG_STEP(st + 3, R0, R2, fa1, fb1, fa0, fb0)
+    if ((st + 3) % SPC == SPC - 1) {  // end of a chunk: flush the exact digit-order sums into W
+      const double qm = qmax_of(NS);
+      const int c = (int)((st + 3) / SPC);
+      const double s2 = chunk_sc[c] * chunk_sc[c] / (qm * qm);
+      int64_t Mo = M;
+      asm volatile("" : "+s"(Mo));
+      // accumulator register r of lane l holds row 8(r/4) + 4(l/32) + r%4, column l%32
+      double* Wt = W + (I0 + wr + 4 * lh) * Mo + (J0 + wc + lr32);
+      const int64_t rlim = M - (I0 + wr + 4 * lh), clim = M - (J0 + wc + lr32);
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int ro = 8 * (r >> 2) + (r & 3);
+        double v = 0.0;
+#pragma unroll
+        for (int g = 0; g < NG; ++g) v = v * 256.0 + (double)acc[g][r];
+#pragma unroll
+        for (int g = NG; g < NFULL; ++g) v *= 256.0;
+        if (ro < rlim && 0 < clim) Wt[ro * Mo] += v * s2;
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+  }
+  // the loads issued past the last stage must land before the workgroup's LDS is released
+  __builtin_amdgcn_s_waitcnt(0);
+#undef AJ_RG_STEP
+#undef AJ_RG_BARRIER
+#undef AJ_RG_READ
+#undef AJ_RG_ISSUE
+}
+
 // Copy the lower triangle to the upper one (32x32 tiles through LDS).
 __global__ void mirror_kernel(double* __restrict__ W, int64_t M) {
   __shared__ double t[32][33];
@@ -1136,6 +1311,7 @@ int run(const double* noise, const double* uv, const double* grid, int64_t K, in
             return AJ_EINVAL;
           }
           break;
+        case 8: hipLaunchKernelGGL((syrk_i8_ring_kernel<NSL, NGL>), g8, dim3(512), 0, st, X8, M, p.Mpad, p.ldx, ng, sc, W, nt8, tiles); break;
         case 7: hipLaunchKernelGGL((syrk_i8_w8_kernel<NSL, NGL>), g8, dim3(512), 0, st, X8, M, p.Mpad, p.ldx, ng, sc, W, nt8, tiles); break;
         case 6: hipLaunchKernelGGL((syrk_i8_k32_kernel<NSL, NGL>), g8, dim3(256), 0, st, X8, M, p.Mpad, p.ldx, ng, sc, W, nt8, tiles); break;
         case 5: hipLaunchKernelGGL((syrk_i8_dma_kernel<NSL, NGL>), g8, dim3(256), 0, st, X8, M, p.Mpad, p.ldx, ng, sc, W, nt8, tiles); break;
