@@ -15,6 +15,12 @@ One step = build W-tilde for the dataset + log-evidence of all 64 samples:
 Both use FP64 for F, H, Cholesky, solves and log-determinants.
 value = log-evidence evaluations / s of the mixed path (64 per step).
 
+Also measured and reported beside it ("lattice_fp64"): the lattice path of
+the reference's own API for regular pixel grids (numba.py:210-255, :860-967):
+the N x (2N-1) table of distinct W-tilde values in exact FP64 (FP64 MFMA over
+per-visibility phasors), then the same batched evaluation reading W-tilde from
+that table.
+
 roofline: the dominant kernel is the W-tilde SYRK; its algorithmic work is
 2 M^2 K flops per W-tilde (lower triangle, cos and sin terms), timed with HIP
 events on the library's stream around every SYRK launch.  "traffic" (HBM
@@ -188,12 +194,31 @@ def main():
         log(f"{label}: {ms:.2f} ms/step; timers " + ", ".join(f"{k}={v[0] / steps:.2f}ms/{v[1]}" for k, v in timers.items()))
         return ms, out, timers
 
+    N = ds.n_side
+    native = dv.to_device(np.ascontiguousarray(ds.native_index_for_slim_index.astype(np.int64)))
+    wc = dv.empty((N, 2 * N - 1))
+    ws_wc = dv.workspace(_lib.query("aj_w_compact_workspace_bytes", K, N))
+
+    def lattice_step():
+        _lib.call("aj_w_compact_curvature_interferometer_from", noise.data_ptr(), uv.data_ptr(), K, N,
+                  float(ds.pixel_scale), wc.data_ptr(), ws_wc.data_ptr(), ws_wc.numel(), dv.stream_ptr())
+        return batch.log_evidence_compact(wc, native, y, c2nn)
+
+    def build_compact():
+        _lib.call("aj_w_compact_curvature_interferometer_from", noise.data_ptr(), uv.data_ptr(), K, N,
+                  float(ds.pixel_scale), wc.data_ptr(), ws_wc.data_ptr(), ws_wc.numel(), dv.stream_ptr())
+
     ms_mixed, out_mixed, tm = timed(mixed_step, args.steps, args.warmup, "mixed")
+    mixed_ev = out_mixed.cpu().numpy()  # the batch's output buffer is reused by the lattice step
+    info_bad = int(batch.info.any(dim=1).sum().item())
+    ms_lat, out_lat, tl = timed(lattice_step, args.steps, args.warmup, "lattice")
+    lat_ev = out_lat.cpu().numpy()
+    lat_bad = int(batch.info.any(dim=1).sum().item())
+    ms_wc, _, _ = timed(build_compact, args.steps, 1, "w_compact")
     ms_plain, out_plain, tp = timed(plain_step, args.plain_steps, 1, "plain")
-    mixed_ev = out_mixed.cpu().numpy()
     plain_ev = out_plain.cpu().numpy()
     dlogl = float(np.abs(mixed_ev - plain_ev).max())
-    info_bad = int(batch.info.any(dim=1).sum().item())
+    info_bad = max(info_bad, lat_bad)
     n = len(ds.samples)
     value = n / (ms_mixed / 1e3)
     plain_value = n / (ms_plain / 1e3)
@@ -239,6 +264,11 @@ def main():
                          "solve": round(tm["solve"][0] / args.steps, 3),
                          "batch_total": round(batch_ms, 3)},
         "evals_per_s_excluding_w_tilde": round(n / (batch_ms / 1e3), 3),
+        "lattice_fp64": {"value": round(n / (ms_lat / 1e3), 3), "ms_per_step": round(ms_lat, 3),
+                         "w_compact_ms": round(ms_wc, 3),
+                         "speedup_vs_plain": round(ms_plain / ms_lat, 3),
+                         "max_abs_dlogL_vs_plain": float(np.abs(lat_ev - plain_ev).max()),
+                         "grid": f"{N} x {N} lattice, table {N} x {2 * N - 1}"},
         "roofline": {"bound": "mfma", "kernel": kernel, "achieved": round(achieved, 3),
                      "peak": peak, "unit": "TFLOP/s", "frac": round(achieved / peak, 5),
                      "raw_mfma_frac": round(achieved * RAW_FACTOR[args.precision] / peak, 5),

@@ -74,6 +74,20 @@ int aj_w_tilde_set_variant(int variant);
 int aj_w_tilde_variant_count(void);
 int aj_w_tilde_set_group_bytes(size_t bytes);
 
+/* numba.py:210-234 w_compact_curvature_interferometer_from(grid_size, noise_map_real f8[K],
+ * uv_wavelengths f8[K,2], pixel_scale [arcsec]) -> f8[N, 2N-1], N = grid_size: the distinct values
+ * of W-tilde on a regular N x N pixel grid, w_compact[m, n] = sum_k n_k^-2 cos(2 pi d ((n - N + 1) u_k0
+ * - m u_k1)), d = pixel_scale in radians (FP64 MFMA over per-visibility phasors). */
+size_t aj_w_compact_workspace_bytes(int64_t K, int64_t grid_size);
+int aj_w_compact_curvature_interferometer_from(const double* noise_map_real, const double* uv_wavelengths, int64_t K,
+                                               int64_t grid_size, double pixel_scale, double* w_compact,
+                                               void* workspace, size_t workspace_bytes, void* stream);
+
+/* numba.py:237-255 w_tilde_via_compact_from(w_compact f8[N,2N-1], native_index_for_slim_index i8[M,2])
+ * -> f8[M,M] (native indices are the (row, column) grid position of each image pixel, in [0, N)). */
+int aj_w_tilde_via_compact_from(const double* w_compact, int64_t grid_size,
+                                const int64_t* native_index_for_slim_index, int64_t M, double* w_tilde, void* stream);
+
 /* numba.py:95-152 w_tilde_data_interferometer_from(visibilities_real f8[K],
  * noise_map_real f8[K], uv_wavelengths f8[K,2], grid_radians_slim f8[M,2]) -> f8[M].
  * Same weight as the reference, (n_k^2 / v_k)^2 (numba.py:149). */
@@ -124,6 +138,14 @@ int aj_curvature_matrix_via_w_wilde_sparse_mapping_matrix_from(const double* w_t
                                                                int64_t B, int64_t pixels, double* curvature_matrix,
                                                                void* workspace, size_t workspace_bytes, void* stream);
 
+/* numba.py:617-647 curvature_matrix_via_w_compact_sparse_mapping_matrix_from(w_compact f8[N,2N-1],
+ * native_index_for_slim_index i8[M,2], pix_indexes i8[M,B], pix_weights f8[M,B], pixels) -> f8[pixels,pixels]:
+ * as the sparse form above with W-tilde read from the lattice table.  Uses aj_curvature_sparse_workspace_bytes. */
+int aj_curvature_matrix_via_w_compact_sparse_mapping_matrix_from(
+    const double* w_compact, int64_t grid_size, const int64_t* native_index_for_slim_index,
+    const int64_t* pix_indexes_for_sub_slim_index, const double* pix_weights_for_sub_slim_index, int64_t M, int64_t B,
+    int64_t pixels, double* curvature_matrix, void* workspace, size_t workspace_bytes, void* stream);
+
 /* numba.py:656-695 constant_regularization_matrix_from(coefficient, neighbors i8[S,P],
  * neighbors_sizes i8[S]) -> f8[S,S]. */
 int aj_constant_regularization_matrix_from(double coefficient, const int64_t* neighbors,
@@ -159,6 +181,16 @@ int aj_log_likelihood_function_via_w_tilde_from(const double* dirty_image, const
                                                 int64_t S, int64_t P, double* log_likelihood, int64_t* info,
                                                 void* workspace, size_t workspace_bytes, void* stream);
 
+/* numba.py:860-967 log_likelihood_function_via_w_compact_from(dirty_image, data, noise_map,
+ * w_compact f8[N,2N-1], native_index_for_slim_index i8[M,2], pix_indexes, pix_weights, neighbors,
+ * neighbors_sizes) -> f8, coefficient 1 as at numba.py:923.  Uses aj_log_likelihood_workspace_bytes. */
+int aj_log_likelihood_function_via_w_compact_from(
+    const double* dirty_image, const double* data, const double* noise_map, const double* w_compact, int64_t grid_size,
+    const int64_t* native_index_for_slim_index, const int64_t* pix_indexes_for_sub_slim_index,
+    const double* pix_weights_for_sub_slim_index, const int64_t* neighbors, const int64_t* neighbors_sizes, int64_t M,
+    int64_t K, int64_t B, int64_t S, int64_t P, double* log_likelihood, int64_t* info, void* workspace,
+    size_t workspace_bytes, void* stream);
+
 /* Batched log-evidence: numba.py:755-857 evaluated for n samples that share one w_tilde, with an
  * explicit regularisation coefficient per sample (the reference fixes it to 1, numba.py:819).
  * Sample b has S_host[b] source pixels; mapping blocks are [n,M,B]; neighbour rows of all samples
@@ -174,6 +206,19 @@ int aj_log_evidence_batched(const double* w_tilde, int64_t M, const double* dirt
                             const double* coefficients_host, const double* deflation_beta_host, double* log_evidence,
                             double* reconstruction, double* logdets, int64_t* info, void* workspace,
                             size_t workspace_bytes, void* stream);
+
+/* aj_log_evidence_batched with W-tilde given by its lattice table (numba.py:860-967 per sample):
+ * same arguments and workspace (aj_log_evidence_batched_workspace_bytes) with w_tilde replaced by
+ * w_compact f8[N,2N-1], grid_size N and native_index_for_slim_index i8[M,2].  No reference counterpart
+ * for the batch. */
+int aj_log_evidence_batched_compact(const double* w_compact, int64_t grid_size,
+                                    const int64_t* native_index_for_slim_index, int64_t M, const double* dirty_image,
+                                    const double* chi2_noise_norm, int64_t n_samples, int64_t B, int64_t P,
+                                    const int64_t* S_host, const int64_t* pix_indexes, const double* pix_weights,
+                                    const int64_t* neighbors, const int64_t* neighbors_sizes,
+                                    const double* coefficients_host, const double* deflation_beta_host,
+                                    double* log_evidence, double* reconstruction, double* logdets, int64_t* info,
+                                    void* workspace, size_t workspace_bytes, void* stream);
 
 /* Backward of aj_log_evidence_batched (no reference counterpart: the reference differentiates its
  * JAX form, jax.py:1032-1058, by autodiff).  Uses the Cholesky factors and Omega = W T left in

@@ -92,6 +92,38 @@ def w_tilde_via_compact_from(w_compact, native_index_for_slim_index):
     return w_compact[np.abs(m), n]
 
 
+def w_compact_matmul_sparse_mapping_matrix_from(w_compact, native_index_for_slim_index, pix_indexes_for_sub_slim_index,
+                                                pix_weights_for_sub_slim_index, pixels):
+    """numba.py:500-549: Omega = W T with W read from the lattice table (loops; small sizes only)."""
+    M, B = pix_indexes_for_sub_slim_index.shape
+    OFFSET = w_compact.shape[0] - 1
+    omega = np.zeros((M, pixels))
+    r = native_index_for_slim_index[:, 0]
+    c = native_index_for_slim_index[:, 1]
+    for m1 in range(M):
+        n1 = r[m1] - r
+        n2 = np.where(n1 >= 0, c[m1] - c, c - c[m1]) + OFFSET
+        w_row = w_compact[np.abs(n1), n2]
+        for m2 in range(M):
+            for b2 in range(B):
+                s2 = pix_indexes_for_sub_slim_index[m2, b2]
+                if s2 == -1:
+                    break
+                omega[m1, s2] += w_row[m2] * pix_weights_for_sub_slim_index[m2, b2]
+    return omega
+
+
+def curvature_matrix_via_w_compact_sparse_mapping_matrix_from(w_compact, native_index_for_slim_index,
+                                                              pix_indexes_for_sub_slim_index,
+                                                              pix_weights_for_sub_slim_index, pixels):
+    """numba.py:617-647: F = T^T Omega with Omega from the lattice table."""
+    omega = w_compact_matmul_sparse_mapping_matrix_from(w_compact, native_index_for_slim_index,
+                                                        pix_indexes_for_sub_slim_index, pix_weights_for_sub_slim_index,
+                                                        pixels)
+    return sparse_mapping_matrix_transpose_matmul(omega, pix_indexes_for_sub_slim_index, pix_weights_for_sub_slim_index,
+                                                  pixels)
+
+
 def mapping_matrix_from(pix_indexes_for_sub_slim_index, pix_weights_for_sub_slim_index, pixels):
     """numba.py:293-376: dense T[m, s] = t (assignment), rows stop at the first -1."""
     M, B = pix_indexes_for_sub_slim_index.shape
@@ -245,3 +277,21 @@ def log_evidence_from_parts(F, D, chi_squared, noise_normalization, neighbors, n
     if logdet_h is None:
         logdet_h = np.linalg.slogdet(H)[1]
     return -0.5 * (chi_squared - s @ D + ld_a - logdet_h + noise_normalization), s, ld_a, logdet_h
+
+
+def log_likelihood_function_via_w_compact_from(dirty_image, data, noise_map, w_compact, native_index_for_slim_index,
+                                               pix_indexes_for_sub_slim_index, pix_weights_for_sub_slim_index,
+                                               neighbors, neighbors_sizes):
+    """numba.py:860-967: the log-likelihood of numba.py:755-857 with F from the lattice table."""
+    S = neighbors_sizes.size
+    F = curvature_matrix_via_w_compact_sparse_mapping_matrix_from(w_compact, native_index_for_slim_index,
+                                                                  pix_indexes_for_sub_slim_index,
+                                                                  pix_weights_for_sub_slim_index, S)
+    H = constant_regularization_matrix_from(1.0, neighbors, neighbors_sizes)
+    A = F + H
+    T = mapping_matrix_from(pix_indexes_for_sub_slim_index, pix_weights_for_sub_slim_index, S)
+    D = data_vector_from(T, dirty_image)
+    s = reconstruction_positive_negative_from(D, A)
+    chi = data.view(np.float64) / noise_map.view(np.float64)
+    return -0.5 * (chi @ chi - s @ D + np.linalg.slogdet(A)[1] - np.linalg.slogdet(H)[1]
+                   + noise_normalization_complex_from(noise_map))

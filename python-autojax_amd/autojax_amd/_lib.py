@@ -35,6 +35,9 @@ SIGNATURES = {
     "aj_w_tilde_set_group_bytes": (_INT, [_S]),
     "aj_w_tilde_workspace_bytes": (_S, [_I, _I, _INT]),
     "aj_w_tilde_curvature_interferometer_from": (_INT, [_P, _P, _P, _I, _I, _INT, _P, _P, _S, _P]),
+    "aj_w_compact_workspace_bytes": (_S, [_I, _I]),
+    "aj_w_compact_curvature_interferometer_from": (_INT, [_P, _P, _I, _I, _D, _P, _P, _S, _P]),
+    "aj_w_tilde_via_compact_from": (_INT, [_P, _I, _P, _I, _P, _P]),
     "aj_visibility_sum_workspace_bytes": (_S, [_I, _I]),
     "aj_w_tilde_data_interferometer_from": (_INT, [_P, _P, _P, _P, _I, _I, _P, _P, _S, _P]),
     "aj_dirty_image_from": (_INT, [_P, _P, _P, _P, _I, _I, _P, _P, _S, _P]),
@@ -46,6 +49,8 @@ SIGNATURES = {
     "aj_sparse_mapping_matrix_transpose_matmul": (_INT, [_P, _P, _P, _I, _I, _I, _I, _P, _P, _S, _P]),
     "aj_curvature_sparse_workspace_bytes": (_S, [_I, _I, _I]),
     "aj_curvature_matrix_via_w_wilde_sparse_mapping_matrix_from": (_INT, [_P, _P, _P, _I, _I, _I, _P, _P, _S, _P]),
+    "aj_curvature_matrix_via_w_compact_sparse_mapping_matrix_from": (
+        _INT, [_P, _I, _P, _P, _P, _I, _I, _I, _P, _P, _S, _P]),
     "aj_constant_regularization_matrix_from": (_INT, [_D, _P, _P, _I, _I, _P, _P]),
     "aj_solve_workspace_bytes": (_S, [_I]),
     "aj_reconstruction_positive_negative_from": (_INT, [_P, _P, _I, _P, _P, _P, _P, _S, _P]),
@@ -54,7 +59,11 @@ SIGNATURES = {
     "aj_log_likelihood_workspace_bytes": (_S, [_I, _I, _I, _I, _I]),
     "aj_log_likelihood_function_via_w_tilde_from": (
         _INT, [_P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _P, _P, _P, _S, _P]),
+    "aj_log_likelihood_function_via_w_compact_from": (
+        _INT, [_P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _P, _P, _P, _S, _P]),
     "aj_log_evidence_batched_workspace_bytes": (_S, [_I, _I, _I, _P]),
+    "aj_log_evidence_batched_compact": (
+        _INT, [_P, _I, _P, _I, _P, _P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _S, _P]),
     "aj_log_evidence_batched": (
         _INT, [_P, _I, _P, _P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _S, _P]),
     "aj_log_evidence_backward_workspace_bytes": (_S, [_I, _I, _P]),

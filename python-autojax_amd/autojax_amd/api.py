@@ -150,6 +150,46 @@ def w_tilde_curvature_interferometer_from(noise_map_real, uv_wavelengths, grid_r
     return _out(out, torch_in)
 
 
+def validate_native_index(native_index_for_slim_index, grid_size):
+    """Host check of the lattice positions: shape [M, 2], every (row, column) in [0, grid_size)."""
+    ni = _host(native_index_for_slim_index)
+    if ni.ndim != 2 or ni.shape[1] != 2:
+        raise ValueError("native_index_for_slim_index must be [M, 2]")
+    if ni.size and (ni.min() < 0 or ni.max() >= int(grid_size)):
+        raise ValueError(f"native_index_for_slim_index outside the {int(grid_size)} x {int(grid_size)} grid")
+
+
+def w_compact_curvature_interferometer_from(grid_size, noise_map_real, uv_wavelengths, pixel_scale):
+    """numba.py:210-234 -> f8[N, 2N-1]: the distinct W-tilde values of a regular N x N grid
+    (pixel_scale in arcsec), exact FP64 on MFMA."""
+    torch_in = _any_t(noise_map_real, uv_wavelengths)
+    n = _dev(noise_map_real, np.float64, "noise_map_real", 1)
+    uv = _dev(uv_wavelengths, np.float64, "uv_wavelengths", 2)
+    K, N = n.shape[0], int(grid_size)
+    if uv.shape != (K, 2) or N <= 0:
+        raise ValueError("shape mismatch: noise_map_real [K], uv_wavelengths [K,2], grid_size > 0")
+    out = dv.empty((N, 2 * N - 1))
+    ws = dv.workspace(_lib.query("aj_w_compact_workspace_bytes", K, N))
+    _lib.call("aj_w_compact_curvature_interferometer_from", n.data_ptr(), uv.data_ptr(), K, N, float(pixel_scale),
+              out.data_ptr(), ws.data_ptr(), ws.numel(), dv.stream_ptr())
+    return _out(out, torch_in)
+
+
+def w_tilde_via_compact_from(w_compact, native_index_for_slim_index):
+    """numba.py:237-255 -> f8[M, M]."""
+    torch_in = _any_t(w_compact, native_index_for_slim_index)
+    C = _dev(w_compact, np.float64, "w_compact", 2)
+    N = C.shape[0]
+    if C.shape[1] != 2 * N - 1:
+        raise ValueError("w_compact must be [N, 2N-1]")
+    validate_native_index(native_index_for_slim_index, N)
+    ni = _dev(native_index_for_slim_index, np.int64, "native_index_for_slim_index", 2)
+    M = ni.shape[0]
+    out = dv.empty((M, M))
+    _lib.call("aj_w_tilde_via_compact_from", C.data_ptr(), N, ni.data_ptr(), M, out.data_ptr(), dv.stream_ptr())
+    return _out(out, torch_in)
+
+
 def w_tilde_data_interferometer_from(visibilities_real, noise_map_real, uv_wavelengths, grid_radians_slim):
     """numba.py:95-152 -> f8[M]."""
     torch_in = _any_t(visibilities_real, noise_map_real, uv_wavelengths, grid_radians_slim)
@@ -262,6 +302,33 @@ def curvature_matrix_via_w_wilde_sparse_mapping_matrix_from(w_tilde, pix_indexes
     return _out(out, torch_in)
 
 
+def curvature_matrix_via_w_compact_sparse_mapping_matrix_from(w_compact, native_index_for_slim_index,
+                                                              pix_indexes_for_sub_slim_index,
+                                                              pix_weights_for_sub_slim_index, pixels):
+    """numba.py:617-647 -> f8[pixels, pixels]: the sparse two-stage product with W-tilde read from
+    the lattice table (never expanded to M x M)."""
+    torch_in = _any_t(w_compact, native_index_for_slim_index, pix_indexes_for_sub_slim_index,
+                      pix_weights_for_sub_slim_index)
+    validate_mapping(pix_indexes_for_sub_slim_index, pixels)
+    C = _dev(w_compact, np.float64, "w_compact", 2)
+    N = C.shape[0]
+    if C.shape[1] != 2 * N - 1:
+        raise ValueError("w_compact must be [N, 2N-1]")
+    validate_native_index(native_index_for_slim_index, N)
+    ni = _dev(native_index_for_slim_index, np.int64, "native_index_for_slim_index", 2)
+    idx = _dev(pix_indexes_for_sub_slim_index, np.int64, "pix_indexes_for_sub_slim_index", 2)
+    w = _dev(pix_weights_for_sub_slim_index, np.float64, "pix_weights_for_sub_slim_index", 2)
+    M, B = idx.shape
+    if ni.shape[0] != M:
+        raise ValueError("native_index_for_slim_index and the mapping disagree on M")
+    out = dv.empty((int(pixels), int(pixels)))
+    ws = dv.workspace(_lib.query("aj_curvature_sparse_workspace_bytes", M, B, int(pixels)))
+    _lib.call("aj_curvature_matrix_via_w_compact_sparse_mapping_matrix_from", C.data_ptr(), N, ni.data_ptr(),
+              idx.data_ptr(), w.data_ptr(), M, B, int(pixels), out.data_ptr(), ws.data_ptr(), ws.numel(),
+              dv.stream_ptr())
+    return _out(out, torch_in)
+
+
 # ---------------------------------------------------------------------------
 # regularisation, solve, normalisation, likelihood
 # ---------------------------------------------------------------------------
@@ -338,6 +405,45 @@ def log_likelihood_function_via_w_tilde_from(dirty_image, data, noise_map, w_til
     _lib.call("aj_log_likelihood_function_via_w_tilde_from", y.data_ptr(), d.data_ptr(), n.data_ptr(), W.data_ptr(),
               idx.data_ptr(), w.data_ptr(), nb.data_ptr(), sz.data_ptr(), M, K, B, S, P, out.data_ptr(),
               info.data_ptr(), ws.data_ptr(), ws.numel(), dv.stream_ptr())
+    inf = info.tolist()
+    if inf[0] or inf[1]:
+        which = "curvature_reg_matrix" if inf[0] else "regularization_matrix"
+        raise np.linalg.LinAlgError(f"{which} is not positive definite (pivot {inf[0] or inf[1]})")
+    return float(out.item())
+
+
+def log_likelihood_function_via_w_compact_from(dirty_image, data, noise_map, w_compact, native_index_for_slim_index,
+                                               pix_indexes_for_sub_slim_index, pix_weights_for_sub_slim_index,
+                                               neighbors, neighbors_sizes):
+    """numba.py:860-967 -> float (coefficient 1, numba.py:923): as
+    log_likelihood_function_via_w_tilde_from with W-tilde read from the lattice table."""
+    S = int(_host(neighbors_sizes).size)
+    validate_mapping(pix_indexes_for_sub_slim_index, S)
+    validate_neighbors(neighbors, neighbors_sizes)
+    C = _dev(w_compact, np.float64, "w_compact", 2)
+    N = C.shape[0]
+    if C.shape[1] != 2 * N - 1:
+        raise ValueError("w_compact must be [N, 2N-1]")
+    validate_native_index(native_index_for_slim_index, N)
+    ni = _dev(native_index_for_slim_index, np.int64, "native_index_for_slim_index", 2)
+    y = _dev(dirty_image, np.float64, "dirty_image", 1)
+    d = _view_f8(_dev(data, np.complex128, "data", 1))
+    n = _view_f8(_dev(noise_map, np.complex128, "noise_map", 1))
+    idx = _dev(pix_indexes_for_sub_slim_index, np.int64, "pix_indexes_for_sub_slim_index", 2)
+    w = _dev(pix_weights_for_sub_slim_index, np.float64, "pix_weights_for_sub_slim_index", 2)
+    nb = _dev(neighbors, np.int64, "neighbors", 2)
+    sz = _dev(neighbors_sizes, np.int64, "neighbors_sizes", 1)
+    M, B = idx.shape
+    K = d.numel() // 2
+    P = nb.shape[1]
+    if ni.shape[0] != M or y.shape[0] != M or n.numel() != 2 * K:
+        raise ValueError("shape mismatch between dirty_image, native indexes, mapping, data and noise_map")
+    out = dv.empty((1,))
+    info = torch.zeros(2, dtype=torch.int64, device=out.device)
+    ws = dv.workspace(_lib.query("aj_log_likelihood_workspace_bytes", M, K, B, S, P))
+    _lib.call("aj_log_likelihood_function_via_w_compact_from", y.data_ptr(), d.data_ptr(), n.data_ptr(), C.data_ptr(),
+              N, ni.data_ptr(), idx.data_ptr(), w.data_ptr(), nb.data_ptr(), sz.data_ptr(), M, K, B, S, P,
+              out.data_ptr(), info.data_ptr(), ws.data_ptr(), ws.numel(), dv.stream_ptr())
     inf = info.tolist()
     if inf[0] or inf[1]:
         which = "curvature_reg_matrix" if inf[0] else "regularization_matrix"

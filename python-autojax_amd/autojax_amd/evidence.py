@@ -131,6 +131,29 @@ class SampleBatch:
                 raise np.linalg.LinAlgError(f"samples {bad}: matrix not positive definite, info={self.info[bad].tolist()}")
         return self.out
 
+    def log_evidence_compact(self, w_compact, native_index_for_slim_index, dirty_image, chi2_noise_norm, check=False):
+        """As ``log_evidence`` with W-tilde given by its lattice table w_compact [N, 2N-1] and the
+        device int64 [M, 2] grid positions of the image pixels (never expanded to M x M)."""
+        N = w_compact.shape[0]
+        if w_compact.shape != (N, 2 * N - 1) or w_compact.dtype != torch.float64:
+            raise ValueError("w_compact must be float64 [N, 2N-1] on the device")
+        if native_index_for_slim_index.shape != (self.M, 2) or native_index_for_slim_index.dtype != torch.int64:
+            raise ValueError("native_index_for_slim_index must be int64 [M, 2] on the device")
+        _lib.call(
+            "aj_log_evidence_batched_compact",
+            w_compact.data_ptr(), N, native_index_for_slim_index.data_ptr(), self.M, dirty_image.data_ptr(),
+            chi2_noise_norm.data_ptr(), self.n, self.B, self.P, self.S_host.ctypes.data_as(_lib.C.c_void_p),
+            self.idx.data_ptr(), self.wts.data_ptr(), self.nbr.data_ptr(), self.nsz.data_ptr(),
+            self.coef_host.ctypes.data_as(_lib.C.c_void_p), self.beta_host.ctypes.data_as(_lib.C.c_void_p),
+            self.out.data_ptr(), self.recon.data_ptr(), self.logdets.data_ptr(), self.info.data_ptr(),
+            self.ws.data_ptr(), self.ws.numel(), dv.stream_ptr(),
+        )
+        if check:
+            bad = torch.nonzero(self.info.any(dim=1)).flatten().tolist()
+            if bad:
+                raise np.linalg.LinAlgError(f"samples {bad}: matrix not positive definite, info={self.info[bad].tolist()}")
+        return self.out
+
     def reconstructions(self):
         off = np.concatenate([[0], np.cumsum(self.S_host)])
         return [self.recon[off[b] : off[b + 1]] for b in range(self.n)]

@@ -155,7 +155,24 @@ __global__ void data_vector_kernel(const SampleDesc* __restrict__ sd, const int*
 constexpr int OMEGA_THREADS = 256;
 constexpr size_t OMEGA_LDS_BUDGET = 150 * 1024;
 
-__global__ void __launch_bounds__(OMEGA_THREADS) omega_kernel(const double* __restrict__ W, int64_t M, int R,
+// Where W-tilde comes from: the dense M x M matrix, or the lattice table w_compact [N, 2N-1] with
+// the (row, column) grid index of every image pixel (numba.py:237-255 gives the element rule).
+struct WSrc {
+  const double* W;
+  const double* C;
+  const int64_t* native;
+  int64_t N;
+};
+
+__device__ inline double wsrc_at(const WSrc& s, int64_t M, int64_t i, int64_t j) {
+  if (s.C == nullptr) return s.W[i * M + j];
+  const int64_t m = s.native[2 * i] - s.native[2 * j];
+  const int64_t n = (m >= 0 ? s.native[2 * i + 1] - s.native[2 * j + 1] : s.native[2 * j + 1] - s.native[2 * i + 1]) +
+                    s.N - 1;
+  return s.C[(m >= 0 ? m : -m) * (2 * s.N - 1) + n];
+}
+
+__global__ void __launch_bounds__(OMEGA_THREADS) omega_kernel(const WSrc W, int64_t M, int R,
                                                                int nsmp, const SampleDesc* __restrict__ sd,
                                                                const int* __restrict__ ptrs,
                                                                const int* __restrict__ ent_m,
@@ -164,7 +181,11 @@ __global__ void __launch_bounds__(OMEGA_THREADS) omega_kernel(const double* __re
   extern __shared__ double wrow[];  // R x M
   const int64_t m0 = (int64_t)blockIdx.x * R;
   const int rows = (int)std::min<int64_t>(R, M - m0);
-  for (int64_t i = threadIdx.x; i < (int64_t)rows * M; i += OMEGA_THREADS) wrow[i] = W[m0 * M + i];
+  if (W.C == nullptr) {
+    for (int64_t i = threadIdx.x; i < (int64_t)rows * M; i += OMEGA_THREADS) wrow[i] = W.W[m0 * M + i];
+  } else {
+    for (int64_t i = threadIdx.x; i < (int64_t)rows * M; i += OMEGA_THREADS) wrow[i] = wsrc_at(W, M, m0 + i / M, i % M);
+  }
   __syncthreads();
   for (int smp = 0; smp < nsmp; ++smp) {
     const SampleDesc d = sd[smp];
@@ -191,18 +212,17 @@ __global__ void __launch_bounds__(OMEGA_THREADS) omega_kernel(const double* __re
 }
 
 // Same product when a row of W does not fit in LDS: gather from global memory.
-__global__ void omega_global_kernel(const double* __restrict__ W, int64_t M, int nsmp,
+__global__ void omega_global_kernel(const WSrc W, int64_t M, int nsmp,
                                     const SampleDesc* __restrict__ sd, const int* __restrict__ ptrs,
                                     const int* __restrict__ ent_m, const double* __restrict__ ent_t,
                                     double* __restrict__ omega) {
   const int64_t m = blockIdx.x;
-  const double* w = W + m * M;
   for (int smp = 0; smp < nsmp; ++smp) {
     const SampleDesc d = sd[smp];
     const int* p = ptrs + d.csc_ptr;
     for (int64_t s = threadIdx.x; s < d.S; s += blockDim.x) {
       double acc = 0.0;
-      for (int e = p[s]; e < p[s + 1]; ++e) acc = fma(ent_t[d.csc_ent + e], w[ent_m[d.csc_ent + e]], acc);
+      for (int e = p[s]; e < p[s + 1]; ++e) acc = fma(ent_t[d.csc_ent + e], wsrc_at(W, M, m, ent_m[d.csc_ent + e]), acc);
       omega[d.omega_off + m * d.S + s] = acc;
     }
   }
@@ -741,7 +761,7 @@ namespace {
 
 // Runs the whole per-sample pipeline for a batch.  Returns the workspace
 // views through `p`; outputs per-sample log-evidence into logl.
-int batch_pipeline(const double* w_tilde, int64_t M, const double* dirty_image, const double* chi2_nn, int64_t n,
+int batch_pipeline(const WSrc w_tilde, int64_t M, const double* dirty_image, const double* chi2_nn, int64_t n,
                    int64_t B, int64_t P, const int64_t* S_host, const int64_t* pix_indexes, const double* pix_weights,
                    const int64_t* neighbors, const int64_t* neighbors_sizes, const double* coef_host,
                    const double* beta_host, double* logl, double* recon, double* logdets_out, int64_t* info_out,
@@ -915,7 +935,8 @@ extern "C" int aj_log_evidence_batched(const double* w_tilde, int64_t M, const d
   AJ_REQUIRE(w_tilde && dirty_image && chi2_noise_norm && S_host && pix_indexes && pix_weights && neighbors &&
                  neighbors_sizes && coefficients_host && log_evidence,
              "null pointer");
-  return batch_pipeline(w_tilde, M, dirty_image, chi2_noise_norm, n_samples, B, P, S_host, pix_indexes, pix_weights,
+  return batch_pipeline(WSrc{w_tilde, nullptr, nullptr, 0}, M, dirty_image, chi2_noise_norm, n_samples, B, P, S_host,
+                        pix_indexes, pix_weights,
                         neighbors, neighbors_sizes, coefficients_host, deflation_beta_host, log_evidence,
                         reconstruction, logdets, info, workspace, workspace_bytes, as_stream(stream));
 }
@@ -944,9 +965,54 @@ extern "C" int aj_log_likelihood_function_via_w_tilde_from(
   hipLaunchKernelGGL(chi2_noise_kernel, dim3(1), dim3(256), 0, st, data, noise_map, 2 * K, chi2nn);
   AJ_LAUNCH_CHECK();
   const double coef = 1.0;  // numba.py:819
-  return batch_pipeline(w_tilde, M, dirty_image, chi2nn, 1, B, P, &S, pix_indexes_for_sub_slim_index,
+  return batch_pipeline(WSrc{w_tilde, nullptr, nullptr, 0}, M, dirty_image, chi2nn, 1, B, P, &S,
+                        pix_indexes_for_sub_slim_index,
                         pix_weights_for_sub_slim_index, neighbors, neighbors_sizes, &coef, nullptr, log_likelihood,
                         nullptr, nullptr, info, workspace, need - 256, st);
+}
+
+extern "C" int aj_log_likelihood_function_via_w_compact_from(
+    const double* dirty_image, const double* data, const double* noise_map, const double* w_compact, int64_t grid_size,
+    const int64_t* native_index_for_slim_index, const int64_t* pix_indexes_for_sub_slim_index,
+    const double* pix_weights_for_sub_slim_index, const int64_t* neighbors, const int64_t* neighbors_sizes, int64_t M,
+    int64_t K, int64_t B, int64_t S, int64_t P, double* log_likelihood, int64_t* info, void* workspace,
+    size_t workspace_bytes, void* stream) {
+  AJ_REQUIRE(M > 0 && K >= 0 && B > 0 && S > 0 && P > 0 && grid_size > 0, "bad sizes");
+  AJ_REQUIRE(dirty_image && data && noise_map && w_compact && native_index_for_slim_index &&
+                 pix_indexes_for_sub_slim_index && pix_weights_for_sub_slim_index && neighbors && neighbors_sizes &&
+                 log_likelihood && info,
+             "null pointer");
+  const size_t need = aj_log_likelihood_workspace_bytes(M, K, B, S, P);
+  AJ_REQUIRE(workspace && workspace_bytes >= need, "workspace too small: need %zu bytes, got %zu", need,
+             workspace_bytes);
+  hipStream_t st = as_stream(stream);
+  double* chi2nn = reinterpret_cast<double*>(static_cast<char*>(workspace) + need - 256);
+  hipLaunchKernelGGL(chi2_noise_kernel, dim3(1), dim3(256), 0, st, data, noise_map, 2 * K, chi2nn);
+  AJ_LAUNCH_CHECK();
+  const double coef = 1.0;  // numba.py:923
+  return batch_pipeline(WSrc{nullptr, w_compact, native_index_for_slim_index, grid_size}, M, dirty_image, chi2nn, 1, B,
+                        P, &S, pix_indexes_for_sub_slim_index, pix_weights_for_sub_slim_index, neighbors,
+                        neighbors_sizes, &coef, nullptr, log_likelihood, nullptr, nullptr, info, workspace, need - 256,
+                        st);
+}
+
+extern "C" int aj_log_evidence_batched_compact(const double* w_compact, int64_t grid_size,
+                                               const int64_t* native_index_for_slim_index, int64_t M,
+                                               const double* dirty_image, const double* chi2_noise_norm,
+                                               int64_t n_samples, int64_t B, int64_t P, const int64_t* S_host,
+                                               const int64_t* pix_indexes, const double* pix_weights,
+                                               const int64_t* neighbors, const int64_t* neighbors_sizes,
+                                               const double* coefficients_host, const double* deflation_beta_host,
+                                               double* log_evidence, double* reconstruction, double* logdets,
+                                               int64_t* info, void* workspace, size_t workspace_bytes, void* stream) {
+  AJ_REQUIRE(M > 0 && B > 0 && P > 0 && n_samples > 0 && grid_size > 0, "bad sizes");
+  AJ_REQUIRE(w_compact && native_index_for_slim_index && dirty_image && chi2_noise_norm && S_host && pix_indexes &&
+                 pix_weights && neighbors && neighbors_sizes && coefficients_host && log_evidence,
+             "null pointer");
+  return batch_pipeline(WSrc{nullptr, w_compact, native_index_for_slim_index, grid_size}, M, dirty_image,
+                        chi2_noise_norm, n_samples, B, P, S_host, pix_indexes, pix_weights, neighbors,
+                        neighbors_sizes, coefficients_host, deflation_beta_host, log_evidence, reconstruction, logdets,
+                        info, workspace, workspace_bytes, as_stream(stream));
 }
 
 // ===========================================================================
@@ -1280,15 +1346,9 @@ extern "C" size_t aj_curvature_sparse_workspace_bytes(int64_t M, int64_t B, int6
   return (M > 0 && B > 0 && S > 0) ? csc_bytes(M, B, S) + sizeof(double) * M * S + 256 : 0;
 }
 
-extern "C" int aj_curvature_matrix_via_w_wilde_sparse_mapping_matrix_from(
-    const double* w_tilde, const int64_t* pix_indexes_for_sub_slim_index, const double* pix_weights_for_sub_slim_index,
-    int64_t M, int64_t B, int64_t pixels, double* curvature_matrix, void* workspace, size_t workspace_bytes,
-    void* stream) {
-  AJ_REQUIRE(M > 0 && B > 0 && pixels > 0, "bad sizes");
-  AJ_REQUIRE(w_tilde && pix_indexes_for_sub_slim_index && pix_weights_for_sub_slim_index && curvature_matrix &&
-                 workspace,
-             "null pointer");
-  hipStream_t st = as_stream(stream);
+static int sparse_curvature(const WSrc w_tilde, const int64_t* pix_indexes_for_sub_slim_index,
+                            const double* pix_weights_for_sub_slim_index, int64_t M, int64_t B, int64_t pixels,
+                            double* curvature_matrix, void* workspace, size_t workspace_bytes, hipStream_t st) {
   Arena ar(workspace, workspace_bytes);
   OneCsc c;
   int rc = build_one_csc(pix_indexes_for_sub_slim_index, pix_weights_for_sub_slim_index, M, B, pixels, ar, c, st);
@@ -1309,6 +1369,32 @@ extern "C" int aj_curvature_matrix_via_w_wilde_sparse_mapping_matrix_from(
                      pixels, curvature_matrix);
   AJ_LAUNCH_CHECK();
   return AJ_OK;
+}
+
+extern "C" int aj_curvature_matrix_via_w_wilde_sparse_mapping_matrix_from(
+    const double* w_tilde, const int64_t* pix_indexes_for_sub_slim_index, const double* pix_weights_for_sub_slim_index,
+    int64_t M, int64_t B, int64_t pixels, double* curvature_matrix, void* workspace, size_t workspace_bytes,
+    void* stream) {
+  AJ_REQUIRE(M > 0 && B > 0 && pixels > 0, "bad sizes");
+  AJ_REQUIRE(w_tilde && pix_indexes_for_sub_slim_index && pix_weights_for_sub_slim_index && curvature_matrix &&
+                 workspace,
+             "null pointer");
+  return sparse_curvature(WSrc{w_tilde, nullptr, nullptr, 0}, pix_indexes_for_sub_slim_index,
+                          pix_weights_for_sub_slim_index, M, B, pixels, curvature_matrix, workspace, workspace_bytes,
+                          as_stream(stream));
+}
+
+extern "C" int aj_curvature_matrix_via_w_compact_sparse_mapping_matrix_from(
+    const double* w_compact, int64_t grid_size, const int64_t* native_index_for_slim_index,
+    const int64_t* pix_indexes_for_sub_slim_index, const double* pix_weights_for_sub_slim_index, int64_t M, int64_t B,
+    int64_t pixels, double* curvature_matrix, void* workspace, size_t workspace_bytes, void* stream) {
+  AJ_REQUIRE(M > 0 && B > 0 && pixels > 0 && grid_size > 0, "bad sizes");
+  AJ_REQUIRE(w_compact && native_index_for_slim_index && pix_indexes_for_sub_slim_index &&
+                 pix_weights_for_sub_slim_index && curvature_matrix && workspace,
+             "null pointer");
+  return sparse_curvature(WSrc{nullptr, w_compact, native_index_for_slim_index, grid_size},
+                          pix_indexes_for_sub_slim_index, pix_weights_for_sub_slim_index, M, B, pixels,
+                          curvature_matrix, workspace, workspace_bytes, as_stream(stream));
 }
 
 extern "C" int aj_constant_regularization_matrix_from(double coefficient, const int64_t* neighbors,

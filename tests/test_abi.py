@@ -31,6 +31,26 @@ def test_library_exports_all_declared_symbols():
     assert set(declared()) == set(_lib.SIGNATURES), set(declared()) ^ set(_lib.SIGNATURES)
 
 
+def declared_arity():
+    """Parameter count of every declared entry point, from the header text."""
+    text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
+    out = {}
+    for m in re.finditer(r"^(?:int|size_t|const char\*)\s+(aj_\w+)\(([^)]*)\)\s*;", text, re.M):
+        params = m.group(2).strip()
+        out[m.group(1)] = 0 if params in ("", "void") else params.count(",") + 1
+    return out
+
+
+def test_ctypes_signatures_match_header_arity():
+    """A short argtypes list makes ctypes pass the trailing arguments (the stream handle) as C int."""
+    from autojax_amd import _lib
+
+    arity = declared_arity()
+    assert set(arity) == set(declared())
+    bad = {n: (len(_lib.SIGNATURES[n][1]), a) for n, a in arity.items() if len(_lib.SIGNATURES[n][1]) != a}
+    assert not bad, bad
+
+
 def test_workspace_queries_without_gpu():
     from autojax_amd import _lib
 
@@ -68,6 +88,10 @@ def test_drop_in_signatures_match_reference_names():
         "reconstruction_positive_negative_from": ["data_vector", "curvature_reg_matrix"],
         "noise_normalization_complex_from": ["noise_map"],
         "log_likelihood_function_via_w_tilde_from": ["dirty_image", "data", "noise_map", "w_tilde", "pix_indexes_for_sub_slim_index", "pix_weights_for_sub_slim_index", "neighbors", "neighbors_sizes"],
+        "w_compact_curvature_interferometer_from": ["grid_size", "noise_map_real", "uv_wavelengths", "pixel_scale"],
+        "w_tilde_via_compact_from": ["w_compact", "native_index_for_slim_index"],
+        "curvature_matrix_via_w_compact_sparse_mapping_matrix_from": ["w_compact", "native_index_for_slim_index", "pix_indexes_for_sub_slim_index", "pix_weights_for_sub_slim_index", "pixels"],
+        "log_likelihood_function_via_w_compact_from": ["dirty_image", "data", "noise_map", "w_compact", "native_index_for_slim_index", "pix_indexes_for_sub_slim_index", "pix_weights_for_sub_slim_index", "neighbors", "neighbors_sizes"],
         "mask_2d_centres_from": ["shape_native", "pixel_scales", "centre"],
         "mask_2d_circular_from": ["shape_native", "pixel_scales", "radius", "centre"],
     }

@@ -74,6 +74,31 @@ def test_w_tilde_matches_compact_lattice_form():
     np.testing.assert_allclose(Wc, W, rtol=0, atol=1e-10 * W.max())
 
 
+def test_compact_curvature_and_likelihood_match_dense_route():
+    # numba.py:500-549 / :617-647 / :860-967 against the dense route with the brute-force W
+    rng = np.random.default_rng(15)
+    N, ps = 8, 0.1
+    grid, native = ds.circular_grid(N, ps, 0.4)
+    M, K, S = grid.shape[0], 25, 9
+    uv = ds.uv_disc_uniform(rng, K, 4e5)
+    n = rng.uniform(0.5, 1.5, K)
+    W = ev.w_tilde_bruteforce(n, uv, grid)
+    wc = R.w_compact_curvature_interferometer_from(N, n, uv, ps)
+    idx = np.array([rng.choice(S, 3, replace=False) for _ in range(M)])  # distinct per row, as in the reference
+    idx[::5, 2] = -1
+    wts = rng.uniform(0.1, 1.0, (M, 3))
+    F = R.curvature_matrix_via_w_compact_sparse_mapping_matrix_from(wc, native, idx, wts, S)
+    T = R.mapping_matrix_from(idx, wts, S)
+    np.testing.assert_allclose(F, T.T @ W @ T, rtol=0, atol=1e-11 * np.abs(F).max())
+    mesh = ds.rectangular_source(3)
+    data = rng.standard_normal(K) + 1j * rng.standard_normal(K)
+    noise = n * (1 + 1j)
+    y = ds.dirty_image_numpy(data, noise, uv, grid)
+    got = R.log_likelihood_function_via_w_compact_from(y, data, noise, wc, native, idx, wts, *mesh)
+    want = R.log_likelihood_function_via_w_tilde_from(y, data, noise, W, idx, wts, *mesh)
+    assert abs(got - want) < 1e-10 * abs(want)
+
+
 def test_w_tilde_data_bruteforce():
     rng = np.random.default_rng(6)
     grid, _ = ds.circular_grid(5, 0.2, 0.45)
