@@ -88,6 +88,12 @@ int aj_w_compact_curvature_interferometer_from(const double* noise_map_real, con
 int aj_w_tilde_via_compact_from(const double* w_compact, int64_t grid_size,
                                 const int64_t* native_index_for_slim_index, int64_t M, double* w_tilde, void* stream);
 
+/* numba.py:259-291 w_tilde_via_preload_from(w_tilde_preload f8[Py,Px], native_index_for_slim_index i8[M,2])
+ * -> f8[M,M], W[i, j] = preload[y_j - y_i, x_j - x_i] with negative offsets wrapping as numpy indexes
+ * (the caller guarantees |offset| < Py, Px). */
+int aj_w_tilde_via_preload_from(const double* w_tilde_preload, int64_t preload_rows, int64_t preload_cols,
+                                const int64_t* native_index_for_slim_index, int64_t M, double* w_tilde, void* stream);
+
 /* numba.py:95-152 w_tilde_data_interferometer_from(visibilities_real f8[K],
  * noise_map_real f8[K], uv_wavelengths f8[K,2], grid_radians_slim f8[M,2]) -> f8[M].
  * Same weight as the reference, (n_k^2 / v_k)^2 (numba.py:149). */

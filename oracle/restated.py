@@ -124,6 +124,34 @@ def curvature_matrix_via_w_compact_sparse_mapping_matrix_from(w_compact, native_
                                                   pixels)
 
 
+def w_tilde_via_preload_from(w_tilde_preload, native_index_for_slim_index):
+    """numba.py:259-291: W[i, j] = preload[y_j - y_i, x_j - x_i] (numpy negative indexes wrap)."""
+    y = native_index_for_slim_index[:, 0]
+    x = native_index_for_slim_index[:, 1]
+    return w_tilde_preload[y[None, :] - y[:, None], x[None, :] - x[:, None]]
+
+
+def curvature_matrix_via_w_compact_sparse_mapping_matrix_direct_from(w_compact, native_index_for_slim_index,
+                                                                     pix_indexes_for_sub_slim_index,
+                                                                     pix_weights_for_sub_slim_index, pixels):
+    """numba.py:439-497: F[s1, s2] += t1 W[m1, m2] t2 over all pixel pairs (four loops; tiny sizes only)."""
+    M, B = pix_indexes_for_sub_slim_index.shape
+    W = w_tilde_via_compact_from(w_compact, native_index_for_slim_index)
+    F = np.zeros((pixels, pixels))
+    for m1 in range(M):
+        for m2 in range(M):
+            for b1 in range(B):
+                s1 = pix_indexes_for_sub_slim_index[m1, b1]
+                if s1 == -1:
+                    break
+                for b2 in range(B):
+                    s2 = pix_indexes_for_sub_slim_index[m2, b2]
+                    if s2 == -1:
+                        break
+                    F[s1, s2] += pix_weights_for_sub_slim_index[m1, b1] * W[m1, m2] * pix_weights_for_sub_slim_index[m2, b2]
+    return F
+
+
 def mapping_matrix_from(pix_indexes_for_sub_slim_index, pix_weights_for_sub_slim_index, pixels):
     """numba.py:293-376: dense T[m, s] = t (assignment), rows stop at the first -1."""
     M, B = pix_indexes_for_sub_slim_index.shape

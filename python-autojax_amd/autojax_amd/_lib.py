@@ -38,6 +38,7 @@ SIGNATURES = {
     "aj_w_compact_workspace_bytes": (_S, [_I, _I]),
     "aj_w_compact_curvature_interferometer_from": (_INT, [_P, _P, _I, _I, _D, _P, _P, _S, _P]),
     "aj_w_tilde_via_compact_from": (_INT, [_P, _I, _P, _I, _P, _P]),
+    "aj_w_tilde_via_preload_from": (_INT, [_P, _I, _I, _P, _I, _P, _P]),
     "aj_visibility_sum_workspace_bytes": (_S, [_I, _I]),
     "aj_w_tilde_data_interferometer_from": (_INT, [_P, _P, _P, _P, _I, _I, _P, _P, _S, _P]),
     "aj_dirty_image_from": (_INT, [_P, _P, _P, _P, _I, _I, _P, _P, _S, _P]),

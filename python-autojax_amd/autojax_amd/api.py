@@ -190,6 +190,24 @@ def w_tilde_via_compact_from(w_compact, native_index_for_slim_index):
     return _out(out, torch_in)
 
 
+def w_tilde_via_preload_from(w_tilde_preload, native_index_for_slim_index):
+    """numba.py:259-291 -> f8[M, M]: W[i, j] = w_tilde_preload[y_j - y_i, x_j - x_i], negative offsets
+    wrapping as numpy indexes do."""
+    torch_in = _any_t(w_tilde_preload, native_index_for_slim_index)
+    Pm = _dev(w_tilde_preload, np.float64, "w_tilde_preload", 2)
+    Py, Px = Pm.shape
+    ni_h = _host(native_index_for_slim_index)
+    if ni_h.ndim != 2 or ni_h.shape[1] != 2:
+        raise ValueError("native_index_for_slim_index must be [M, 2]")
+    if ni_h.size and (np.ptp(ni_h[:, 0]) >= Py or np.ptp(ni_h[:, 1]) >= Px):
+        raise IndexError("pixel offsets exceed the preload table")
+    ni = _dev(native_index_for_slim_index, np.int64, "native_index_for_slim_index", 2)
+    M = ni.shape[0]
+    out = dv.empty((M, M))
+    _lib.call("aj_w_tilde_via_preload_from", Pm.data_ptr(), Py, Px, ni.data_ptr(), M, out.data_ptr(), dv.stream_ptr())
+    return _out(out, torch_in)
+
+
 def w_tilde_data_interferometer_from(visibilities_real, noise_map_real, uv_wavelengths, grid_radians_slim):
     """numba.py:95-152 -> f8[M]."""
     torch_in = _any_t(visibilities_real, noise_map_real, uv_wavelengths, grid_radians_slim)
@@ -327,6 +345,17 @@ def curvature_matrix_via_w_compact_sparse_mapping_matrix_from(w_compact, native_
               idx.data_ptr(), w.data_ptr(), M, B, int(pixels), out.data_ptr(), ws.data_ptr(), ws.numel(),
               dv.stream_ptr())
     return _out(out, torch_in)
+
+
+def curvature_matrix_via_w_compact_sparse_mapping_matrix_direct_from(w_compact, native_index_for_slim_index,
+                                                                     pix_indexes_for_sub_slim_index,
+                                                                     pix_weights_for_sub_slim_index, pixels):
+    """numba.py:439-497 -> f8[pixels, pixels]: the same F as
+    curvature_matrix_via_w_compact_sparse_mapping_matrix_from (the reference's four-loop form); the
+    device computes it by the same two sparse products from the lattice table."""
+    return curvature_matrix_via_w_compact_sparse_mapping_matrix_from(
+        w_compact, native_index_for_slim_index, pix_indexes_for_sub_slim_index, pix_weights_for_sub_slim_index,
+        pixels)
 
 
 # ---------------------------------------------------------------------------

@@ -124,6 +124,20 @@ __global__ void compact_expand_kernel(const double* __restrict__ C, int64_t N, c
   }
 }
 
+// W[i, j] = preload[y_j - y_i, x_j - x_i] with negative offsets wrapping (numba.py:284-290).
+__global__ void preload_expand_kernel(const double* __restrict__ P, int64_t Py, int64_t Px,
+                                      const int64_t* __restrict__ native, int64_t M, double* __restrict__ W) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= M) return;
+  const int64_t yj = native[2 * j], xj = native[2 * j + 1];
+  for (int64_t i = blockIdx.y; i < M; i += gridDim.y) {
+    int64_t dy = yj - native[2 * i], dx = xj - native[2 * i + 1];
+    dy += dy < 0 ? Py : 0;
+    dx += dx < 0 ? Px : 0;
+    W[i * M + j] = P[dy * Px + dx];
+  }
+}
+
 struct Plan {
   int64_t ntn, ntm, nsplit, k_per_split;
   size_t bytes;
@@ -188,6 +202,19 @@ extern "C" int aj_w_tilde_via_compact_from(const double* w_compact, int64_t grid
   hipLaunchKernelGGL(cp::compact_expand_kernel,
                      dim3((unsigned)ceil_div(M, (int64_t)256), (unsigned)std::min<int64_t>(M, 32768)), dim3(256), 0,
                      as_stream(stream), w_compact, grid_size, native_index_for_slim_index, M, w_tilde);
+  AJ_LAUNCH_CHECK();
+  return AJ_OK;
+}
+
+extern "C" int aj_w_tilde_via_preload_from(const double* w_tilde_preload, int64_t preload_rows, int64_t preload_cols,
+                                           const int64_t* native_index_for_slim_index, int64_t M, double* w_tilde,
+                                           void* stream) {
+  AJ_REQUIRE(preload_rows > 0 && preload_cols > 0 && M > 0 && M < (1ll << 31), "bad sizes");
+  AJ_REQUIRE(w_tilde_preload && native_index_for_slim_index && w_tilde, "null pointer");
+  hipLaunchKernelGGL(cp::preload_expand_kernel,
+                     dim3((unsigned)ceil_div(M, (int64_t)256), (unsigned)std::min<int64_t>(M, 32768)), dim3(256), 0,
+                     as_stream(stream), w_tilde_preload, preload_rows, preload_cols, native_index_for_slim_index, M,
+                     w_tilde);
   AJ_LAUNCH_CHECK();
   return AJ_OK;
 }

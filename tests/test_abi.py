@@ -90,6 +90,8 @@ def test_drop_in_signatures_match_reference_names():
         "log_likelihood_function_via_w_tilde_from": ["dirty_image", "data", "noise_map", "w_tilde", "pix_indexes_for_sub_slim_index", "pix_weights_for_sub_slim_index", "neighbors", "neighbors_sizes"],
         "w_compact_curvature_interferometer_from": ["grid_size", "noise_map_real", "uv_wavelengths", "pixel_scale"],
         "w_tilde_via_compact_from": ["w_compact", "native_index_for_slim_index"],
+        "w_tilde_via_preload_from": ["w_tilde_preload", "native_index_for_slim_index"],
+        "curvature_matrix_via_w_compact_sparse_mapping_matrix_direct_from": ["w_compact", "native_index_for_slim_index", "pix_indexes_for_sub_slim_index", "pix_weights_for_sub_slim_index", "pixels"],
         "curvature_matrix_via_w_compact_sparse_mapping_matrix_from": ["w_compact", "native_index_for_slim_index", "pix_indexes_for_sub_slim_index", "pix_weights_for_sub_slim_index", "pixels"],
         "log_likelihood_function_via_w_compact_from": ["dirty_image", "data", "noise_map", "w_compact", "native_index_for_slim_index", "pix_indexes_for_sub_slim_index", "pix_weights_for_sub_slim_index", "neighbors", "neighbors_sizes"],
         "mask_2d_centres_from": ["shape_native", "pixel_scales", "centre"],

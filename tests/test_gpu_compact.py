@@ -67,6 +67,16 @@ def test_w_tilde_via_compact_is_the_lattice_gather(gpu, lattice):
     np.testing.assert_array_equal(got, R.w_tilde_via_compact_from(L["wc"], L["native"]))
 
 
+def test_w_tilde_via_preload_is_the_wrapped_gather(gpu, lattice):
+    from autojax_amd import api
+
+    L = lattice
+    N = L["N"]
+    pre = np.random.default_rng(3).standard_normal((2 * N, 2 * N))
+    got = api.w_tilde_via_preload_from(pre, L["native"])
+    np.testing.assert_array_equal(got, R.w_tilde_via_preload_from(pre, L["native"]))
+
+
 def test_compact_w_tilde_equals_feature_syrk(gpu, lattice):
     """Two independent device routes to the same W: lattice table + gather vs the FP64 feature SYRK."""
     from autojax_amd import api

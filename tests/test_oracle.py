@@ -90,6 +90,15 @@ def test_compact_curvature_and_likelihood_match_dense_route():
     F = R.curvature_matrix_via_w_compact_sparse_mapping_matrix_from(wc, native, idx, wts, S)
     T = R.mapping_matrix_from(idx, wts, S)
     np.testing.assert_allclose(F, T.T @ W @ T, rtol=0, atol=1e-11 * np.abs(F).max())
+    Fd = R.curvature_matrix_via_w_compact_sparse_mapping_matrix_direct_from(wc, native, idx, wts, S)
+    np.testing.assert_allclose(Fd, F, rtol=0, atol=1e-11 * np.abs(F).max())
+    # a preload table holding the same lattice values gives the same W (numba.py:259-291)
+    O = N - 1
+    pre = np.zeros((2 * N, 2 * N))
+    for a in range(-O, N):
+        for b in range(-O, N):
+            pre[-a, -b] = wc[a, b + O] if a >= 0 else wc[-a, -b + O]
+    np.testing.assert_allclose(R.w_tilde_via_preload_from(pre, native), W, rtol=0, atol=1e-10 * W.max())
     mesh = ds.rectangular_source(3)
     data = rng.standard_normal(K) + 1j * rng.standard_normal(K)
     noise = n * (1 + 1j)
