@@ -152,7 +152,7 @@ __global__ void data_vector_kernel(const SampleDesc* __restrict__ sd, const int*
 // once from HBM for the whole batch), then each thread forms Omega[m, s] for
 // one source pixel s and the R rows by gathering its bucket from LDS.
 // ---------------------------------------------------------------------------
-constexpr int OMEGA_THREADS = 256;
+constexpr int OMEGA_THREADS = 1024;
 constexpr size_t OMEGA_LDS_BUDGET = 150 * 1024;
 
 // Where W-tilde comes from: the dense M x M matrix, or the lattice table w_compact [N, 2N-1] with
@@ -197,7 +197,20 @@ __global__ void __launch_bounds__(OMEGA_THREADS) omega_kernel(const WSrc W, int6
       double acc[8];
 #pragma unroll
       for (int r = 0; r < 8; ++r) acc[r] = 0.0;
-      for (int e = p[s]; e < p[s + 1]; ++e) {
+      int e = p[s];
+      const int end = p[s + 1];
+      // four bucket entries loaded together (independent loads in flight), summed in entry order
+      for (; e + 4 <= end; e += 4) {
+        const int m0 = em[e], m1 = em[e + 1], m2 = em[e + 2], m3 = em[e + 3];
+        const double t0 = et[e], t1 = et[e + 1], t2 = et[e + 2], t3 = et[e + 3];
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+          if (r < rows) {
+            const double* wr = wrow + r * M;
+            acc[r] = fma(t3, wr[m3], fma(t2, wr[m2], fma(t1, wr[m1], fma(t0, wr[m0], acc[r]))));
+          }
+      }
+      for (; e < end; ++e) {
         const int m = em[e];
         const double t = et[e];
 #pragma unroll
@@ -451,53 +464,53 @@ __device__ inline double* mat_ptr(const SampleDesc* __restrict__ sd, double* __r
   return mats + d.mat_off + ((j & 1) ? d.S * d.S : 0);
 }
 
-__global__ void __launch_bounds__(256) chol_diag_kernel(const SampleDesc* __restrict__ sd,
-                                                        const StepEntry* __restrict__ e, int64_t k0, int kstep,
-                                                        int maxsteps, double* __restrict__ mats,
-                                                        double* __restrict__ ldp, int64_t* __restrict__ failp) {
-  __shared__ double Lkk[NB][NB + 1];
-  __shared__ int fail;
-  __shared__ double ld_acc;
+__device__ inline double readlane_d(double v, int l) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+
+// One wave per matrix: lane l < 32 holds row l of the 32 x 32 diagonal block in registers; for each
+// column the pivot and the column entries are broadcast with readlane (no barriers, no LDS).
+__global__ void __launch_bounds__(64) chol_diag_kernel(const SampleDesc* __restrict__ sd,
+                                                       const StepEntry* __restrict__ e, int64_t k0, int kstep,
+                                                       int maxsteps, double* __restrict__ mats,
+                                                       double* __restrict__ ldp, int64_t* __restrict__ failp) {
   const int j = e[blockIdx.x].mat;
   int64_t n;
   double* A = mat_ptr(sd, mats, j, n);
   const int nb = (int)std::min<int64_t>(NB, n - k0);
-  const int tid = threadIdx.x;
-  if (tid == 0) {
-    fail = 0;
-    ld_acc = 0.0;
-  }
-  for (int x = tid; x < NB * NB; x += 256) {
-    const int i = x / NB, c = x % NB;
-    Lkk[i][c] = (i < nb && c <= i) ? A[(k0 + i) * n + k0 + c] : 0.0;
-  }
-  __syncthreads();
-  for (int c = 0; c < nb; ++c) {
-    if (tid == 0) {
-      double dj = Lkk[c][c];
-      if (!(dj > 0.0)) {
-        if (!fail) fail = (int)(k0 + c + 1);
-        dj = 1.0;  // keep the arithmetic finite; the result is flagged
+  const int lane = threadIdx.x;
+  double r[NB];
+#pragma unroll
+  for (int c = 0; c < NB; ++c) r[c] = (lane < nb && c <= lane) ? A[(k0 + lane) * n + k0 + c] : 0.0;
+  double ld = 0.0;
+  int64_t fail = 0;
+#pragma unroll
+  for (int c = 0; c < NB; ++c) {
+    if (c < nb) {
+      double d = readlane_d(r[c], c);
+      if (!(d > 0.0)) {
+        if (!fail) fail = k0 + c + 1;
+        d = 1.0;  // keep the arithmetic finite; the result is flagged
       }
-      dj = sqrt(dj);
-      Lkk[c][c] = dj;
-      ld_acc += log(dj);
+      d = sqrt(d);
+      ld += log(d);
+      if (lane == c) r[c] = d;
+      if (lane > c) r[c] /= d;
+#pragma unroll
+      for (int q = c + 1; q < NB; ++q) {
+        const double lqc = readlane_d(r[c], q);  // L[q][c]
+        if (lane >= q) r[q] = fma(-r[c], lqc, r[q]);
+      }
     }
-    __syncthreads();
-    if (tid > c && tid < nb) Lkk[tid][c] /= Lkk[c][c];
-    __syncthreads();
-    for (int x = tid; x < NB * NB; x += 256) {
-      const int i = x / NB, l = x % NB;
-      if (i > c && l > c && l <= i && i < nb) Lkk[i][l] -= Lkk[i][c] * Lkk[l][c];
-    }
-    __syncthreads();
   }
-  for (int x = tid; x < NB * NB; x += 256) {
-    const int i = x / NB, c = x % NB;
-    if (i < nb && c <= i) A[(k0 + i) * n + k0 + c] = Lkk[i][c];
-  }
-  if (tid == 0) {
-    ldp[(int64_t)j * maxsteps + kstep] = ld_acc;
+#pragma unroll
+  for (int c = 0; c < NB; ++c)
+    if (lane < nb && c <= lane) A[(k0 + lane) * n + k0 + c] = r[c];
+  if (lane == 0) {
+    ldp[(int64_t)j * maxsteps + kstep] = ld;
     failp[(int64_t)j * maxsteps + kstep] = fail;
   }
 }
@@ -613,6 +626,8 @@ __global__ void chol_finish_kernel(const SampleDesc* __restrict__ sd, int nmat, 
   info[j] = f;
 }
 
+constexpr int SB = 64;  // row block of the triangular solves
+
 // ---------------------------------------------------------------------------
 // Solves L y = D, L^T s = y and the log-evidence, one workgroup per sample.
 //   logL = -1/2 (chi.chi - s.D + log|A| - log|H| + noise_norm)   (numba.py:852-857)
@@ -625,26 +640,86 @@ __global__ void __launch_bounds__(256) solve_loglike_kernel(const SampleDesc* __
                                                             double* __restrict__ recon, double* __restrict__ logl) {
   extern __shared__ double y[];
   __shared__ double red[256];
+  __shared__ double Ld[SB][SB + 1];
+  __shared__ double part[4][SB];
   const SampleDesc d = sd[blockIdx.x];
   const double* L = mats + d.mat_off;
   const int64_t n = d.S;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   for (int64_t i = tid; i < n; i += 256) y[i] = D[d.s_off + i];
   __syncthreads();
-  // forward: column-oriented
-  for (int64_t j = 0; j < n; ++j) {
-    const double yj = y[j] / L[j * n + j];
+  // Blocked substitution over 64-row blocks: the off-diagonal part of each block is a GEMV on
+  // contiguous row segments of L (all four waves), the 64 x 64 diagonal block is solved from LDS by
+  // one wave with the solved value broadcast by a lane shuffle.
+  // forward, L y = D: y_blk -= L[blk, 0:k0] y[0:k0]; then the diagonal block
+  for (int64_t k0 = 0; k0 < n; k0 += SB) {
+    const int nb = (int)std::min<int64_t>(SB, n - k0);
+    for (int e = tid; e < SB * SB; e += 256) {
+      const int r = e / SB, c = e % SB;
+      Ld[r][c] = (r < nb && c <= r) ? L[(k0 + r) * n + k0 + c] : 0.0;
+    }
+    // wave w: rows w, w+4, ..., four rows at a time (four independent loads in flight per lane)
+    for (int r0 = wave; r0 < nb; r0 += 16) {
+      double a[4] = {0.0, 0.0, 0.0, 0.0};
+      const double* Li[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) Li[q] = L + (k0 + std::min(r0 + 4 * q, nb - 1)) * n;
+      for (int64_t j = lane; j < k0; j += 64) {
+        const double yj = y[j];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) a[q] = fma(Li[q][j], yj, a[q]);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const double t = wave_sum(a[q]);
+        if (lane == 0 && r0 + 4 * q < nb) part[0][r0 + 4 * q] = t;
+      }
+    }
     __syncthreads();
-    if (tid == 0) y[j] = yj;
-    for (int64_t i = j + 1 + tid; i < n; i += 256) y[i] -= L[i * n + j] * yj;
+    if (wave == 0) {
+      double v = lane < nb ? y[k0 + lane] - part[0][lane] : 0.0;
+      for (int j = 0; j < nb; ++j) {
+        if (lane == j) v = v / Ld[j][j];
+        const double yj = __shfl(v, j);
+        if (lane > j) v = fma(-Ld[lane][j], yj, v);
+      }
+      if (lane < nb) y[k0 + lane] = v;
+    }
     __syncthreads();
   }
-  // backward: row j of L is column j of L^T
-  for (int64_t j = n - 1; j >= 0; --j) {
-    const double sj = y[j] / L[j * n + j];
+  // backward, L^T s = y: y_blk -= L[k0+nb:n, blk]^T s[k0+nb:n] (four row-interleaved partial sums,
+  // added in a fixed order); then the transposed diagonal block
+  for (int64_t k0 = ((n - 1) / SB) * SB; k0 >= 0; k0 -= SB) {
+    const int nb = (int)std::min<int64_t>(SB, n - k0);
+    for (int e = tid; e < SB * SB; e += 256) {
+      const int r = e / SB, c = e % SB;
+      Ld[r][c] = (r < nb && c <= r) ? L[(k0 + r) * n + k0 + c] : 0.0;
+    }
+    // thread (wave w, lane c): column k0 + c over rows j = k0 + nb + w, +4, ..., in four interleaved
+    // accumulators (four independent loads in flight), combined in a fixed order
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    if (lane < nb) {
+      const double* col = L + k0 + lane;
+      int64_t j = k0 + nb + wave;
+      for (; j + 12 < n; j += 16) {
+        a0 = fma(col[j * n], y[j], a0);
+        a1 = fma(col[(j + 4) * n], y[j + 4], a1);
+        a2 = fma(col[(j + 8) * n], y[j + 8], a2);
+        a3 = fma(col[(j + 12) * n], y[j + 12], a3);
+      }
+      for (; j < n; j += 4) a0 = fma(col[j * n], y[j], a0);
+    }
+    part[wave][lane] = (a0 + a1) + (a2 + a3);
     __syncthreads();
-    if (tid == 0) y[j] = sj;
-    for (int64_t i = tid; i < j; i += 256) y[i] -= L[j * n + i] * sj;
+    if (wave == 0) {
+      double v = lane < nb ? y[k0 + lane] - (((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane]) : 0.0;
+      for (int j = nb - 1; j >= 0; --j) {
+        if (lane == j) v = v / Ld[j][j];
+        const double sj = __shfl(v, j);
+        if (lane < j) v = fma(-Ld[j][lane], sj, v);
+      }
+      if (lane < nb) y[k0 + lane] = v;
+    }
     __syncthreads();
   }
   double acc = 0.0;
@@ -877,7 +952,7 @@ int batch_pipeline(const WSrc w_tilde, int64_t M, const double* dirty_image, con
   for (int64_t k = 0; k < p.maxsteps; ++k) {
     const StepEntry* e = steps + step_off[k];
     const int ne = (int)step_cnt[k];
-    hipLaunchKernelGGL(chol_diag_kernel, dim3((unsigned)ne), dim3(256), 0, st, desc, e, k * NB, (int)k,
+    hipLaunchKernelGGL(chol_diag_kernel, dim3((unsigned)ne), dim3(64), 0, st, desc, e, k * NB, (int)k,
                        (int)p.maxsteps, mats, ldp, failp);
     AJ_LAUNCH_CHECK();
     if (grid_p[k] > 0) {
