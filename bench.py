@@ -6,10 +6,11 @@ Delaunay source pixels each (own mesh, own barycentric mapping, B = 3) and
 lambda log-uniform in [0.01, 100].  Synthetic data (datasets.config5).
 
 One step = build W-tilde for the dataset + log-evidence of all 64 samples:
-  mixed  : W-tilde at --precision (default int8x5: exact int32 MFMA on five
-           signed 8-bit digits of FP64 cos/sin features, FP64 flush per
-           4096-visibility chunk; kernel variant picked by the autotuner before
-           the timed region), then one batched evaluation of all samples.
+  mixed  : W-tilde at --precision (default int8crt: FP64 cos/sin features on
+           the five-digit fixed-point grid, their exact integer Gram matrix per
+           16384-visibility chunk from its residues modulo 12 moduli, one int8
+           MFMA product each, recombined in FP64), then one batched evaluation
+           of all samples.
   plain  : W-tilde in FP64 (FP64 MFMA), then 64 sequential single-sample
            evaluations.
 Both use FP64 for F, H, Cholesky, solves and log-determinants.
@@ -46,8 +47,9 @@ for p in (ROOT, ROOT / "python-autojax_amd"):
 
 import numpy as np  # noqa: E402
 
-PEAKS_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0, "int8x3": 5000.0, "int8x4": 5000.0, "int8x5": 5000.0}
-RAW_FACTOR = {"fp64": 1, "fp32": 1, "bf16x3": 3, "int8x3": 9, "int8x4": 15, "int8x5": 22}
+PEAKS_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0, "int8x3": 5000.0, "int8x4": 5000.0, "int8x5": 5000.0,
+                "int8crt": 5000.0}
+RAW_FACTOR = {"fp64": 1, "fp32": 1, "bf16x3": 3, "int8x3": 9, "int8x4": 15, "int8x5": 22, "int8crt": 12}
 
 
 def log(msg):
@@ -61,6 +63,8 @@ I8_DIGITS = {"int8x3": (3, 5), "int8x4": (4, 6), "int8x5": (5, 7)}
 
 
 def syrk_kernel_name(precision, variant):
+    if precision == "int8crt":
+        return "crt::syrk_pipe8_kernel"
     if precision not in I8_DIGITS:
         return f"syrk_kernel<{precision}>"
     ns, ng = I8_DIGITS[precision]
@@ -120,7 +124,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", default="int8x5", choices=list(PEAKS_TFLOPS))
+    ap.add_argument("--precision", default="int8crt", choices=list(PEAKS_TFLOPS))
     ap.add_argument("--no-autotune", action="store_true", help="keep the default SYRK kernel variant")
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--samples", type=int, default=64)
@@ -229,7 +233,7 @@ def main():
     achieved = algo_flops_step * args.steps / (syrk_ms / 1e3) / 1e12
     peak = PEAKS_TFLOPS[args.precision]
     kernel = syrk_kernel_name(args.precision, min(tuned, key=tuned.get) if tuned else None)
-    wt_ms = (tm["wt_syrk"][0] + tm["wt_features"][0]) / args.steps
+    wt_ms = (tm["wt_syrk"][0] + tm["wt_features"][0] + tm["wt_combine"][0]) / args.steps
     batch_ms = ms_mixed - wt_ms
 
     line = {
@@ -258,6 +262,7 @@ def main():
         "nonpd_samples": info_bad,
         "breakdown_ms": {"w_tilde_syrk": round(tm["wt_syrk"][0] / args.steps, 3),
                          "w_tilde_features": round(tm["wt_features"][0] / args.steps, 3),
+                         "w_tilde_combine": round(tm["wt_combine"][0] / args.steps, 3),
                          "omega": round(tm["omega"][0] / args.steps, 3),
                          "fgather": round(tm["fgather"][0] / args.steps, 3),
                          "cholesky": round(tm["chol"][0] / args.steps, 3),

@@ -38,6 +38,9 @@ extern "C" {
 #define AJ_PREC_INT8X3 3 /* 3 signed 8-bit digits per feature, exact int32 accumulation */
 #define AJ_PREC_INT8X4 4 /* 4 signed 8-bit digits per feature, exact int32 accumulation */
 #define AJ_PREC_INT8X5 5 /* 5 signed 8-bit digits per feature, exact int32 accumulation */
+#define AJ_PREC_INT8CRT 6 /* features on the 5-digit fixed-point grid, exact integer Gram per chunk of
+                             visibilities from its residues modulo 12 moduli <= 256 (one int8 product
+                             each), recombined in FP64 (Chinese remainder theorem) */
 
 const char* aj_last_error(void);
 int aj_abi_version(void);
@@ -52,7 +55,8 @@ int aj_abi_version(void);
 #define AJ_TIMER_FGATHER 3     /* F = T^T Omega gather (batched) */
 #define AJ_TIMER_CHOL 4        /* batched Cholesky of A and H */
 #define AJ_TIMER_SOLVE 5       /* triangular solves + log-evidence */
-#define AJ_TIMER_COUNT 6
+#define AJ_TIMER_WT_COMBINE 6  /* W-tilde residue recombination (AJ_PREC_INT8CRT) */
+#define AJ_TIMER_COUNT 7
 int aj_timing_enable(int on);
 int aj_timing_read(int timer, double* total_ms, int64_t* count);
 
@@ -73,6 +77,15 @@ int aj_w_tilde_curvature_interferometer_from(const double* noise_map_real, const
 int aj_w_tilde_set_variant(int variant);
 int aj_w_tilde_variant_count(void);
 int aj_w_tilde_set_group_bytes(size_t bytes);
+
+/* Constants of the AJ_PREC_INT8CRT recombination, as the kernels use them (host-only query, no
+ * reference counterpart): moduli[12]; the CRT basis y_i = (P/m_i) ((P/m_i)^-1 mod m_i) mod P split as
+ * y_hi[i] (a multiple of 2^53) + y_lo[i] (an integer below 2^53); pq = {P_hi, P_lo, 1/P}. */
+int aj_crt_constants(double* moduli, double* y_hi, double* y_lo, double* pq);
+/* Tuning of AJ_PREC_INT8CRT (process-wide): residue SYRK kernel variant 0..3 (-1 = default 3; all
+ * compute identical residues) and visibilities per exact integer chunk (a multiple of 4096 up to
+ * 32768; 0 = default 16384).  The chunk size changes aj_w_tilde_workspace_bytes for this mode. */
+int aj_w_tilde_crt_configure(int variant, int64_t chunk_visibilities);
 
 /* numba.py:210-234 w_compact_curvature_interferometer_from(grid_size, noise_map_real f8[K],
  * uv_wavelengths f8[K,2], pixel_scale [arcsec]) -> f8[N, 2N-1], N = grid_size: the distinct values

@@ -3,11 +3,14 @@
     python -m autojax_amd.kernel_bench wtilde --M 5000 --K 131072 --precision int8x4 --reps 3
     python -m autojax_amd.kernel_bench batch --M 5000 --samples 64 --K 8192
     python -m autojax_amd.kernel_bench copy --mib 1024
+    python -m autojax_amd.kernel_bench intmm --M 8192
 
 ``wtilde`` builds W-tilde on synthetic data of the batching workload's
 generator; ``batch`` evaluates the batched log-evidence of a sample batch on
 one W-tilde; ``copy`` runs a plain device copy of known size, the reference
-for scaling the FETCH_SIZE / WRITE_SIZE counters.  Each prints one JSON line
+for scaling the FETCH_SIZE / WRITE_SIZE counters; ``intmm`` times the vendor
+library's int8 GEMM (torch._int_mm, M x M x M) as the device's achievable
+int8 matrix rate.  Each prints one JSON line
 with the per-call time measured with HIP events.
 """
 
@@ -22,7 +25,7 @@ import numpy as np
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["wtilde", "batch", "copy"])
+    ap.add_argument("what", choices=["wtilde", "batch", "copy", "intmm"])
     ap.add_argument("--M", type=int, default=5000)
     ap.add_argument("--K", type=int, default=131072)
     ap.add_argument("--precision", default="int8x4")
@@ -30,8 +33,11 @@ def main(argv=None):
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--mib", type=int, default=1024)
     ap.add_argument("--variant", type=int, default=None, help="integer SYRK kernel variant (LDS layout)")
+    ap.add_argument("--chunk", type=int, default=0, help="int8crt: visibilities per exact chunk (0 = default)")
     ap.add_argument("--check", action="store_true", help="compare W-tilde with kernel variant 0 (bitwise)")
     ap.add_argument("--group-mib", type=int, default=0, help="cap on the features of one SYRK launch (0 = default)")
+    ap.add_argument("--against", default=None, help="also build W-tilde at this precision and report the "
+                                                    "normwise difference max|dW| / max|W|")
     a = ap.parse_args(argv)
 
     import torch
@@ -52,7 +58,9 @@ def main(argv=None):
         torch.cuda.synchronize()
         return s.elapsed_time(e) / reps
 
-    if a.variant is not None:
+    if a.precision == "int8crt":
+        _lib.call("aj_w_tilde_crt_configure", -1 if a.variant is None else a.variant, a.chunk)
+    elif a.variant is not None:
         _lib.call("aj_w_tilde_set_variant", a.variant)
     _lib.call("aj_w_tilde_set_group_bytes", a.group_mib << 20)
     if a.what == "copy":
@@ -62,6 +70,13 @@ def main(argv=None):
         ms = timed(lambda: y.copy_(x), a.reps)
         print(json.dumps({"kernel": "copy", "bytes_read": n * 8, "bytes_written": n * 8, "ms": ms,
                           "GBps": 2 * n * 8 / ms / 1e6}), flush=True)
+        return
+    if a.what == "intmm":
+        n = a.M
+        x = torch.randint(-128, 128, (n, n), dtype=torch.int8, device="cuda")
+        y = torch.randint(-128, 128, (n, n), dtype=torch.int8, device="cuda")
+        ms = timed(lambda: torch._int_mm(x, y), a.reps)
+        print(json.dumps({"kernel": "torch._int_mm", "M": n, "ms": ms, "TOPS": 2.0 * n ** 3 / ms / 1e9}), flush=True)
         return
     rng = np.random.default_rng(5)
     grid, _ = dsm.grid_for_pixel_count(a.M, 0.05)
@@ -75,11 +90,19 @@ def main(argv=None):
     if a.what == "wtilde":
         ws = dv.workspace(_lib.query("aj_w_tilde_workspace_bytes", K, M, _lib.PRECISIONS[a.precision]))
         ms = timed(lambda: E.w_tilde(n_d, uv_d, g_d, a.precision, out=W, workspace=ws), a.reps)
-        rec = {"kernel": "w_tilde", "precision": a.precision, "variant": a.variant, "group_mib": a.group_mib,
+        rec = {"kernel": "w_tilde", "precision": a.precision, "variant": a.variant, "chunk": a.chunk,
+               "group_mib": a.group_mib,
                "M": M, "K": K, "ms": ms,
                "algorithmic_tflops": 2.0 * M * M * K / ms / 1e9}
+        if a.against:
+            Wb = E.w_tilde(n_d, uv_d, g_d, a.against)
+            rec["against"] = a.against
+            rec["normwise_diff"] = float(((W - Wb).abs().max() / Wb.abs().max()).item())
         if a.check:
-            _lib.call("aj_w_tilde_set_variant", 0)
+            if a.precision == "int8crt":
+                _lib.call("aj_w_tilde_crt_configure", 0, a.chunk)
+            else:
+                _lib.call("aj_w_tilde_set_variant", 0)
             W0 = E.w_tilde(n_d, uv_d, g_d, a.precision, workspace=ws)
             rec["max_abs_diff_vs_variant0"] = float((W - W0).abs().max().item())
         print(json.dumps(rec), flush=True)

@@ -1331,13 +1331,41 @@ int run(const double* noise, const double* uv, const double* grid, int64_t K, in
   return AJ_OK;
 }
 
+// The integer-residue (CRT) mode lives in wtilde_crt.hip: it accumulates the lower triangle of a
+// zeroed W; centring and the mirror are shared with the modes above.
+size_t crt_workspace_bytes(int64_t M);
+int crt_accumulate(const double* noise, const double* uv, const double* grid, const double* centre, int64_t K,
+                   int64_t M, double* W, void* workspace, size_t workspace_bytes, hipStream_t st);
+
+inline size_t crt_mode_bytes(int64_t M) { return 512 + crt_workspace_bytes(M); }
+
+int run_crt_mode(const double* noise, const double* uv, const double* grid, int64_t K, int64_t M, double* W,
+                 void* workspace, size_t workspace_bytes, hipStream_t st) {
+  if (workspace_bytes < crt_mode_bytes(M)) {
+    set_error("workspace too small: need %zu bytes, got %zu", crt_mode_bytes(M), workspace_bytes);
+    return AJ_ESPACE;
+  }
+  double* centre = static_cast<double*>(workspace);
+  hipLaunchKernelGGL(grid_mean_kernel, dim3(1), dim3(256), 0, st, grid, M, centre);
+  AJ_LAUNCH_CHECK();
+  AJ_CHECK_HIP(hipMemsetAsync(W, 0, sizeof(double) * M * M, st));
+  const int rc = crt_accumulate(noise, uv, grid, centre, K, M, W, static_cast<char*>(workspace) + 512,
+                                workspace_bytes - 512, st);
+  if (rc != AJ_OK) return rc;
+  dim3 mg((unsigned)ceil_div(M, 32), (unsigned)ceil_div(M, 32));
+  hipLaunchKernelGGL(mirror_kernel, mg, dim3(256), 0, st, W, M);
+  AJ_LAUNCH_CHECK();
+  return AJ_OK;
+}
+
 }  // namespace wt
 }  // namespace aj
 
 using namespace aj;
 
 extern "C" size_t aj_w_tilde_workspace_bytes(int64_t K, int64_t M, int precision) {
-  if (K < 0 || M <= 0 || precision < AJ_PREC_FP64 || precision > AJ_PREC_INT8X5) return 0;
+  if (K < 0 || M <= 0 || precision < AJ_PREC_FP64 || precision > AJ_PREC_INT8CRT) return 0;
+  if (precision == AJ_PREC_INT8CRT) return wt::crt_mode_bytes(M);
   return wt::make_plan(K, M, precision).bytes;
 }
 
@@ -1346,8 +1374,11 @@ extern "C" int aj_w_tilde_curvature_interferometer_from(const double* noise_map_
                                                         int precision, double* w_tilde, void* workspace,
                                                         size_t workspace_bytes, void* stream) {
   AJ_REQUIRE(M > 0 && K >= 0, "bad sizes M=%lld K=%lld", (long long)M, (long long)K);
-  AJ_REQUIRE(precision >= AJ_PREC_FP64 && precision <= AJ_PREC_INT8X5, "bad precision %d", precision);
+  AJ_REQUIRE(precision >= AJ_PREC_FP64 && precision <= AJ_PREC_INT8CRT, "bad precision %d", precision);
   AJ_REQUIRE(w_tilde && grid_radians_slim && (K == 0 || (noise_map_real && uv_wavelengths)), "null pointer");
+  if (precision == AJ_PREC_INT8CRT)
+    return wt::run_crt_mode(noise_map_real, uv_wavelengths, grid_radians_slim, K, M, w_tilde, workspace,
+                            workspace_bytes, as_stream(stream));
   wt::Plan p = wt::make_plan(K, M, precision);
   AJ_REQUIRE(workspace_bytes >= p.bytes && workspace, "workspace too small: need %zu bytes, got %zu", p.bytes,
              workspace_bytes);

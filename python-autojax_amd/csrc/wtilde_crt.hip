@@ -1,0 +1,858 @@
+// W-tilde as an exact integer Gram matrix assembled from its residues modulo twelve small moduli,
+// for gfx950.
+//
+// Reference: src/autojax/numba.py:155-207 (w_tilde_curvature_interferometer_from)
+//   W_ij = sum_k n_k^-2 cos(2 pi [(g_i1 - g_j1) u_k0 + (g_i0 - g_j0) u_k1])
+//
+// As in the other modes, W = [C S][C S]^T over cos/sin features of the FP64-reduced phases.  Per chunk
+// of kc visibilities (16384 by default, at most 32768) every feature sqrt(w_k) cos(.) (and sin) is divided by the chunk's largest
+// sqrt(w_k) and rounded to an integer q with |q| <= Q = 127 * 256^4 (the same fixed-point grid as the
+// five-digit mode).  The chunk's integer Gram matrix G_ij = sum_f q_if q_jf is then bounded by
+// kc (Q^2 + 2Q) < P/2, P = prod_i m_i over the moduli below, so G is determined by its residues
+// G mod m_i (Chinese remainder theorem).  Each residue is one int8 Gram product of the features'
+// balanced residues (|r| <= 128) on v_mfma_i32_16x16x64_i8, exact in int32 because
+// 2 kc * 128^2 < 2^31.  That is 12 int8 products per FP64-equivalent product, where splitting
+// q into five base-256 digits needs 22.  The residues are recombined in FP64 with the CRT basis
+// split into an exact high part (a multiple of 2^53) and a low part below 2^53, the integer-modular
+// emulation of Ozaki, Uchino and Imamura ("Ozaki Scheme II: a GEMM-oriented emulation of
+// floating-point matrix multiplication using an integer modular technique", 2025).  The result
+// is the Gram matrix of the quantised features to FP64 rounding, so it stays positive
+// semi-definite.
+//
+// Launches per chunk: scale (max 1/|n_k|), features (12 residue planes), SYRK (one workgroup per
+// (256 x 256 lower-triangle tile, modulus) unit, writing int8 residues of G), combine (CRT + FP64
+// accumulation into the lower triangle of W).
+
+#include "aj_common.h"
+
+#include <vector>
+
+namespace aj {
+namespace crt {
+
+typedef int i4 __attribute__((ext_vector_type(4)));
+
+constexpr int NMOD = 12;
+constexpr int KC_DEFAULT = 16384;  // visibilities per exact integer Gram matrix (a chunk)
+constexpr int KC_MAX = 32768;      // chunk bound: KC_MAX (Q^2 + 2Q) < P/2 and 2 KC_MAX 128^2 < 2^31
+constexpr int CT = 256;            // output tile edge (rows of A and of B per workgroup)
+constexpr int KB = 64;             // features per 1 KB block (16 rows x 64 bytes) and per LDS stage
+// A chunk of kc visibilities has 2 kc features (cos block, then sin block) = nkb = kc / 32 blocks
+// per row-block.
+constexpr int UNIT_BYTES = CT * CT;  // int8 residues of one tile
+
+// Pairwise coprime moduli, all <= 256 so that balanced residues fit in int8.
+constexpr int MODULI[NMOD] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217};
+
+// Largest feature integer: 127 * 256^4.
+constexpr double QMAX = 127.0 * 256.0 * 256.0 * 256.0 * 256.0;
+
+struct Consts {
+  double m[NMOD];      // moduli
+  double inv_m[NMOD];  // 1 / m_i (FP64)
+  double y_hi[NMOD];   // CRT basis y_i = (P/m_i) * ((P/m_i)^-1 mod m_i) mod P, high part: multiple of 2^53
+  double y_lo[NMOD];   // y_i - y_hi, an integer below 2^53 (exact)
+  double p_hi, p_lo;   // P split the same way
+  double inv_p;        // 1 / P
+};
+
+typedef unsigned __int128 u128;
+
+inline int64_t inv_mod(int64_t a, int64_t m) {
+  // extended Euclid; a and m coprime
+  int64_t t = 0, nt = 1, r = m, nr = a % m;
+  while (nr != 0) {
+    const int64_t q = r / nr;
+    int64_t tmp = t - q * nt;
+    t = nt;
+    nt = tmp;
+    tmp = r - q * nr;
+    r = nr;
+    nr = tmp;
+  }
+  return t < 0 ? t + m : t;
+}
+
+inline double u128_to_double_exact_hi(u128 v) {  // v is a multiple of 2^53 below 2^106
+  return std::ldexp((double)(uint64_t)(v >> 53), 53);
+}
+
+// CRT constants, computed once in exact 128-bit integer arithmetic (P < 2^95).
+inline Consts make_consts() {
+  Consts c{};
+  u128 P = 1;
+  for (int i = 0; i < NMOD; ++i) P *= (u128)MODULI[i];
+  const u128 mask53 = (((u128)1) << 53) - 1;
+  for (int i = 0; i < NMOD; ++i) {
+    const int64_t m = MODULI[i];
+    const u128 Pi = P / (u128)m;
+    const int64_t inv = inv_mod((int64_t)(Pi % (u128)m), m);
+    const u128 y = (Pi * (u128)inv) % P;
+    c.m[i] = (double)m;
+    c.inv_m[i] = 1.0 / (double)m;
+    c.y_hi[i] = u128_to_double_exact_hi(y & ~mask53);
+    c.y_lo[i] = (double)(uint64_t)(y & mask53);
+  }
+  c.p_hi = u128_to_double_exact_hi(P & ~mask53);
+  c.p_lo = (double)(uint64_t)(P & mask53);
+  c.inv_p = 1.0 / ((double)(uint64_t)(P >> 64) * 18446744073709551616.0 + (double)(uint64_t)P);
+  return c;
+}
+
+// ---------------------------------------------------------------------------
+// Chunk scale: max_k 1/|n_k| over the chunk's visibilities (1 if the chunk is empty).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) scale_kernel(const double* __restrict__ noise, int64_t K, int64_t k_begin,
+                                                    int kc, double* __restrict__ sc) {
+  __shared__ double red[256];
+  double m = 0.0;
+  for (int k = threadIdx.x; k < kc; k += 256) {
+    const int64_t kg = k_begin + k;
+    if (kg < K) m = fmax(m, 1.0 / fabs(noise[kg]));
+  }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sc[0] = red[0] > 0.0 ? red[0] : 1.0;
+}
+
+// ---------------------------------------------------------------------------
+// Features: residue planes X[m][rb][kb][16][64] (1 KB blocks: 16 image rows x 64 features), cos
+// features at kb < nkb/2, sin features above.  Thread = 4 consecutive visibilities x the 16 rows of
+// row-block blockIdx.y.  Padded rows and visibilities past K hold zeros.
+// ---------------------------------------------------------------------------
+__device__ inline int residue_of(double q, int i, const Consts& c) {
+  if (i == 0) return (int)(q - 256.0 * floor((q + 128.0) * (1.0 / 256.0)));  // [-128, 127]
+  // m odd: q/m is never a half-integer, and the product's rounding (|q| < 2^40) is far below the
+  // 1/(2m) gap, so rint selects the balanced residue in [-(m-1)/2, (m-1)/2].
+  const double n = rint(q * c.inv_m[i]);
+  return (int)fma(-n, c.m[i], q);
+}
+
+__global__ void __launch_bounds__(256) features_kernel(const double* __restrict__ noise,
+                                                       const double* __restrict__ uv,
+                                                       const double* __restrict__ grid,
+                                                       const double* __restrict__ centre, int64_t K, int64_t M,
+                                                       int64_t Mpad, int64_t k_begin, int kc,
+                                                       const double* __restrict__ sc, Consts c,
+                                                       int8_t* __restrict__ X) {
+  const int nkb = kc / 32;
+  const int k0 = (blockIdx.x * 256 + threadIdx.x) * 4;  // first of 4 visibilities (chunk-local)
+  const int64_t rb = blockIdx.y;
+  double u0[4], u1[4], qs[4];
+  const double inv_sc = QMAX / sc[0];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int64_t kg = k_begin + k0 + v;
+    const bool ok = kg < K;
+    u0[v] = ok ? uv[2 * kg] : 0.0;
+    u1[v] = ok ? uv[2 * kg + 1] : 0.0;
+    qs[v] = ok ? inv_sc / fabs(noise[kg]) : 0.0;  // zero scale: features of absent visibilities are 0
+  }
+  const double cy = centre[0], cx = centre[1];
+  const int64_t plane = Mpad * 2 * (int64_t)kc;
+  const int kbc = k0 / KB, kin = k0 % KB;
+  int8_t* base = X + rb * (int64_t)nkb * 1024 + kin;
+  for (int r = 0; r < 16; ++r) {
+    const int64_t i = rb * 16 + r;
+    double qc[4], qn[4];
+    if (i < M) {
+      const double gx = grid[2 * i + 1] - cx, gy = grid[2 * i] - cy;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        double rv = fma(gx, u0[v], gy * u1[v]);
+        rv -= rint(rv);
+        double s, co;
+        sincospi(2.0 * rv, &s, &co);
+        qc[v] = rint(co * qs[v]);
+        qn[v] = rint(s * qs[v]);
+      }
+    } else {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) qc[v] = qn[v] = 0.0;
+    }
+    int8_t* rowc = base + (int64_t)kbc * 1024 + r * 64;
+    int8_t* rows = rowc + (int64_t)(nkb / 2) * 1024;
+#pragma unroll
+    for (int m = 0; m < NMOD; ++m) {
+      uint32_t wc = 0, ws = 0;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        wc |= (uint32_t)(uint8_t)(int8_t)residue_of(qc[v], m, c) << (8 * v);
+        ws |= (uint32_t)(uint8_t)(int8_t)residue_of(qn[v], m, c) << (8 * v);
+      }
+      *reinterpret_cast<uint32_t*>(rowc + m * plane) = wc;
+      *reinterpret_cast<uint32_t*>(rows + m * plane) = ws;
+    }
+  }
+}
+
+// Residues of a wave's exact int32 Gram block (|acc| < 2^30), balanced, 4 per word.  Layout of a
+// unit: R[unit][rb][cb][lane] for the 16 x 16 block (rb, cb) of the 256 x 256 tile, each block's 64
+// words in MFMA output order (lane holds rows 4*(lane/16) + 0..3, column lane%16).  The wave's
+// block (a, b) is tile block (rb0 + a, cb0 + b).
+template <int NA, int NB>
+__device__ inline void store_residues(const i4 (&acc)[NA][NB], const Consts& c, int mi, int ntiles, int t, int rb0,
+                                      int cb0, int lane, int8_t* __restrict__ R) {
+  const double m = c.m[mi], inv_m = c.inv_m[mi];
+  uint32_t* out = reinterpret_cast<uint32_t*>(R + ((int64_t)mi * ntiles + t) * UNIT_BYTES) + lane;
+#pragma unroll
+  for (int a = 0; a < NA; ++a)
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      uint32_t w = 0;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int v = acc[a][b][r];
+        int res;
+        if (mi == 0) {
+          res = (int)(int8_t)(v & 0xFF);
+        } else {
+          const double d = (double)v;
+          res = (int)fma(-rint(d * inv_m), m, d);
+        }
+        w |= (uint32_t)(uint8_t)(int8_t)res << (8 * r);
+      }
+      out[((rb0 + a) * 16 + cb0 + b) * 64] = w;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// SYRK of one (tile, modulus) unit: the int32 Gram block of A = rows [I*256, +256) and B = rows
+// [J*256, +256) over the chunk's 2 kc features of residue plane m, reduced to balanced int8
+// residues mod m_m.  4 waves as 2 x 2, each a 128 x 128 block = 8 x 8 MFMA 16x16x64 blocks (256
+// int32 accumulators per lane).  Per stage of 64 features the workgroup stages 32 blocks of 1 KB
+// (16 of A, 16 of B) in LDS through registers, one stage ahead, rows of 64 bytes with 16-byte units
+// XOR-swizzled by (row/2)%4 (free of bank conflicts for the fragment reads).
+// Units: blockIdx % 8 selects the XCD; XCD x works through its run of `per` tiles (spatially
+// compact in the tile list) for modulus 0, then modulus 1, ..., so that all XCDs read the same
+// residue plane at a time and the tiles an XCD holds share row panels in its L2.
+// Residues are stored in MFMA output order: R[unit][wave][a][b][lane] as 4 packed int8 (rows
+// 4*(lane/16) + 0..3 of the 16x16 block, column lane%16).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256, 1) syrk_kernel(const int8_t* __restrict__ X, int64_t plane, int nkb,
+                                                      int ntiles, int per, const int* __restrict__ tiles,
+                                                      Consts c, int8_t* __restrict__ R) {
+  __shared__ __attribute__((aligned(1024))) int8_t lds[2][32][1024];
+  const int xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
+  const int mi = slot / per, loc = slot % per;
+  if (mi >= NMOD) return;
+  const int t = xcd * per + loc;
+  if (t >= ntiles) return;
+  const int64_t I = tiles[2 * t], J = tiles[2 * t + 1];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  // loads: thread moves 16 bytes (unit lane of a 1 KB block) of blocks wave + 4 j, j = 0..7
+  // (blocks 0-15: A row-blocks I*16 + b, 16-31: B row-blocks J*16 + b - 16)
+  const int8_t* srcA = X + mi * plane + (I * 16 + wave) * (int64_t)nkb * 1024 + lane * 16;
+  const int8_t* srcB = X + mi * plane + (J * 16 + wave) * (int64_t)nkb * 1024 + lane * 16;
+  const int64_t jstride = 4 * (int64_t)nkb * 1024;  // 4 row-blocks
+  const int wrow = lane >> 2, wunit = lane & 3;
+  const int woff = wrow * 64 + 16 * (wunit ^ ((wrow >> 1) & 3));
+  const int roff = lr * 64 + 16 * (lk ^ ((lr >> 1) & 3));
+  const int ablk = (wave >> 1) * 8, bblk = 16 + (wave & 1) * 8;
+
+  int8_t* const lbase = &lds[0][0][0];
+  i4 p[8];
+#define AJ_CRT_LOAD(ST)                                                                       \
+  _Pragma("unroll") for (int j = 0; j < 4; ++j) {                                             \
+    p[j] = *reinterpret_cast<const i4*>(srcA + j * jstride + (int64_t)(ST) * 1024);           \
+    p[4 + j] = *reinterpret_cast<const i4*>(srcB + j * jstride + (int64_t)(ST) * 1024);       \
+  }
+#define AJ_CRT_STORE(OFF)                                                                              \
+  _Pragma("unroll") for (int j = 0; j < 4; ++j) {                                                      \
+    *reinterpret_cast<i4*>(lbase + (OFF) + (wave + 4 * j) * 1024 + woff) = p[j];                       \
+    *reinterpret_cast<i4*>(lbase + (OFF) + (16 + wave + 4 * j) * 1024 + woff) = p[4 + j];              \
+  }
+
+  i4 acc[8][8];
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int b = 0; b < 8; ++b) acc[a][b] = i4{0, 0, 0, 0};
+
+  AJ_CRT_LOAD(0)
+  AJ_CRT_STORE(0)
+  __syncthreads();
+  for (int st = 0; st < nkb; ++st) {
+    const int cur = (st & 1) * 32768, nxt = cur ^ 32768;
+    const bool more = st + 1 < nkb;
+    // order: next stage's global loads, this stage's fragment reads and MFMAs, then the LDS
+    // stores of the next stage (which wait for its loads)
+    if (more) { AJ_CRT_LOAD(st + 1) }
+    __builtin_amdgcn_sched_barrier(0);
+    i4 fa[8], fb[8];
+    fa[0] = *reinterpret_cast<const i4*>(lbase + cur + ablk * 1024 + roff);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) fb[b] = *reinterpret_cast<const i4*>(lbase + cur + (bblk + b) * 1024 + roff);
+#pragma unroll
+    for (int a = 1; a < 8; ++a) fa[a] = *reinterpret_cast<const i4*>(lbase + cur + (ablk + a) * 1024 + roff);
+    // all 16 fragment reads are issued before the first MFMA (the MFMAs then wait only for the
+    // fragments they use, in read order)
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+      for (int b = 0; b < 8; ++b) acc[a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[a], fb[b], acc[a][b], 0, 0, 0);
+    __builtin_amdgcn_sched_barrier(0);
+    if (more) { AJ_CRT_STORE(nxt) }
+    __syncthreads();
+  }
+#undef AJ_CRT_LOAD
+#undef AJ_CRT_STORE
+
+  store_residues(acc, c, mi, ntiles, t, (wave >> 1) * 8, (wave & 1) * 8, lane, R);
+}
+
+// ---------------------------------------------------------------------------
+// The same unit with the feature blocks moved by direct global->LDS loads (global_load_lds, 16 bytes
+// per lane; lane L of a wave writes byte 16 L of the destination block, so the XOR swizzle is applied
+// by choosing each lane's source address) into a ring of four 32 KB stage buffers.  The loop runs
+// over intervals of two stages: one barrier per interval, the loads of interval j+1 issued right
+// after it (into the buffers interval j-1 used), then the fragment reads of both stages, the second
+// set landing while the MFMAs of the first run.
+// ---------------------------------------------------------------------------
+__device__ inline void lds_dma16(const int8_t* g, int8_t* lds_block) {
+  __builtin_amdgcn_global_load_lds((const void*)g,
+                                   (void __attribute__((address_space(3)))*)(
+                                       (__attribute__((address_space(3))) int8_t*)lds_block),
+                                   16, 0, 0);
+}
+
+__global__ void __launch_bounds__(256, 1) syrk_ring_kernel(const int8_t* __restrict__ X, int64_t plane, int nkb,
+                                                           int ntiles, int per, const int* __restrict__ tiles,
+                                                           Consts c, int8_t* __restrict__ R) {
+  __shared__ __attribute__((aligned(1024))) int8_t lds[4][32][1024];
+  const int xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
+  const int mi = slot / per, loc = slot % per;
+  if (mi >= NMOD) return;
+  const int t = xcd * per + loc;
+  if (t >= ntiles) return;
+  const int64_t I = tiles[2 * t], J = tiles[2 * t + 1];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  // source of lane L in a block: row L/4, global unit (L%4) ^ ((row/2)%4)
+  const int srow = lane >> 2;
+  const int soff = srow * 64 + 16 * ((lane & 3) ^ ((srow >> 1) & 3));
+  const int8_t* gA = X + mi * plane + (I * 16 + wave) * (int64_t)nkb * 1024 + soff;
+  const int8_t* gB = X + mi * plane + (J * 16 + wave) * (int64_t)nkb * 1024 + soff;
+  const int64_t jstride = 4 * (int64_t)nkb * 1024;
+  const int roff = lr * 64 + 16 * (lk ^ ((lr >> 1) & 3));
+  const int ablk = (wave >> 1) * 8, bblk = 16 + (wave & 1) * 8;
+
+#define AJ_RING_ISSUE(ST)                                                                    \
+  {                                                                                          \
+    const int64_t ko_ = (int64_t)(ST) * 1024;                                                \
+    int8_t* lb_ = &lds[(ST) & 3][0][0];                                                      \
+    _Pragma("unroll") for (int j = 0; j < 4; ++j) {                                          \
+      lds_dma16(gA + j * jstride + ko_, lb_ + (wave + 4 * j) * 1024);                        \
+      lds_dma16(gB + j * jstride + ko_, lb_ + (16 + wave + 4 * j) * 1024);                   \
+    }                                                                                        \
+  }
+
+  i4 acc[8][8];
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int b = 0; b < 8; ++b) acc[a][b] = i4{0, 0, 0, 0};
+
+  AJ_RING_ISSUE(0)
+  AJ_RING_ISSUE(1)
+  const int NI = nkb / 2;
+  for (int j = 0; j < NI; ++j) {
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's loads of interval j are in LDS
+    __syncthreads();                 // everyone's are; everyone finished reading interval j-1
+    if (j + 1 < NI) {
+      AJ_RING_ISSUE(2 * j + 2)
+      AJ_RING_ISSUE(2 * j + 3)
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    const int8_t* l0 = &lds[(2 * j) & 3][0][0];
+    const int8_t* l1 = &lds[(2 * j + 1) & 3][0][0];
+    i4 fa[8], fb[8], ga[8], gb[8];
+    fa[0] = *reinterpret_cast<const i4*>(l0 + ablk * 1024 + roff);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) fb[b] = *reinterpret_cast<const i4*>(l0 + (bblk + b) * 1024 + roff);
+#pragma unroll
+    for (int a = 1; a < 8; ++a) fa[a] = *reinterpret_cast<const i4*>(l0 + (ablk + a) * 1024 + roff);
+    ga[0] = *reinterpret_cast<const i4*>(l1 + ablk * 1024 + roff);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) gb[b] = *reinterpret_cast<const i4*>(l1 + (bblk + b) * 1024 + roff);
+#pragma unroll
+    for (int a = 1; a < 8; ++a) ga[a] = *reinterpret_cast<const i4*>(l1 + (ablk + a) * 1024 + roff);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+      for (int b = 0; b < 8; ++b) acc[a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[a], fb[b], acc[a][b], 0, 0, 0);
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+      for (int b = 0; b < 8; ++b) acc[a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ga[a], gb[b], acc[a][b], 0, 0, 0);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+#undef AJ_RING_ISSUE
+  store_residues(acc, c, mi, ntiles, t, (wave >> 1) * 8, (wave & 1) * 8, lane, R);
+}
+
+// ---------------------------------------------------------------------------
+// The ring kernel with eight waves (two per SIMD), each a 128 x 64 block of the tile (8 x 4 MFMA
+// blocks, 128 int32 accumulators per lane): the second wave on a SIMD runs its MFMAs while the
+// first waits for fragments, at the price of 1.5x the LDS fragment reads.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(512, 1) syrk_ring8_kernel(const int8_t* __restrict__ X, int64_t plane, int nkb,
+                                                            int ntiles, int per, const int* __restrict__ tiles,
+                                                            Consts c, int8_t* __restrict__ R) {
+  __shared__ __attribute__((aligned(1024))) int8_t lds[4][32][1024];
+  const int xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
+  const int mi = slot / per, loc = slot % per;
+  if (mi >= NMOD) return;
+  const int t = xcd * per + loc;
+  if (t >= ntiles) return;
+  const int64_t I = tiles[2 * t], J = tiles[2 * t + 1];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;  // 8 waves
+  const int lr = lane & 15, lk = lane >> 4;
+  const int srow = lane >> 2;
+  const int soff = srow * 64 + 16 * ((lane & 3) ^ ((srow >> 1) & 3));
+  // wave w moves blocks w and w + 8 of A (w < 8 covers 16 blocks) and of B
+  const int8_t* gA = X + mi * plane + (I * 16 + wave) * (int64_t)nkb * 1024 + soff;
+  const int8_t* gB = X + mi * plane + (J * 16 + wave) * (int64_t)nkb * 1024 + soff;
+  const int64_t jstride = 8 * (int64_t)nkb * 1024;
+  const int roff = lr * 64 + 16 * (lk ^ ((lr >> 1) & 3));
+  const int ablk = (wave >> 2) * 8, bblk = 16 + (wave & 3) * 4;
+
+#define AJ_RING8_ISSUE(ST)                                                                   \
+  {                                                                                          \
+    const int64_t ko_ = (int64_t)(ST) * 1024;                                                \
+    int8_t* lb_ = &lds[(ST) & 3][0][0];                                                      \
+    _Pragma("unroll") for (int j = 0; j < 2; ++j) {                                          \
+      lds_dma16(gA + j * jstride + ko_, lb_ + (wave + 8 * j) * 1024);                        \
+      lds_dma16(gB + j * jstride + ko_, lb_ + (16 + wave + 8 * j) * 1024);                   \
+    }                                                                                        \
+  }
+
+  i4 acc[8][4];
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = i4{0, 0, 0, 0};
+
+  AJ_RING8_ISSUE(0)
+  AJ_RING8_ISSUE(1)
+  const int NI = nkb / 2;
+  for (int j = 0; j < NI; ++j) {
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (j + 1 < NI) {
+      AJ_RING8_ISSUE(2 * j + 2)
+      AJ_RING8_ISSUE(2 * j + 3)
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      __builtin_amdgcn_sched_barrier(0);
+      const int8_t* l0 = &lds[(2 * j + h) & 3][0][0];
+      i4 fa[8], fb[4];
+      fa[0] = *reinterpret_cast<const i4*>(l0 + ablk * 1024 + roff);
+#pragma unroll
+      for (int b = 0; b < 4; ++b) fb[b] = *reinterpret_cast<const i4*>(l0 + (bblk + b) * 1024 + roff);
+#pragma unroll
+      for (int a = 1; a < 8; ++a) fa[a] = *reinterpret_cast<const i4*>(l0 + (ablk + a) * 1024 + roff);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          acc[a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[a], fb[b], acc[a][b], 0, 0, 0);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+#undef AJ_RING8_ISSUE
+  store_residues(acc, c, mi, ntiles, t, (wave >> 2) * 8, (wave & 3) * 4, lane, R);
+}
+
+// ---------------------------------------------------------------------------
+// Eight waves, one barrier per stage, loads three stages ahead in the four-buffer ring: the loads
+// of stage s + 3 go into the buffer stage s - 1 used, issued right after the barrier that ends
+// every wave's reads of it.  Before that barrier each wave waits until its loads of stage s have
+// landed (at most the loads of the two later stages still in flight).
+// ---------------------------------------------------------------------------
+constexpr unsigned vmcnt_imm(unsigned n) { return (n & 0xF) | ((n >> 4) << 14) | (0x7u << 4) | (0xFu << 8); }
+
+__global__ void __launch_bounds__(512, 1) syrk_deep8_kernel(const int8_t* __restrict__ X, int64_t plane, int nkb,
+                                                            int ntiles, int per, const int* __restrict__ tiles,
+                                                            Consts c, int8_t* __restrict__ R) {
+  __shared__ __attribute__((aligned(1024))) int8_t lds[4][32][1024];
+  const int xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
+  const int mi = slot / per, loc = slot % per;
+  if (mi >= NMOD) return;
+  const int t = xcd * per + loc;
+  if (t >= ntiles) return;
+  const int64_t I = tiles[2 * t], J = tiles[2 * t + 1];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int srow = lane >> 2;
+  const int soff = srow * 64 + 16 * ((lane & 3) ^ ((srow >> 1) & 3));
+  const int8_t* gA = X + mi * plane + (I * 16 + wave) * (int64_t)nkb * 1024 + soff;
+  const int8_t* gB = X + mi * plane + (J * 16 + wave) * (int64_t)nkb * 1024 + soff;
+  const int64_t jstride = 8 * (int64_t)nkb * 1024;
+  const int roff = lr * 64 + 16 * (lk ^ ((lr >> 1) & 3));
+  const int ablk = (wave >> 2) * 8, bblk = 16 + (wave & 3) * 4;
+
+#define AJ_DEEP_ISSUE(ST)                                                                    \
+  {                                                                                          \
+    const int64_t ko_ = (int64_t)(ST) * 1024;                                                \
+    int8_t* lb_ = &lds[(ST) & 3][0][0];                                                      \
+    _Pragma("unroll") for (int j = 0; j < 2; ++j) {                                          \
+      lds_dma16(gA + j * jstride + ko_, lb_ + (wave + 8 * j) * 1024);                        \
+      lds_dma16(gB + j * jstride + ko_, lb_ + (16 + wave + 8 * j) * 1024);                   \
+    }                                                                                        \
+  }
+
+  i4 acc[8][4];
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = i4{0, 0, 0, 0};
+
+  // nkb >= 128 (chunks of at least 4096 visibilities), so the three leading stages exist
+  AJ_DEEP_ISSUE(0)
+  AJ_DEEP_ISSUE(1)
+  AJ_DEEP_ISSUE(2)
+  for (int st = 0; st < nkb; ++st) {
+    // 4 loads per stage per thread, completing in order
+    const int later = min(2, nkb - 1 - st);
+    if (later == 2) __builtin_amdgcn_s_waitcnt(vmcnt_imm(8));
+    else if (later == 1) __builtin_amdgcn_s_waitcnt(vmcnt_imm(4));
+    else __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
+    // workgroup barrier without the release fence of __syncthreads, which would also wait for the
+    // loads of the two later stages; the memory clobber keeps LDS accesses on their side of it
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    if (st + 3 < nkb) { AJ_DEEP_ISSUE(st + 3) }
+    __builtin_amdgcn_sched_barrier(0);
+    const int8_t* l0 = &lds[st & 3][0][0];
+    i4 fa[8], fb[4];
+    fa[0] = *reinterpret_cast<const i4*>(l0 + ablk * 1024 + roff);
+#pragma unroll
+    for (int b = 0; b < 4; ++b) fb[b] = *reinterpret_cast<const i4*>(l0 + (bblk + b) * 1024 + roff);
+#pragma unroll
+    for (int a = 1; a < 8; ++a) fa[a] = *reinterpret_cast<const i4*>(l0 + (ablk + a) * 1024 + roff);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[a], fb[b], acc[a][b], 0, 0, 0);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+#undef AJ_DEEP_ISSUE
+  store_residues(acc, c, mi, ntiles, t, (wave >> 2) * 8, (wave & 3) * 4, lane, R);
+}
+
+// ---------------------------------------------------------------------------
+// Eight waves with the fragment reads software-pipelined across stages: after the barrier of
+// stage s every wave issues the loads of stage s + 3 (raw buffer loads straight to LDS, into the
+// buffer of stage s - 1) and the fragment reads of stage s + 1, then runs the MFMAs of stage s on
+// the fragments it read during stage s - 1.  The barrier of stage s follows each wave's wait for
+// its loads of stage s + 1 (the loads of stage s + 2 may still be in flight).
+// ---------------------------------------------------------------------------
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+
+__device__ void raw_buffer_load_lds(i32x4 rsrc, __attribute__((address_space(3))) uint32_t* lds_ptr, int size,
+                                    int voffset, int soffset, int offset, int aux)
+    __asm("llvm.amdgcn.raw.buffer.load.lds");
+
+// Buffer resource for byte offsets below 2^32 from base (stride 0, raw addressing).
+__device__ inline i32x4 make_rsrc(const void* base, uint32_t bytes) {
+  const uint64_t a = reinterpret_cast<uint64_t>(base);
+  i32x4 r;
+  r[0] = (int)(uint32_t)a;
+  r[1] = (int)(uint32_t)(a >> 32) & 0xFFFF;
+  r[2] = (int)bytes;
+  r[3] = 0x00020000;
+  return r;
+}
+
+__global__ void __launch_bounds__(512, 1) syrk_pipe8_kernel(const int8_t* __restrict__ X, int64_t plane, int nkb,
+                                                            int ntiles, int per, const int* __restrict__ tiles,
+                                                            Consts c, int8_t* __restrict__ R) {
+  __shared__ __attribute__((aligned(1024))) int8_t lds[4][32][1024];
+  const int xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
+  const int mi = slot / per, loc = slot % per;
+  if (mi >= NMOD) return;
+  const int t = xcd * per + loc;
+  if (t >= ntiles) return;
+  const int I = tiles[2 * t], J = tiles[2 * t + 1];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int srow = lane >> 2;
+  const int soff = srow * 64 + 16 * ((lane & 3) ^ ((srow >> 1) & 3));
+  const i32x4 rsrc = make_rsrc(X + mi * plane, (uint32_t)plane);
+  const uint32_t rbytes = (uint32_t)nkb * 1024u;  // bytes per row-block
+  const uint32_t oA = (uint32_t)(I * 16 + wave) * rbytes + soff;
+  const uint32_t oB = (uint32_t)(J * 16 + wave) * rbytes + soff;
+  const uint32_t jstr = 8u * rbytes;
+  const int roff = lr * 64 + 16 * (lk ^ ((lr >> 1) & 3));
+  const int ablk = (wave >> 2) * 8, bblk = 16 + (wave & 3) * 4;
+  typedef __attribute__((address_space(3))) uint32_t lds_u32;
+
+#define AJ_PIPE_ISSUE(ST)                                                                                \
+  {                                                                                                      \
+    const uint32_t ko_ = (uint32_t)(ST) * 1024u;                                                         \
+    int8_t* lb_ = &lds[(ST) & 3][0][0];                                                                  \
+    _Pragma("unroll") for (int j = 0; j < 2; ++j) {                                                      \
+      raw_buffer_load_lds(rsrc, (lds_u32*)(lb_ + (wave + 8 * j) * 1024), 16, oA + j * jstr + ko_, 0, 0, 0);      \
+      raw_buffer_load_lds(rsrc, (lds_u32*)(lb_ + (16 + wave + 8 * j) * 1024), 16, oB + j * jstr + ko_, 0, 0, 0); \
+    }                                                                                                    \
+  }
+#define AJ_PIPE_READ(FA, FB, ST)                                                                         \
+  {                                                                                                      \
+    const int8_t* l_ = &lds[(ST) & 3][0][0];                                                             \
+    FA[0] = *reinterpret_cast<const i4*>(l_ + ablk * 1024 + roff);                                       \
+    _Pragma("unroll") for (int b = 0; b < 4; ++b)                                                        \
+      FB[b] = *reinterpret_cast<const i4*>(l_ + (bblk + b) * 1024 + roff);                               \
+    _Pragma("unroll") for (int a = 1; a < 8; ++a)                                                        \
+      FA[a] = *reinterpret_cast<const i4*>(l_ + (ablk + a) * 1024 + roff);                               \
+  }
+#define AJ_PIPE_MFMA(FA, FB)                                                                             \
+  _Pragma("unroll") for (int a = 0; a < 8; ++a)                                                          \
+  _Pragma("unroll") for (int b = 0; b < 4; ++b)                                                          \
+    acc[a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(FA[a], FB[b], acc[a][b], 0, 0, 0);
+  // wait until at most N of this wave's buffer loads are in flight, then the workgroup barrier (no
+  // release fence: it would wait for every load); the memory clobber keeps LDS accesses in place
+#define AJ_PIPE_SYNC(N)                                                                                  \
+  {                                                                                                      \
+    __builtin_amdgcn_s_waitcnt(vmcnt_imm(N));                                                            \
+    asm volatile("s_barrier" ::: "memory");                                                              \
+  }
+// Branch-free, so that the waits before the MFMAs count only the reads issued one step earlier.  Past
+// the chunk's last stage the loads fetch the next row-block's features, or past the end of the plane
+// (the buffer range) zeros, into buffers that are no longer read, and the reads of stage nkb are
+// never used.
+#define AJ_PIPE_STEP(FA, FB, GA, GB, ST)                                                                 \
+  {                                                                                                      \
+    const int s_ = (ST);                                                                                 \
+    AJ_PIPE_SYNC(4)                                                                                      \
+    AJ_PIPE_ISSUE(s_ + 3)                                                                                \
+    AJ_PIPE_READ(GA, GB, s_ + 1)                                                                         \
+    __builtin_amdgcn_sched_barrier(0);                                                                   \
+    AJ_PIPE_MFMA(FA, FB)                                                                                 \
+    __builtin_amdgcn_sched_barrier(0);                                                                   \
+  }
+
+  i4 acc[8][4];
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = i4{0, 0, 0, 0};
+
+  // nkb >= 128 and even
+  i4 fa0[8], fb0[4], fa1[8], fb1[4];
+  AJ_PIPE_ISSUE(0)
+  AJ_PIPE_ISSUE(1)
+  AJ_PIPE_ISSUE(2)
+  // stage 0 landed everywhere; waiting for every counter here also leaves nothing but this wave's
+  // LDS reads pending at the loop head, so the waits before the MFMAs can count them
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  AJ_PIPE_READ(fa0, fb0, 0)
+  for (int st = 0; st < nkb; st += 2) {
+    AJ_PIPE_STEP(fa0, fb0, fa1, fb1, st)
+    AJ_PIPE_STEP(fa1, fb1, fa0, fb0, st + 1)
+  }
+#undef AJ_PIPE_ISSUE
+#undef AJ_PIPE_READ
+#undef AJ_PIPE_MFMA
+#undef AJ_PIPE_SYNC
+#undef AJ_PIPE_STEP
+  // the loads issued past the last stage must land before the workgroup's LDS is released
+  __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
+  store_residues(acc, c, mi, ntiles, t, (wave >> 2) * 8, (wave & 3) * 4, lane, R);
+}
+
+// ---------------------------------------------------------------------------
+// Combine: per element of the lower triangle, G from its 12 residues, W += G * scale.
+//   S_hi = sum_i r_i y_hi_i  (exact: every term and partial sum is a multiple of 2^53 below 2^106)
+//   S_lo = sum_i r_i y_lo_i  (|.| < 2^64, rounding error below 2^16)
+//   n    = rint((S_hi + S_lo) / P)
+//   G    = (S_hi - n p_hi) + (S_lo - n p_lo)   (first bracket exact)
+// Thread = one packed word of one unit: 4 elements (one column, 4 rows) of a 16 x 16 block.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) combine_kernel(const int8_t* __restrict__ R, int ntiles,
+                                                      const int* __restrict__ tiles, Consts c,
+                                                      const double* __restrict__ sc, double* __restrict__ W,
+                                                      int64_t M) {
+  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int t = (int)(g >> 14);
+  if (t >= ntiles) return;
+  const int rem = (int)(g & 16383);
+  const int rb = rem >> 10, cb = (rem >> 6) & 15, lane = rem & 63;
+  const int64_t i0 = (int64_t)tiles[2 * t] * CT + rb * 16 + 4 * (lane >> 4);
+  const int64_t j = (int64_t)tiles[2 * t + 1] * CT + cb * 16 + (lane & 15);
+  if (j >= M || i0 >= M || i0 + 3 < j) return;
+  uint32_t w[NMOD];
+#pragma unroll
+  for (int m = 0; m < NMOD; ++m)
+    w[m] = reinterpret_cast<const uint32_t*>(R + ((int64_t)m * ntiles + t) * UNIT_BYTES)[rem];
+  const double s2 = sc[0] * sc[0] / (QMAX * QMAX);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int64_t i = i0 + r;
+    if (i >= M || i < j) continue;
+    double shi = 0.0, slo = 0.0;
+#pragma unroll
+    for (int m = 0; m < NMOD; ++m) {
+      const double rv = (double)(int8_t)(w[m] >> (8 * r));
+      shi = fma(rv, c.y_hi[m], shi);
+      slo = fma(rv, c.y_lo[m], slo);
+    }
+    const double n = rint((shi + slo) * c.inv_p);
+    const double G = fma(-n, c.p_hi, shi) + fma(-n, c.p_lo, slo);
+    W[i * M + j] += G * s2;
+  }
+}
+
+// Lower-triangle 256-tiles in 4 x 4 super-tiles, so that a run of consecutive tiles is compact.
+inline void build_tiles(int64_t nI, std::vector<int>& out) {
+  out.clear();
+  const int64_t nS = (nI + 3) / 4;
+  for (int64_t SI = 0; SI < nS; ++SI)
+    for (int64_t SJ = 0; SJ <= SI; ++SJ)
+      for (int64_t i = 0; i < 4; ++i)
+        for (int64_t j = 0; j < 4; ++j) {
+          const int64_t I = 4 * SI + i, J = 4 * SJ + j;
+          if (I < nI && J <= I) {
+            out.push_back((int)I);
+            out.push_back((int)J);
+          }
+        }
+}
+
+struct Plan {
+  int64_t Mpad, nI, ntiles, per;
+  size_t feat_bytes, res_bytes, bytes;
+};
+
+inline Plan make_plan(int64_t M, int kc) {
+  Plan p{};
+  p.Mpad = ceil_div(M, CT) * CT;
+  p.nI = p.Mpad / CT;
+  p.ntiles = p.nI * (p.nI + 1) / 2;
+  p.per = ceil_div(p.ntiles, 8);
+  p.feat_bytes = (size_t)NMOD * p.Mpad * 2 * kc;
+  p.res_bytes = (size_t)NMOD * p.ntiles * UNIT_BYTES;
+  p.bytes = align_up(p.feat_bytes, 256) + align_up(p.res_bytes, 256) + align_up(sizeof(int) * 2 * p.ntiles, 256) +
+            256 /*scale*/ + 256;
+  return p;
+}
+
+}  // namespace crt
+
+namespace wt {
+
+// Residue SYRK kernel (0 = direct-to-LDS ring, four waves; 1 = register-staged double buffer; 2 =
+// ring, eight waves, two stages per barrier; 3 = ring, eight waves, one stage per barrier, loads three
+// stages ahead; 4 = as 3 with buffer loads and the fragment reads one stage ahead) and visibilities
+// per chunk; process-wide tuning.
+int g_crt_variant = 4;
+int g_crt_kc = crt::KC_DEFAULT;
+
+size_t crt_workspace_bytes(int64_t M) { return crt::make_plan(M, g_crt_kc).bytes; }
+
+// Accumulate the lower triangle of W (zeroed by the caller) over all visibilities.
+int crt_accumulate(const double* noise, const double* uv, const double* grid, const double* centre, int64_t K,
+                   int64_t M, double* W, void* workspace, size_t workspace_bytes, hipStream_t st) {
+  using namespace crt;
+  const int kc = g_crt_kc, nkb = kc / 32;
+  const Plan p = make_plan(M, kc);
+  Arena ar(workspace, workspace_bytes);
+  int8_t* X = ar.take<int8_t>(p.feat_bytes);
+  int8_t* R = ar.take<int8_t>(p.res_bytes);
+  int* tiles = ar.take<int>(2 * p.ntiles);
+  double* sc = ar.take<double>(1);
+  if (!ar.ok()) {
+    set_error("workspace too small: need %zu bytes, got %zu", p.bytes, workspace_bytes);
+    return AJ_ESPACE;
+  }
+  static const Consts cst = make_consts();
+  static thread_local std::vector<int> tl;
+  build_tiles(p.nI, tl);
+  AJ_CHECK_HIP(hipMemcpyAsync(tiles, tl.data(), sizeof(int) * tl.size(), hipMemcpyHostToDevice, st));
+  const int64_t nch = ceil_div(K, kc);
+  const int64_t plane = p.Mpad * 2 * (int64_t)kc;
+  const dim3 fg((unsigned)(kc / 1024), (unsigned)(p.Mpad / 16));
+  const dim3 sg((unsigned)(8 * p.per * NMOD));
+  const dim3 cg((unsigned)ceil_div(p.ntiles * 16384, 256));
+  const int nt = (int)p.ntiles, per = (int)p.per;
+  for (int64_t ch = 0; ch < nch; ++ch) {
+    const int64_t k_begin = ch * kc;
+    hipLaunchKernelGGL(scale_kernel, dim3(1), dim3(256), 0, st, noise, K, k_begin, kc, sc);
+    AJ_LAUNCH_CHECK();
+    timer_begin(AJ_TIMER_WT_FEATURES, st);
+    hipLaunchKernelGGL(features_kernel, fg, dim3(256), 0, st, noise, uv, grid, centre, K, M, p.Mpad, k_begin, kc,
+                       sc, cst, X);
+    timer_end(AJ_TIMER_WT_FEATURES, st);
+    AJ_LAUNCH_CHECK();
+    timer_begin(AJ_TIMER_WT_SYRK, st);
+    switch (g_crt_variant) {
+      case 0: hipLaunchKernelGGL(syrk_ring_kernel, sg, dim3(256), 0, st, X, plane, nkb, nt, per, tiles, cst, R); break;
+      case 1: hipLaunchKernelGGL(syrk_kernel, sg, dim3(256), 0, st, X, plane, nkb, nt, per, tiles, cst, R); break;
+      case 2: hipLaunchKernelGGL(syrk_ring8_kernel, sg, dim3(512), 0, st, X, plane, nkb, nt, per, tiles, cst, R); break;
+      case 3: hipLaunchKernelGGL(syrk_deep8_kernel, sg, dim3(512), 0, st, X, plane, nkb, nt, per, tiles, cst, R); break;
+      default: hipLaunchKernelGGL(syrk_pipe8_kernel, sg, dim3(512), 0, st, X, plane, nkb, nt, per, tiles, cst, R); break;
+    }
+    timer_end(AJ_TIMER_WT_SYRK, st);
+    AJ_LAUNCH_CHECK();
+    timer_begin(AJ_TIMER_WT_COMBINE, st);
+    hipLaunchKernelGGL(combine_kernel, cg, dim3(256), 0, st, R, nt, tiles, cst, sc, W, M);
+    timer_end(AJ_TIMER_WT_COMBINE, st);
+    AJ_LAUNCH_CHECK();
+  }
+  return AJ_OK;
+}
+
+}  // namespace wt
+}  // namespace aj
+
+// Residue SYRK kernel variant (0..4, -1 = default) and visibilities per chunk (a multiple of 4096 up to
+// 32768, 0 = default 16384); tuning, no reference counterpart.
+extern "C" int aj_w_tilde_crt_configure(int variant, int64_t chunk_visibilities) {
+  if (variant < -1 || variant > 4) {
+    aj::set_error("residue SYRK variant %d outside [-1, 4]", variant);
+    return AJ_EINVAL;
+  }
+  if (chunk_visibilities != 0 &&
+      (chunk_visibilities < 4096 || chunk_visibilities > aj::crt::KC_MAX || chunk_visibilities % 4096 != 0)) {
+    aj::set_error("chunk of %lld visibilities: need a multiple of 4096 in [4096, %d]", (long long)chunk_visibilities,
+                  aj::crt::KC_MAX);
+    return AJ_EINVAL;
+  }
+  aj::wt::g_crt_variant = variant < 0 ? 4 : variant;
+  aj::wt::g_crt_kc = chunk_visibilities ? (int)chunk_visibilities : aj::crt::KC_DEFAULT;
+  return AJ_OK;
+}
+
+// CRT constants as the kernels use them (host query, no reference counterpart): fills
+// moduli[12], y_hi[12], y_lo[12] and pq[3] = {p_hi, p_lo, inv_p}.
+extern "C" int aj_crt_constants(double* moduli, double* y_hi, double* y_lo, double* pq) {
+  if (!moduli || !y_hi || !y_lo || !pq) {
+    aj::set_error("null pointer");
+    return AJ_EINVAL;
+  }
+  const aj::crt::Consts c = aj::crt::make_consts();
+  for (int i = 0; i < aj::crt::NMOD; ++i) {
+    moduli[i] = c.m[i];
+    y_hi[i] = c.y_hi[i];
+    y_lo[i] = c.y_lo[i];
+  }
+  pq[0] = c.p_hi;
+  pq[1] = c.p_lo;
+  pq[2] = c.inv_p;
+  return AJ_OK;
+}

@@ -13,7 +13,8 @@ pytestmark = pytest.mark.gpu
 GOLD = Path(__file__).resolve().parent / "golden"
 
 # normwise tolerances of W-tilde against the FP64 oracle, relative to max|W| (= the diagonal)
-WT_TOL = {"fp64": 1e-12, "int8x4": 5e-8, "int8x5": 5e-12, "int8x3": 3e-7, "fp32": 3e-6, "bf16x3": 3e-4}
+WT_TOL = {"fp64": 1e-12, "int8x4": 5e-8, "int8x5": 5e-12, "int8crt": 5e-12, "int8x3": 3e-7, "fp32": 3e-6,
+          "bf16x3": 3e-4}
 
 
 @pytest.fixture(scope="module")
@@ -46,7 +47,7 @@ def test_w_tilde_fp64_matches_fsum_golden(gpu):
     assert _rel(W, g["w_tilde"]) < 1e-13
 
 
-@pytest.mark.parametrize("precision", ["fp64", "int8x4", "int8x5", "int8x3", "fp32", "bf16x3"])
+@pytest.mark.parametrize("precision", ["fp64", "int8x4", "int8x5", "int8crt", "int8x3", "fp32", "bf16x3"])
 def test_w_tilde_modes_vs_oracle(gpu, cfg1_small, precision):
     from autojax_amd import api
 
@@ -73,6 +74,53 @@ def test_w_tilde_multichunk_rows(gpu):
         ph = dg[:, 1:2] * d.uv_wavelengths[None, :, 0] + dg[:, 0:1] * d.uv_wavelengths[None, :, 1]
         want = np.cos(ph) @ (1 / n**2)
         assert np.abs(W[i] - want).max() < 1e-12 * want.max()
+
+
+def test_w_tilde_crt_multichunk(gpu):
+    """The residue (CRT) mode over three 16384-visibility chunks, the last one ragged: against the
+    oracle on a row subset, against the five-digit mode (same fixed-point grid) and the FP64 mode on
+    the whole matrix, and bitwise run to run."""
+    from autojax_amd import api
+
+    d = ds.config1(K=40000)
+    args = (d.noise_map_real, d.uv_wavelengths, d.grid_radians_slim)
+    W = api.w_tilde_curvature_interferometer_from(*args, precision="int8crt")
+    np.testing.assert_array_equal(W, api.w_tilde_curvature_interferometer_from(*args, precision="int8crt"))
+    np.testing.assert_array_equal(W, W.T)
+    W5 = api.w_tilde_curvature_interferometer_from(*args, precision="int8x5")
+    W64 = api.w_tilde_curvature_interferometer_from(*args, precision="fp64")
+    print(f"int8crt vs int8x5 {_rel(W, W5):.3e}, vs fp64 {_rel(W, W64):.3e}", flush=True)
+    assert _rel(W, W5) < 5e-12 and _rel(W, W64) < 5e-12
+    g, n = d.grid_radians_slim, d.noise_map_real
+    for i in (0, 1, 255, 256, 511, 715):
+        dg = 2 * np.pi * (g[i] - g)
+        ph = dg[:, 1:2] * d.uv_wavelengths[None, :, 0] + dg[:, 0:1] * d.uv_wavelengths[None, :, 1]
+        want = np.cos(ph) @ (1 / n**2)
+        assert np.abs(W[i] - want).max() < 5e-12 * want.max()
+
+
+def test_w_tilde_crt_variants_agree(gpu):
+    """All residue SYRK kernels and chunk sizes give the same W-tilde: bitwise for one chunk size
+    (two chunks, the last one ragged, M not a multiple of 256), to FP64 rounding across chunk sizes
+    (the per-chunk scale and the FP64 accumulation order change)."""
+    from autojax_amd import _lib, api
+
+    d = ds.config1(K=20000)
+    args = (d.noise_map_real, d.uv_wavelengths, d.grid_radians_slim)
+    out = []
+    try:
+        for v in (0, 1, 2, 3, 4):
+            _lib.call("aj_w_tilde_crt_configure", v, 16384)
+            out.append(api.w_tilde_curvature_interferometer_from(*args, precision="int8crt"))
+        for kc in (4096, 32768):
+            _lib.call("aj_w_tilde_crt_configure", -1, kc)
+            W = api.w_tilde_curvature_interferometer_from(*args, precision="int8crt")
+            print(f"chunk {kc}: {_rel(W, out[0]):.3e}", flush=True)
+            assert _rel(W, out[0]) < 1e-13
+    finally:
+        _lib.call("aj_w_tilde_crt_configure", -1, 0)
+    for W in out[1:]:
+        np.testing.assert_array_equal(W, out[0])
 
 
 def test_w_tilde_run_to_run_bitwise(gpu, cfg1_small):
