@@ -82,7 +82,7 @@ int aj_w_tilde_set_group_bytes(size_t bytes);
  * reference counterpart): moduli[12]; the CRT basis y_i = (P/m_i) ((P/m_i)^-1 mod m_i) mod P split as
  * y_hi[i] (a multiple of 2^53) + y_lo[i] (an integer below 2^53); pq = {P_hi, P_lo, 1/P}. */
 int aj_crt_constants(double* moduli, double* y_hi, double* y_lo, double* pq);
-/* Tuning of AJ_PREC_INT8CRT (process-wide): residue SYRK kernel variant 0..3 (-1 = default 3; all
+/* Tuning of AJ_PREC_INT8CRT (process-wide): residue SYRK kernel variant 0..2 (-1 = default 0; all
  * compute identical residues) and visibilities per exact integer chunk (a multiple of 4096 up to
  * 32768; 0 = default 16384).  The chunk size changes aj_w_tilde_workspace_bytes for this mode. */
 int aj_w_tilde_crt_configure(int variant, int64_t chunk_visibilities);

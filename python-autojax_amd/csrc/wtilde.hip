@@ -1333,16 +1333,16 @@ int run(const double* noise, const double* uv, const double* grid, int64_t K, in
 
 // The integer-residue (CRT) mode lives in wtilde_crt.hip: it accumulates the lower triangle of a
 // zeroed W; centring and the mirror are shared with the modes above.
-size_t crt_workspace_bytes(int64_t M);
+size_t crt_workspace_bytes(int64_t M, int64_t K);
 int crt_accumulate(const double* noise, const double* uv, const double* grid, const double* centre, int64_t K,
                    int64_t M, double* W, void* workspace, size_t workspace_bytes, hipStream_t st);
 
-inline size_t crt_mode_bytes(int64_t M) { return 512 + crt_workspace_bytes(M); }
+inline size_t crt_mode_bytes(int64_t M, int64_t K) { return 512 + crt_workspace_bytes(M, K); }
 
 int run_crt_mode(const double* noise, const double* uv, const double* grid, int64_t K, int64_t M, double* W,
                  void* workspace, size_t workspace_bytes, hipStream_t st) {
-  if (workspace_bytes < crt_mode_bytes(M)) {
-    set_error("workspace too small: need %zu bytes, got %zu", crt_mode_bytes(M), workspace_bytes);
+  if (workspace_bytes < crt_mode_bytes(M, K)) {
+    set_error("workspace too small: need %zu bytes, got %zu", crt_mode_bytes(M, K), workspace_bytes);
     return AJ_ESPACE;
   }
   double* centre = static_cast<double*>(workspace);
@@ -1365,7 +1365,7 @@ using namespace aj;
 
 extern "C" size_t aj_w_tilde_workspace_bytes(int64_t K, int64_t M, int precision) {
   if (K < 0 || M <= 0 || precision < AJ_PREC_FP64 || precision > AJ_PREC_INT8CRT) return 0;
-  if (precision == AJ_PREC_INT8CRT) return wt::crt_mode_bytes(M);
+  if (precision == AJ_PREC_INT8CRT) return wt::crt_mode_bytes(M, K);
   return wt::make_plan(K, M, precision).bytes;
 }
 

@@ -100,11 +100,13 @@ inline Consts make_consts() {
 }
 
 // ---------------------------------------------------------------------------
-// Chunk scale: max_k 1/|n_k| over the chunk's visibilities (1 if the chunk is empty).
+// Chunk scales: sc[c] = max_k 1/|n_k| over chunk c's visibilities (1 if the chunk is empty); one
+// workgroup per chunk.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) scale_kernel(const double* __restrict__ noise, int64_t K, int64_t k_begin,
-                                                    int kc, double* __restrict__ sc) {
+__global__ void __launch_bounds__(256) scale_kernel(const double* __restrict__ noise, int64_t K, int kc,
+                                                    double* __restrict__ sc) {
   __shared__ double red[256];
+  const int64_t k_begin = (int64_t)blockIdx.x * kc;
   double m = 0.0;
   for (int k = threadIdx.x; k < kc; k += 256) {
     const int64_t kg = k_begin + k;
@@ -116,36 +118,90 @@ __global__ void __launch_bounds__(256) scale_kernel(const double* __restrict__ n
     if (threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) sc[0] = red[0] > 0.0 ? red[0] : 1.0;
+  if (threadIdx.x == 0) sc[blockIdx.x] = red[0] > 0.0 ? red[0] : 1.0;
 }
 
-// ---------------------------------------------------------------------------
-// Features: residue planes X[m][rb][kb][16][64] (1 KB blocks: 16 image rows x 64 features), cos
-// features at kb < nkb/2, sin features above.  Thread = 4 consecutive visibilities x the 16 rows of
-// row-block blockIdx.y.  Padded rows and visibilities past K hold zeros.
-// ---------------------------------------------------------------------------
-__device__ inline int residue_of(double q, int i, const Consts& c) {
-  if (i == 0) return (int)(q - 256.0 * floor((q + 128.0) * (1.0 / 256.0)));  // [-128, 127]
+// sin(pi t) and cos(pi t) for |t| <= 1: t = x + n/2 with n = rint(2t), |x| <= 1/4, then the Taylor
+// series of sin(pi x) and cos(pi x) to degree 19 and 18 (truncation below 1e-17 at |pi x| <= pi/4;
+// absolute error about 1e-16 overall) and the quadrant n mod 4.  Coefficients (-1)^k pi^(2k+1)/(2k+1)!
+// and (-1)^k pi^(2k)/(2k)! rounded to FP64.
+__device__ inline void sincospi_small(double t, double& sn, double& cs) {
+  const double n = rint(2.0 * t);
+  const double x = fma(-0.5, n, t);
+  const double y = x * x;
+  double ps = -2.2948428997269873e-08;
+  ps = fma(ps, y, 7.952054001475513e-07);
+  ps = fma(ps, y, -2.1915353447830217e-05);
+  ps = fma(ps, y, 0.00046630280576761255);
+  ps = fma(ps, y, -0.0073704309457143504);
+  ps = fma(ps, y, 0.08214588661112823);
+  ps = fma(ps, y, -0.5992645293207921);
+  ps = fma(ps, y, 2.5501640398773455);
+  ps = fma(ps, y, -5.16771278004997);
+  ps = fma(ps, y, 3.141592653589793);
+  double pc = -1.3878952462213771e-07;
+  pc = fma(pc, y, 4.303069587032947e-06);
+  pc = fma(pc, y, -0.0001046381049248457);
+  pc = fma(pc, y, 0.0019295743094039231);
+  pc = fma(pc, y, -0.02580689139001406);
+  pc = fma(pc, y, 0.2353306303588932);
+  pc = fma(pc, y, -1.3352627688545895);
+  pc = fma(pc, y, 4.0587121264167685);
+  pc = fma(pc, y, -4.934802200544679);
+  pc = fma(pc, y, 1.0);
+  const double sx = x * ps;
+  const int q = (int)n & 3;  // quadrant: pi t = pi x + q pi/2 (mod 2 pi)
+  const double a = (q & 1) ? pc : sx;   // sin for q = 0, 2 (negated for 2); cos for q = 1, 3
+  const double b = (q & 1) ? sx : pc;
+  sn = (q & 2) ? -a : a;
+  cs = ((q + 1) & 2) ? -b : b;
+}
+
+// Residue of an integer-valued double q (|q| < 2^40) modulo MODULI[i] (a compile-time index).
+template <int i>
+__device__ inline int residue_of(double q) {
+  constexpr double m = (double)MODULI[i], inv_m = 1.0 / (double)MODULI[i];
+  if constexpr (i == 0) return (int)(q - 256.0 * floor((q + 128.0) * (1.0 / 256.0)));  // [-128, 127]
   // m odd: q/m is never a half-integer, and the product's rounding (|q| < 2^40) is far below the
   // 1/(2m) gap, so rint selects the balanced residue in [-(m-1)/2, (m-1)/2].
-  const double n = rint(q * c.inv_m[i]);
-  return (int)fma(-n, c.m[i], q);
+  const double n = rint(q * inv_m);
+  return (int)fma(-n, m, q);
 }
 
+// Packed residues of (cos, sin) of two visibilities: bytes cos0, sin0, cos1, sin1.
+template <int i>
+__device__ inline uint32_t packed_residues(const double (&qc)[2], const double (&qn)[2]) {
+  return (uint32_t)(uint8_t)(int8_t)residue_of<i>(qc[0]) | ((uint32_t)(uint8_t)(int8_t)residue_of<i>(qn[0]) << 8) |
+         ((uint32_t)(uint8_t)(int8_t)residue_of<i>(qc[1]) << 16) |
+         ((uint32_t)(uint8_t)(int8_t)residue_of<i>(qn[1]) << 24);
+}
+
+template <int i>
+__device__ inline void store_residues4(int8_t* __restrict__ X, int64_t plane, uint32_t off, const double (&qc)[2],
+                                       const double (&qn)[2]) {
+  *reinterpret_cast<uint32_t*>(X + i * plane + off) = packed_residues<i>(qc, qn);
+  if constexpr (i + 1 < NMOD) store_residues4<i + 1>(X, plane, off, qc, qn);
+}
+
+// ---------------------------------------------------------------------------
+// Features: residue planes X[m][rb][kb][16][64] (1 KB blocks: 16 image rows x 64 features); feature
+// 2k is the cos and 2k + 1 the sin feature of the chunk's visibility k.  Thread = 2 consecutive
+// visibilities (4 bytes per row and plane) x the 16 rows of row-block blockIdx.y.  Padded rows and
+// visibilities past K hold zeros.
+// ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) features_kernel(const double* __restrict__ noise,
                                                        const double* __restrict__ uv,
                                                        const double* __restrict__ grid,
                                                        const double* __restrict__ centre, int64_t K, int64_t M,
                                                        int64_t Mpad, int64_t k_begin, int kc,
-                                                       const double* __restrict__ sc, Consts c,
-                                                       int8_t* __restrict__ X) {
+                                                       const double* __restrict__ sc, int8_t* __restrict__ X) {
   const int nkb = kc / 32;
-  const int k0 = (blockIdx.x * 256 + threadIdx.x) * 4;  // first of 4 visibilities (chunk-local)
-  const int64_t rb = blockIdx.y;
-  double u0[4], u1[4], qs[4];
+  const int k0 = (blockIdx.x * 256 + threadIdx.x) * 2;  // first of 2 visibilities (chunk-local)
+  const int rb = blockIdx.y;
+  double u0[2], u1[2], qs[2];
   const double inv_sc = QMAX / sc[0];
 #pragma unroll
-  for (int v = 0; v < 4; ++v) {
+  for (int v = 0; v < 2; ++v) {
     const int64_t kg = k_begin + k0 + v;
     const bool ok = kg < K;
     u0[v] = ok ? uv[2 * kg] : 0.0;
@@ -154,39 +210,28 @@ __global__ void __launch_bounds__(256) features_kernel(const double* __restrict_
   }
   const double cy = centre[0], cx = centre[1];
   const int64_t plane = Mpad * 2 * (int64_t)kc;
-  const int kbc = k0 / KB, kin = k0 % KB;
-  int8_t* base = X + rb * (int64_t)nkb * 1024 + kin;
-  for (int r = 0; r < 16; ++r) {
-    const int64_t i = rb * 16 + r;
-    double qc[4], qn[4];
+  // byte offset within a plane (a plane holds fewer than 2^32 bytes: checked on the host)
+  uint32_t off = ((uint32_t)rb * (uint32_t)nkb + (uint32_t)(k0 / 32)) * 1024u + (uint32_t)((2 * k0) % 64);
+#pragma unroll 1
+  for (int r = 0; r < 16; ++r, off += 64) {
+    const int64_t i = (int64_t)rb * 16 + r;
+    double qc[2], qn[2];
     if (i < M) {
       const double gx = grid[2 * i + 1] - cx, gy = grid[2 * i] - cy;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
+      for (int v = 0; v < 2; ++v) {
         double rv = fma(gx, u0[v], gy * u1[v]);
         rv -= rint(rv);
         double s, co;
-        sincospi(2.0 * rv, &s, &co);
+        sincospi_small(2.0 * rv, s, co);
         qc[v] = rint(co * qs[v]);
         qn[v] = rint(s * qs[v]);
       }
     } else {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) qc[v] = qn[v] = 0.0;
+      for (int v = 0; v < 2; ++v) qc[v] = qn[v] = 0.0;
     }
-    int8_t* rowc = base + (int64_t)kbc * 1024 + r * 64;
-    int8_t* rows = rowc + (int64_t)(nkb / 2) * 1024;
-#pragma unroll
-    for (int m = 0; m < NMOD; ++m) {
-      uint32_t wc = 0, ws = 0;
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        wc |= (uint32_t)(uint8_t)(int8_t)residue_of(qc[v], m, c) << (8 * v);
-        ws |= (uint32_t)(uint8_t)(int8_t)residue_of(qn[v], m, c) << (8 * v);
-      }
-      *reinterpret_cast<uint32_t*>(rowc + m * plane) = wc;
-      *reinterpret_cast<uint32_t*>(rows + m * plane) = ws;
-    }
+    store_residues4<0>(X, plane, off, qc, qn);
   }
 }
 
@@ -220,102 +265,8 @@ __device__ inline void store_residues(const i4 (&acc)[NA][NB], const Consts& c, 
     }
 }
 
-// ---------------------------------------------------------------------------
-// SYRK of one (tile, modulus) unit: the int32 Gram block of A = rows [I*256, +256) and B = rows
-// [J*256, +256) over the chunk's 2 kc features of residue plane m, reduced to balanced int8
-// residues mod m_m.  4 waves as 2 x 2, each a 128 x 128 block = 8 x 8 MFMA 16x16x64 blocks (256
-// int32 accumulators per lane).  Per stage of 64 features the workgroup stages 32 blocks of 1 KB
-// (16 of A, 16 of B) in LDS through registers, one stage ahead, rows of 64 bytes with 16-byte units
-// XOR-swizzled by (row/2)%4 (free of bank conflicts for the fragment reads).
-// Units: blockIdx % 8 selects the XCD; XCD x works through its run of `per` tiles (spatially
-// compact in the tile list) for modulus 0, then modulus 1, ..., so that all XCDs read the same
-// residue plane at a time and the tiles an XCD holds share row panels in its L2.
-// Residues are stored in MFMA output order: R[unit][wave][a][b][lane] as 4 packed int8 (rows
-// 4*(lane/16) + 0..3 of the 16x16 block, column lane%16).
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256, 1) syrk_kernel(const int8_t* __restrict__ X, int64_t plane, int nkb,
-                                                      int ntiles, int per, const int* __restrict__ tiles,
-                                                      Consts c, int8_t* __restrict__ R) {
-  __shared__ __attribute__((aligned(1024))) int8_t lds[2][32][1024];
-  const int xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
-  const int mi = slot / per, loc = slot % per;
-  if (mi >= NMOD) return;
-  const int t = xcd * per + loc;
-  if (t >= ntiles) return;
-  const int64_t I = tiles[2 * t], J = tiles[2 * t + 1];
-
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int lr = lane & 15, lk = lane >> 4;
-  // loads: thread moves 16 bytes (unit lane of a 1 KB block) of blocks wave + 4 j, j = 0..7
-  // (blocks 0-15: A row-blocks I*16 + b, 16-31: B row-blocks J*16 + b - 16)
-  const int8_t* srcA = X + mi * plane + (I * 16 + wave) * (int64_t)nkb * 1024 + lane * 16;
-  const int8_t* srcB = X + mi * plane + (J * 16 + wave) * (int64_t)nkb * 1024 + lane * 16;
-  const int64_t jstride = 4 * (int64_t)nkb * 1024;  // 4 row-blocks
-  const int wrow = lane >> 2, wunit = lane & 3;
-  const int woff = wrow * 64 + 16 * (wunit ^ ((wrow >> 1) & 3));
-  const int roff = lr * 64 + 16 * (lk ^ ((lr >> 1) & 3));
-  const int ablk = (wave >> 1) * 8, bblk = 16 + (wave & 1) * 8;
-
-  int8_t* const lbase = &lds[0][0][0];
-  i4 p[8];
-#define AJ_CRT_LOAD(ST)                                                                       \
-  _Pragma("unroll") for (int j = 0; j < 4; ++j) {                                             \
-    p[j] = *reinterpret_cast<const i4*>(srcA + j * jstride + (int64_t)(ST) * 1024);           \
-    p[4 + j] = *reinterpret_cast<const i4*>(srcB + j * jstride + (int64_t)(ST) * 1024);       \
-  }
-#define AJ_CRT_STORE(OFF)                                                                              \
-  _Pragma("unroll") for (int j = 0; j < 4; ++j) {                                                      \
-    *reinterpret_cast<i4*>(lbase + (OFF) + (wave + 4 * j) * 1024 + woff) = p[j];                       \
-    *reinterpret_cast<i4*>(lbase + (OFF) + (16 + wave + 4 * j) * 1024 + woff) = p[4 + j];              \
-  }
-
-  i4 acc[8][8];
-#pragma unroll
-  for (int a = 0; a < 8; ++a)
-#pragma unroll
-    for (int b = 0; b < 8; ++b) acc[a][b] = i4{0, 0, 0, 0};
-
-  AJ_CRT_LOAD(0)
-  AJ_CRT_STORE(0)
-  __syncthreads();
-  for (int st = 0; st < nkb; ++st) {
-    const int cur = (st & 1) * 32768, nxt = cur ^ 32768;
-    const bool more = st + 1 < nkb;
-    // order: next stage's global loads, this stage's fragment reads and MFMAs, then the LDS
-    // stores of the next stage (which wait for its loads)
-    if (more) { AJ_CRT_LOAD(st + 1) }
-    __builtin_amdgcn_sched_barrier(0);
-    i4 fa[8], fb[8];
-    fa[0] = *reinterpret_cast<const i4*>(lbase + cur + ablk * 1024 + roff);
-#pragma unroll
-    for (int b = 0; b < 8; ++b) fb[b] = *reinterpret_cast<const i4*>(lbase + cur + (bblk + b) * 1024 + roff);
-#pragma unroll
-    for (int a = 1; a < 8; ++a) fa[a] = *reinterpret_cast<const i4*>(lbase + cur + (ablk + a) * 1024 + roff);
-    // all 16 fragment reads are issued before the first MFMA (the MFMAs then wait only for the
-    // fragments they use, in read order)
-    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (int a = 0; a < 8; ++a)
-#pragma unroll
-      for (int b = 0; b < 8; ++b) acc[a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[a], fb[b], acc[a][b], 0, 0, 0);
-    __builtin_amdgcn_sched_barrier(0);
-    if (more) { AJ_CRT_STORE(nxt) }
-    __syncthreads();
-  }
-#undef AJ_CRT_LOAD
-#undef AJ_CRT_STORE
-
-  store_residues(acc, c, mi, ntiles, t, (wave >> 1) * 8, (wave & 1) * 8, lane, R);
-}
-
-// ---------------------------------------------------------------------------
-// The same unit with the feature blocks moved by direct global->LDS loads (global_load_lds, 16 bytes
-// per lane; lane L of a wave writes byte 16 L of the destination block, so the XOR swizzle is applied
-// by choosing each lane's source address) into a ring of four 32 KB stage buffers.  The loop runs
-// over intervals of two stages: one barrier per interval, the loads of interval j+1 issued right
-// after it (into the buffers interval j-1 used), then the fragment reads of both stages, the second
-// set landing while the MFMAs of the first run.
-// ---------------------------------------------------------------------------
+// Direct global->LDS load of 16 bytes per lane: lane L of the wave writes bytes 16 L .. 16 L + 15 of
+// the wave-uniform destination block.
 __device__ inline void lds_dma16(const int8_t* g, int8_t* lds_block) {
   __builtin_amdgcn_global_load_lds((const void*)g,
                                    (void __attribute__((address_space(3)))*)(
@@ -323,87 +274,18 @@ __device__ inline void lds_dma16(const int8_t* g, int8_t* lds_block) {
                                    16, 0, 0);
 }
 
-__global__ void __launch_bounds__(256, 1) syrk_ring_kernel(const int8_t* __restrict__ X, int64_t plane, int nkb,
-                                                           int ntiles, int per, const int* __restrict__ tiles,
-                                                           Consts c, int8_t* __restrict__ R) {
-  __shared__ __attribute__((aligned(1024))) int8_t lds[4][32][1024];
-  const int xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
-  const int mi = slot / per, loc = slot % per;
-  if (mi >= NMOD) return;
-  const int t = xcd * per + loc;
-  if (t >= ntiles) return;
-  const int64_t I = tiles[2 * t], J = tiles[2 * t + 1];
-
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int lr = lane & 15, lk = lane >> 4;
-  // source of lane L in a block: row L/4, global unit (L%4) ^ ((row/2)%4)
-  const int srow = lane >> 2;
-  const int soff = srow * 64 + 16 * ((lane & 3) ^ ((srow >> 1) & 3));
-  const int8_t* gA = X + mi * plane + (I * 16 + wave) * (int64_t)nkb * 1024 + soff;
-  const int8_t* gB = X + mi * plane + (J * 16 + wave) * (int64_t)nkb * 1024 + soff;
-  const int64_t jstride = 4 * (int64_t)nkb * 1024;
-  const int roff = lr * 64 + 16 * (lk ^ ((lr >> 1) & 3));
-  const int ablk = (wave >> 1) * 8, bblk = 16 + (wave & 1) * 8;
-
-#define AJ_RING_ISSUE(ST)                                                                    \
-  {                                                                                          \
-    const int64_t ko_ = (int64_t)(ST) * 1024;                                                \
-    int8_t* lb_ = &lds[(ST) & 3][0][0];                                                      \
-    _Pragma("unroll") for (int j = 0; j < 4; ++j) {                                          \
-      lds_dma16(gA + j * jstride + ko_, lb_ + (wave + 4 * j) * 1024);                        \
-      lds_dma16(gB + j * jstride + ko_, lb_ + (16 + wave + 4 * j) * 1024);                   \
-    }                                                                                        \
-  }
-
-  i4 acc[8][8];
-#pragma unroll
-  for (int a = 0; a < 8; ++a)
-#pragma unroll
-    for (int b = 0; b < 8; ++b) acc[a][b] = i4{0, 0, 0, 0};
-
-  AJ_RING_ISSUE(0)
-  AJ_RING_ISSUE(1)
-  const int NI = nkb / 2;
-  for (int j = 0; j < NI; ++j) {
-    __builtin_amdgcn_s_waitcnt(0);  // this wave's loads of interval j are in LDS
-    __syncthreads();                 // everyone's are; everyone finished reading interval j-1
-    if (j + 1 < NI) {
-      AJ_RING_ISSUE(2 * j + 2)
-      AJ_RING_ISSUE(2 * j + 3)
-    }
-    __builtin_amdgcn_sched_barrier(0);
-    const int8_t* l0 = &lds[(2 * j) & 3][0][0];
-    const int8_t* l1 = &lds[(2 * j + 1) & 3][0][0];
-    i4 fa[8], fb[8], ga[8], gb[8];
-    fa[0] = *reinterpret_cast<const i4*>(l0 + ablk * 1024 + roff);
-#pragma unroll
-    for (int b = 0; b < 8; ++b) fb[b] = *reinterpret_cast<const i4*>(l0 + (bblk + b) * 1024 + roff);
-#pragma unroll
-    for (int a = 1; a < 8; ++a) fa[a] = *reinterpret_cast<const i4*>(l0 + (ablk + a) * 1024 + roff);
-    ga[0] = *reinterpret_cast<const i4*>(l1 + ablk * 1024 + roff);
-#pragma unroll
-    for (int b = 0; b < 8; ++b) gb[b] = *reinterpret_cast<const i4*>(l1 + (bblk + b) * 1024 + roff);
-#pragma unroll
-    for (int a = 1; a < 8; ++a) ga[a] = *reinterpret_cast<const i4*>(l1 + (ablk + a) * 1024 + roff);
-    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (int a = 0; a < 8; ++a)
-#pragma unroll
-      for (int b = 0; b < 8; ++b) acc[a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[a], fb[b], acc[a][b], 0, 0, 0);
-#pragma unroll
-    for (int a = 0; a < 8; ++a)
-#pragma unroll
-      for (int b = 0; b < 8; ++b) acc[a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ga[a], gb[b], acc[a][b], 0, 0, 0);
-    __builtin_amdgcn_sched_barrier(0);
-  }
-#undef AJ_RING_ISSUE
-  store_residues(acc, c, mi, ntiles, t, (wave >> 1) * 8, (wave & 1) * 8, lane, R);
-}
-
 // ---------------------------------------------------------------------------
-// The ring kernel with eight waves (two per SIMD), each a 128 x 64 block of the tile (8 x 4 MFMA
-// blocks, 128 int32 accumulators per lane): the second wave on a SIMD runs its MFMAs while the
-// first waits for fragments, at the price of 1.5x the LDS fragment reads.
+// SYRK of one (tile, modulus) unit: the int32 Gram block of A = rows [I*256, +256) and B = rows
+// [J*256, +256) over the chunk's 2 kc features of residue plane m, reduced to balanced int8 residues
+// mod m_m.  Eight waves (two per SIMD), each a 128 x 64 block of the tile (8 x 4 MFMA 16x16x64
+// blocks, 128 int32 accumulators per lane).  The feature blocks (1 KB: 16 rows x 64 bytes; 32 per
+// stage, 16 of A and 16 of B) move by direct global->LDS loads into a ring of four 32 KB stage
+// buffers, the XOR swizzle (16-byte unit u of row r stored at u ^ (r/2)%4, free of bank conflicts
+// for the fragment reads) applied through each lane's source address.  One barrier per interval of
+// two stages, the loads of the next interval issued right after it.
+// Units: blockIdx % 8 selects the XCD; XCD x works through its run of `per` tiles (spatially compact
+// in the tile list) for modulus 0, then modulus 1, ..., so that all XCDs read the same residue plane
+// at a time and the tiles an XCD holds share row panels in its L2.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(512, 1) syrk_ring8_kernel(const int8_t* __restrict__ X, int64_t plane, int nkb,
                                                             int ntiles, int per, const int* __restrict__ tiles,
@@ -476,83 +358,9 @@ __global__ void __launch_bounds__(512, 1) syrk_ring8_kernel(const int8_t* __rest
   store_residues(acc, c, mi, ntiles, t, (wave >> 2) * 8, (wave & 3) * 4, lane, R);
 }
 
-// ---------------------------------------------------------------------------
-// Eight waves, one barrier per stage, loads three stages ahead in the four-buffer ring: the loads
-// of stage s + 3 go into the buffer stage s - 1 used, issued right after the barrier that ends
-// every wave's reads of it.  Before that barrier each wave waits until its loads of stage s have
-// landed (at most the loads of the two later stages still in flight).
-// ---------------------------------------------------------------------------
+// s_waitcnt immediate that waits for vmcnt <= n and leaves expcnt and lgkmcnt unconstrained
+// (gfx9 encoding: vmcnt[3:0] and [15:14], expcnt [6:4], lgkmcnt [11:8]).
 constexpr unsigned vmcnt_imm(unsigned n) { return (n & 0xF) | ((n >> 4) << 14) | (0x7u << 4) | (0xFu << 8); }
-
-__global__ void __launch_bounds__(512, 1) syrk_deep8_kernel(const int8_t* __restrict__ X, int64_t plane, int nkb,
-                                                            int ntiles, int per, const int* __restrict__ tiles,
-                                                            Consts c, int8_t* __restrict__ R) {
-  __shared__ __attribute__((aligned(1024))) int8_t lds[4][32][1024];
-  const int xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
-  const int mi = slot / per, loc = slot % per;
-  if (mi >= NMOD) return;
-  const int t = xcd * per + loc;
-  if (t >= ntiles) return;
-  const int64_t I = tiles[2 * t], J = tiles[2 * t + 1];
-
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int lr = lane & 15, lk = lane >> 4;
-  const int srow = lane >> 2;
-  const int soff = srow * 64 + 16 * ((lane & 3) ^ ((srow >> 1) & 3));
-  const int8_t* gA = X + mi * plane + (I * 16 + wave) * (int64_t)nkb * 1024 + soff;
-  const int8_t* gB = X + mi * plane + (J * 16 + wave) * (int64_t)nkb * 1024 + soff;
-  const int64_t jstride = 8 * (int64_t)nkb * 1024;
-  const int roff = lr * 64 + 16 * (lk ^ ((lr >> 1) & 3));
-  const int ablk = (wave >> 2) * 8, bblk = 16 + (wave & 3) * 4;
-
-#define AJ_DEEP_ISSUE(ST)                                                                    \
-  {                                                                                          \
-    const int64_t ko_ = (int64_t)(ST) * 1024;                                                \
-    int8_t* lb_ = &lds[(ST) & 3][0][0];                                                      \
-    _Pragma("unroll") for (int j = 0; j < 2; ++j) {                                          \
-      lds_dma16(gA + j * jstride + ko_, lb_ + (wave + 8 * j) * 1024);                        \
-      lds_dma16(gB + j * jstride + ko_, lb_ + (16 + wave + 8 * j) * 1024);                   \
-    }                                                                                        \
-  }
-
-  i4 acc[8][4];
-#pragma unroll
-  for (int a = 0; a < 8; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) acc[a][b] = i4{0, 0, 0, 0};
-
-  // nkb >= 128 (chunks of at least 4096 visibilities), so the three leading stages exist
-  AJ_DEEP_ISSUE(0)
-  AJ_DEEP_ISSUE(1)
-  AJ_DEEP_ISSUE(2)
-  for (int st = 0; st < nkb; ++st) {
-    // 4 loads per stage per thread, completing in order
-    const int later = min(2, nkb - 1 - st);
-    if (later == 2) __builtin_amdgcn_s_waitcnt(vmcnt_imm(8));
-    else if (later == 1) __builtin_amdgcn_s_waitcnt(vmcnt_imm(4));
-    else __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
-    // workgroup barrier without the release fence of __syncthreads, which would also wait for the
-    // loads of the two later stages; the memory clobber keeps LDS accesses on their side of it
-    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
-    if (st + 3 < nkb) { AJ_DEEP_ISSUE(st + 3) }
-    __builtin_amdgcn_sched_barrier(0);
-    const int8_t* l0 = &lds[st & 3][0][0];
-    i4 fa[8], fb[4];
-    fa[0] = *reinterpret_cast<const i4*>(l0 + ablk * 1024 + roff);
-#pragma unroll
-    for (int b = 0; b < 4; ++b) fb[b] = *reinterpret_cast<const i4*>(l0 + (bblk + b) * 1024 + roff);
-#pragma unroll
-    for (int a = 1; a < 8; ++a) fa[a] = *reinterpret_cast<const i4*>(l0 + (ablk + a) * 1024 + roff);
-    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (int a = 0; a < 8; ++a)
-#pragma unroll
-      for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[a], fb[b], acc[a][b], 0, 0, 0);
-    __builtin_amdgcn_sched_barrier(0);
-  }
-#undef AJ_DEEP_ISSUE
-  store_residues(acc, c, mi, ntiles, t, (wave >> 2) * 8, (wave & 3) * 4, lane, R);
-}
 
 // ---------------------------------------------------------------------------
 // Eight waves with the fragment reads software-pipelined across stages: after the barrier of
@@ -578,6 +386,9 @@ __device__ inline i32x4 make_rsrc(const void* base, uint32_t bytes) {
   return r;
 }
 
+// SPLIT = 1 runs the first half of a stage's MFMAs before the barrier (their fragments are already in
+// registers), so a SIMD has MFMA work while its waves wait at the barrier and issue the next loads.
+template <int SPLIT>
 __global__ void __launch_bounds__(512, 1) syrk_pipe8_kernel(const int8_t* __restrict__ X, int64_t plane, int nkb,
                                                             int ntiles, int per, const int* __restrict__ tiles,
                                                             Consts c, int8_t* __restrict__ R) {
@@ -620,8 +431,8 @@ __global__ void __launch_bounds__(512, 1) syrk_pipe8_kernel(const int8_t* __rest
     _Pragma("unroll") for (int a = 1; a < 8; ++a)                                                        \
       FA[a] = *reinterpret_cast<const i4*>(l_ + (ablk + a) * 1024 + roff);                               \
   }
-#define AJ_PIPE_MFMA(FA, FB)                                                                             \
-  _Pragma("unroll") for (int a = 0; a < 8; ++a)                                                          \
+#define AJ_PIPE_MFMA(FA, FB, A0, A1)                                                                     \
+  _Pragma("unroll") for (int a = (A0); a < (A1); ++a)                                                    \
   _Pragma("unroll") for (int b = 0; b < 4; ++b)                                                          \
     acc[a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(FA[a], FB[b], acc[a][b], 0, 0, 0);
   // wait until at most N of this wave's buffer loads are in flight, then the workgroup barrier (no
@@ -638,11 +449,15 @@ __global__ void __launch_bounds__(512, 1) syrk_pipe8_kernel(const int8_t* __rest
 #define AJ_PIPE_STEP(FA, FB, GA, GB, ST)                                                                 \
   {                                                                                                      \
     const int s_ = (ST);                                                                                 \
+    if constexpr (SPLIT) {                                                                               \
+      AJ_PIPE_MFMA(FA, FB, 0, 4)                                                                         \
+      __builtin_amdgcn_sched_barrier(0);                                                                 \
+    }                                                                                                    \
     AJ_PIPE_SYNC(4)                                                                                      \
     AJ_PIPE_ISSUE(s_ + 3)                                                                                \
     AJ_PIPE_READ(GA, GB, s_ + 1)                                                                         \
     __builtin_amdgcn_sched_barrier(0);                                                                   \
-    AJ_PIPE_MFMA(FA, FB)                                                                                 \
+    AJ_PIPE_MFMA(FA, FB, SPLIT ? 4 : 0, 8)                                                               \
     __builtin_amdgcn_sched_barrier(0);                                                                   \
   }
 
@@ -739,16 +554,17 @@ struct Plan {
   size_t feat_bytes, res_bytes, bytes;
 };
 
-inline Plan make_plan(int64_t M, int kc) {
+inline Plan make_plan(int64_t M, int64_t K, int kc) {
   Plan p{};
   p.Mpad = ceil_div(M, CT) * CT;
   p.nI = p.Mpad / CT;
   p.ntiles = p.nI * (p.nI + 1) / 2;
   p.per = ceil_div(p.ntiles, 8);
-  p.feat_bytes = (size_t)NMOD * p.Mpad * 2 * kc;
+  p.feat_bytes = (size_t)NMOD * p.Mpad * 2 * kc;  // one chunk
   p.res_bytes = (size_t)NMOD * p.ntiles * UNIT_BYTES;
-  p.bytes = align_up(p.feat_bytes, 256) + align_up(p.res_bytes, 256) + align_up(sizeof(int) * 2 * p.ntiles, 256) +
-            256 /*scale*/ + 256;
+  const int64_t nch = std::max<int64_t>(1, ceil_div(K, kc));
+  p.bytes = align_up(p.feat_bytes, 256) + align_up(p.res_bytes, 256) +
+            align_up(sizeof(int) * 2 * p.ntiles, 256) + align_up(sizeof(double) * nch, 256) + 256;
   return p;
 }
 
@@ -756,61 +572,62 @@ inline Plan make_plan(int64_t M, int kc) {
 
 namespace wt {
 
-// Residue SYRK kernel (0 = direct-to-LDS ring, four waves; 1 = register-staged double buffer; 2 =
-// ring, eight waves, two stages per barrier; 3 = ring, eight waves, one stage per barrier, loads three
-// stages ahead; 4 = as 3 with buffer loads and the fragment reads one stage ahead) and visibilities
-// per chunk; process-wide tuning.
-int g_crt_variant = 4;
+// Residue SYRK kernel (0 = syrk_pipe8_kernel<1>, the default; 1 = syrk_pipe8_kernel<0>; 2 =
+// syrk_ring8_kernel) and visibilities per chunk; process-wide tuning.
+int g_crt_variant = 0;
 int g_crt_kc = crt::KC_DEFAULT;
 
-size_t crt_workspace_bytes(int64_t M) { return crt::make_plan(M, g_crt_kc).bytes; }
+size_t crt_workspace_bytes(int64_t M, int64_t K) { return crt::make_plan(M, K, g_crt_kc).bytes; }
 
 // Accumulate the lower triangle of W (zeroed by the caller) over all visibilities.
 int crt_accumulate(const double* noise, const double* uv, const double* grid, const double* centre, int64_t K,
                    int64_t M, double* W, void* workspace, size_t workspace_bytes, hipStream_t st) {
   using namespace crt;
   const int kc = g_crt_kc, nkb = kc / 32;
-  const Plan p = make_plan(M, kc);
+  const Plan p = make_plan(M, K, kc);
+  const int64_t nch = std::max<int64_t>(1, ceil_div(K, kc));
   Arena ar(workspace, workspace_bytes);
   int8_t* X = ar.take<int8_t>(p.feat_bytes);
   int8_t* R = ar.take<int8_t>(p.res_bytes);
   int* tiles = ar.take<int>(2 * p.ntiles);
-  double* sc = ar.take<double>(1);
+  double* sc = ar.take<double>(nch);
   if (!ar.ok()) {
     set_error("workspace too small: need %zu bytes, got %zu", p.bytes, workspace_bytes);
     return AJ_ESPACE;
   }
+  if ((uint64_t)p.Mpad * 2 * (uint64_t)kc >= (uint64_t(1) << 32)) {
+    set_error("int8crt: %lld image pixels x %d visibilities per chunk exceed the 32-bit plane offsets; use "
+              "a smaller chunk (aj_w_tilde_crt_configure)", (long long)M, kc);
+    return AJ_EINVAL;
+  }
+  if (K == 0) return AJ_OK;
   static const Consts cst = make_consts();
   static thread_local std::vector<int> tl;
   build_tiles(p.nI, tl);
   AJ_CHECK_HIP(hipMemcpyAsync(tiles, tl.data(), sizeof(int) * tl.size(), hipMemcpyHostToDevice, st));
-  const int64_t nch = ceil_div(K, kc);
+  hipLaunchKernelGGL(scale_kernel, dim3((unsigned)nch), dim3(256), 0, st, noise, K, kc, sc);
+  AJ_LAUNCH_CHECK();
   const int64_t plane = p.Mpad * 2 * (int64_t)kc;
-  const dim3 fg((unsigned)(kc / 1024), (unsigned)(p.Mpad / 16));
+  const dim3 fg((unsigned)(kc / 512), (unsigned)(p.Mpad / 16));
   const dim3 sg((unsigned)(8 * p.per * NMOD));
   const dim3 cg((unsigned)ceil_div(p.ntiles * 16384, 256));
   const int nt = (int)p.ntiles, per = (int)p.per;
   for (int64_t ch = 0; ch < nch; ++ch) {
-    const int64_t k_begin = ch * kc;
-    hipLaunchKernelGGL(scale_kernel, dim3(1), dim3(256), 0, st, noise, K, k_begin, kc, sc);
-    AJ_LAUNCH_CHECK();
     timer_begin(AJ_TIMER_WT_FEATURES, st);
-    hipLaunchKernelGGL(features_kernel, fg, dim3(256), 0, st, noise, uv, grid, centre, K, M, p.Mpad, k_begin, kc,
-                       sc, cst, X);
+    hipLaunchKernelGGL(features_kernel, fg, dim3(256), 0, st, noise, uv, grid, centre, K, M, p.Mpad, ch * kc, kc,
+                       sc + ch, X);
     timer_end(AJ_TIMER_WT_FEATURES, st);
     AJ_LAUNCH_CHECK();
     timer_begin(AJ_TIMER_WT_SYRK, st);
     switch (g_crt_variant) {
-      case 0: hipLaunchKernelGGL(syrk_ring_kernel, sg, dim3(256), 0, st, X, plane, nkb, nt, per, tiles, cst, R); break;
-      case 1: hipLaunchKernelGGL(syrk_kernel, sg, dim3(256), 0, st, X, plane, nkb, nt, per, tiles, cst, R); break;
+      case 1: hipLaunchKernelGGL(syrk_pipe8_kernel<0>, sg, dim3(512), 0, st, X, plane, nkb, nt, per, tiles, cst, R); break;
       case 2: hipLaunchKernelGGL(syrk_ring8_kernel, sg, dim3(512), 0, st, X, plane, nkb, nt, per, tiles, cst, R); break;
-      case 3: hipLaunchKernelGGL(syrk_deep8_kernel, sg, dim3(512), 0, st, X, plane, nkb, nt, per, tiles, cst, R); break;
-      default: hipLaunchKernelGGL(syrk_pipe8_kernel, sg, dim3(512), 0, st, X, plane, nkb, nt, per, tiles, cst, R); break;
+      default: hipLaunchKernelGGL(syrk_pipe8_kernel<1>, sg, dim3(512), 0, st, X, plane, nkb, nt, per, tiles, cst, R); break;
     }
     timer_end(AJ_TIMER_WT_SYRK, st);
     AJ_LAUNCH_CHECK();
     timer_begin(AJ_TIMER_WT_COMBINE, st);
-    hipLaunchKernelGGL(combine_kernel, cg, dim3(256), 0, st, R, nt, tiles, cst, sc, W, M);
+    hipLaunchKernelGGL(combine_kernel, cg, dim3(256), 0, st, R, nt, tiles, cst, sc + ch, W, M);
     timer_end(AJ_TIMER_WT_COMBINE, st);
     AJ_LAUNCH_CHECK();
   }
@@ -820,11 +637,11 @@ int crt_accumulate(const double* noise, const double* uv, const double* grid, co
 }  // namespace wt
 }  // namespace aj
 
-// Residue SYRK kernel variant (0..4, -1 = default) and visibilities per chunk (a multiple of 4096 up to
+// Residue SYRK kernel variant (0..2, -1 = default 0) and visibilities per chunk (a multiple of 4096 up to
 // 32768, 0 = default 16384); tuning, no reference counterpart.
 extern "C" int aj_w_tilde_crt_configure(int variant, int64_t chunk_visibilities) {
-  if (variant < -1 || variant > 4) {
-    aj::set_error("residue SYRK variant %d outside [-1, 4]", variant);
+  if (variant < -1 || variant > 2) {
+    aj::set_error("residue SYRK variant %d outside [-1, 2]", variant);
     return AJ_EINVAL;
   }
   if (chunk_visibilities != 0 &&
@@ -833,7 +650,7 @@ extern "C" int aj_w_tilde_crt_configure(int variant, int64_t chunk_visibilities)
                   aj::crt::KC_MAX);
     return AJ_EINVAL;
   }
-  aj::wt::g_crt_variant = variant < 0 ? 4 : variant;
+  aj::wt::g_crt_variant = variant < 0 ? 0 : variant;
   aj::wt::g_crt_kc = chunk_visibilities ? (int)chunk_visibilities : aj::crt::KC_DEFAULT;
   return AJ_OK;
 }

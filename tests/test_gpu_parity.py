@@ -100,16 +100,16 @@ def test_w_tilde_crt_multichunk(gpu):
 
 
 def test_w_tilde_crt_variants_agree(gpu):
-    """All residue SYRK kernels and chunk sizes give the same W-tilde: bitwise for one chunk size
-    (two chunks, the last one ragged, M not a multiple of 256), to FP64 rounding across chunk sizes
-    (the per-chunk scale and the FP64 accumulation order change)."""
+    """All residue SYRK kernels give bitwise-identical W-tilde for one chunk size (two chunks, the last
+    one ragged, M not a multiple of 256); other chunk sizes agree to FP64 rounding (the per-chunk scale
+    and the FP64 accumulation order change)."""
     from autojax_amd import _lib, api
 
     d = ds.config1(K=20000)
     args = (d.noise_map_real, d.uv_wavelengths, d.grid_radians_slim)
     out = []
     try:
-        for v in (0, 1, 2, 3, 4):
+        for v in (0, 1, 2):
             _lib.call("aj_w_tilde_crt_configure", v, 16384)
             out.append(api.w_tilde_curvature_interferometer_from(*args, precision="int8crt"))
         for kc in (4096, 32768):
