@@ -6,9 +6,9 @@ Delaunay source pixels each (own mesh, own barycentric mapping, B = 3) and
 lambda log-uniform in [0.01, 100].  Synthetic data (datasets.config5).
 
 One step = build W-tilde for the dataset + log-evidence of all 64 samples:
-  mixed  : W-tilde at --precision (default int8crt: FP64 cos/sin features on
-           the five-digit fixed-point grid, their exact integer Gram matrix per
-           16384-visibility chunk from its residues modulo 12 moduli, one int8
+  mixed  : W-tilde at --precision (default int8crt11: FP64 cos/sin features
+           rounded to 36-bit integers, their exact integer Gram matrix per
+           16384-visibility chunk from its residues modulo 11 moduli, one int8
            MFMA product each, recombined in FP64), then one batched evaluation
            of all samples.
   plain  : W-tilde in FP64 (FP64 MFMA), then 64 sequential single-sample
@@ -48,8 +48,9 @@ for p in (ROOT, ROOT / "python-autojax_amd"):
 import numpy as np  # noqa: E402
 
 PEAKS_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0, "int8x3": 5000.0, "int8x4": 5000.0, "int8x5": 5000.0,
-                "int8crt": 5000.0}
-RAW_FACTOR = {"fp64": 1, "fp32": 1, "bf16x3": 3, "int8x3": 9, "int8x4": 15, "int8x5": 22, "int8crt": 12}
+                "int8crt": 5000.0, "int8crt11": 5000.0}
+RAW_FACTOR = {"fp64": 1, "fp32": 1, "bf16x3": 3, "int8x3": 9, "int8x4": 15, "int8x5": 22, "int8crt": 12,
+              "int8crt11": 11}
 
 
 def log(msg):
@@ -63,7 +64,7 @@ I8_DIGITS = {"int8x3": (3, 5), "int8x4": (4, 6), "int8x5": (5, 7)}
 
 
 def syrk_kernel_name(precision, variant):
-    if precision == "int8crt":
+    if precision.startswith("int8crt"):
         return "crt::syrk_pipe8_kernel<1>"
     if precision not in I8_DIGITS:
         return f"syrk_kernel<{precision}>"
@@ -124,7 +125,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", default="int8crt", choices=list(PEAKS_TFLOPS))
+    ap.add_argument("--precision", default="int8crt11", choices=list(PEAKS_TFLOPS))
     ap.add_argument("--no-autotune", action="store_true", help="keep the default SYRK kernel variant")
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--samples", type=int, default=64)

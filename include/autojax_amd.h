@@ -41,6 +41,8 @@ extern "C" {
 #define AJ_PREC_INT8CRT 6 /* features on the 5-digit fixed-point grid, exact integer Gram per chunk of
                              visibilities from its residues modulo 12 moduli <= 256 (one int8 product
                              each), recombined in FP64 (Chinese remainder theorem) */
+#define AJ_PREC_INT8CRT11 7 /* as AJ_PREC_INT8CRT on a grid of 60 * 2^30 (36 bits) with 11 moduli:
+                               11 int8 products per FP64-equivalent product, chunks up to 16384 */
 
 const char* aj_last_error(void);
 int aj_abi_version(void);
@@ -78,13 +80,15 @@ int aj_w_tilde_set_variant(int variant);
 int aj_w_tilde_variant_count(void);
 int aj_w_tilde_set_group_bytes(size_t bytes);
 
-/* Constants of the AJ_PREC_INT8CRT recombination, as the kernels use them (host-only query, no
- * reference counterpart): moduli[12]; the CRT basis y_i = (P/m_i) ((P/m_i)^-1 mod m_i) mod P split as
- * y_hi[i] (a multiple of 2^53) + y_lo[i] (an integer below 2^53); pq = {P_hi, P_lo, 1/P}. */
-int aj_crt_constants(double* moduli, double* y_hi, double* y_lo, double* pq);
-/* Tuning of AJ_PREC_INT8CRT (process-wide): residue SYRK kernel variant 0..2 (-1 = default 0; all
- * compute identical residues) and visibilities per exact integer chunk (a multiple of 4096 up to
- * 32768; 0 = default 16384).  The chunk size changes aj_w_tilde_workspace_bytes for this mode. */
+/* Constants of the AJ_PREC_INT8CRT (nmod = 12) or AJ_PREC_INT8CRT11 (nmod = 11) recombination, as the
+ * kernels use them (host-only query, no reference counterpart): moduli[nmod]; the CRT basis
+ * y_i = (P/m_i) ((P/m_i)^-1 mod m_i) mod P split as y_hi[i] (a multiple of 2^53) + y_lo[i] (an integer
+ * below 2^53); pq = {P_hi, P_lo, 1/P, Q} with Q the largest feature integer of the mode. */
+int aj_crt_constants(int nmod, double* moduli, double* y_hi, double* y_lo, double* pq);
+/* Tuning of AJ_PREC_INT8CRT and AJ_PREC_INT8CRT11 (process-wide): residue SYRK kernel variant 0..2
+ * (-1 = default 0; all compute identical residues) and visibilities per exact integer chunk (a
+ * multiple of 4096 up to 32768, capped at 16384 with 11 moduli; 0 = default 16384).  The chunk size
+ * changes aj_w_tilde_workspace_bytes for these modes. */
 int aj_w_tilde_crt_configure(int variant, int64_t chunk_visibilities);
 
 /* numba.py:210-234 w_compact_curvature_interferometer_from(grid_size, noise_map_real f8[K],

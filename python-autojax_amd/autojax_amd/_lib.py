@@ -16,7 +16,7 @@ import numpy as np
 LIB_PATH = Path(__file__).resolve().parent / "lib" / "libautojax_amd.so"
 
 AJ_OK, AJ_EINVAL, AJ_ENOTPD, AJ_EHIP, AJ_ESPACE = 0, 1, 2, 3, 4
-PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x3": 2, "int8x3": 3, "int8x4": 4, "int8x5": 5, "int8crt": 6}
+PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x3": 2, "int8x3": 3, "int8x4": 4, "int8x5": 5, "int8crt": 6, "int8crt11": 7}
 
 _P = C.c_void_p
 _I = C.c_int64
@@ -33,7 +33,7 @@ SIGNATURES = {
     "aj_w_tilde_set_variant": (_INT, [_INT]),
     "aj_w_tilde_variant_count": (_INT, []),
     "aj_w_tilde_set_group_bytes": (_INT, [_S]),
-    "aj_crt_constants": (_INT, [_P, _P, _P, _P]),
+    "aj_crt_constants": (_INT, [_INT, _P, _P, _P, _P]),
     "aj_w_tilde_crt_configure": (_INT, [_INT, _I]),
     "aj_w_tilde_workspace_bytes": (_S, [_I, _I, _INT]),
     "aj_w_tilde_curvature_interferometer_from": (_INT, [_P, _P, _P, _I, _I, _INT, _P, _P, _S, _P]),

@@ -134,7 +134,8 @@ def w_tilde_curvature_interferometer_from(noise_map_real, uv_wavelengths, grid_r
     "int8x4", "int8x5" (exact integer MFMA on 3, 4 or 5 signed 8-bit digits of FP64
     cos/sin features, all digit products that reach FP64); "int8crt" (the
     five-digit fixed-point features, their exact integer Gram matrix from its
-    residues modulo 12 moduli, one int8 MFMA product each); "fp32" or
+    residues modulo 12 moduli, one int8 MFMA product each); "int8crt11" (36-bit
+    features, 11 moduli); "fp32" or
     "bf16x3" (FP32 cos/sin, FP32-accumulated MFMA).
     """
     torch_in = _any_t(noise_map_real, uv_wavelengths, grid_radians_slim)

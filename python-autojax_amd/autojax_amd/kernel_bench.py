@@ -58,7 +58,7 @@ def main(argv=None):
         torch.cuda.synchronize()
         return s.elapsed_time(e) / reps
 
-    if a.precision == "int8crt":
+    if a.precision.startswith("int8crt"):
         _lib.call("aj_w_tilde_crt_configure", -1 if a.variant is None else a.variant, a.chunk)
     elif a.variant is not None:
         _lib.call("aj_w_tilde_set_variant", a.variant)
@@ -99,7 +99,7 @@ def main(argv=None):
             rec["against"] = a.against
             rec["normwise_diff"] = float(((W - Wb).abs().max() / Wb.abs().max()).item())
         if a.check:
-            if a.precision == "int8crt":
+            if a.precision.startswith("int8crt"):
                 _lib.call("aj_w_tilde_crt_configure", 0, a.chunk)
             else:
                 _lib.call("aj_w_tilde_set_variant", 0)

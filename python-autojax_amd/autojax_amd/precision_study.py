@@ -1,6 +1,6 @@
 """Precision study: per-stage error of each W-tilde arithmetic mode against the FP64 GPU path.
 
-    python -m autojax_amd.precision_study [--config cfg3] [--modes int8crt,int8x5,int8x4,int8x3,fp32,bf16x3]
+    python -m autojax_amd.precision_study [--config cfg3] [--modes int8crt,int8crt11,int8x5,int8x4,int8x3,fp32,bf16x3]
                                           [--lams 0.01,1,100]
 
 For every mode, W-tilde is rebuilt at that precision and everything after it runs in FP64 (as in
@@ -29,7 +29,7 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--config", default="cfg3", choices=["cfg3", "cfg5"])
     ap.add_argument("--K", type=int, default=1_000_000)
-    ap.add_argument("--modes", default="int8crt,int8x5,int8x4,int8x3,fp32,bf16x3")
+    ap.add_argument("--modes", default="int8crt,int8crt11,int8x5,int8x4,int8x3,fp32,bf16x3")
     ap.add_argument("--lams", default="0.01,1,100")
     a = ap.parse_args(argv)
 

@@ -1333,23 +1333,24 @@ int run(const double* noise, const double* uv, const double* grid, int64_t K, in
 
 // The integer-residue (CRT) mode lives in wtilde_crt.hip: it accumulates the lower triangle of a
 // zeroed W; centring and the mirror are shared with the modes above.
-size_t crt_workspace_bytes(int64_t M, int64_t K);
+size_t crt_workspace_bytes(int64_t M, int64_t K, int nm);
 int crt_accumulate(const double* noise, const double* uv, const double* grid, const double* centre, int64_t K,
-                   int64_t M, double* W, void* workspace, size_t workspace_bytes, hipStream_t st);
+                   int64_t M, int nm, double* W, void* workspace, size_t workspace_bytes, hipStream_t st);
 
-inline size_t crt_mode_bytes(int64_t M, int64_t K) { return 512 + crt_workspace_bytes(M, K); }
+inline int crt_moduli(int precision) { return precision == AJ_PREC_INT8CRT11 ? 11 : 12; }
+inline size_t crt_mode_bytes(int64_t M, int64_t K, int nm) { return 512 + crt_workspace_bytes(M, K, nm); }
 
-int run_crt_mode(const double* noise, const double* uv, const double* grid, int64_t K, int64_t M, double* W,
+int run_crt_mode(const double* noise, const double* uv, const double* grid, int64_t K, int64_t M, int nm, double* W,
                  void* workspace, size_t workspace_bytes, hipStream_t st) {
-  if (workspace_bytes < crt_mode_bytes(M, K)) {
-    set_error("workspace too small: need %zu bytes, got %zu", crt_mode_bytes(M, K), workspace_bytes);
+  if (workspace_bytes < crt_mode_bytes(M, K, nm)) {
+    set_error("workspace too small: need %zu bytes, got %zu", crt_mode_bytes(M, K, nm), workspace_bytes);
     return AJ_ESPACE;
   }
   double* centre = static_cast<double*>(workspace);
   hipLaunchKernelGGL(grid_mean_kernel, dim3(1), dim3(256), 0, st, grid, M, centre);
   AJ_LAUNCH_CHECK();
   AJ_CHECK_HIP(hipMemsetAsync(W, 0, sizeof(double) * M * M, st));
-  const int rc = crt_accumulate(noise, uv, grid, centre, K, M, W, static_cast<char*>(workspace) + 512,
+  const int rc = crt_accumulate(noise, uv, grid, centre, K, M, nm, W, static_cast<char*>(workspace) + 512,
                                 workspace_bytes - 512, st);
   if (rc != AJ_OK) return rc;
   dim3 mg((unsigned)ceil_div(M, 32), (unsigned)ceil_div(M, 32));
@@ -1364,8 +1365,8 @@ int run_crt_mode(const double* noise, const double* uv, const double* grid, int6
 using namespace aj;
 
 extern "C" size_t aj_w_tilde_workspace_bytes(int64_t K, int64_t M, int precision) {
-  if (K < 0 || M <= 0 || precision < AJ_PREC_FP64 || precision > AJ_PREC_INT8CRT) return 0;
-  if (precision == AJ_PREC_INT8CRT) return wt::crt_mode_bytes(M, K);
+  if (K < 0 || M <= 0 || precision < AJ_PREC_FP64 || precision > AJ_PREC_INT8CRT11) return 0;
+  if (precision >= AJ_PREC_INT8CRT) return wt::crt_mode_bytes(M, K, wt::crt_moduli(precision));
   return wt::make_plan(K, M, precision).bytes;
 }
 
@@ -1374,11 +1375,11 @@ extern "C" int aj_w_tilde_curvature_interferometer_from(const double* noise_map_
                                                         int precision, double* w_tilde, void* workspace,
                                                         size_t workspace_bytes, void* stream) {
   AJ_REQUIRE(M > 0 && K >= 0, "bad sizes M=%lld K=%lld", (long long)M, (long long)K);
-  AJ_REQUIRE(precision >= AJ_PREC_FP64 && precision <= AJ_PREC_INT8CRT, "bad precision %d", precision);
+  AJ_REQUIRE(precision >= AJ_PREC_FP64 && precision <= AJ_PREC_INT8CRT11, "bad precision %d", precision);
   AJ_REQUIRE(w_tilde && grid_radians_slim && (K == 0 || (noise_map_real && uv_wavelengths)), "null pointer");
-  if (precision == AJ_PREC_INT8CRT)
-    return wt::run_crt_mode(noise_map_real, uv_wavelengths, grid_radians_slim, K, M, w_tilde, workspace,
-                            workspace_bytes, as_stream(stream));
+  if (precision >= AJ_PREC_INT8CRT)
+    return wt::run_crt_mode(noise_map_real, uv_wavelengths, grid_radians_slim, K, M, wt::crt_moduli(precision),
+                            w_tilde, workspace, workspace_bytes, as_stream(stream));
   wt::Plan p = wt::make_plan(K, M, precision);
   AJ_REQUIRE(workspace_bytes >= p.bytes && workspace, "workspace too small: need %zu bytes, got %zu", p.bytes,
              workspace_bytes);

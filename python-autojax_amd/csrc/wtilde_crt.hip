@@ -32,7 +32,7 @@ namespace crt {
 
 typedef int i4 __attribute__((ext_vector_type(4)));
 
-constexpr int NMOD = 12;
+constexpr int NMOD = 12;           // moduli of the default mode; the 11-moduli mode uses the first 11
 constexpr int KC_DEFAULT = 16384;  // visibilities per exact integer Gram matrix (a chunk)
 constexpr int KC_MAX = 32768;      // chunk bound: KC_MAX (Q^2 + 2Q) < P/2 and 2 KC_MAX 128^2 < 2^31
 constexpr int CT = 256;            // output tile edge (rows of A and of B per workgroup)
@@ -44,8 +44,14 @@ constexpr int UNIT_BYTES = CT * CT;  // int8 residues of one tile
 // Pairwise coprime moduli, all <= 256 so that balanced residues fit in int8.
 constexpr int MODULI[NMOD] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217};
 
-// Largest feature integer: 127 * 256^4.
-constexpr double QMAX = 127.0 * 256.0 * 256.0 * 256.0 * 256.0;
+// Largest feature integer: 127 * 256^4 (about 2^39, the five-digit grid) with all 12 moduli
+// (P about 2^94.8, chunks up to 58,091 visibilities); 60 * 2^30 (about 2^35.9) with the first 11
+// (P about 2^87.0, chunks up to 23,300 visibilities).
+constexpr double QMAX12 = 127.0 * 256.0 * 256.0 * 256.0 * 256.0;
+constexpr double QMAX11 = 60.0 * 1073741824.0;
+__host__ __device__ constexpr double qmax_for(int nm) { return nm == 12 ? QMAX12 : QMAX11; }
+// largest chunk (visibilities) whose integer Gram stays inside (-P/2, P/2)
+constexpr int kc_max_for(int nm) { return nm == 12 ? 32768 : 16384; }
 
 struct Consts {
   double m[NMOD];      // moduli
@@ -54,6 +60,7 @@ struct Consts {
   double y_lo[NMOD];   // y_i - y_hi, an integer below 2^53 (exact)
   double p_hi, p_lo;   // P split the same way
   double inv_p;        // 1 / P
+  int nm;              // moduli in use (the first nm of MODULI)
 };
 
 typedef unsigned __int128 u128;
@@ -77,13 +84,14 @@ inline double u128_to_double_exact_hi(u128 v) {  // v is a multiple of 2^53 belo
   return std::ldexp((double)(uint64_t)(v >> 53), 53);
 }
 
-// CRT constants, computed once in exact 128-bit integer arithmetic (P < 2^95).
-inline Consts make_consts() {
+// CRT constants of the first nm moduli, computed once in exact 128-bit integer arithmetic (P < 2^95).
+inline Consts make_consts(int nm) {
   Consts c{};
+  c.nm = nm;
   u128 P = 1;
-  for (int i = 0; i < NMOD; ++i) P *= (u128)MODULI[i];
+  for (int i = 0; i < nm; ++i) P *= (u128)MODULI[i];
   const u128 mask53 = (((u128)1) << 53) - 1;
-  for (int i = 0; i < NMOD; ++i) {
+  for (int i = 0; i < nm; ++i) {
     const int64_t m = MODULI[i];
     const u128 Pi = P / (u128)m;
     const int64_t inv = inv_mod((int64_t)(Pi % (u128)m), m);
@@ -176,11 +184,11 @@ __device__ inline uint32_t packed_residues(const double (&qc)[2], const double (
          ((uint32_t)(uint8_t)(int8_t)residue_of<i>(qn[1]) << 24);
 }
 
-template <int i>
+template <int i, int NM>
 __device__ inline void store_residues4(int8_t* __restrict__ X, int64_t plane, uint32_t off, const double (&qc)[2],
                                        const double (&qn)[2]) {
   *reinterpret_cast<uint32_t*>(X + i * plane + off) = packed_residues<i>(qc, qn);
-  if constexpr (i + 1 < NMOD) store_residues4<i + 1>(X, plane, off, qc, qn);
+  if constexpr (i + 1 < NM) store_residues4<i + 1, NM>(X, plane, off, qc, qn);
 }
 
 // ---------------------------------------------------------------------------
@@ -189,6 +197,7 @@ __device__ inline void store_residues4(int8_t* __restrict__ X, int64_t plane, ui
 // visibilities (4 bytes per row and plane) x the 16 rows of row-block blockIdx.y.  Padded rows and
 // visibilities past K hold zeros.
 // ---------------------------------------------------------------------------
+template <int NM>
 __global__ void __launch_bounds__(256) features_kernel(const double* __restrict__ noise,
                                                        const double* __restrict__ uv,
                                                        const double* __restrict__ grid,
@@ -199,7 +208,7 @@ __global__ void __launch_bounds__(256) features_kernel(const double* __restrict_
   const int k0 = (blockIdx.x * 256 + threadIdx.x) * 2;  // first of 2 visibilities (chunk-local)
   const int rb = blockIdx.y;
   double u0[2], u1[2], qs[2];
-  const double inv_sc = QMAX / sc[0];
+  const double inv_sc = qmax_for(NM) / sc[0];
 #pragma unroll
   for (int v = 0; v < 2; ++v) {
     const int64_t kg = k_begin + k0 + v;
@@ -231,7 +240,7 @@ __global__ void __launch_bounds__(256) features_kernel(const double* __restrict_
 #pragma unroll
       for (int v = 0; v < 2; ++v) qc[v] = qn[v] = 0.0;
     }
-    store_residues4<0>(X, plane, off, qc, qn);
+    store_residues4<0, NM>(X, plane, off, qc, qn);
   }
 }
 
@@ -276,7 +285,8 @@ __device__ inline void lds_dma16(const int8_t* g, int8_t* lds_block) {
 
 // ---------------------------------------------------------------------------
 // SYRK of one (tile, modulus) unit: the int32 Gram block of A = rows [I*256, +256) and B = rows
-// [J*256, +256) over the chunk's 2 kc features of residue plane m, reduced to balanced int8 residues
+// [J*256, +256) over the first nst stages (64 features each; all 2 kc / 64 = nkb of them except in a
+// short last chunk) of residue plane m, reduced to balanced int8 residues
 // mod m_m.  Eight waves (two per SIMD), each a 128 x 64 block of the tile (8 x 4 MFMA 16x16x64
 // blocks, 128 int32 accumulators per lane).  The feature blocks (1 KB: 16 rows x 64 bytes; 32 per
 // stage, 16 of A and 16 of B) move by direct global->LDS loads into a ring of four 32 KB stage
@@ -287,13 +297,13 @@ __device__ inline void lds_dma16(const int8_t* g, int8_t* lds_block) {
 // in the tile list) for modulus 0, then modulus 1, ..., so that all XCDs read the same residue plane
 // at a time and the tiles an XCD holds share row panels in its L2.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(512, 1) syrk_ring8_kernel(const int8_t* __restrict__ X, int64_t plane, int nkb,
+__global__ void __launch_bounds__(512, 1) syrk_ring8_kernel(const int8_t* __restrict__ X, int64_t plane, int nkb, int nst,
                                                             int ntiles, int per, const int* __restrict__ tiles,
                                                             Consts c, int8_t* __restrict__ R) {
   __shared__ __attribute__((aligned(1024))) int8_t lds[4][32][1024];
   const int xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
   const int mi = slot / per, loc = slot % per;
-  if (mi >= NMOD) return;
+  if (mi >= c.nm) return;
   const int t = xcd * per + loc;
   if (t >= ntiles) return;
   const int64_t I = tiles[2 * t], J = tiles[2 * t + 1];
@@ -327,7 +337,7 @@ __global__ void __launch_bounds__(512, 1) syrk_ring8_kernel(const int8_t* __rest
 
   AJ_RING8_ISSUE(0)
   AJ_RING8_ISSUE(1)
-  const int NI = nkb / 2;
+  const int NI = nst / 2;
   for (int j = 0; j < NI; ++j) {
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
@@ -389,13 +399,13 @@ __device__ inline i32x4 make_rsrc(const void* base, uint32_t bytes) {
 // SPLIT = 1 runs the first half of a stage's MFMAs before the barrier (their fragments are already in
 // registers), so a SIMD has MFMA work while its waves wait at the barrier and issue the next loads.
 template <int SPLIT>
-__global__ void __launch_bounds__(512, 1) syrk_pipe8_kernel(const int8_t* __restrict__ X, int64_t plane, int nkb,
+__global__ void __launch_bounds__(512, 1) syrk_pipe8_kernel(const int8_t* __restrict__ X, int64_t plane, int nkb, int nst,
                                                             int ntiles, int per, const int* __restrict__ tiles,
                                                             Consts c, int8_t* __restrict__ R) {
   __shared__ __attribute__((aligned(1024))) int8_t lds[4][32][1024];
   const int xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
   const int mi = slot / per, loc = slot % per;
-  if (mi >= NMOD) return;
+  if (mi >= c.nm) return;
   const int t = xcd * per + loc;
   if (t >= ntiles) return;
   const int I = tiles[2 * t], J = tiles[2 * t + 1];
@@ -444,7 +454,7 @@ __global__ void __launch_bounds__(512, 1) syrk_pipe8_kernel(const int8_t* __rest
   }
 // Branch-free, so that the waits before the MFMAs count only the reads issued one step earlier.  Past
 // the chunk's last stage the loads fetch the next row-block's features, or past the end of the plane
-// (the buffer range) zeros, into buffers that are no longer read, and the reads of stage nkb are
+// (the buffer range) zeros, into buffers that are no longer read, and the reads of stage nst are
 // never used.
 #define AJ_PIPE_STEP(FA, FB, GA, GB, ST)                                                                 \
   {                                                                                                      \
@@ -467,7 +477,7 @@ __global__ void __launch_bounds__(512, 1) syrk_pipe8_kernel(const int8_t* __rest
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[a][b] = i4{0, 0, 0, 0};
 
-  // nkb >= 128 and even
+  // nst even (>= 2); nkb (the row-block stride in blocks) >= nst
   i4 fa0[8], fb0[4], fa1[8], fb1[4];
   AJ_PIPE_ISSUE(0)
   AJ_PIPE_ISSUE(1)
@@ -477,7 +487,7 @@ __global__ void __launch_bounds__(512, 1) syrk_pipe8_kernel(const int8_t* __rest
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   AJ_PIPE_READ(fa0, fb0, 0)
-  for (int st = 0; st < nkb; st += 2) {
+  for (int st = 0; st < nst; st += 2) {
     AJ_PIPE_STEP(fa0, fb0, fa1, fb1, st)
     AJ_PIPE_STEP(fa1, fb1, fa0, fb0, st + 1)
   }
@@ -492,13 +502,14 @@ __global__ void __launch_bounds__(512, 1) syrk_pipe8_kernel(const int8_t* __rest
 }
 
 // ---------------------------------------------------------------------------
-// Combine: per element of the lower triangle, G from its 12 residues, W += G * scale.
+// Combine: per element of the lower triangle, G from its NM residues, W += G * scale.
 //   S_hi = sum_i r_i y_hi_i  (exact: every term and partial sum is a multiple of 2^53 below 2^106)
 //   S_lo = sum_i r_i y_lo_i  (|.| < 2^64, rounding error below 2^16)
 //   n    = rint((S_hi + S_lo) / P)
 //   G    = (S_hi - n p_hi) + (S_lo - n p_lo)   (first bracket exact)
 // Thread = one packed word of one unit: 4 elements (one column, 4 rows) of a 16 x 16 block.
 // ---------------------------------------------------------------------------
+template <int NM>
 __global__ void __launch_bounds__(256) combine_kernel(const int8_t* __restrict__ R, int ntiles,
                                                       const int* __restrict__ tiles, Consts c,
                                                       const double* __restrict__ sc, double* __restrict__ W,
@@ -511,18 +522,19 @@ __global__ void __launch_bounds__(256) combine_kernel(const int8_t* __restrict__
   const int64_t i0 = (int64_t)tiles[2 * t] * CT + rb * 16 + 4 * (lane >> 4);
   const int64_t j = (int64_t)tiles[2 * t + 1] * CT + cb * 16 + (lane & 15);
   if (j >= M || i0 >= M || i0 + 3 < j) return;
-  uint32_t w[NMOD];
+  uint32_t w[NM];
 #pragma unroll
-  for (int m = 0; m < NMOD; ++m)
+  for (int m = 0; m < NM; ++m)
     w[m] = reinterpret_cast<const uint32_t*>(R + ((int64_t)m * ntiles + t) * UNIT_BYTES)[rem];
-  const double s2 = sc[0] * sc[0] / (QMAX * QMAX);
+  constexpr double Q = qmax_for(NM);
+  const double s2 = sc[0] * sc[0] / (Q * Q);
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int64_t i = i0 + r;
     if (i >= M || i < j) continue;
     double shi = 0.0, slo = 0.0;
 #pragma unroll
-    for (int m = 0; m < NMOD; ++m) {
+    for (int m = 0; m < NM; ++m) {
       const double rv = (double)(int8_t)(w[m] >> (8 * r));
       shi = fma(rv, c.y_hi[m], shi);
       slo = fma(rv, c.y_lo[m], slo);
@@ -554,14 +566,14 @@ struct Plan {
   size_t feat_bytes, res_bytes, bytes;
 };
 
-inline Plan make_plan(int64_t M, int64_t K, int kc) {
+inline Plan make_plan(int64_t M, int64_t K, int kc, int nm) {
   Plan p{};
   p.Mpad = ceil_div(M, CT) * CT;
   p.nI = p.Mpad / CT;
   p.ntiles = p.nI * (p.nI + 1) / 2;
   p.per = ceil_div(p.ntiles, 8);
-  p.feat_bytes = (size_t)NMOD * p.Mpad * 2 * kc;  // one chunk
-  p.res_bytes = (size_t)NMOD * p.ntiles * UNIT_BYTES;
+  p.feat_bytes = (size_t)nm * p.Mpad * 2 * kc;  // one chunk
+  p.res_bytes = (size_t)nm * p.ntiles * UNIT_BYTES;
   const int64_t nch = std::max<int64_t>(1, ceil_div(K, kc));
   p.bytes = align_up(p.feat_bytes, 256) + align_up(p.res_bytes, 256) +
             align_up(sizeof(int) * 2 * p.ntiles, 256) + align_up(sizeof(double) * nch, 256) + 256;
@@ -577,14 +589,21 @@ namespace wt {
 int g_crt_variant = 0;
 int g_crt_kc = crt::KC_DEFAULT;
 
-size_t crt_workspace_bytes(int64_t M, int64_t K) { return crt::make_plan(M, K, g_crt_kc).bytes; }
+// Visibilities per chunk for nm moduli: the configured chunk, capped by the modulus product's range.
+inline int crt_kc(int nm) { return std::min(g_crt_kc, crt::kc_max_for(nm)); }
+
+size_t crt_workspace_bytes(int64_t M, int64_t K, int nm) { return crt::make_plan(M, K, crt_kc(nm), nm).bytes; }
 
 // Accumulate the lower triangle of W (zeroed by the caller) over all visibilities.
 int crt_accumulate(const double* noise, const double* uv, const double* grid, const double* centre, int64_t K,
-                   int64_t M, double* W, void* workspace, size_t workspace_bytes, hipStream_t st) {
+                   int64_t M, int nm, double* W, void* workspace, size_t workspace_bytes, hipStream_t st) {
   using namespace crt;
-  const int kc = g_crt_kc, nkb = kc / 32;
-  const Plan p = make_plan(M, K, kc);
+  if (nm != 11 && nm != 12) {
+    set_error("int8crt: %d moduli (11 or 12 supported)", nm);
+    return AJ_EINVAL;
+  }
+  const int kc = crt_kc(nm), nkb = kc / 32;
+  const Plan p = make_plan(M, K, kc, nm);
   const int64_t nch = std::max<int64_t>(1, ceil_div(K, kc));
   Arena ar(workspace, workspace_bytes);
   int8_t* X = ar.take<int8_t>(p.feat_bytes);
@@ -601,7 +620,8 @@ int crt_accumulate(const double* noise, const double* uv, const double* grid, co
     return AJ_EINVAL;
   }
   if (K == 0) return AJ_OK;
-  static const Consts cst = make_consts();
+  static const Consts cst12 = make_consts(12), cst11 = make_consts(11);
+  const Consts& cst = nm == 12 ? cst12 : cst11;
   static thread_local std::vector<int> tl;
   build_tiles(p.nI, tl);
   AJ_CHECK_HIP(hipMemcpyAsync(tiles, tl.data(), sizeof(int) * tl.size(), hipMemcpyHostToDevice, st));
@@ -609,25 +629,37 @@ int crt_accumulate(const double* noise, const double* uv, const double* grid, co
   AJ_LAUNCH_CHECK();
   const int64_t plane = p.Mpad * 2 * (int64_t)kc;
   const dim3 fg((unsigned)(kc / 512), (unsigned)(p.Mpad / 16));
-  const dim3 sg((unsigned)(8 * p.per * NMOD));
+  const dim3 sg((unsigned)(8 * p.per * nm));
   const dim3 cg((unsigned)ceil_div(p.ntiles * 16384, 256));
   const int nt = (int)p.ntiles, per = (int)p.per;
   for (int64_t ch = 0; ch < nch; ++ch) {
+    // a short last chunk runs only the stages that hold its visibilities (an even count; the features
+    // kernel zero-fills every visibility of those stages past K)
+    const int64_t kv = std::min<int64_t>(kc, K - ch * kc);
+    const int nst = std::min<int>(nkb, (int)(2 * ceil_div(kv, 64)));
+    const dim3 fgc((unsigned)std::min<int64_t>(fg.x, ceil_div((int64_t)nst * 32, 512)), fg.y);
     timer_begin(AJ_TIMER_WT_FEATURES, st);
-    hipLaunchKernelGGL(features_kernel, fg, dim3(256), 0, st, noise, uv, grid, centre, K, M, p.Mpad, ch * kc, kc,
-                       sc + ch, X);
+    if (nm == 12)
+      hipLaunchKernelGGL(features_kernel<12>, fgc, dim3(256), 0, st, noise, uv, grid, centre, K, M, p.Mpad, ch * kc,
+                         kc, sc + ch, X);
+    else
+      hipLaunchKernelGGL(features_kernel<11>, fgc, dim3(256), 0, st, noise, uv, grid, centre, K, M, p.Mpad, ch * kc,
+                         kc, sc + ch, X);
     timer_end(AJ_TIMER_WT_FEATURES, st);
     AJ_LAUNCH_CHECK();
     timer_begin(AJ_TIMER_WT_SYRK, st);
     switch (g_crt_variant) {
-      case 1: hipLaunchKernelGGL(syrk_pipe8_kernel<0>, sg, dim3(512), 0, st, X, plane, nkb, nt, per, tiles, cst, R); break;
-      case 2: hipLaunchKernelGGL(syrk_ring8_kernel, sg, dim3(512), 0, st, X, plane, nkb, nt, per, tiles, cst, R); break;
-      default: hipLaunchKernelGGL(syrk_pipe8_kernel<1>, sg, dim3(512), 0, st, X, plane, nkb, nt, per, tiles, cst, R); break;
+      case 1: hipLaunchKernelGGL(syrk_pipe8_kernel<0>, sg, dim3(512), 0, st, X, plane, nkb, nst, nt, per, tiles, cst, R); break;
+      case 2: hipLaunchKernelGGL(syrk_ring8_kernel, sg, dim3(512), 0, st, X, plane, nkb, nst, nt, per, tiles, cst, R); break;
+      default: hipLaunchKernelGGL(syrk_pipe8_kernel<1>, sg, dim3(512), 0, st, X, plane, nkb, nst, nt, per, tiles, cst, R); break;
     }
     timer_end(AJ_TIMER_WT_SYRK, st);
     AJ_LAUNCH_CHECK();
     timer_begin(AJ_TIMER_WT_COMBINE, st);
-    hipLaunchKernelGGL(combine_kernel, cg, dim3(256), 0, st, R, nt, tiles, cst, sc + ch, W, M);
+    if (nm == 12)
+      hipLaunchKernelGGL(combine_kernel<12>, cg, dim3(256), 0, st, R, nt, tiles, cst, sc + ch, W, M);
+    else
+      hipLaunchKernelGGL(combine_kernel<11>, cg, dim3(256), 0, st, R, nt, tiles, cst, sc + ch, W, M);
     timer_end(AJ_TIMER_WT_COMBINE, st);
     AJ_LAUNCH_CHECK();
   }
@@ -655,15 +687,16 @@ extern "C" int aj_w_tilde_crt_configure(int variant, int64_t chunk_visibilities)
   return AJ_OK;
 }
 
-// CRT constants as the kernels use them (host query, no reference counterpart): fills
-// moduli[12], y_hi[12], y_lo[12] and pq[3] = {p_hi, p_lo, inv_p}.
-extern "C" int aj_crt_constants(double* moduli, double* y_hi, double* y_lo, double* pq) {
-  if (!moduli || !y_hi || !y_lo || !pq) {
-    aj::set_error("null pointer");
+// CRT constants as the kernels use them for nmod = 12 or 11 moduli (host query, no reference
+// counterpart): fills moduli[nmod], y_hi[nmod], y_lo[nmod] and pq[4] = {p_hi, p_lo, inv_p, Q} with Q
+// the largest feature integer of that mode.
+extern "C" int aj_crt_constants(int nmod, double* moduli, double* y_hi, double* y_lo, double* pq) {
+  if (!moduli || !y_hi || !y_lo || !pq || (nmod != 11 && nmod != 12)) {
+    aj::set_error("aj_crt_constants: null pointer or nmod %d not 11 or 12", nmod);
     return AJ_EINVAL;
   }
-  const aj::crt::Consts c = aj::crt::make_consts();
-  for (int i = 0; i < aj::crt::NMOD; ++i) {
+  const aj::crt::Consts c = aj::crt::make_consts(nmod);
+  for (int i = 0; i < nmod; ++i) {
     moduli[i] = c.m[i];
     y_hi[i] = c.y_hi[i];
     y_lo[i] = c.y_lo[i];
@@ -671,5 +704,6 @@ extern "C" int aj_crt_constants(double* moduli, double* y_hi, double* y_lo, doub
   pq[0] = c.p_hi;
   pq[1] = c.p_lo;
   pq[2] = c.inv_p;
+  pq[3] = aj::crt::qmax_for(nmod);
   return AJ_OK;
 }

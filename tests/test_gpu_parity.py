@@ -13,8 +13,8 @@ pytestmark = pytest.mark.gpu
 GOLD = Path(__file__).resolve().parent / "golden"
 
 # normwise tolerances of W-tilde against the FP64 oracle, relative to max|W| (= the diagonal)
-WT_TOL = {"fp64": 1e-12, "int8x4": 5e-8, "int8x5": 5e-12, "int8crt": 5e-12, "int8x3": 3e-7, "fp32": 3e-6,
-          "bf16x3": 3e-4}
+WT_TOL = {"fp64": 1e-12, "int8x4": 5e-8, "int8x5": 5e-12, "int8crt": 5e-12, "int8crt11": 3e-11, "int8x3": 3e-7,
+          "fp32": 3e-6, "bf16x3": 3e-4}
 
 
 @pytest.fixture(scope="module")
@@ -47,7 +47,7 @@ def test_w_tilde_fp64_matches_fsum_golden(gpu):
     assert _rel(W, g["w_tilde"]) < 1e-13
 
 
-@pytest.mark.parametrize("precision", ["fp64", "int8x4", "int8x5", "int8crt", "int8x3", "fp32", "bf16x3"])
+@pytest.mark.parametrize("precision", ["fp64", "int8x4", "int8x5", "int8crt", "int8crt11", "int8x3", "fp32", "bf16x3"])
 def test_w_tilde_modes_vs_oracle(gpu, cfg1_small, precision):
     from autojax_amd import api
 
@@ -89,8 +89,11 @@ def test_w_tilde_crt_multichunk(gpu):
     np.testing.assert_array_equal(W, W.T)
     W5 = api.w_tilde_curvature_interferometer_from(*args, precision="int8x5")
     W64 = api.w_tilde_curvature_interferometer_from(*args, precision="fp64")
-    print(f"int8crt vs int8x5 {_rel(W, W5):.3e}, vs fp64 {_rel(W, W64):.3e}", flush=True)
-    assert _rel(W, W5) < 5e-12 and _rel(W, W64) < 5e-12
+    W11 = api.w_tilde_curvature_interferometer_from(*args, precision="int8crt11")
+    print(f"int8crt vs int8x5 {_rel(W, W5):.3e}, vs fp64 {_rel(W, W64):.3e}; int8crt11 vs fp64 {_rel(W11, W64):.3e}",
+          flush=True)
+    assert _rel(W, W5) < 5e-12 and _rel(W, W64) < 5e-12 and _rel(W11, W64) < 3e-11
+    np.testing.assert_array_equal(W11, W11.T)
     g, n = d.grid_radians_slim, d.noise_map_real
     for i in (0, 1, 255, 256, 511, 715):
         dg = 2 * np.pi * (g[i] - g)
