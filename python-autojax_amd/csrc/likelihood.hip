@@ -16,6 +16,7 @@
 // ordered sum with no floating-point atomics.
 #include "aj_common.h"
 
+#include <climits>
 #include <vector>
 
 namespace aj {
@@ -107,28 +108,43 @@ __global__ void csc_fill_kernel(const SampleDesc* __restrict__ sd, const int64_t
 }
 
 // Sort each bucket by key (m*B + b): insertion sort, buckets are short.
-__global__ void csc_sort_kernel(const SampleDesc* __restrict__ sd, const int* __restrict__ ptrs, int64_t B,
-                                int* __restrict__ ent_key, double* __restrict__ ent_t) {
+// One wave per bucket: buckets of at most 64 entries are ordered by rank (each lane counts the keys
+// below its own over the wave; keys m*B+b are distinct); larger buckets by insertion sort in lane 0.
+// Keys become m for the gathers.  256 threads = 4 buckets per workgroup.
+__global__ void __launch_bounds__(256) csc_sort_kernel(const SampleDesc* __restrict__ sd,
+                                                       const int* __restrict__ ptrs, int64_t B,
+                                                       int* __restrict__ ent_key, double* __restrict__ ent_t) {
   const SampleDesc d = sd[blockIdx.y];
   const int* p = ptrs + d.csc_ptr;
-  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < d.S; s += (int64_t)gridDim.x * blockDim.x) {
-    const int lo = p[s], hi = p[s + 1];
+  const int lane = threadIdx.x & 63;
+  for (int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); s < d.S; s += (int64_t)gridDim.x * 4) {
+    const int lo = p[s], hi = p[s + 1], len = hi - lo;
     int* k = ent_key + d.csc_ent;
     double* t = ent_t + d.csc_ent;
-    for (int i = lo + 1; i < hi; ++i) {
-      int kk = k[i];
-      double tt = t[i];
-      int j = i - 1;
-      while (j >= lo && k[j] > kk) {
-        k[j + 1] = k[j];
-        t[j + 1] = t[j];
-        --j;
+    if (len <= 64) {
+      const int key = lane < len ? k[lo + lane] : INT_MAX;
+      const double tv = lane < len ? t[lo + lane] : 0.0;
+      int rank = 0;
+      for (int j = 0; j < 64; ++j) rank += __shfl(key, j, 64) < key ? 1 : 0;
+      if (lane < len) {
+        k[lo + rank] = (int)(key / B);
+        t[lo + rank] = tv;
       }
-      k[j + 1] = kk;
-      t[j + 1] = tt;
+    } else if (lane == 0) {
+      for (int i = lo + 1; i < hi; ++i) {
+        int kk = k[i];
+        double tt = t[i];
+        int j = i - 1;
+        while (j >= lo && k[j] > kk) {
+          k[j + 1] = k[j];
+          t[j + 1] = t[j];
+          --j;
+        }
+        k[j + 1] = kk;
+        t[j + 1] = tt;
+      }
+      for (int i = lo; i < hi; ++i) k[i] = (int)(k[i] / B);
     }
-    // keys now hold m*B+b; convert to m for the gathers
-    for (int i = lo; i < hi; ++i) k[i] = (int)(k[i] / B);
   }
 }
 
@@ -153,7 +169,7 @@ __global__ void data_vector_kernel(const SampleDesc* __restrict__ sd, const int*
 // one source pixel s and the R rows by gathering its bucket from LDS.
 // ---------------------------------------------------------------------------
 constexpr int OMEGA_THREADS = 1024;
-constexpr size_t OMEGA_LDS_BUDGET = 150 * 1024;
+constexpr size_t OMEGA_LDS_BUDGET = 160 * 1024;  // the whole LDS of a gfx950 CU: 4 rows of W at M = 5,000
 
 // Where W-tilde comes from: the dense M x M matrix, or the lattice table w_compact [N, 2N-1] with
 // the (row, column) grid index of every image pixel (numba.py:237-255 gives the element rule).
@@ -887,7 +903,8 @@ int batch_pipeline(const WSrc w_tilde, int64_t M, const double* dirty_image, con
                      cur, ekey, et);
   AJ_LAUNCH_CHECK();
   const unsigned gs = (unsigned)std::min<int64_t>(ceil_div(maxS, 256), 1024);
-  hipLaunchKernelGGL(csc_sort_kernel, dim3(gs, (unsigned)n), dim3(256), 0, st, desc, ptrs, B, ekey, et);
+  hipLaunchKernelGGL(csc_sort_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(maxS, 4), 65535), (unsigned)n),
+                     dim3(256), 0, st, desc, ptrs, B, ekey, et);
   AJ_LAUNCH_CHECK();
   hipLaunchKernelGGL(data_vector_kernel, dim3(gs, (unsigned)n), dim3(256), 0, st, desc, ptrs, ekey, et, dirty_image,
                      D);
@@ -1338,11 +1355,11 @@ int build_one_csc(const int64_t* idx, const double* w, int64_t M, int64_t B, int
   AJ_CHECK_HIP(hipMemcpyAsync(o.desc, &h, sizeof(h), hipMemcpyHostToDevice, st));
   AJ_CHECK_HIP(hipMemsetAsync(o.ptrs, 0, sizeof(int) * (S + 1), st));
   const unsigned gm = (unsigned)std::min<int64_t>(ceil_div(M, 256), 1024);
-  const unsigned gs = (unsigned)std::min<int64_t>(ceil_div(S, 256), 1024);
   hipLaunchKernelGGL(csc_count_kernel, dim3(gm, 1), dim3(256), 0, st, o.desc, idx, M, B, o.ptrs);
   hipLaunchKernelGGL(csc_scan_kernel, dim3(1), dim3(1024), 0, st, o.desc, o.ptrs, cur);
   hipLaunchKernelGGL(csc_fill_kernel, dim3(gm, 1), dim3(256), 0, st, o.desc, idx, w, M, B, cur, o.ekey, o.et);
-  hipLaunchKernelGGL(csc_sort_kernel, dim3(gs, 1), dim3(256), 0, st, o.desc, o.ptrs, B, o.ekey, o.et);
+  hipLaunchKernelGGL(csc_sort_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(S, 4), 65535), 1), dim3(256), 0, st,
+                     o.desc, o.ptrs, B, o.ekey, o.et);
   AJ_LAUNCH_CHECK();
   return AJ_OK;
 }
