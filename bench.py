@@ -65,7 +65,7 @@ I8_DIGITS = {"int8x3": (3, 5), "int8x4": (4, 6), "int8x5": (5, 7)}
 
 def syrk_kernel_name(precision, variant):
     if precision.startswith("int8crt"):
-        return "crt::syrk_pipe8_kernel<1>"
+        return "crt::syrk_sk_kernel<1>"
     if precision not in I8_DIGITS:
         return f"syrk_kernel<{precision}>"
     ns, ng = I8_DIGITS[precision]

@@ -85,8 +85,8 @@ int aj_w_tilde_set_group_bytes(size_t bytes);
  * y_i = (P/m_i) ((P/m_i)^-1 mod m_i) mod P split as y_hi[i] (a multiple of 2^53) + y_lo[i] (an integer
  * below 2^53); pq = {P_hi, P_lo, 1/P, Q} with Q the largest feature integer of the mode. */
 int aj_crt_constants(int nmod, double* moduli, double* y_hi, double* y_lo, double* pq);
-/* Tuning of AJ_PREC_INT8CRT and AJ_PREC_INT8CRT11 (process-wide): residue SYRK kernel variant 0..2
- * (-1 = default 0; all compute identical residues) and visibilities per exact integer chunk (a
+/* Tuning of AJ_PREC_INT8CRT and AJ_PREC_INT8CRT11 (process-wide): residue SYRK kernel variant 0..3
+ * (-1 = default 3, the persistent form with one workgroup per CU; all compute identical residues) and visibilities per exact integer chunk (a
  * multiple of 4096 up to 32768, capped at 16384 with 11 moduli; 0 = default 16384).  The chunk size
  * changes aj_w_tilde_workspace_bytes for these modes. */
 int aj_w_tilde_crt_configure(int variant, int64_t chunk_visibilities);

@@ -248,30 +248,35 @@ __global__ void __launch_bounds__(256) features_kernel(const double* __restrict_
 // unit: R[unit][rb][cb][lane] for the 16 x 16 block (rb, cb) of the 256 x 256 tile, each block's 64
 // words in MFMA output order (lane holds rows 4*(lane/16) + 0..3, column lane%16).  The wave's
 // block (a, b) is tile block (rb0 + a, cb0 + b).
+// Balanced residue of v mod m_mi (modulus 0 is 256: the low byte as int8; the others odd, in
+// [-(m - 1)/2, (m - 1)/2]); exact for |v| < 2^30.
+__device__ inline int balanced_residue(int v, int mi, double m, double inv_m) {
+  if (mi == 0) return (int)(int8_t)(v & 0xFF);
+  const double d = (double)v;
+  return (int)fma(-rint(d * inv_m), m, d);
+}
+
 template <int NA, int NB>
-__device__ inline void store_residues(const i4 (&acc)[NA][NB], const Consts& c, int mi, int ntiles, int t, int rb0,
-                                      int cb0, int lane, int8_t* __restrict__ R) {
+__device__ inline void store_residues_to(const i4 (&acc)[NA][NB], const Consts& c, int mi, int8_t* __restrict__ unit,
+                                         int rb0, int cb0, int lane) {
   const double m = c.m[mi], inv_m = c.inv_m[mi];
-  uint32_t* out = reinterpret_cast<uint32_t*>(R + ((int64_t)mi * ntiles + t) * UNIT_BYTES) + lane;
+  uint32_t* out = reinterpret_cast<uint32_t*>(unit) + lane;
 #pragma unroll
   for (int a = 0; a < NA; ++a)
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       uint32_t w = 0;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int v = acc[a][b][r];
-        int res;
-        if (mi == 0) {
-          res = (int)(int8_t)(v & 0xFF);
-        } else {
-          const double d = (double)v;
-          res = (int)fma(-rint(d * inv_m), m, d);
-        }
-        w |= (uint32_t)(uint8_t)(int8_t)res << (8 * r);
-      }
+      for (int r = 0; r < 4; ++r)
+        w |= (uint32_t)(uint8_t)(int8_t)balanced_residue(acc[a][b][r], mi, m, inv_m) << (8 * r);
       out[((rb0 + a) * 16 + cb0 + b) * 64] = w;
     }
+}
+
+template <int NA, int NB>
+__device__ inline void store_residues(const i4 (&acc)[NA][NB], const Consts& c, int mi, int ntiles, int t, int rb0,
+                                      int cb0, int lane, int8_t* __restrict__ R) {
+  store_residues_to(acc, c, mi, R + ((int64_t)mi * ntiles + t) * UNIT_BYTES, rb0, cb0, lane);
 }
 
 // Direct global->LDS load of 16 bytes per lane: lane L of the wave writes bytes 16 L .. 16 L + 15 of
@@ -396,33 +401,10 @@ __device__ inline i32x4 make_rsrc(const void* base, uint32_t bytes) {
   return r;
 }
 
-// SPLIT = 1 runs the first half of a stage's MFMAs before the barrier (their fragments are already in
-// registers), so a SIMD has MFMA work while its waves wait at the barrier and issue the next loads.
-template <int SPLIT>
-__global__ void __launch_bounds__(512, 1) syrk_pipe8_kernel(const int8_t* __restrict__ X, int64_t plane, int nkb, int nst,
-                                                            int ntiles, int per, const int* __restrict__ tiles,
-                                                            Consts c, int8_t* __restrict__ R) {
-  __shared__ __attribute__((aligned(1024))) int8_t lds[4][32][1024];
-  const int xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
-  const int mi = slot / per, loc = slot % per;
-  if (mi >= c.nm) return;
-  const int t = xcd * per + loc;
-  if (t >= ntiles) return;
-  const int I = tiles[2 * t], J = tiles[2 * t + 1];
-
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int lr = lane & 15, lk = lane >> 4;
-  const int srow = lane >> 2;
-  const int soff = srow * 64 + 16 * ((lane & 3) ^ ((srow >> 1) & 3));
-  const i32x4 rsrc = make_rsrc(X + mi * plane, (uint32_t)plane);
-  const uint32_t rbytes = (uint32_t)nkb * 1024u;  // bytes per row-block
-  const uint32_t oA = (uint32_t)(I * 16 + wave) * rbytes + soff;
-  const uint32_t oB = (uint32_t)(J * 16 + wave) * rbytes + soff;
-  const uint32_t jstr = 8u * rbytes;
-  const int roff = lr * 64 + 16 * (lk ^ ((lr >> 1) & 3));
-  const int ablk = (wave >> 2) * 8, bblk = 16 + (wave & 3) * 4;
-  typedef __attribute__((address_space(3))) uint32_t lds_u32;
-
+// Pipeline steps shared by the residue SYRK kernels.  They use the kernel's locals: lds (the ring of
+// four 32 KB stage buffers), rsrc (buffer resource of the residue plane), oA / oB (this wave's byte
+// offsets of its A and B blocks at stage 0), jstr, wave, ablk, bblk, roff, acc and the SPLIT parameter.
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
 #define AJ_PIPE_ISSUE(ST)                                                                                \
   {                                                                                                      \
     const uint32_t ko_ = (uint32_t)(ST) * 1024u;                                                         \
@@ -471,6 +453,33 @@ __global__ void __launch_bounds__(512, 1) syrk_pipe8_kernel(const int8_t* __rest
     __builtin_amdgcn_sched_barrier(0);                                                                   \
   }
 
+
+// SPLIT = 1 runs the first half of a stage's MFMAs before the barrier (their fragments are already in
+// registers), so a SIMD has MFMA work while its waves wait at the barrier and issue the next loads.
+template <int SPLIT>
+__global__ void __launch_bounds__(512, 1) syrk_pipe8_kernel(const int8_t* __restrict__ X, int64_t plane, int nkb, int nst,
+                                                            int ntiles, int per, const int* __restrict__ tiles,
+                                                            Consts c, int8_t* __restrict__ R) {
+  __shared__ __attribute__((aligned(1024))) int8_t lds[4][32][1024];
+  const int xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
+  const int mi = slot / per, loc = slot % per;
+  if (mi >= c.nm) return;
+  const int t = xcd * per + loc;
+  if (t >= ntiles) return;
+  const int I = tiles[2 * t], J = tiles[2 * t + 1];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int srow = lane >> 2;
+  const int soff = srow * 64 + 16 * ((lane & 3) ^ ((srow >> 1) & 3));
+  const i32x4 rsrc = make_rsrc(X + mi * plane, (uint32_t)plane);
+  const uint32_t rbytes = (uint32_t)nkb * 1024u;  // bytes per row-block
+  const uint32_t oA = (uint32_t)(I * 16 + wave) * rbytes + soff;
+  const uint32_t oB = (uint32_t)(J * 16 + wave) * rbytes + soff;
+  const uint32_t jstr = 8u * rbytes;
+  const int roff = lr * 64 + 16 * (lk ^ ((lr >> 1) & 3));
+  const int ablk = (wave >> 2) * 8, bblk = 16 + (wave & 3) * 4;
+
   i4 acc[8][4];
 #pragma unroll
   for (int a = 0; a < 8; ++a)
@@ -491,14 +500,100 @@ __global__ void __launch_bounds__(512, 1) syrk_pipe8_kernel(const int8_t* __rest
     AJ_PIPE_STEP(fa0, fb0, fa1, fb1, st)
     AJ_PIPE_STEP(fa1, fb1, fa0, fb0, st + 1)
   }
+  // the loads issued past the last stage must land before the workgroup's LDS is released
+  __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
+  store_residues(acc, c, mi, ntiles, t, (wave >> 2) * 8, (wave & 3) * 4, lane, R);
+}
+
+// ---------------------------------------------------------------------------
+// Persistent form of syrk_pipe8_kernel<SPLIT>: one workgroup per CU (blockIdx % 8 selects the XCD),
+// each running a host-built list of segments beg[blockIdx.x] .. beg[blockIdx.x + 1] - 1.  A segment
+// is a run of stages [k0, k0 + ns) of one (tile, modulus) unit; its residues go to the unit's place
+// in R, or, for a piece of a unit split between workgroups, to a partial slot of Rp that
+// fixup_kernel sums.  The host deals every CU the same number of stages (whole units in rounds of
+// consecutive units per XCD, as a dispatch of one workgroup per unit would, then the remainder split
+// into pieces), so that no CU idles through a last, partial round.
+// ---------------------------------------------------------------------------
+struct __attribute__((aligned(16))) SkSeg {
+  int t, mi, k0, ns;  // tile, modulus, first stage, stages (even)
+  int slot;           // -1: whole unit, residues to R; else partial slot in Rp
+  int pad[3];
+};
+
+struct __attribute__((aligned(16))) SkFix {
+  int t, mi, slot0, n;  // unit (tile, modulus) whose residues are the sum of Rp slots slot0 .. slot0 + n - 1
+};
+
+template <int SPLIT>
+__global__ void __launch_bounds__(512, 1) syrk_sk_kernel(const int8_t* __restrict__ X, int64_t plane, int nkb,
+                                                         const int* __restrict__ tiles, const SkSeg* __restrict__ segs,
+                                                         const int* __restrict__ beg, Consts c, int ntiles,
+                                                         int8_t* __restrict__ R, int8_t* __restrict__ Rp) {
+  __shared__ __attribute__((aligned(1024))) int8_t lds[4][32][1024];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int srow = lane >> 2;
+  const int soff = srow * 64 + 16 * ((lane & 3) ^ ((srow >> 1) & 3));
+  const uint32_t rbytes = (uint32_t)nkb * 1024u;  // bytes per row-block
+  const uint32_t jstr = 8u * rbytes;
+  const int roff = lr * 64 + 16 * (lk ^ ((lr >> 1) & 3));
+  const int ablk = (wave >> 2) * 8, bblk = 16 + (wave & 3) * 4;
+  const int s1 = beg[blockIdx.x + 1];
+  for (int s = beg[blockIdx.x]; s < s1; ++s) {
+    const SkSeg g = segs[s];
+    const int I = tiles[2 * g.t], J = tiles[2 * g.t + 1];
+    const i32x4 rsrc = make_rsrc(X + g.mi * plane, (uint32_t)plane);
+    const uint32_t oA = (uint32_t)(I * 16 + wave) * rbytes + (uint32_t)g.k0 * 1024u + soff;
+    const uint32_t oB = (uint32_t)(J * 16 + wave) * rbytes + (uint32_t)g.k0 * 1024u + soff;
+
+    i4 acc[8][4];
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[a][b] = i4{0, 0, 0, 0};
+
+    i4 fa0[8], fb0[4], fa1[8], fb1[4];
+    AJ_PIPE_ISSUE(0)
+    AJ_PIPE_ISSUE(1)
+    AJ_PIPE_ISSUE(2)
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    AJ_PIPE_READ(fa0, fb0, 0)
+    for (int st = 0; st < g.ns; st += 2) {
+      AJ_PIPE_STEP(fa0, fb0, fa1, fb1, st)
+      AJ_PIPE_STEP(fa1, fb1, fa0, fb0, st + 1)
+    }
+    __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
+    int8_t* out = g.slot < 0 ? R + ((int64_t)g.mi * ntiles + g.t) * UNIT_BYTES : Rp + (int64_t)g.slot * UNIT_BYTES;
+    store_residues_to(acc, c, g.mi, out, (wave >> 2) * 8, (wave & 3) * 4, lane);
+    // every wave is past its reads of this segment's stages before the next segment's loads
+    __syncthreads();
+  }
+}
 #undef AJ_PIPE_ISSUE
 #undef AJ_PIPE_READ
 #undef AJ_PIPE_MFMA
 #undef AJ_PIPE_SYNC
 #undef AJ_PIPE_STEP
-  // the loads issued past the last stage must land before the workgroup's LDS is released
-  __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
-  store_residues(acc, c, mi, ntiles, t, (wave >> 2) * 8, (wave & 3) * 4, lane, R);
+
+// Residues of each split unit: the balanced residue of the sum of its pieces' balanced residues (the
+// same value as the residue of the unit's whole int32 Gram block).  Grid (64, split units); a thread
+// is one packed word (4 residues) of the unit.
+__global__ void __launch_bounds__(256) fixup_kernel(const SkFix* __restrict__ fix, const int8_t* __restrict__ Rp,
+                                                    Consts c, int ntiles, int8_t* __restrict__ R) {
+  const SkFix f = fix[blockIdx.y];
+  const int wd = blockIdx.x * 256 + threadIdx.x;
+  int v[4] = {0, 0, 0, 0};
+  for (int p = 0; p < f.n; ++p) {
+    const uint32_t w = reinterpret_cast<const uint32_t*>(Rp + (int64_t)(f.slot0 + p) * UNIT_BYTES)[wd];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) v[r] += (int)(int8_t)(w >> (8 * r));
+  }
+  const double m = c.m[f.mi], inv_m = c.inv_m[f.mi];
+  uint32_t w = 0;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) w |= (uint32_t)(uint8_t)(int8_t)balanced_residue(v[r], f.mi, m, inv_m) << (8 * r);
+  reinterpret_cast<uint32_t*>(R + ((int64_t)f.mi * ntiles + f.t) * UNIT_BYTES)[wd] = w;
 }
 
 // ---------------------------------------------------------------------------
@@ -561,9 +656,123 @@ inline void build_tiles(int64_t nI, std::vector<int>& out) {
         }
 }
 
+// Schedule of syrk_sk_kernel: the ntiles * nm units of `pairs` stage pairs each dealt to G workgroups
+// (G a multiple of 8; workgroup w runs on XCD w % 8).  XCD x owns the tiles [x ntiles / 8,
+// (x + 1) ntiles / 8) and their units in modulus-major order; its C = G / 8 workgroups take them in
+// rounds of C consecutive units (so the units in flight on an XCD share row panels in its L2).  The
+// units left over after the last full round of every XCD are then dealt, as runs of stage pairs in
+// XCD order, so that every workgroup ends at the same stage count (to one pair): a unit covered by
+// one run is written whole, a unit cut between runs is written as partial residues to consecutive
+// slots and summed by fixup_kernel.
+constexpr int SK_WG_MAX = 256;  // workgroups of the persistent SYRK (one per CU on MI355X)
+
+struct SkSched {
+  std::vector<SkSeg> segs;
+  std::vector<int> beg;  // G + 1 offsets into segs
+  std::vector<SkFix> fix;
+  int nslots = 0;
+};
+
+inline void build_sk(int ntiles, int nm, int pairs, int G, SkSched& S) {
+  const int C = G / 8;
+  std::vector<std::vector<SkSeg>> per(G);
+  std::vector<int64_t> base(G, 0);
+  struct Unit {
+    int t, mi;
+  };
+  std::vector<Unit> left;
+  for (int x = 0; x < 8; ++x) {
+    const int t0 = (int)((int64_t)x * ntiles / 8), nx = (int)((int64_t)(x + 1) * ntiles / 8) - t0;
+    const int U = nm * nx, F = U / C;
+    for (int j = 0; j < C; ++j) {
+      const int w = j * 8 + x;
+      for (int r = 0; r < F; ++r) {
+        const int u = r * C + j;
+        per[w].push_back(SkSeg{t0 + u % nx, u / nx, 0, 2 * pairs, -1, {0, 0, 0}});
+      }
+      base[w] = (int64_t)F * pairs;
+    }
+    for (int u = F * C; u < U; ++u) left.push_back(Unit{t0 + u % nx, u / nx});
+  }
+  // leftover pairs to each workgroup: fill every workgroup up to the common level H - 1, then one more
+  // pair to as many workgroups below H as needed
+  const int64_t LP = (int64_t)left.size() * pairs;
+  auto fill = [&](int64_t H) {
+    int64_t f = 0;
+    for (int w = 0; w < G; ++w) f += std::max<int64_t>(0, H - base[w]);
+    return f;
+  };
+  int64_t lo = 0, hi = LP + *std::max_element(base.begin(), base.end());
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) / 2;
+    if (fill(mid) >= LP)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  std::vector<int64_t> d(G);
+  int64_t rem = LP;
+  for (int w = 0; w < G; ++w) {
+    d[w] = std::max<int64_t>(0, lo - 1 - base[w]);
+    rem -= d[w];
+  }
+  for (int x = 0; x < 8; ++x)
+    for (int j = 0; j < C && rem > 0; ++j)
+      if (base[j * 8 + x] < lo) {
+        ++d[j * 8 + x];
+        --rem;
+      }
+  struct Piece {
+    int w;
+    int64_t u;
+    int p0, np;
+  };
+  std::vector<Piece> pcs;
+  int64_t pos = 0;
+  for (int x = 0; x < 8; ++x)
+    for (int j = 0; j < C; ++j) {
+      const int w = j * 8 + x;
+      for (int64_t a = pos, b = pos + d[w]; a < b;) {
+        const int64_t u = a / pairs;
+        const int p0 = (int)(a % pairs), np = (int)std::min<int64_t>(pairs - p0, b - a);
+        pcs.push_back(Piece{w, u, p0, np});
+        a += np;
+      }
+      pos += d[w];
+    }
+  S.fix.clear();
+  S.nslots = 0;
+  for (size_t i = 0; i < pcs.size();) {
+    size_t k = i;
+    while (k < pcs.size() && pcs[k].u == pcs[i].u) ++k;
+    const Unit lu = left[pcs[i].u];
+    if (k - i == 1) {
+      per[pcs[i].w].push_back(SkSeg{lu.t, lu.mi, 0, 2 * pairs, -1, {0, 0, 0}});
+    } else {
+      S.fix.push_back(SkFix{lu.t, lu.mi, S.nslots, (int)(k - i)});
+      for (size_t q = i; q < k; ++q)
+        per[pcs[q].w].push_back(SkSeg{lu.t, lu.mi, 2 * pcs[q].p0, 2 * pcs[q].np, S.nslots++, {0, 0, 0}});
+    }
+    i = k;
+  }
+  S.segs.clear();
+  S.beg.assign(G + 1, 0);
+  for (int w = 0; w < G; ++w) {
+    S.beg[w] = (int)S.segs.size();
+    S.segs.insert(S.segs.end(), per[w].begin(), per[w].end());
+  }
+  S.beg[G] = (int)S.segs.size();
+}
+
+// Capacities of one schedule's tables for at most SK_WG_MAX workgroups: every unit once plus at most
+// one piece per workgroup; leftover units are fewer than 8 per round slot (at most 8 (C - 1)).
+inline int64_t sk_seg_cap(int64_t units) { return units + SK_WG_MAX; }
+inline int64_t sk_slot_cap(int64_t units) { return std::min<int64_t>(units, 8 * (SK_WG_MAX / 8 - 1)) + SK_WG_MAX; }
+
 struct Plan {
   int64_t Mpad, nI, ntiles, per;
   size_t feat_bytes, res_bytes, bytes;
+  int64_t seg_cap, slot_cap;  // persistent SYRK schedule tables (two: full chunks, last chunk)
 };
 
 inline Plan make_plan(int64_t M, int64_t K, int kc, int nm) {
@@ -575,8 +784,14 @@ inline Plan make_plan(int64_t M, int64_t K, int kc, int nm) {
   p.feat_bytes = (size_t)nm * p.Mpad * 2 * kc;  // one chunk
   p.res_bytes = (size_t)nm * p.ntiles * UNIT_BYTES;
   const int64_t nch = std::max<int64_t>(1, ceil_div(K, kc));
+  const int64_t units = (int64_t)nm * p.ntiles;
+  p.seg_cap = sk_seg_cap(units);
+  p.slot_cap = sk_slot_cap(units);
+  const size_t sched_bytes = align_up(sizeof(SkSeg) * p.seg_cap, 256) + align_up(sizeof(int) * (SK_WG_MAX + 1), 256) +
+                             align_up(sizeof(SkFix) * units, 256);
   p.bytes = align_up(p.feat_bytes, 256) + align_up(p.res_bytes, 256) +
-            align_up(sizeof(int) * 2 * p.ntiles, 256) + align_up(sizeof(double) * nch, 256) + 256;
+            align_up(sizeof(int) * 2 * p.ntiles, 256) + align_up(sizeof(double) * nch, 256) + 2 * sched_bytes +
+            align_up((size_t)p.slot_cap * UNIT_BYTES, 256) + 256;
   return p;
 }
 
@@ -584,9 +799,9 @@ inline Plan make_plan(int64_t M, int64_t K, int kc, int nm) {
 
 namespace wt {
 
-// Residue SYRK kernel (0 = syrk_pipe8_kernel<1>, the default; 1 = syrk_pipe8_kernel<0>; 2 =
-// syrk_ring8_kernel) and visibilities per chunk; process-wide tuning.
-int g_crt_variant = 0;
+// Residue SYRK kernel (0 = syrk_pipe8_kernel<1>; 1 = syrk_pipe8_kernel<0>; 2 = syrk_ring8_kernel;
+// 3 = syrk_sk_kernel<1> + fixup_kernel, the default) and visibilities per chunk; process-wide tuning.
+int g_crt_variant = 3;
 int g_crt_kc = crt::KC_DEFAULT;
 
 // Visibilities per chunk for nm moduli: the configured chunk, capped by the modulus product's range.
@@ -610,6 +825,16 @@ int crt_accumulate(const double* noise, const double* uv, const double* grid, co
   int8_t* R = ar.take<int8_t>(p.res_bytes);
   int* tiles = ar.take<int>(2 * p.ntiles);
   double* sc = ar.take<double>(nch);
+  const int64_t units = (int64_t)nm * p.ntiles;
+  SkSeg* dsegs[2];
+  int* dbeg[2];
+  SkFix* dfix[2];
+  for (int i = 0; i < 2; ++i) {
+    dsegs[i] = ar.take<SkSeg>(p.seg_cap);
+    dbeg[i] = ar.take<int>(SK_WG_MAX + 1);
+    dfix[i] = ar.take<SkFix>(units);
+  }
+  int8_t* Rp = ar.take<int8_t>((size_t)p.slot_cap * UNIT_BYTES);
   if (!ar.ok()) {
     set_error("workspace too small: need %zu bytes, got %zu", p.bytes, workspace_bytes);
     return AJ_ESPACE;
@@ -625,6 +850,34 @@ int crt_accumulate(const double* noise, const double* uv, const double* grid, co
   static thread_local std::vector<int> tl;
   build_tiles(p.nI, tl);
   AJ_CHECK_HIP(hipMemcpyAsync(tiles, tl.data(), sizeof(int) * tl.size(), hipMemcpyHostToDevice, st));
+  // persistent SYRK: schedules for a full chunk (i = 0) and for the last chunk (i = 1)
+  int G = 0;
+  SkSched* sk[2] = {nullptr, nullptr};
+  static thread_local SkSched skt[2];
+  if (g_crt_variant == 3) {
+    int dev = 0, cus = 0;
+    AJ_CHECK_HIP(hipGetDevice(&dev));
+    AJ_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    G = std::max(8, std::min(SK_WG_MAX, cus / 8 * 8));
+    for (int i = 0; i < 2; ++i) {
+      const int64_t kv = i == 0 ? std::min<int64_t>(kc, K) : K - (nch - 1) * kc;
+      const int nst = std::min<int>(nkb, (int)(2 * ceil_div(kv, 64)));
+      build_sk((int)p.ntiles, nm, nst / 2, G, skt[i]);
+      if ((int64_t)skt[i].segs.size() > p.seg_cap || (int64_t)skt[i].nslots > p.slot_cap) {
+        set_error("int8crt: SYRK schedule exceeds its tables (%zu segments, %d slots)", skt[i].segs.size(),
+                  skt[i].nslots);
+        return AJ_EINVAL;
+      }
+      AJ_CHECK_HIP(hipMemcpyAsync(dsegs[i], skt[i].segs.data(), sizeof(SkSeg) * skt[i].segs.size(),
+                                  hipMemcpyHostToDevice, st));
+      AJ_CHECK_HIP(hipMemcpyAsync(dbeg[i], skt[i].beg.data(), sizeof(int) * skt[i].beg.size(), hipMemcpyHostToDevice,
+                                  st));
+      if (!skt[i].fix.empty())
+        AJ_CHECK_HIP(hipMemcpyAsync(dfix[i], skt[i].fix.data(), sizeof(SkFix) * skt[i].fix.size(),
+                                    hipMemcpyHostToDevice, st));
+      sk[i] = &skt[i];
+    }
+  }
   hipLaunchKernelGGL(scale_kernel, dim3((unsigned)nch), dim3(256), 0, st, noise, K, kc, sc);
   AJ_LAUNCH_CHECK();
   const int64_t plane = p.Mpad * 2 * (int64_t)kc;
@@ -651,6 +904,17 @@ int crt_accumulate(const double* noise, const double* uv, const double* grid, co
     switch (g_crt_variant) {
       case 1: hipLaunchKernelGGL(syrk_pipe8_kernel<0>, sg, dim3(512), 0, st, X, plane, nkb, nst, nt, per, tiles, cst, R); break;
       case 2: hipLaunchKernelGGL(syrk_ring8_kernel, sg, dim3(512), 0, st, X, plane, nkb, nst, nt, per, tiles, cst, R); break;
+      case 3: {
+        const int i = ch + 1 < nch ? 0 : 1;
+        hipLaunchKernelGGL(syrk_sk_kernel<1>, dim3((unsigned)G), dim3(512), 0, st, X, plane, nkb, tiles, dsegs[i], dbeg[i],
+                           cst, nt, R, Rp);
+        if (!sk[i]->fix.empty()) {
+          AJ_LAUNCH_CHECK();
+          hipLaunchKernelGGL(fixup_kernel, dim3(UNIT_BYTES / 4 / 256, (unsigned)sk[i]->fix.size()), dim3(256), 0, st,
+                             dfix[i], Rp, cst, nt, R);
+        }
+        break;
+      }
       default: hipLaunchKernelGGL(syrk_pipe8_kernel<1>, sg, dim3(512), 0, st, X, plane, nkb, nst, nt, per, tiles, cst, R); break;
     }
     timer_end(AJ_TIMER_WT_SYRK, st);
@@ -669,11 +933,11 @@ int crt_accumulate(const double* noise, const double* uv, const double* grid, co
 }  // namespace wt
 }  // namespace aj
 
-// Residue SYRK kernel variant (0..2, -1 = default 0) and visibilities per chunk (a multiple of 4096 up to
+// Residue SYRK kernel variant (0..3, -1 = default 3) and visibilities per chunk (a multiple of 4096 up to
 // 32768, 0 = default 16384); tuning, no reference counterpart.
 extern "C" int aj_w_tilde_crt_configure(int variant, int64_t chunk_visibilities) {
-  if (variant < -1 || variant > 2) {
-    aj::set_error("residue SYRK variant %d outside [-1, 2]", variant);
+  if (variant < -1 || variant > 3) {
+    aj::set_error("residue SYRK variant %d outside [-1, 3]", variant);
     return AJ_EINVAL;
   }
   if (chunk_visibilities != 0 &&
@@ -682,7 +946,7 @@ extern "C" int aj_w_tilde_crt_configure(int variant, int64_t chunk_visibilities)
                   aj::crt::KC_MAX);
     return AJ_EINVAL;
   }
-  aj::wt::g_crt_variant = variant < 0 ? 0 : variant;
+  aj::wt::g_crt_variant = variant < 0 ? 3 : variant;
   aj::wt::g_crt_kc = chunk_visibilities ? (int)chunk_visibilities : aj::crt::KC_DEFAULT;
   return AJ_OK;
 }

@@ -112,7 +112,7 @@ def test_w_tilde_crt_variants_agree(gpu):
     args = (d.noise_map_real, d.uv_wavelengths, d.grid_radians_slim)
     out = []
     try:
-        for v in (0, 1, 2):
+        for v in (0, 1, 2, 3, 3):
             _lib.call("aj_w_tilde_crt_configure", v, 16384)
             out.append(api.w_tilde_curvature_interferometer_from(*args, precision="int8crt"))
         for kc in (4096, 32768):
@@ -124,6 +124,32 @@ def test_w_tilde_crt_variants_agree(gpu):
         _lib.call("aj_w_tilde_crt_configure", -1, 0)
     for W in out[1:]:
         np.testing.assert_array_equal(W, out[0])
+
+
+def test_w_tilde_crt_persistent_schedule(gpu):
+    """The persistent residue SYRK (variant 3: every CU runs whole units in rounds, then pieces of the
+    units left over, summed by a fix-up) equals the one-workgroup-per-unit kernel bitwise, at a size
+    with full rounds and split units (M = 2500: 55 tiles, 605 units of 11 moduli) and a ragged last
+    chunk, in both residue modes, on two calls in a row."""
+    from autojax_amd import _lib, api
+
+    rng = np.random.default_rng(11)
+    grid, _ = ds.grid_for_pixel_count(2500, 0.05)
+    K = 20000
+    uv = ds.uv_loguniform(rng, K, 1e3, 5e5)
+    _, noise = ds.visibilities(rng, K, 0.5, 2.0)
+    args = (np.ascontiguousarray(noise.real), uv, grid)
+    try:
+        for precision in ("int8crt11", "int8crt"):
+            _lib.call("aj_w_tilde_crt_configure", 0, 16384)
+            ref = api.w_tilde_curvature_interferometer_from(*args, precision=precision)
+            _lib.call("aj_w_tilde_crt_configure", 3, 16384)
+            for _ in range(2):
+                W = api.w_tilde_curvature_interferometer_from(*args, precision=precision)
+                print(f"{precision}: max |dW| {np.abs(W - ref).max():.3e}", flush=True)
+                np.testing.assert_array_equal(W, ref)
+    finally:
+        _lib.call("aj_w_tilde_crt_configure", -1, 0)
 
 
 def test_w_tilde_run_to_run_bitwise(gpu, cfg1_small):
