@@ -106,10 +106,23 @@ int aj_w_tilde_via_compact_from(const double* w_compact, int64_t grid_size,
                                 const int64_t* native_index_for_slim_index, int64_t M, double* w_tilde, void* stream);
 
 /* numba.py:259-291 w_tilde_via_preload_from(w_tilde_preload f8[Py,Px], native_index_for_slim_index i8[M,2])
- * -> f8[M,M], W[i, j] = preload[y_j - y_i, x_j - x_i] with negative offsets wrapping as numpy indexes
- * (the caller guarantees |offset| < Py, Px). */
+ * -> f8[M,M], W[i, j] = preload[y_j - y_i, x_j - x_i] for j >= i with negative offsets wrapping as numpy
+ * indexes, W[j, i] = W[i, j] (the caller guarantees |offset| < Py, Px). */
 int aj_w_tilde_via_preload_from(const double* w_tilde_preload, int64_t preload_rows, int64_t preload_cols,
                                 const int64_t* native_index_for_slim_index, int64_t M, double* w_tilde, void* stream);
+
+/* original.py:230-403 w_tilde_curvature_preload_interferometer_from(noise_map_real f8[K],
+ * uv_wavelengths f8[K,2], shape_masked_pixels_2d (Y, X), grid_radians_2d f8[Gy,Gx,2]) -> f8[2Y,2X]: for
+ * the Y x X corner of the grid, entry [i, j] = sum_k n_k^-2 cos(2 pi (dx u_k0 + dy u_k1)) with (dy, dx)
+ * = g[0,0] - g[i,j]; [i, -j], [-i, j], [-i, -j] (numpy indexes) take the offsets from the corners
+ * g[0,X-1], g[Y-1,0], g[Y-1,X-1] to g[i,X-1-j], g[Y-1-i,j], g[Y-1-i,X-1-j]; row Y and column X are 0.
+ * Y <= Gy, X <= Gx.  Workspace from aj_w_tilde_preload_workspace_bytes. */
+size_t aj_w_tilde_preload_workspace_bytes(int64_t K, int64_t y_shape, int64_t x_shape);
+int aj_w_tilde_curvature_preload_interferometer_from(const double* noise_map_real, const double* uv_wavelengths,
+                                                     int64_t K, int64_t y_shape, int64_t x_shape,
+                                                     const double* grid_radians_2d, int64_t grid_rows,
+                                                     int64_t grid_cols, double* curvature_preload, void* workspace,
+                                                     size_t workspace_bytes, void* stream);
 
 /* numba.py:95-152 w_tilde_data_interferometer_from(visibilities_real f8[K],
  * noise_map_real f8[K], uv_wavelengths f8[K,2], grid_radians_slim f8[M,2]) -> f8[M].
@@ -168,6 +181,17 @@ int aj_curvature_matrix_via_w_compact_sparse_mapping_matrix_from(
     const double* w_compact, int64_t grid_size, const int64_t* native_index_for_slim_index,
     const int64_t* pix_indexes_for_sub_slim_index, const double* pix_weights_for_sub_slim_index, int64_t M, int64_t B,
     int64_t pixels, double* curvature_matrix, void* workspace, size_t workspace_bytes, void* stream);
+
+/* original.py:600-700 curvature_matrix_via_w_tilde_curvature_preload_interferometer_from(curvature_preload
+ * f8[Py,Px], pix_indexes i8[M,B], pix_size i8[M], pix_weights f8[M,B], native_index_for_slim_index i8[M,2],
+ * pix_pixels) -> f8[pixels,pixels] = T^T W T with W read from the preload table as
+ * aj_w_tilde_via_preload_from reads it.  The mapping is in this library's format (-1 ends a row; the
+ * Python layer turns pix_size into it).  Uses aj_curvature_sparse_workspace_bytes. */
+int aj_curvature_matrix_via_w_tilde_curvature_preload_interferometer_from(
+    const double* curvature_preload, int64_t preload_rows, int64_t preload_cols,
+    const int64_t* native_index_for_slim_index, const int64_t* pix_indexes_for_sub_slim_index,
+    const double* pix_weights_for_sub_slim_index, int64_t M, int64_t B, int64_t pixels, double* curvature_matrix,
+    void* workspace, size_t workspace_bytes, void* stream);
 
 /* numba.py:656-695 constant_regularization_matrix_from(coefficient, neighbors i8[S,P],
  * neighbors_sizes i8[S]) -> f8[S,S]. */

@@ -1,7 +1,8 @@
 """FP64 numpy restatement of the reference's likelihood path (test oracle only).
 
 Each function keeps the name, parameter names and parameter order of the
-function in ``src/autojax/numba.py`` it restates, and cites the lines.  The
+function in ``src/autojax/numba.py`` (for the preload form, ``original.py``) it
+restates, and cites the lines.  The
 arithmetic is plain FP64 numpy/LAPACK; loops that the reference writes as
 explicit loops are vectorised where that cannot change the value beyond FP64
 rounding order.
@@ -125,10 +126,85 @@ def curvature_matrix_via_w_compact_sparse_mapping_matrix_from(w_compact, native_
 
 
 def w_tilde_via_preload_from(w_tilde_preload, native_index_for_slim_index):
-    """numba.py:259-291: W[i, j] = preload[y_j - y_i, x_j - x_i] (numpy negative indexes wrap)."""
+    """numba.py:259-291: W[i, j] = preload[y_j - y_i, x_j - x_i] for j >= i (numpy negative indexes
+    wrap), then W[j, i] = W[i, j]."""
     y = native_index_for_slim_index[:, 0]
     x = native_index_for_slim_index[:, 1]
-    return w_tilde_preload[y[None, :] - y[:, None], x[None, :] - x[:, None]]
+    T = w_tilde_preload[y[None, :] - y[:, None], x[None, :] - x[:, None]]
+    return np.triu(T) + np.triu(T, 1).T
+
+
+def _preload_quadrant(noise_map_real, uv_wavelengths, dy, dx):
+    """sum_k n_k^-2 cos(2 pi (dx u_k0 + dy u_k1)) for offset arrays dy, dx (original.py:356-361)."""
+    w = noise_map_real**-2.0
+    ph = TWO_PI * (dx.reshape(-1, 1) * uv_wavelengths[None, :, 0] + dy.reshape(-1, 1) * uv_wavelengths[None, :, 1])
+    return (np.cos(ph) @ w).reshape(dy.shape)
+
+
+def w_tilde_curvature_preload_interferometer_from(noise_map_real, uv_wavelengths, shape_masked_pixels_2d,
+                                                  grid_radians_2d):
+    """original.py:230-403: the 2Y x 2X preload table over the Y x X corner of the grid; each of the four
+    loops of the reference is one quadrant, with offsets from its own corner (row Y and column X stay 0)."""
+    Y, X = shape_masked_pixels_2d
+    g = grid_radians_2d[0:Y, 0:X]
+    out = np.zeros((2 * Y, 2 * X))
+    # original.py:355-361: [i, j], offsets from g[0, 0]
+    out[:Y, :X] = _preload_quadrant(noise_map_real, uv_wavelengths, g[0, 0, 0] - g[:, :, 0], g[0, 0, 1] - g[:, :, 1])
+    j = np.arange(1, X)
+    i = np.arange(1, Y)
+    # original.py:363-374: [i, -j], j >= 1, offsets from g[0, -1] to g[i, X - j - 1]
+    if X > 1:
+        gp = g[:, X - j - 1]
+        out[:Y, 2 * X - j] = _preload_quadrant(noise_map_real, uv_wavelengths, g[0, -1, 0] - gp[:, :, 0],
+                                               g[0, -1, 1] - gp[:, :, 1])
+    # original.py:376-387: [-i, j], i >= 1, offsets from g[-1, 0] to g[Y - i - 1, j]
+    if Y > 1:
+        gp = g[Y - i - 1, :]
+        out[2 * Y - i, :X] = _preload_quadrant(noise_map_real, uv_wavelengths, g[-1, 0, 0] - gp[:, :, 0],
+                                               g[-1, 0, 1] - gp[:, :, 1])
+    # original.py:389-400: [-i, -j], i, j >= 1, offsets from g[-1, -1] to g[Y - i - 1, X - j - 1]
+    if X > 1 and Y > 1:
+        gp = g[np.ix_(Y - i - 1, X - j - 1)]
+        out[np.ix_(2 * Y - i, 2 * X - j)] = _preload_quadrant(noise_map_real, uv_wavelengths,
+                                                              g[-1, -1, 0] - gp[:, :, 0], g[-1, -1, 1] - gp[:, :, 1])
+    return out
+
+
+def curvature_matrix_via_w_tilde_curvature_preload_interferometer_from(curvature_preload,
+                                                                       pix_indexes_for_sub_slim_index,
+                                                                       pix_size_for_sub_slim_index,
+                                                                       pix_weights_for_sub_slim_index,
+                                                                       native_index_for_slim_index, pix_pixels):
+    """original.py:600-700, loop for loop: pixel pairs ip0 < ip1 read the table at the offset of ip1
+    from ip0, the result is added to its transpose, then each pixel's own pairs add preload[0, 0]."""
+    idx, size, wts, nat = (pix_indexes_for_sub_slim_index, pix_size_for_sub_slim_index,
+                           pix_weights_for_sub_slim_index, native_index_for_slim_index)
+    M = nat.shape[0]
+    F = np.zeros((pix_pixels, pix_pixels))
+    for ip0 in range(M):
+        y0, x0 = nat[ip0]
+        # all ip1 > ip0 at once: the table value of each pair, then each (ip0_pix, ip1_pix) mapping pair
+        ip1 = np.arange(ip0 + 1, M)
+        if ip1.size == 0:
+            continue
+        val = curvature_preload[nat[ip1, 0] - y0, nat[ip1, 1] - x0]
+        for a in range(size[ip0]):
+            for b in range(idx.shape[1]):
+                live = b < size[ip1]
+                np.add.at(F[idx[ip0, a]], idx[ip1[live], b], val[live] * wts[ip0, a] * wts[ip1[live], b])
+    F = F + F.T
+    pre = curvature_preload[0, 0]
+    for ip0 in range(M):
+        for a in range(size[ip0]):
+            for b in range(size[ip0]):
+                s0, s1 = idx[ip0, a], idx[ip0, b]
+                v = pre * wts[ip0, a] * wts[ip0, b]
+                if s0 > s1:
+                    F[s0, s1] += v
+                    F[s1, s0] += v
+                elif s0 == s1:
+                    F[s0, s1] += v
+    return F
 
 
 def curvature_matrix_via_w_compact_sparse_mapping_matrix_direct_from(w_compact, native_index_for_slim_index,

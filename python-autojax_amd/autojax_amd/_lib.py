@@ -41,6 +41,8 @@ SIGNATURES = {
     "aj_w_compact_curvature_interferometer_from": (_INT, [_P, _P, _I, _I, _D, _P, _P, _S, _P]),
     "aj_w_tilde_via_compact_from": (_INT, [_P, _I, _P, _I, _P, _P]),
     "aj_w_tilde_via_preload_from": (_INT, [_P, _I, _I, _P, _I, _P, _P]),
+    "aj_w_tilde_preload_workspace_bytes": (_S, [_I, _I, _I]),
+    "aj_w_tilde_curvature_preload_interferometer_from": (_INT, [_P, _P, _I, _I, _I, _P, _I, _I, _P, _P, _S, _P]),
     "aj_visibility_sum_workspace_bytes": (_S, [_I, _I]),
     "aj_w_tilde_data_interferometer_from": (_INT, [_P, _P, _P, _P, _I, _I, _P, _P, _S, _P]),
     "aj_dirty_image_from": (_INT, [_P, _P, _P, _P, _I, _I, _P, _P, _S, _P]),
@@ -54,6 +56,8 @@ SIGNATURES = {
     "aj_curvature_matrix_via_w_wilde_sparse_mapping_matrix_from": (_INT, [_P, _P, _P, _I, _I, _I, _P, _P, _S, _P]),
     "aj_curvature_matrix_via_w_compact_sparse_mapping_matrix_from": (
         _INT, [_P, _I, _P, _P, _P, _I, _I, _I, _P, _P, _S, _P]),
+    "aj_curvature_matrix_via_w_tilde_curvature_preload_interferometer_from": (
+        _INT, [_P, _I, _I, _P, _P, _P, _I, _I, _I, _P, _P, _S, _P]),
     "aj_constant_regularization_matrix_from": (_INT, [_D, _P, _P, _I, _I, _P, _P]),
     "aj_solve_workspace_bytes": (_S, [_I]),
     "aj_reconstruction_positive_negative_from": (_INT, [_P, _P, _I, _P, _P, _P, _P, _S, _P]),

@@ -211,6 +211,29 @@ def w_tilde_via_preload_from(w_tilde_preload, native_index_for_slim_index):
     return _out(out, torch_in)
 
 
+def w_tilde_curvature_preload_interferometer_from(noise_map_real, uv_wavelengths, shape_masked_pixels_2d,
+                                                  grid_radians_2d):
+    """original.py:230-403 -> f8[2Y, 2X]: the preload table of W-tilde values by pixel offset over the
+    (Y, X) = shape_masked_pixels_2d corner of grid_radians_2d f8[Gy, Gx, 2] ((y, x) radians), one FP64
+    sum over the visibilities per entry; read back by w_tilde_via_preload_from and
+    curvature_matrix_via_w_tilde_curvature_preload_interferometer_from."""
+    torch_in = _any_t(noise_map_real, uv_wavelengths, grid_radians_2d)
+    n = _dev(noise_map_real, np.float64, "noise_map_real", 1)
+    uv = _dev(uv_wavelengths, np.float64, "uv_wavelengths", 2)
+    g = _dev(grid_radians_2d, np.float64, "grid_radians_2d", 3)
+    K = n.shape[0]
+    Y, X = (int(v) for v in shape_masked_pixels_2d)
+    if uv.shape != (K, 2) or g.shape[2] != 2:
+        raise ValueError("shape mismatch: noise_map_real [K], uv_wavelengths [K,2], grid_radians_2d [Gy,Gx,2]")
+    if not (0 < Y <= g.shape[0] and 0 < X <= g.shape[1]):
+        raise ValueError(f"shape_masked_pixels_2d {(Y, X)} outside the {tuple(g.shape[:2])} grid")
+    out = dv.empty((2 * Y, 2 * X))
+    ws = dv.workspace(_lib.query("aj_w_tilde_preload_workspace_bytes", K, Y, X))
+    _lib.call("aj_w_tilde_curvature_preload_interferometer_from", n.data_ptr(), uv.data_ptr(), K, Y, X, g.data_ptr(),
+              g.shape[0], g.shape[1], out.data_ptr(), ws.data_ptr(), ws.numel(), dv.stream_ptr())
+    return _out(out, torch_in)
+
+
 def w_tilde_data_interferometer_from(visibilities_real, noise_map_real, uv_wavelengths, grid_radians_slim):
     """numba.py:95-152 -> f8[M]."""
     torch_in = _any_t(visibilities_real, noise_map_real, uv_wavelengths, grid_radians_slim)
@@ -346,6 +369,50 @@ def curvature_matrix_via_w_compact_sparse_mapping_matrix_from(w_compact, native_
     ws = dv.workspace(_lib.query("aj_curvature_sparse_workspace_bytes", M, B, int(pixels)))
     _lib.call("aj_curvature_matrix_via_w_compact_sparse_mapping_matrix_from", C.data_ptr(), N, ni.data_ptr(),
               idx.data_ptr(), w.data_ptr(), M, B, int(pixels), out.data_ptr(), ws.data_ptr(), ws.numel(),
+              dv.stream_ptr())
+    return _out(out, torch_in)
+
+
+def curvature_matrix_via_w_tilde_curvature_preload_interferometer_from(curvature_preload,
+                                                                       pix_indexes_for_sub_slim_index,
+                                                                       pix_size_for_sub_slim_index,
+                                                                       pix_weights_for_sub_slim_index,
+                                                                       native_index_for_slim_index, pix_pixels):
+    """original.py:600-700 -> f8[pix_pixels, pix_pixels] = T^T W T with W read from the preload table
+    as w_tilde_via_preload_from reads it (never expanded to M x M).  Mapping entries at or past
+    pix_size_for_sub_slim_index[m] are not used, as in the reference's loops."""
+    torch_in = _any_t(curvature_preload, pix_indexes_for_sub_slim_index, pix_weights_for_sub_slim_index,
+                      native_index_for_slim_index)
+    idx_h = _host(pix_indexes_for_sub_slim_index)
+    size_h = _host(pix_size_for_sub_slim_index)
+    for a, name in ((idx_h, "pix_indexes_for_sub_slim_index"), (size_h, "pix_size_for_sub_slim_index")):
+        if a.dtype != np.int64:
+            raise TypeError(f"{name}: expected int64, got {a.dtype}")
+    idx_h = np.array(idx_h, copy=True)
+    if idx_h.ndim != 2 or size_h.shape != (idx_h.shape[0],):
+        raise ValueError("pix_indexes_for_sub_slim_index [M, B] and pix_size_for_sub_slim_index [M] disagree")
+    if size_h.size and (size_h.min() < 0 or size_h.max() > idx_h.shape[1]):
+        raise ValueError("pix_size_for_sub_slim_index outside [0, B]")
+    idx_h[np.arange(idx_h.shape[1])[None, :] >= size_h[:, None]] = -1
+    validate_mapping(idx_h, pix_pixels)
+    Pm = _dev(curvature_preload, np.float64, "curvature_preload", 2)
+    Py, Px = Pm.shape
+    ni_h = _host(native_index_for_slim_index)
+    if ni_h.ndim != 2 or ni_h.shape != (idx_h.shape[0], 2):
+        raise ValueError("native_index_for_slim_index must be [M, 2]")
+    if ni_h.size and (np.ptp(ni_h[:, 0]) >= Py or np.ptp(ni_h[:, 1]) >= Px):
+        raise IndexError("pixel offsets exceed the preload table")
+    ni = _dev(native_index_for_slim_index, np.int64, "native_index_for_slim_index", 2)
+    idx = _dev(idx_h, np.int64, "pix_indexes_for_sub_slim_index", 2)
+    w = _dev(pix_weights_for_sub_slim_index, np.float64, "pix_weights_for_sub_slim_index", 2)
+    M, B = idx.shape
+    if w.shape != (M, B):
+        raise ValueError("pix_weights_for_sub_slim_index must match pix_indexes_for_sub_slim_index")
+    S = int(pix_pixels)
+    out = dv.empty((S, S))
+    ws = dv.workspace(_lib.query("aj_curvature_sparse_workspace_bytes", M, B, S))
+    _lib.call("aj_curvature_matrix_via_w_tilde_curvature_preload_interferometer_from", Pm.data_ptr(), Py, Px,
+              ni.data_ptr(), idx.data_ptr(), w.data_ptr(), M, B, S, out.data_ptr(), ws.data_ptr(), ws.numel(),
               dv.stream_ptr())
     return _out(out, torch_in)
 

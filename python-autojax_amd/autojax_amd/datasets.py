@@ -207,6 +207,7 @@ class Dataset:
     noise_map: np.ndarray
     samples: list = field(default_factory=list)
     dirty_image: np.ndarray | None = None
+    grid_radians_2d: np.ndarray | None = None  # [N, N, 2] (y, x) radians, when the file convention gives one
 
     @property
     def M(self):
@@ -287,4 +288,93 @@ def small_delaunay(seed, n_pix, K, S, n_samples=1, lam_range=None, uv_max=5e5):
         idx, w = barycentric_mapping(rng, grid.shape[0], mesh.triangles)
         lam = 1.0 if lam_range is None else float(np.exp(rng.uniform(np.log(lam_range[0]), np.log(lam_range[1]))))
         ds.samples.append(Sample(idx, w, mesh.neighbors, mesh.neighbors_sizes, lam, mesh))
+    return ds
+
+
+# ---------------------------------------------------------------------------
+# The reference's dataset file (tests.py:396-452, DataLoaded): data.npz with the arrays below; the
+# grid, mask and pixel-to-grid indexes are not stored but fixed by tests.py:155-306.
+# ---------------------------------------------------------------------------
+NPZ_KEYS = ("dirty_image", "data", "noise_map", "uv_wavelengths", "neighbors", "neighbors_sizes",
+            "pix_indexes_for_sub_slim_index", "pix_weights_for_sub_slim_index")
+NPZ_GRID_SIZE = 30  # tests.py:26 (AUTOJAX_GRID_SIZE default)
+NPZ_PIXEL_SCALE = 0.2  # tests.py:158
+NPZ_CORNER_ARCSEC = 9.9  # tests.py:296, the dataset's first grid coordinate
+
+
+def reference_grid_radians_2d(n_side, pixel_scale, corner_arcsec=NPZ_CORNER_ARCSEC):
+    """tests.py:291-302: g[I, J] = (-d I + c, d J - c) in radians, d = pixel_scale arcsec, c = corner."""
+    d = pixel_scale * ARCSEC
+    c = corner_arcsec * ARCSEC
+    I, J = np.mgrid[:n_side, :n_side]
+    g = np.empty((n_side, n_side, 2))
+    g[:, :, 0] = -d * I + c
+    g[:, :, 1] = d * J - c
+    return g
+
+
+def reference_mask(n_side, pixel_scale, radius, centre=(0.0, 0.0)):
+    """original.py:8-95 mask_2d_circular_from on an n x n grid (True = masked): unmasked where the
+    distance of the pixel centre from `centre`, sqrt(x^2 + y^2), is at most radius."""
+    cy = (n_side - 1) / 2.0 - centre[0] / pixel_scale
+    cx = (n_side - 1) / 2.0 + centre[1] / pixel_scale
+    ys, xs = np.indices((n_side, n_side))
+    r = np.sqrt(((xs - cx) * pixel_scale) ** 2 + ((ys - cy) * pixel_scale) ** 2)
+    return ~(r <= radius)
+
+
+def load_npz(path, grid_size=NPZ_GRID_SIZE, pixel_scale=NPZ_PIXEL_SCALE, coefficient=1.0):
+    """A Dataset from a file holding the reference's data.npz arrays (NPZ_KEYS), with the grid the
+    reference's harness builds for it: grid_size x grid_size at pixel_scale, circular mask of radius
+    ((grid_size + 1) // 2) * pixel_scale (tests.py:210-212), slim order argwhere(~mask).  Loaded with
+    allow_pickle=False.  Raises ValueError when a key is missing or the mapping's row count differs
+    from the mask's pixel count (another grid_size / pixel_scale)."""
+    with np.load(path, allow_pickle=False) as z:
+        missing = [k for k in NPZ_KEYS if k not in z.files]
+        if missing:
+            raise ValueError(f"{path}: missing arrays {missing}")
+        a = {k: np.ascontiguousarray(z[k]) for k in NPZ_KEYS}
+    radius = ((grid_size + 1) // 2) * pixel_scale
+    mask = reference_mask(grid_size, pixel_scale, radius)
+    g2 = reference_grid_radians_2d(grid_size, pixel_scale)
+    native = np.ascontiguousarray(np.argwhere(~mask)).astype(np.int64)
+    grid = np.ascontiguousarray(g2[~mask])
+    M = a["pix_indexes_for_sub_slim_index"].shape[0]
+    if grid.shape[0] != M:
+        raise ValueError(f"{path}: mapping has {M} image pixels, the {grid_size}x{grid_size} mask at "
+                         f"{pixel_scale}\" has {grid.shape[0]}")
+    sample = Sample(a["pix_indexes_for_sub_slim_index"].astype(np.int64),
+                    a["pix_weights_for_sub_slim_index"].astype(np.float64), a["neighbors"].astype(np.int64),
+                    a["neighbors_sizes"].astype(np.int64), float(coefficient))
+    ds = Dataset("npz", -1, grid, native, pixel_scale, grid_size, a["uv_wavelengths"].astype(np.float64),
+                 a["data"].astype(np.complex128), a["noise_map"].astype(np.complex128), [sample],
+                 a["dirty_image"].astype(np.float64), g2)
+    return ds
+
+
+def save_npz(ds, path, sample=0):
+    """Write a Dataset (its `sample`-th lens-model sample) with the reference's data.npz keys."""
+    s = ds.samples[sample]
+    np.savez(path, dirty_image=ds.ensure_dirty_image(), data=ds.data, noise_map=ds.noise_map,
+             uv_wavelengths=ds.uv_wavelengths, neighbors=s.neighbors, neighbors_sizes=s.neighbors_sizes,
+             pix_indexes_for_sub_slim_index=s.pix_indexes_for_sub_slim_index,
+             pix_weights_for_sub_slim_index=s.pix_weights_for_sub_slim_index)
+
+
+def reference_convention_dataset(seed=0, K=1024, S=64, grid_size=NPZ_GRID_SIZE, pixel_scale=NPZ_PIXEL_SCALE):
+    """A small dataset on the reference harness's grid (reference_grid_radians_2d + reference_mask):
+    uv uniform in a 1e5-lambda disc, N(0,1) data, sigma ~ U(0.5, 1.5), a Delaunay source of S vertices
+    spanning the grid with barycentric mapping (B = 3)."""
+    rng = np.random.default_rng(seed)
+    radius = ((grid_size + 1) // 2) * pixel_scale
+    mask = reference_mask(grid_size, pixel_scale, radius)
+    g2 = reference_grid_radians_2d(grid_size, pixel_scale)
+    native = np.ascontiguousarray(np.argwhere(~mask)).astype(np.int64)
+    grid = np.ascontiguousarray(g2[~mask])
+    uv = uv_disc_uniform(rng, K, 1e5)
+    data, noise = visibilities(rng, K, 0.5, 1.5)
+    mesh = delaunay_source(rng, S, radius_arcsec=radius)
+    idx, w = barycentric_mapping(rng, grid.shape[0], mesh.triangles)
+    ds = Dataset("refgrid", seed, grid, native, pixel_scale, grid_size, uv, data, noise,
+                 [Sample(idx, w, mesh.neighbors, mesh.neighbors_sizes, 1.0, mesh)], None, g2)
     return ds

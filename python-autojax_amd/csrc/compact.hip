@@ -124,18 +124,116 @@ __global__ void compact_expand_kernel(const double* __restrict__ C, int64_t N, c
   }
 }
 
-// W[i, j] = preload[y_j - y_i, x_j - x_i] with negative offsets wrapping (numba.py:284-290).
+// W[i, j] = preload[y_j - y_i, x_j - x_i] for j >= i, negative offsets wrapping as numpy indexes, and
+// W[j, i] = W[i, j] (numba.py:280-290: the upper triangle from the table, then mirrored).
 __global__ void preload_expand_kernel(const double* __restrict__ P, int64_t Py, int64_t Px,
                                       const int64_t* __restrict__ native, int64_t M, double* __restrict__ W) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= M) return;
   const int64_t yj = native[2 * j], xj = native[2 * j + 1];
   for (int64_t i = blockIdx.y; i < M; i += gridDim.y) {
-    int64_t dy = yj - native[2 * i], dx = xj - native[2 * i + 1];
+    const int64_t yi = native[2 * i], xi = native[2 * i + 1];
+    int64_t dy = i <= j ? yj - yi : yi - yj, dx = i <= j ? xj - xi : xi - xj;
     dy += dy < 0 ? Py : 0;
     dx += dx < 0 ? Px : 0;
     W[i * M + j] = P[dy * Px + dx];
   }
+}
+
+// ---------------------------------------------------------------------------
+// Preload table of original.py:230-403 (w_tilde_curvature_preload_interferometer_from) on the Y x X
+// corner of the grid g (f8[Gy, Gx, 2], (y, x) in radians): a 2Y x 2X table whose four quadrants hold
+//   [i, j]          offset g[0, 0]         - g[i, j]                    (i < Y, j < X)
+//   [i, 2X - j]     offset g[0, X-1]       - g[i, X-1-j]                (j >= 1)
+//   [2Y - i, j]     offset g[Y-1, 0]       - g[Y-1-i, j]                (i >= 1)
+//   [2Y - i, 2X - j] offset g[Y-1, X-1]    - g[Y-1-i, X-1-j]            (i, j >= 1)
+// each sum_k n_k^-2 cos(2 pi (dx u_k0 + dy u_k1)); row Y and column X stay zero.  Thread = one table
+// entry; the workgroup stages PRE_KV visibilities at a time in LDS; blockIdx.y is a slice of the
+// visibilities whose partial table compact_reduce_kernel sums in slice order.
+// ---------------------------------------------------------------------------
+constexpr int PRE_KV = 256;
+constexpr int PRE_TARGET_BLOCKS = 2048;
+
+__device__ inline bool preload_offset(const double* __restrict__ g, int64_t Gx, int64_t Y, int64_t X, int64_t r,
+                                      int64_t c, double& dy, double& dx) {
+  // corner (cy, cx) and pixel (py, px) of the entry's quadrant; false for row Y / column X
+  int64_t cy, cx, py, px;
+  if (r < Y) {
+    cy = 0;
+    py = r;
+  } else if (r > Y) {
+    const int64_t i = 2 * Y - r;
+    cy = Y - 1;
+    py = Y - 1 - i;
+  } else {
+    return false;
+  }
+  if (c < X) {
+    cx = 0;
+    px = c;
+  } else if (c > X) {
+    const int64_t j = 2 * X - c;
+    cx = X - 1;
+    px = X - 1 - j;
+  } else {
+    return false;
+  }
+  dy = g[2 * (cy * Gx + cx)] - g[2 * (py * Gx + px)];
+  dx = g[2 * (cy * Gx + cx) + 1] - g[2 * (py * Gx + px) + 1];
+  return true;
+}
+
+__global__ void __launch_bounds__(256) preload_build_kernel(const double* __restrict__ noise,
+                                                            const double* __restrict__ uv, int64_t K,
+                                                            const double* __restrict__ g, int64_t Gx, int64_t Y,
+                                                            int64_t X, int64_t k_per_split,
+                                                            double* __restrict__ partial) {
+  __shared__ double sw[PRE_KV], su0[PRE_KV], su1[PRE_KV];
+  const int64_t E = 4 * Y * X;
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t kb = (int64_t)blockIdx.y * k_per_split, ke = std::min<int64_t>(K, kb + k_per_split);
+  double dy = 0.0, dx = 0.0;
+  const bool live = e < E && preload_offset(g, Gx, Y, X, e / (2 * X), e % (2 * X), dy, dx);
+  double acc = 0.0;
+  for (int64_t k0 = kb; k0 < ke; k0 += PRE_KV) {
+    const int n = (int)std::min<int64_t>(PRE_KV, ke - k0);
+    if (threadIdx.x < n) {
+      const double s = noise[k0 + threadIdx.x];
+      sw[threadIdx.x] = 1.0 / (s * s);
+      su0[threadIdx.x] = uv[2 * (k0 + threadIdx.x)];
+      su1[threadIdx.x] = uv[2 * (k0 + threadIdx.x) + 1];
+    }
+    __syncthreads();
+    if (live) {
+      for (int k = 0; k < n; ++k) {
+        double rv = fma(dx, su0[k], dy * su1[k]);
+        rv -= rint(rv);
+        acc = fma(sw[k], cospi(2.0 * rv), acc);
+      }
+    }
+    __syncthreads();
+  }
+  if (e < E) partial[(int64_t)blockIdx.y * E + e] = acc;
+}
+
+// Visibility slices of the preload build: enough workgroups to fill the device, at least PRE_KV
+// visibilities per slice.
+struct PrePlan {
+  int64_t nblk, nsplit, k_per_split;
+  size_t bytes;
+};
+
+inline PrePlan make_pre_plan(int64_t K, int64_t Y, int64_t X) {
+  PrePlan p{};
+  const int64_t E = 4 * Y * X;
+  p.nblk = ceil_div(E, (int64_t)256);
+  const int64_t chunks = std::max<int64_t>(1, ceil_div(K, (int64_t)PRE_KV));
+  p.nsplit = std::min<int64_t>(chunks, std::max<int64_t>(1, ceil_div((int64_t)PRE_TARGET_BLOCKS, p.nblk)));
+  p.nsplit = std::min<int64_t>(p.nsplit, 65535);
+  p.k_per_split = ceil_div(chunks, p.nsplit) * PRE_KV;
+  p.nsplit = std::max<int64_t>(1, ceil_div(std::max<int64_t>(K, 1), p.k_per_split));
+  p.bytes = align_up(sizeof(double) * p.nsplit * E, 256);
+  return p;
 }
 
 struct Plan {
@@ -215,6 +313,44 @@ extern "C" int aj_w_tilde_via_preload_from(const double* w_tilde_preload, int64_
                      dim3((unsigned)ceil_div(M, (int64_t)256), (unsigned)std::min<int64_t>(M, 32768)), dim3(256), 0,
                      as_stream(stream), w_tilde_preload, preload_rows, preload_cols, native_index_for_slim_index, M,
                      w_tilde);
+  AJ_LAUNCH_CHECK();
+  return AJ_OK;
+}
+
+extern "C" size_t aj_w_tilde_preload_workspace_bytes(int64_t K, int64_t y_shape, int64_t x_shape) {
+  if (K < 0 || y_shape <= 0 || x_shape <= 0) return 0;
+  return cp::make_pre_plan(K, y_shape, x_shape).bytes;
+}
+
+extern "C" int aj_w_tilde_curvature_preload_interferometer_from(const double* noise_map_real,
+                                                                const double* uv_wavelengths, int64_t K,
+                                                                int64_t y_shape, int64_t x_shape,
+                                                                const double* grid_radians_2d, int64_t grid_rows,
+                                                                int64_t grid_cols, double* curvature_preload,
+                                                                void* workspace, size_t workspace_bytes,
+                                                                void* stream) {
+  AJ_REQUIRE(K >= 0 && y_shape > 0 && x_shape > 0 && y_shape <= grid_rows && x_shape <= grid_cols &&
+                 4 * y_shape * x_shape < (1ll << 40),
+             "bad sizes K=%lld shape (%lld, %lld) grid (%lld, %lld)", (long long)K, (long long)y_shape,
+             (long long)x_shape, (long long)grid_rows, (long long)grid_cols);
+  AJ_REQUIRE(curvature_preload && grid_radians_2d && (K == 0 || (noise_map_real && uv_wavelengths)), "null pointer");
+  hipStream_t st = as_stream(stream);
+  const int64_t E = 4 * y_shape * x_shape;
+  if (K == 0) {
+    AJ_CHECK_HIP(hipMemsetAsync(curvature_preload, 0, sizeof(double) * E, st));
+    return AJ_OK;
+  }
+  const cp::PrePlan p = cp::make_pre_plan(K, y_shape, x_shape);
+  AJ_REQUIRE(workspace && workspace_bytes >= p.bytes, "workspace too small: need %zu bytes, got %zu", p.bytes,
+             workspace_bytes);
+  AJ_REQUIRE(p.nblk < (1ll << 31), "preload table too large");
+  double* partial = static_cast<double*>(workspace);
+  hipLaunchKernelGGL(cp::preload_build_kernel, dim3((unsigned)p.nblk, (unsigned)p.nsplit), dim3(256), 0, st,
+                     noise_map_real, uv_wavelengths, K, grid_radians_2d, grid_cols, y_shape, x_shape, p.k_per_split,
+                     partial);
+  AJ_LAUNCH_CHECK();
+  hipLaunchKernelGGL(cp::compact_reduce_kernel, dim3((unsigned)ceil_div(E, (int64_t)256)), dim3(256), 0, st, partial,
+                     p.nsplit, E, curvature_preload);
   AJ_LAUNCH_CHECK();
   return AJ_OK;
 }

@@ -174,13 +174,27 @@ constexpr size_t OMEGA_LDS_BUDGET = 160 * 1024;  // the whole LDS of a gfx950 CU
 // Where W-tilde comes from: the dense M x M matrix, or the lattice table w_compact [N, 2N-1] with
 // the (row, column) grid index of every image pixel (numba.py:237-255 gives the element rule).
 struct WSrc {
-  const double* W;
-  const double* C;
-  const int64_t* native;
+  const double* W;       // dense M x M W-tilde, or
+  const double* C;       // the lattice table w_compact (N x (2N - 1)), or
+  const int64_t* native;  // (pixel grid positions, for C and Pre)
   int64_t N;
+  const double* Pre;  // the preload table (Py x Px)
+  int64_t Py, Px;
 };
 
+// W-tilde[i, j] from the preload table as original.py:431-449 reads it: the entry at the offset of the
+// later pixel in slim order from the earlier one, negative offsets wrapping as numpy indexes.
+__device__ inline double preload_at(const double* P, int64_t Py, int64_t Px, const int64_t* native, int64_t i,
+                                    int64_t j) {
+  const int64_t sg = i <= j ? 1 : -1;
+  int64_t dy = sg * (native[2 * j] - native[2 * i]), dx = sg * (native[2 * j + 1] - native[2 * i + 1]);
+  dy += dy < 0 ? Py : 0;
+  dx += dx < 0 ? Px : 0;
+  return P[dy * Px + dx];
+}
+
 __device__ inline double wsrc_at(const WSrc& s, int64_t M, int64_t i, int64_t j) {
+  if (s.Pre != nullptr) return preload_at(s.Pre, s.Py, s.Px, s.native, i, j);
   if (s.C == nullptr) return s.W[i * M + j];
   const int64_t m = s.native[2 * i] - s.native[2 * j];
   const int64_t n = (m >= 0 ? s.native[2 * i + 1] - s.native[2 * j + 1] : s.native[2 * j + 1] - s.native[2 * i + 1]) +
@@ -197,7 +211,7 @@ __global__ void __launch_bounds__(OMEGA_THREADS) omega_kernel(const WSrc W, int6
   extern __shared__ double wrow[];  // R x M
   const int64_t m0 = (int64_t)blockIdx.x * R;
   const int rows = (int)std::min<int64_t>(R, M - m0);
-  if (W.C == nullptr) {
+  if (W.C == nullptr && W.Pre == nullptr) {
     for (int64_t i = threadIdx.x; i < (int64_t)rows * M; i += OMEGA_THREADS) wrow[i] = W.W[m0 * M + i];
   } else {
     for (int64_t i = threadIdx.x; i < (int64_t)rows * M; i += OMEGA_THREADS) wrow[i] = wsrc_at(W, M, m0 + i / M, i % M);
@@ -1485,6 +1499,21 @@ extern "C" int aj_curvature_matrix_via_w_compact_sparse_mapping_matrix_from(
                  pix_weights_for_sub_slim_index && curvature_matrix && workspace,
              "null pointer");
   return sparse_curvature(WSrc{nullptr, w_compact, native_index_for_slim_index, grid_size},
+                          pix_indexes_for_sub_slim_index, pix_weights_for_sub_slim_index, M, B, pixels,
+                          curvature_matrix, workspace, workspace_bytes, as_stream(stream));
+}
+
+extern "C" int aj_curvature_matrix_via_w_tilde_curvature_preload_interferometer_from(
+    const double* curvature_preload, int64_t preload_rows, int64_t preload_cols,
+    const int64_t* native_index_for_slim_index, const int64_t* pix_indexes_for_sub_slim_index,
+    const double* pix_weights_for_sub_slim_index, int64_t M, int64_t B, int64_t pixels, double* curvature_matrix,
+    void* workspace, size_t workspace_bytes, void* stream) {
+  AJ_REQUIRE(M > 0 && B > 0 && pixels > 0 && preload_rows > 0 && preload_cols > 0, "bad sizes");
+  AJ_REQUIRE(curvature_preload && native_index_for_slim_index && pix_indexes_for_sub_slim_index &&
+                 pix_weights_for_sub_slim_index && curvature_matrix && workspace,
+             "null pointer");
+  return sparse_curvature(WSrc{nullptr, nullptr, native_index_for_slim_index, 0, curvature_preload, preload_rows,
+                               preload_cols},
                           pix_indexes_for_sub_slim_index, pix_weights_for_sub_slim_index, M, B, pixels,
                           curvature_matrix, workspace, workspace_bytes, as_stream(stream));
 }

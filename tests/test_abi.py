@@ -70,8 +70,9 @@ def test_bad_arguments_rejected_before_any_gpu_work():
 
 
 def test_drop_in_signatures_match_reference_names():
-    """Parameter names and order of the drop-in equal those of src/autojax/numba.py (listed here,
-    since that module cannot be imported without numba)."""
+    """Parameter names and order of the drop-in equal those of src/autojax/numba.py and, for the
+    preload form, src/autojax/original.py (listed here, since those modules cannot be imported without
+    numba)."""
     import inspect
 
     from autojax_amd import api
@@ -96,6 +97,9 @@ def test_drop_in_signatures_match_reference_names():
         "log_likelihood_function_via_w_compact_from": ["dirty_image", "data", "noise_map", "w_compact", "native_index_for_slim_index", "pix_indexes_for_sub_slim_index", "pix_weights_for_sub_slim_index", "neighbors", "neighbors_sizes"],
         "mask_2d_centres_from": ["shape_native", "pixel_scales", "centre"],
         "mask_2d_circular_from": ["shape_native", "pixel_scales", "radius", "centre"],
+        # original.py:230-235 and :600-607 (the preload interchange form)
+        "w_tilde_curvature_preload_interferometer_from": ["noise_map_real", "uv_wavelengths", "shape_masked_pixels_2d", "grid_radians_2d"],
+        "curvature_matrix_via_w_tilde_curvature_preload_interferometer_from": ["curvature_preload", "pix_indexes_for_sub_slim_index", "pix_size_for_sub_slim_index", "pix_weights_for_sub_slim_index", "native_index_for_slim_index", "pix_pixels"],
     }
     for name, params in ref.items():
         sig = inspect.signature(getattr(api, name))

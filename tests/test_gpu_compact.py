@@ -1,4 +1,5 @@
-"""GPU parity of the lattice (w_compact) path: numba.py:210-255, :617-647, :860-967."""
+"""GPU parity of the lattice (w_compact) path, numba.py:210-255, :617-647, :860-967, and of the preload
+form, original.py:230-403, :600-700."""
 
 import numpy as np
 import pytest
@@ -75,6 +76,65 @@ def test_w_tilde_via_preload_is_the_wrapped_gather(gpu, lattice):
     pre = np.random.default_rng(3).standard_normal((2 * N, 2 * N))
     got = api.w_tilde_via_preload_from(pre, L["native"])
     np.testing.assert_array_equal(got, R.w_tilde_via_preload_from(pre, L["native"]))
+
+
+@pytest.fixture(scope="module")
+def refgrid():
+    """The reference harness's grid (30 x 30 at 0.2", corner 9.9", radius 3": 716 pixels), 3,000
+    visibilities, a 120-vertex Delaunay source with rows cut short by pix_size."""
+    d = ds.reference_convention_dataset(seed=31, K=3000, S=120)
+    N = d.n_side
+    pre = R.w_tilde_curvature_preload_interferometer_from(d.noise_map_real, d.uv_wavelengths, (N, N),
+                                                          d.grid_radians_2d)
+    size = np.full(d.M, 3, dtype=np.int64)
+    size[::5] = 2
+    size[::13] = 1
+    return d, pre, size
+
+
+def test_preload_table_vs_oracle(gpu, refgrid):
+    """original.py:230-403 on the device: the 2N x 2N table against the oracle (FP64, norm-wise 1e-10),
+    a sub-corner of the grid (Y, X) = (17, 23), and no visibilities (all zero)."""
+    from autojax_amd import api
+
+    d, pre, _ = refgrid
+    N = d.n_side
+    got = api.w_tilde_curvature_preload_interferometer_from(d.noise_map_real, d.uv_wavelengths, (N, N),
+                                                            d.grid_radians_2d)
+    err = _rel(got, pre)
+    print(f"preload table rel err {err:.3e}", flush=True)
+    assert got.shape == (2 * N, 2 * N) and err < 1e-10
+    assert not got[N].any() and not got[:, N].any()
+    sub = api.w_tilde_curvature_preload_interferometer_from(d.noise_map_real, d.uv_wavelengths, (17, 23),
+                                                            d.grid_radians_2d)
+    want = R.w_tilde_curvature_preload_interferometer_from(d.noise_map_real, d.uv_wavelengths, (17, 23),
+                                                           d.grid_radians_2d)
+    assert _rel(sub, want) < 1e-10
+    empty = api.w_tilde_curvature_preload_interferometer_from(np.zeros(0), np.zeros((0, 2)), (N, N),
+                                                              d.grid_radians_2d)
+    assert empty.shape == (2 * N, 2 * N) and not empty.any()
+
+
+def test_preload_curvature_vs_oracle_and_expanded(gpu, refgrid):
+    """original.py:600-700 on the device (W read from the table, rows cut by pix_size): against the
+    oracle's loop form to 1e-10 norm-wise, and bitwise equal to the sparse product over the expanded
+    W-tilde (the same table values in the same order)."""
+    from autojax_amd import api
+
+    d, pre, size = refgrid
+    s = d.samples[0]
+    args = (pre, s.pix_indexes_for_sub_slim_index, size, s.pix_weights_for_sub_slim_index,
+            d.native_index_for_slim_index, s.pixels)
+    got = api.curvature_matrix_via_w_tilde_curvature_preload_interferometer_from(*args)
+    want = R.curvature_matrix_via_w_tilde_curvature_preload_interferometer_from(*args)
+    err = _rel(got, want)
+    print(f"F via preload rel err {err:.3e}", flush=True)
+    assert err < 1e-10
+    idx = s.pix_indexes_for_sub_slim_index.copy()
+    idx[np.arange(3)[None, :] >= size[:, None]] = -1
+    W = api.w_tilde_via_preload_from(pre, d.native_index_for_slim_index)
+    F2 = api.curvature_matrix_via_w_wilde_sparse_mapping_matrix_from(W, idx, s.pix_weights_for_sub_slim_index, s.pixels)
+    np.testing.assert_array_equal(got, F2)
 
 
 def test_compact_w_tilde_equals_feature_syrk(gpu, lattice):

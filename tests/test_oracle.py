@@ -270,3 +270,60 @@ def test_wrong_sign_dirty_image_breaks_evidence():
                                                      g["pix_weights_for_sub_slim_index"], g["neighbors"],
                                                      g["neighbors_sizes"])
     assert abs(got - g["log_evidence"][1]) > 1e-6
+
+
+# ------------------------------------------------------- preload form (original.py)
+
+def test_preload_table_reproduces_w_tilde():
+    # original.py:230-403 then numba.py:259-291 against the brute-force W on the reference harness's
+    # grid (tests.py:291-306; the reference's own equivalence test, tests.py:720-731).  A table whose
+    # second quadrant takes its offsets from the wrong corner must fail.
+    d = ds.reference_convention_dataset(seed=21, K=60, S=20)
+    n, uv, N = d.noise_map_real, d.uv_wavelengths, d.n_side
+    W = ev.w_tilde_bruteforce(n, uv, d.grid_radians_slim)
+    pre = R.w_tilde_curvature_preload_interferometer_from(n, uv, (N, N), d.grid_radians_2d)
+    assert pre.shape == (2 * N, 2 * N)
+    assert not pre[N].any() and not pre[:, N].any()
+    Wp = R.w_tilde_via_preload_from(pre, d.native_index_for_slim_index)
+    np.testing.assert_allclose(Wp, W, rtol=0, atol=1e-10 * W.max())
+    bad = pre.copy()
+    g, j = d.grid_radians_2d, np.arange(1, N)
+    gp = g[:, N - j - 1]
+    bad[:N, 2 * N - j] = R._preload_quadrant(n, uv, g[0, 0, 0] - gp[:, :, 0], g[0, 0, 1] - gp[:, :, 1])
+    assert not np.allclose(R.w_tilde_via_preload_from(bad, d.native_index_for_slim_index), W, rtol=0,
+                           atol=1e-6 * W.max())
+
+
+def test_preload_curvature_matches_dense_route():
+    # original.py:600-700 (pixel pairs, transpose, own-pixel terms; rows cut by pix_size) against
+    # T^T W T with the brute-force W (the reference's TestCurvatureMatrix equivalence, tests.py:797-809)
+    d = ds.reference_convention_dataset(seed=22, K=40, S=16)
+    n, uv, N = d.noise_map_real, d.uv_wavelengths, d.n_side
+    W = ev.w_tilde_bruteforce(n, uv, d.grid_radians_slim)
+    pre = R.w_tilde_curvature_preload_interferometer_from(n, uv, (N, N), d.grid_radians_2d)
+    s = d.samples[0]
+    M, S = d.M, s.pixels
+    size = np.full(M, 3, dtype=np.int64)
+    size[::7] = 2
+    size[::11] = 1
+    F = R.curvature_matrix_via_w_tilde_curvature_preload_interferometer_from(
+        pre, s.pix_indexes_for_sub_slim_index, size, s.pix_weights_for_sub_slim_index, d.native_index_for_slim_index, S)
+    idx = s.pix_indexes_for_sub_slim_index.copy()
+    idx[np.arange(3)[None, :] >= size[:, None]] = -1
+    T = R.mapping_matrix_from(idx, s.pix_weights_for_sub_slim_index, S)
+    np.testing.assert_allclose(F, T.T @ W @ T, rtol=0, atol=1e-10 * np.abs(F).max())
+
+
+def test_preload_gather_mirrors_the_upper_triangle():
+    # numba.py:280-290 reads the table for j >= i only and mirrors: for any table (not only a
+    # symmetric one) W is symmetric and its upper triangle is the plain wrapped gather
+    d = ds.reference_convention_dataset(seed=23, K=8, S=10)
+    N = d.n_side
+    nat = d.native_index_for_slim_index[:50]
+    pre = np.random.default_rng(0).standard_normal((2 * N, 2 * N))
+    W = R.w_tilde_via_preload_from(pre, nat)
+    np.testing.assert_array_equal(W, W.T)
+    y, x = nat[:, 0], nat[:, 1]
+    G = pre[y[None, :] - y[:, None], x[None, :] - x[:, None]]
+    np.testing.assert_array_equal(np.triu(W), np.triu(G))
+    assert not np.array_equal(np.tril(W, -1), np.tril(G, -1))
