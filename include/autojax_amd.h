@@ -292,6 +292,30 @@ int aj_barycentric_vjp(const double* vertices, const double* positions, const in
                        const double* dweights, double* dvertices, void* workspace, size_t workspace_bytes,
                        void* stream);
 
+/* Source-mesh mapper (the reference builds it upstream, external/get_dataset.py:38-60: a Delaunay mesh
+ * of the source vertices, its neighbour lists and the image pixels' barycentric mapping; no function of
+ * the reference's modules).  Samples s = 0 .. n_samples-1 own vertices [vertex_offsets[s],
+ * vertex_offsets[s+1]) of vertices f8[*, 2] (y, x), 3 to 3072 each (vertex_offsets on the device and
+ * the same offsets on the host).
+ * aj_delaunay_mesh_batched: the Delaunay triangulation of each sample's vertices (one workgroup per
+ * sample): finite triangles i8[2 * total vertices, 3] (sample s from row 2 * vertex_offsets[s], local
+ * vertex ids, counter-clockwise in (x, y)), n_triangles[s]; neighbors i8[total, max_neighbors] (sorted,
+ * -1 padded, local ids) and neighbors_sizes[total] in the format of numba.py:672-677; status[s] = 0 ok,
+ * 1 too many vertices, 2 all vertices on a line, 3 / 4 an insertion failed (capacity / geometry), 5 a
+ * vertex has more than max_neighbors neighbours.  Coincident vertices after the first are left out.
+ * aj_delaunay_locate_batched: each pixel position f8[n_samples, M, 2] (sample s at positions +
+ * 2 * s * positions_stride; stride 0 shares one set) located in its sample's triangles:
+ * pix_indexes_for_sub_slim_index i8[n_samples, M, 3] and pix_weights_for_sub_slim_index f8[n_samples,
+ * M, 3] (barycentric, as aj_barycentric_weights; outside the hull the nearest vertex with weight 1, then
+ * -1 / 0), the mapping format of numba.py:355-360. */
+int aj_delaunay_mesh_batched(const double* vertices, const int64_t* vertex_offsets, const int64_t* vertex_offsets_host,
+                             int64_t n_samples, int64_t max_neighbors, int64_t* triangles, int64_t* n_triangles,
+                             int64_t* neighbors, int64_t* neighbors_sizes, int* status, void* stream);
+int aj_delaunay_locate_batched(const double* vertices, const int64_t* vertex_offsets, int64_t n_samples,
+                               const int64_t* triangles, const int64_t* n_triangles, const double* positions,
+                               int64_t positions_stride, int64_t M, int64_t* pix_indexes_for_sub_slim_index,
+                               double* pix_weights_for_sub_slim_index, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

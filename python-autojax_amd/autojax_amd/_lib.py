@@ -79,6 +79,8 @@ SIGNATURES = {
     "aj_barycentric_weights": (_INT, [_P, _P, _P, _I, _P, _P]),
     "aj_barycentric_vjp_workspace_bytes": (_S, [_I, _I]),
     "aj_barycentric_vjp": (_INT, [_P, _P, _P, _I, _I, _P, _P, _P, _S, _P]),
+    "aj_delaunay_mesh_batched": (_INT, [_P, _P, _P, _I, _I, _P, _P, _P, _P, _P, _P]),
+    "aj_delaunay_locate_batched": (_INT, [_P, _P, _I, _P, _P, _P, _I, _I, _P, _P, _P]),
 }
 
 _lib = None

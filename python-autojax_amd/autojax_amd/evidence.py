@@ -85,16 +85,42 @@ class SampleBatch:
             nbr[off : off + S, : s.neighbors.shape[1]] = s.neighbors
             nsz[off : off + S] = s.neighbors_sizes
             off += S
-        self.coef_host = np.array([float(s.coefficient) for s in samples], np.float64)
+        coef = np.array([float(s.coefficient) for s in samples], np.float64)
         # mean degree of deflatable graphs (0 = plain factorisation of H)
-        self.deg_host = np.array(
+        deg = np.array(
             [deflation_beta(s.neighbors, s.neighbors_sizes, 1.0) if deflate else 0.0 for s in samples], np.float64
         )
+        self._setup(dv.to_device(idx), dv.to_device(wts), dv.to_device(nbr), dv.to_device(nsz), coef, deg)
+
+    @classmethod
+    def from_mappers(cls, mappers, coefficients, deflate=True):
+        """A batch over the device mapper's output (`autojax_amd.mapper.build_mappers`): the mapping and
+        neighbour lists stay on the device; only the neighbour lists' graph check for the deflated
+        log|H| reads them back."""
+        self = cls.__new__(cls)
+        self.n = len(mappers.offsets) - 1
+        self.M = int(mappers.pix_indexes.shape[1])
+        self.B = int(mappers.pix_indexes.shape[2])
+        self.P = int(mappers.neighbors.shape[1])
+        self.S_host = np.diff(mappers.offsets).astype(np.int64)
+        coef = np.asarray(coefficients, np.float64).reshape(self.n).copy()
+        deg = np.zeros(self.n)
+        if deflate:
+            nb, sz = dv.to_host(mappers.neighbors), dv.to_host(mappers.neighbors_sizes)
+            for b in range(self.n):
+                a, e = int(mappers.offsets[b]), int(mappers.offsets[b + 1])
+                deg[b] = deflation_beta(nb[a:e], sz[a:e], 1.0)
+        self._setup(mappers.pix_indexes, mappers.pix_weights, mappers.neighbors, mappers.neighbors_sizes, coef, deg)
+        return self
+
+    def _setup(self, idx, wts, nbr, nsz, coef, deg):
+        self.coef_host = coef
+        self.deg_host = deg
         self.beta_host = self.coef_host**2 * self.deg_host
-        self.idx = dv.to_device(idx)
-        self.wts = dv.to_device(wts)
-        self.nbr = dv.to_device(nbr)
-        self.nsz = dv.to_device(nsz)
+        self.idx = idx.contiguous()
+        self.wts = wts.contiguous()
+        self.nbr = nbr.contiguous()
+        self.nsz = nsz.contiguous()
         self.ws = dv.workspace(
             _lib.query("aj_log_evidence_batched_workspace_bytes", self.M, self.B, self.n,
                        self.S_host.ctypes.data_as(_lib.C.c_void_p))

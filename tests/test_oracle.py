@@ -327,3 +327,56 @@ def test_preload_gather_mirrors_the_upper_triangle():
     G = pre[y[None, :] - y[:, None], x[None, :] - x[:, None]]
     np.testing.assert_array_equal(np.triu(W), np.triu(G))
     assert not np.array_equal(np.tril(W, -1), np.tril(G, -1))
+
+
+# ------------------------------------------------------- source-mesh mapper (upstream of the reference)
+
+def test_mapper_oracle_delaunay_property_and_euler():
+    from scipy.spatial import ConvexHull
+
+    from oracle import mapper as MP
+
+    rng = np.random.default_rng(40)
+    V = rng.standard_normal((150, 2))
+    T = MP.delaunay_triangles(V)
+    h = len(ConvexHull(V).vertices)
+    assert T.shape[0] == 2 * len(V) - h - 2
+    # no vertex strictly inside any triangle's circumcircle
+    for t in T:
+        a, b, c = V[t]
+        ax, ay, bx, by, cx, cy = a[1], a[0], b[1], b[0], c[1], c[0]
+        d = 2 * (ax * (by - cy) + bx * (cy - ay) + cx * (ay - by))
+        ux = ((ax**2 + ay**2) * (by - cy) + (bx**2 + by**2) * (cy - ay) + (cx**2 + cy**2) * (ay - by)) / d
+        uy = ((ax**2 + ay**2) * (cx - bx) + (bx**2 + by**2) * (ax - cx) + (cx**2 + cy**2) * (bx - ax)) / d
+        r2 = (ax - ux) ** 2 + (ay - uy) ** 2
+        d2 = (V[:, 1] - ux) ** 2 + (V[:, 0] - uy) ** 2
+        assert np.all(d2 >= r2 * (1 - 1e-9))
+    # neighbour lists from triangles equal those from the edge list
+    nb, sz = MP.neighbors_from_triangles(len(V), T)
+    edges = np.concatenate([T[:, [0, 1]], T[:, [1, 2]], T[:, [2, 0]]])
+    nb2, sz2 = ds.neighbors_from_edges(len(V), edges)
+    np.testing.assert_array_equal(sz, sz2)
+    np.testing.assert_array_equal(nb, nb2)
+
+
+def test_mapper_oracle_barycentric_reproduces_points_and_nearest_outside():
+    from oracle import mapper as MP
+
+    rng = np.random.default_rng(41)
+    V = rng.random((80, 2))
+    T = MP.delaunay_triangles(V)
+    P = rng.uniform(-0.2, 1.2, (400, 2))
+    idx, w = MP.barycentric_mapping(V, T, P)
+    live = idx[:, 1] >= 0
+    assert 0 < live.sum() < len(P)
+    np.testing.assert_allclose(w.sum(axis=1), 1.0, rtol=0, atol=1e-12)
+    rec = np.einsum("mb,mbk->mk", w[live], V[idx[live]])
+    np.testing.assert_allclose(rec, P[live], rtol=0, atol=1e-12)
+    assert np.all(w[live] >= -1e-15)
+    far = ~live
+    d = ((P[far, None, :] - V[None, :, :]) ** 2).sum(-1)
+    np.testing.assert_array_equal(idx[far, 0], d.argmin(axis=1))
+    assert np.all(w[far, 0] == 1.0) and np.all(idx[far, 1:] == -1)
+    # a plant: weights in the wrong vertex order no longer reproduce the points
+    rec_bad = np.einsum("mb,mbk->mk", w[live][:, [1, 0, 2]], V[idx[live]])
+    assert not np.allclose(rec_bad, P[live], atol=1e-6)
