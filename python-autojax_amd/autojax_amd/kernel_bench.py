@@ -4,13 +4,15 @@
     python -m autojax_amd.kernel_bench batch --M 5000 --samples 64 --K 8192
     python -m autojax_amd.kernel_bench copy --mib 1024
     python -m autojax_amd.kernel_bench intmm --M 8192
+    python -m autojax_amd.kernel_bench mapper --M 5000 --samples 64
 
 ``wtilde`` builds W-tilde on synthetic data of the batching workload's
 generator; ``batch`` evaluates the batched log-evidence of a sample batch on
 one W-tilde; ``copy`` runs a plain device copy of known size, the reference
 for scaling the FETCH_SIZE / WRITE_SIZE counters; ``intmm`` times the vendor
 library's int8 GEMM (torch._int_mm, M x M x M) as the device's achievable
-int8 matrix rate.  Each prints one JSON line
+int8 matrix rate; ``mapper`` builds the source meshes and mappings of a sample batch
+(S ~ U{300..1500} vertices each, the M image pixels as positions) on the device.  Each prints one JSON line
 with the per-call time measured with HIP events.
 """
 
@@ -25,7 +27,7 @@ import numpy as np
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["wtilde", "batch", "copy", "intmm"])
+    ap.add_argument("what", choices=["wtilde", "batch", "copy", "intmm", "mapper"])
     ap.add_argument("--M", type=int, default=5000)
     ap.add_argument("--K", type=int, default=131072)
     ap.add_argument("--precision", default="int8x4")
@@ -106,6 +108,22 @@ def main(argv=None):
             W0 = E.w_tilde(n_d, uv_d, g_d, a.precision, workspace=ws)
             rec["max_abs_diff_vs_variant0"] = float((W - W0).abs().max().item())
         print(json.dumps(rec), flush=True)
+        return
+    if a.what == "mapper":
+        from . import mapper as mp
+
+        r = float(np.abs(grid).max())
+        verts = []
+        for _ in range(a.samples):
+            S = int(rng.integers(300, 1501))
+            rad = r * np.sqrt(rng.random(S))
+            th = 2.0 * np.pi * rng.random(S)
+            verts.append(np.stack([rad * np.sin(th), rad * np.cos(th)], 1))
+        mb = mp.MapperBatch(verts, grid)
+        ms = timed(mb.launch, a.reps)
+        mb.result()
+        print(json.dumps({"kernel": "mapper", "M": M, "samples": a.samples, "sum_S": int(mb.offsets[-1]), "ms": ms}),
+              flush=True)
         return
     E.w_tilde(n_d, uv_d, g_d, "fp64", out=W)
     samples = []

@@ -51,52 +51,69 @@ class DeviceMappers:
                     pix_weights_for_sub_slim_index=dv.to_host(self.pix_weights[s]))
 
 
+class MapperBatch:
+    """Device buffers for one set of samples; `launch` enqueues the two kernels on the current stream
+    (no synchronisation), `result` checks every sample's status and returns the outputs."""
+
+    def __init__(self, vertices, positions, max_neighbors: int = 32):
+        n = len(vertices)
+        if n == 0:
+            raise ValueError("no samples")
+        for v in vertices:
+            if np.asarray(v).ndim != 2 or np.asarray(v).shape[1] != 2:
+                raise ValueError("vertices must be [S, 2] (y, x)")
+        sizes = np.array([np.asarray(v).shape[0] for v in vertices], dtype=np.int64)
+        if sizes.min() < 3 or sizes.max() > MAX_VERTICES:
+            raise ValueError(f"each sample needs 3..{MAX_VERTICES} vertices, got {sizes.min()}..{sizes.max()}")
+        self.offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+        self.V = dv.to_device(np.concatenate([np.asarray(v, dtype=np.float64) for v in vertices]))
+        if isinstance(positions, (list, tuple)):
+            if len(positions) != n:
+                raise ValueError("one position set per sample")
+            pos = np.stack([np.asarray(p, dtype=np.float64) for p in positions])
+            self.stride = pos.shape[1]
+        else:
+            pos = np.asarray(positions, dtype=np.float64)
+            self.stride = 0 if pos.ndim == 2 else pos.shape[1]
+            if pos.ndim == 3 and pos.shape[0] != n:
+                raise ValueError("positions [n, M, 2] must have one set per sample")
+        self.M = pos.shape[-2]
+        if pos.shape[-1] != 2 or self.M == 0:
+            raise ValueError("positions must be [..., M, 2] (y, x) with M > 0")
+        self.n = n
+        self.pos = dv.to_device(pos)
+        self.off_d = dv.to_device(self.offsets)
+        total = int(self.offsets[-1])
+        self.P = int(max_neighbors)
+        self.tri = dv.empty((2 * total, 3), torch.int64)
+        self.ntri = dv.empty((n,), torch.int64)
+        self.nbr = dv.empty((total, self.P), torch.int64)
+        self.nsz = dv.empty((total,), torch.int64)
+        self.status = dv.empty((n,), torch.int32)
+        self.idx = dv.empty((n, self.M, 3), torch.int64)
+        self.wts = dv.empty((n, self.M, 3))
+
+    def launch(self):
+        _lib.call("aj_delaunay_mesh_batched", self.V.data_ptr(), self.off_d.data_ptr(), self.offsets.ctypes.data,
+                  self.n, self.P, self.tri.data_ptr(), self.ntri.data_ptr(), self.nbr.data_ptr(), self.nsz.data_ptr(),
+                  self.status.data_ptr(), dv.stream_ptr())
+        _lib.call("aj_delaunay_locate_batched", self.V.data_ptr(), self.off_d.data_ptr(), self.n, self.tri.data_ptr(),
+                  self.ntri.data_ptr(), self.pos.data_ptr(), self.stride, self.M, self.idx.data_ptr(),
+                  self.wts.data_ptr(), dv.stream_ptr())
+
+    def result(self) -> DeviceMappers:
+        st = dv.to_host(self.status)
+        bad = np.nonzero(st)[0]
+        if bad.size:
+            s = int(bad[0])
+            raise ValueError(f"sample {s}: {STATUS.get(int(st[s]), f'status {int(st[s])}')}")
+        return DeviceMappers(self.offsets, self.tri, self.ntri, self.nbr, self.nsz, self.idx, self.wts)
+
+
 def build_mappers(vertices, positions, max_neighbors: int = 32) -> DeviceMappers:
     """vertices: list of f8[S_s, 2] (3 <= S_s <= 3072); positions: f8[M, 2] shared by all samples, or a
     list of f8[M, 2] (one per sample), or f8[n, M, 2].  Raises ValueError with the failing sample when
     a triangulation cannot be built."""
-    n = len(vertices)
-    if n == 0:
-        raise ValueError("no samples")
-    sizes = np.array([np.asarray(v).shape[0] for v in vertices], dtype=np.int64)
-    if sizes.min() < 3 or sizes.max() > MAX_VERTICES:
-        raise ValueError(f"each sample needs 3..{MAX_VERTICES} vertices, got {sizes.min()}..{sizes.max()}")
-    offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
-    for v in vertices:
-        if np.asarray(v).ndim != 2 or np.asarray(v).shape[1] != 2:
-            raise ValueError("vertices must be [S, 2] (y, x)")
-    V = dv.to_device(np.concatenate([np.asarray(v, dtype=np.float64) for v in vertices]))
-    if isinstance(positions, (list, tuple)):
-        if len(positions) != n:
-            raise ValueError("one position set per sample")
-        pos = np.stack([np.asarray(p, dtype=np.float64) for p in positions])
-        stride = pos.shape[1]
-    else:
-        pos = np.asarray(positions, dtype=np.float64)
-        stride = 0 if pos.ndim == 2 else pos.shape[1]
-        if pos.ndim == 3 and pos.shape[0] != n:
-            raise ValueError("positions [n, M, 2] must have one set per sample")
-    M = pos.shape[-2]
-    if pos.shape[-1] != 2 or M == 0:
-        raise ValueError("positions must be [..., M, 2] (y, x) with M > 0")
-    P_d = dv.to_device(pos)
-    off_d = dv.to_device(offsets)
-    total = int(offsets[-1])
-    P = int(max_neighbors)
-    tri = dv.empty((2 * total, 3), torch.int64)
-    ntri = dv.empty((n,), torch.int64)
-    nbr = dv.empty((total, P), torch.int64)
-    nsz = dv.empty((total,), torch.int64)
-    status = dv.empty((n,), torch.int32)
-    _lib.call("aj_delaunay_mesh_batched", V.data_ptr(), off_d.data_ptr(), offsets.ctypes.data, n, P, tri.data_ptr(),
-              ntri.data_ptr(), nbr.data_ptr(), nsz.data_ptr(), status.data_ptr(), dv.stream_ptr())
-    idx = dv.empty((n, M, 3), torch.int64)
-    wts = dv.empty((n, M, 3))
-    _lib.call("aj_delaunay_locate_batched", V.data_ptr(), off_d.data_ptr(), n, tri.data_ptr(), ntri.data_ptr(),
-              P_d.data_ptr(), stride, M, idx.data_ptr(), wts.data_ptr(), dv.stream_ptr())
-    st = dv.to_host(status)
-    bad = np.nonzero(st)[0]
-    if bad.size:
-        s = int(bad[0])
-        raise ValueError(f"sample {s}: {STATUS.get(int(st[s]), f'status {int(st[s])}')}")
-    return DeviceMappers(offsets, tri, ntri, nbr, nsz, idx, wts)
+    mb = MapperBatch(vertices, positions, max_neighbors)
+    mb.launch()
+    return mb.result()
