@@ -3,6 +3,10 @@
     python -m autojax_amd.precision_study [--config cfg3] [--modes int8crt,int8crt11,int8x5,int8x4,int8x3,fp32,bf16x3]
                                           [--lams 0.01,1,100]
 
+Mode ``lattice_fp64`` builds the FP64 W-tilde by an independent route (the exact lattice table of
+numba.py:210-255, expanded): its differences from the FP64 feature SYRK are the FP64 path's own
+rounding spread, the floor against which the other modes' errors are read.
+
 For every mode, W-tilde is rebuilt at that precision and everything after it runs in FP64 (as in
 the product path).  Reported, against the same quantities from the FP64 W-tilde:
 
@@ -55,9 +59,16 @@ def main(argv=None):
     D = api.sparse_mapping_matrix_transpose_matmul(y.reshape(-1, 1), idx, wts, S).reshape(-1)
     H = [api.constant_regularization_matrix_from(c, smp.neighbors, smp.neighbors_sizes) for c in lams]
 
+    native = dv.to_device(ds.native_index_for_slim_index)
+
     def stages(prec):
         t = time.perf_counter()
-        W = E.w_tilde(noise, uv, grid, prec)
+        if prec == "lattice_fp64":
+            # the same FP64 W-tilde by another summation order: the lattice table, expanded
+            wc = api.w_compact_curvature_interferometer_from(ds.n_side, noise, uv, ds.pixel_scale)
+            W = api.w_tilde_via_compact_from(wc, native)
+        else:
+            W = E.w_tilde(noise, uv, grid, prec)
         torch.cuda.synchronize()
         t_w = time.perf_counter() - t
         F = api.curvature_matrix_via_w_wilde_sparse_mapping_matrix_from(W, idx, wts, S)
