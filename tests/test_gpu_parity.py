@@ -60,6 +60,59 @@ def test_w_tilde_modes_vs_oracle(gpu, cfg1_small, precision):
     np.testing.assert_array_equal(W, W.T)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "int8x5", "int8crt", "int8crt11"])
+def test_w_tilde_edge_sizes(gpu, precision):
+    """Image-pixel counts around the 256-row tile edge and tiny ones (1, 2, 255, 256, 257) with 0, 1 and 65
+    visibilities: zeros without visibilities, the oracle's W otherwise (a single visibility gives the
+    rank-2 matrix w cos(phi_i - phi_j))."""
+    from autojax_amd import api
+
+    rng = np.random.default_rng(60)
+    for M in (1, 2, 255, 256, 257):
+        grid, _ = ds.grid_for_pixel_count(max(M, 3), 0.05)
+        grid = np.ascontiguousarray(grid[:M])
+        for K in (0, 1, 65):
+            uv = ds.uv_loguniform(rng, K, 1e3, 5e5) if K else np.zeros((0, 2))
+            n = rng.uniform(0.5, 2.0, K)
+            W = api.w_tilde_curvature_interferometer_from(n, uv, grid, precision=precision)
+            assert W.shape == (M, M)
+            if K == 0:
+                assert not W.any()
+                continue
+            Wo = R.w_tilde_curvature_interferometer_from(n, uv, grid)
+            err = _rel(W, Wo)
+            assert err < WT_TOL[precision], (M, K, err)
+            np.testing.assert_array_equal(W, W.T)
+
+
+def test_w_tilde_full_size_properties(gpu):
+    """The batching workload's size (M 5,000, K 1e6): the mixed path's W (int8crt11) against the FP64
+    feature SYRK (norm-wise 2e-13), exactly symmetric, its diagonal equal to sum_k n_k^-2 (1e-12
+    relative), and two builds bitwise equal."""
+    import torch
+
+    from autojax_amd import device as dv
+    from autojax_amd import evidence as E
+
+    rng = np.random.default_rng(5)
+    grid, _ = ds.grid_for_pixel_count(5000, 0.05)
+    K = 1_000_000
+    uv = ds.uv_loguniform(rng, K, 1e3, 5e5)
+    _, noise = ds.visibilities(rng, K, 0.5, 2.0)
+    nr = np.ascontiguousarray(noise.real)
+    n_d, uv_d, g_d = dv.to_device(nr), dv.to_device(uv), dv.to_device(grid)
+    W = E.w_tilde(n_d, uv_d, g_d, "int8crt11")
+    W2 = E.w_tilde(n_d, uv_d, g_d, "int8crt11")
+    W64 = E.w_tilde(n_d, uv_d, g_d, "fp64")
+    torch.cuda.synchronize()
+    err = float(((W - W64).abs().max() / W64.abs().max()).item())
+    diag = float(np.sum(nr**-2.0))
+    derr = float(((torch.diagonal(W) - diag).abs().max() / diag).item())
+    print(f"int8crt11 vs fp64 {err:.3e}, diagonal {derr:.3e}", flush=True)
+    assert err < 2e-13 and derr < 1e-12
+    assert torch.equal(W, W.T) and torch.equal(W, W2)
+
+
 def test_w_tilde_multichunk_rows(gpu):
     """K spanning several 4096-visibility chunks; oracle on a row subset at full K."""
     from autojax_amd import api
