@@ -130,16 +130,14 @@ __global__ void __launch_bounds__(256) scale_kernel(const double* __restrict__ n
 }
 
 // sin(pi t) and cos(pi t) for |t| <= 1: t = x + n/2 with n = rint(2t), |x| <= 1/4, then the Taylor
-// series of sin(pi x) and cos(pi x) to degree 19 and 18 (truncation below 1e-17 at |pi x| <= pi/4;
-// absolute error about 1e-16 overall) and the quadrant n mod 4.  Coefficients (-1)^k pi^(2k+1)/(2k+1)!
-// and (-1)^k pi^(2k)/(2k)! rounded to FP64.
+// series of sin(pi x) and cos(pi x) to degree 15 and 16 (truncation below 5e-17 at |pi x| <= pi/4,
+// under FP64 rounding; absolute error about 1e-16 overall) and the quadrant n mod 4.  Coefficients
+// (-1)^k pi^(2k+1)/(2k+1)! and (-1)^k pi^(2k)/(2k)! rounded to FP64.
 __device__ inline void sincospi_small(double t, double& sn, double& cs) {
   const double n = rint(2.0 * t);
   const double x = fma(-0.5, n, t);
   const double y = x * x;
-  double ps = -2.2948428997269873e-08;
-  ps = fma(ps, y, 7.952054001475513e-07);
-  ps = fma(ps, y, -2.1915353447830217e-05);
+  double ps = -2.1915353447830217e-05;
   ps = fma(ps, y, 0.00046630280576761255);
   ps = fma(ps, y, -0.0073704309457143504);
   ps = fma(ps, y, 0.08214588661112823);
@@ -147,8 +145,7 @@ __device__ inline void sincospi_small(double t, double& sn, double& cs) {
   ps = fma(ps, y, 2.5501640398773455);
   ps = fma(ps, y, -5.16771278004997);
   ps = fma(ps, y, 3.141592653589793);
-  double pc = -1.3878952462213771e-07;
-  pc = fma(pc, y, 4.303069587032947e-06);
+  double pc = 4.303069587032947e-06;
   pc = fma(pc, y, -0.0001046381049248457);
   pc = fma(pc, y, 0.0019295743094039231);
   pc = fma(pc, y, -0.02580689139001406);
@@ -165,30 +162,39 @@ __device__ inline void sincospi_small(double t, double& sn, double& cs) {
   cs = ((q + 1) & 2) ? -b : b;
 }
 
-// Residue of an integer-valued double q (|q| < 2^40) modulo MODULI[i] (a compile-time index).
+// Residues of an integer-valued double q (|q| < 2^40) as int8 bytes.  qm = q + 1.5 * 2^52 (exact) holds q
+// in the low bits of its significand: the low 32 bits of qm's bit pattern are q mod 2^32 in two's
+// complement, so its low byte is the balanced residue modulo 256, and for an odd modulus m
+// fma(-n, m, qm) with n = rint(q / m) is exactly qm - n m = 1.5 * 2^52 + r (|r| <= (m - 1)/2; q/m is
+// never a half-integer and the product's rounding is far below the 1/(2m) gap), whose low byte is r.
+constexpr double RES_SHIFT = 6755399441055744.0;  // 1.5 * 2^52
+
+__device__ inline uint32_t low_word(double d) { return (uint32_t)__double_as_longlong(d); }
+
 template <int i>
-__device__ inline int residue_of(double q) {
+__device__ inline uint32_t residue_word(double q, double qm) {
+  if constexpr (i == 0) return low_word(qm);
   constexpr double m = (double)MODULI[i], inv_m = 1.0 / (double)MODULI[i];
-  if constexpr (i == 0) return (int)(q - 256.0 * floor((q + 128.0) * (1.0 / 256.0)));  // [-128, 127]
-  // m odd: q/m is never a half-integer, and the product's rounding (|q| < 2^40) is far below the
-  // 1/(2m) gap, so rint selects the balanced residue in [-(m-1)/2, (m-1)/2].
-  const double n = rint(q * inv_m);
-  return (int)fma(-n, m, q);
+  return low_word(fma(-rint(q * inv_m), m, qm));
 }
 
 // Packed residues of (cos, sin) of two visibilities: bytes cos0, sin0, cos1, sin1.
 template <int i>
-__device__ inline uint32_t packed_residues(const double (&qc)[2], const double (&qn)[2]) {
-  return (uint32_t)(uint8_t)(int8_t)residue_of<i>(qc[0]) | ((uint32_t)(uint8_t)(int8_t)residue_of<i>(qn[0]) << 8) |
-         ((uint32_t)(uint8_t)(int8_t)residue_of<i>(qc[1]) << 16) |
-         ((uint32_t)(uint8_t)(int8_t)residue_of<i>(qn[1]) << 24);
+__device__ inline uint32_t packed_residues(const double (&qc)[2], const double (&qn)[2], const double (&qcm)[2],
+                                           const double (&qnm)[2]) {
+  const uint32_t a = residue_word<i>(qc[0], qcm[0]), b = residue_word<i>(qn[0], qnm[0]);
+  const uint32_t c = residue_word<i>(qc[1], qcm[1]), d = residue_word<i>(qn[1], qnm[1]);
+  // v_perm_b32 selects bytes of {src0, src1} (src1 = bytes 0-3, src0 = bytes 4-7; selector 12 = zero)
+  const uint32_t ab = __builtin_amdgcn_perm(b, a, 0x0c0c0400u);  // a0 b0 0 0
+  const uint32_t cd = __builtin_amdgcn_perm(d, c, 0x0c0c0400u);  // c0 d0 0 0
+  return __builtin_amdgcn_perm(cd, ab, 0x05040100u);             // a0 b0 c0 d0
 }
 
 template <int i, int NM>
 __device__ inline void store_residues4(int8_t* __restrict__ X, int64_t plane, uint32_t off, const double (&qc)[2],
-                                       const double (&qn)[2]) {
-  *reinterpret_cast<uint32_t*>(X + i * plane + off) = packed_residues<i>(qc, qn);
-  if constexpr (i + 1 < NM) store_residues4<i + 1, NM>(X, plane, off, qc, qn);
+                                       const double (&qn)[2], const double (&qcm)[2], const double (&qnm)[2]) {
+  *reinterpret_cast<uint32_t*>(X + i * plane + off) = packed_residues<i>(qc, qn, qcm, qnm);
+  if constexpr (i + 1 < NM) store_residues4<i + 1, NM>(X, plane, off, qc, qn, qcm, qnm);
 }
 
 // ---------------------------------------------------------------------------
@@ -240,7 +246,13 @@ __global__ void __launch_bounds__(256) features_kernel(const double* __restrict_
 #pragma unroll
       for (int v = 0; v < 2; ++v) qc[v] = qn[v] = 0.0;
     }
-    store_residues4<0, NM>(X, plane, off, qc, qn);
+    double qcm[2], qnm[2];
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+      qcm[v] = qc[v] + RES_SHIFT;
+      qnm[v] = qn[v] + RES_SHIFT;
+    }
+    store_residues4<0, NM>(X, plane, off, qc, qn, qcm, qnm);
   }
 }
 
