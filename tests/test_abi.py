@@ -4,6 +4,7 @@ import ctypes
 import re
 from pathlib import Path
 
+import numpy as np
 import pytest
 
 ROOT = Path(__file__).resolve().parents[1]
@@ -67,6 +68,57 @@ def test_bad_arguments_rejected_before_any_gpu_work():
         _lib.call("aj_w_tilde_curvature_interferometer_from", None, None, None, 10, 0, 0, None, None, 0, None)
     with pytest.raises(ValueError):
         _lib.call("aj_w_tilde_curvature_interferometer_from", None, None, None, 10, 5, 7, None, None, 0, None)
+
+
+def test_new_entry_points_reject_bad_arguments_without_gpu():
+    """Argument checks of the residue-mode tuning, the preload form and the mapper run before any HIP
+    call: each returns AJ_EINVAL (ValueError) on the CPU."""
+    import ctypes
+
+    import numpy as np
+
+    from autojax_amd import _lib
+
+    for variant, chunk in ((5, 0), (0, 1000), (0, 65536), (-2, 0)):
+        with pytest.raises(ValueError):
+            _lib.call("aj_w_tilde_crt_configure", variant, chunk)
+    _lib.call("aj_w_tilde_crt_configure", -1, 0)  # defaults
+    # preload table: corner larger than the grid, negative K
+    with pytest.raises(ValueError):
+        _lib.call("aj_w_tilde_curvature_preload_interferometer_from", None, None, 0, 31, 30, 1, 30, 30, 1, None, 0,
+                  None)
+    with pytest.raises(ValueError):
+        _lib.call("aj_w_tilde_curvature_preload_interferometer_from", None, None, -1, 30, 30, 1, 30, 30, 1, None, 0,
+                  None)
+    with pytest.raises(ValueError):
+        _lib.call("aj_curvature_matrix_via_w_tilde_curvature_preload_interferometer_from", 1, 60, 60, 1, 1, 1, 0, 3,
+                  10, 1, 1, 1, None)
+    # mapper: no samples; a sample of two vertices (host offsets); no pixels
+    offs = np.array([0, 2], dtype=np.int64)
+    with pytest.raises(ValueError):
+        _lib.call("aj_delaunay_mesh_batched", 1, 1, offs.ctypes.data, 0, 32, 1, 1, 1, 1, 1, None)
+    with pytest.raises(ValueError, match="vertices"):
+        _lib.call("aj_delaunay_mesh_batched", 1, 1, offs.ctypes.data, 1, 32, 1, 1, 1, 1, 1, None)
+    with pytest.raises(ValueError):
+        _lib.call("aj_delaunay_locate_batched", 1, 1, 1, 1, 1, 1, 0, 0, 1, 1, None)
+    # CRT constants: host-only query, 11 or 12 moduli
+    buf = [np.zeros(12) for _ in range(3)] + [np.zeros(4)]
+    with pytest.raises(ValueError):
+        _lib.call("aj_crt_constants", 10, *(b.ctypes.data for b in buf))
+    _lib.call("aj_crt_constants", 11, *(b.ctypes.data for b in buf))
+    assert buf[0][:11].tolist() == [256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223]
+    assert buf[3][3] == 60.0 * 2.0**30
+
+
+def test_mapper_rejects_bad_vertex_counts_on_host():
+    from autojax_amd import mapper
+
+    with pytest.raises(ValueError, match="3..3072"):
+        mapper.build_mappers([np.zeros((2, 2))], np.zeros((4, 2)))
+    with pytest.raises(ValueError, match="3..3072"):
+        mapper.build_mappers([np.zeros((3073, 2))], np.zeros((4, 2)))
+    with pytest.raises(ValueError, match="no samples"):
+        mapper.build_mappers([], np.zeros((4, 2)))
 
 
 def test_drop_in_signatures_match_reference_names():
