@@ -73,10 +73,6 @@ def test_bad_arguments_rejected_before_any_gpu_work():
 def test_new_entry_points_reject_bad_arguments_without_gpu():
     """Argument checks of the residue-mode tuning, the preload form and the mapper run before any HIP
     call: each returns AJ_EINVAL (ValueError) on the CPU."""
-    import ctypes
-
-    import numpy as np
-
     from autojax_amd import _lib
 
     for variant, chunk in ((5, 0), (0, 1000), (0, 65536), (-2, 0)):
