@@ -1,27 +1,28 @@
-// W-tilde as an exact integer Gram matrix assembled from its residues modulo twelve small moduli,
+// W-tilde as an exact integer Gram matrix assembled from its residues modulo 12 (or 11) small moduli,
 // for gfx950.
 //
 // Reference: src/autojax/numba.py:155-207 (w_tilde_curvature_interferometer_from)
 //   W_ij = sum_k n_k^-2 cos(2 pi [(g_i1 - g_j1) u_k0 + (g_i0 - g_j0) u_k1])
 //
 // As in the other modes, W = [C S][C S]^T over cos/sin features of the FP64-reduced phases.  Per chunk
-// of kc visibilities (16384 by default, at most 32768) every feature sqrt(w_k) cos(.) (and sin) is divided by the chunk's largest
-// sqrt(w_k) and rounded to an integer q with |q| <= Q = 127 * 256^4 (the same fixed-point grid as the
-// five-digit mode).  The chunk's integer Gram matrix G_ij = sum_f q_if q_jf is then bounded by
-// kc (Q^2 + 2Q) < P/2, P = prod_i m_i over the moduli below, so G is determined by its residues
+// of kc visibilities every feature sqrt(w_k) cos(.) (and sin) is divided by the chunk's largest
+// sqrt(w_k) and rounded to an integer q with |q| <= Q: Q = 127 * 256^4 (the five-digit mode's grid)
+// with all 12 moduli (chunks of 16384 by default, at most 32768), Q = 60 * 2^30 with the first 11
+// (chunks of at most 16384).  The chunk's integer Gram matrix G_ij = sum_f q_if q_jf is then bounded
+// by kc (Q^2 + 2Q) < P/2, P = prod_i m_i over the moduli in use, so G is determined by its residues
 // G mod m_i (Chinese remainder theorem).  Each residue is one int8 Gram product of the features'
 // balanced residues (|r| <= 128) on v_mfma_i32_16x16x64_i8, exact in int32 because
-// 2 kc * 128^2 < 2^31.  That is 12 int8 products per FP64-equivalent product, where splitting
-// q into five base-256 digits needs 22.  The residues are recombined in FP64 with the CRT basis
-// split into an exact high part (a multiple of 2^53) and a low part below 2^53, the integer-modular
-// emulation of Ozaki, Uchino and Imamura ("Ozaki Scheme II: a GEMM-oriented emulation of
-// floating-point matrix multiplication using an integer modular technique", 2025).  The result
-// is the Gram matrix of the quantised features to FP64 rounding, so it stays positive
-// semi-definite.
+// 2 kc * 128^2 < 2^31.  That is 12 (or 11) int8 products per FP64-equivalent product, where
+// splitting q into five base-256 digits needs 22.  The residues are recombined in FP64 with the CRT
+// basis split into an exact high part (a multiple of 2^53) and a low part below 2^53, the
+// integer-modular emulation of Ozaki, Uchino and Imamura ("Ozaki Scheme II: a GEMM-oriented emulation
+// of floating-point matrix multiplication using an integer modular technique", 2025).  The result is
+// the Gram matrix of the quantised features to FP64 rounding, so it stays positive semi-definite.
 //
-// Launches per chunk: scale (max 1/|n_k|), features (12 residue planes), SYRK (one workgroup per
-// (256 x 256 lower-triangle tile, modulus) unit, writing int8 residues of G), combine (CRT + FP64
-// accumulation into the lower triangle of W).
+// Launches per chunk: scale (max 1/|n_k|, once for all chunks), features (one residue plane per
+// modulus), the SYRK (by default persistent: one workgroup per CU working through (256 x 256
+// lower-triangle tile, modulus) units and pieces of units, writing int8 residues of G; a fix-up sums the
+// pieces' residues), combine (CRT + FP64 accumulation into the lower triangle of W).
 
 #include "aj_common.h"
 
