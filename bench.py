@@ -91,10 +91,24 @@ def counter_traffic(precision, M, kernel):
     return float(fields["traffic_bytes_per_launch"])
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for row in f:
+                if row.startswith("model name"):
+                    return row.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(ds, k_sub, n_like, threads):
     """Oracle timed on the host: C/OpenMP W-tilde on k_sub visibilities (scaled to K) and the
-    numpy likelihood (dense T, two DGEMMs, LU solve, two slogdets) on n_like samples (scaled)."""
+    numpy likelihood (dense T, two DGEMMs, LU solve, two slogdets) on n_like samples (scaled), with
+    `threads` OpenMP threads and the BLAS limited to the same count."""
     import ctypes
+
+    from threadpoolctl import threadpool_limits
 
     from oracle import restated as R
 
@@ -111,12 +125,13 @@ def cpu_baseline(ds, k_sub, n_like, threads):
     t_w = (time.perf_counter() - t0) * (K / k_sub)
     assert rc == 0
     y = ds.dirty_image
-    t0 = time.perf_counter()
-    for smp in ds.samples[:n_like]:
-        R.log_likelihood_function_via_w_tilde_from(y, ds.data, ds.noise_map, W, smp.pix_indexes_for_sub_slim_index,
-                                                   smp.pix_weights_for_sub_slim_index, smp.neighbors,
-                                                   smp.neighbors_sizes)
-    t_like = (time.perf_counter() - t0) / n_like * len(ds.samples)
+    with threadpool_limits(limits=threads):
+        t0 = time.perf_counter()
+        for smp in ds.samples[:n_like]:
+            R.log_likelihood_function_via_w_tilde_from(y, ds.data, ds.noise_map, W, smp.pix_indexes_for_sub_slim_index,
+                                                       smp.pix_weights_for_sub_slim_index, smp.neighbors,
+                                                       smp.neighbors_sizes)
+        t_like = (time.perf_counter() - t0) / n_like * len(ds.samples)
     return len(ds.samples) / (t_w + t_like), t_w, t_like
 
 
@@ -237,6 +252,7 @@ def main():
     wt_ms = (tm["wt_syrk"][0] + tm["wt_features"][0] + tm["wt_combine"][0]) / args.steps
     batch_ms = ms_mixed - wt_ms
 
+    chol_flops = float(sum(2.0 * float(S) ** 3 / 3.0 for S in batch.S_host))
     line = {
         "metric": "log-evidence evals/sec (mixed+batched vs plain FP64 GPU) at <=1e-3 |dlogL|",
         "value": round(value, 3),
@@ -270,6 +286,9 @@ def main():
                          "solve": round(tm["solve"][0] / args.steps, 3),
                          "batch_total": round(batch_ms, 3)},
         "evals_per_s_excluding_w_tilde": round(n / (batch_ms / 1e3), 3),
+        # blocked Cholesky of every A = F + H and every H of the batch: S^3 / 3 flops each
+        "factorisation": {"flops": chol_flops, "tflops": round(chol_flops / tm["chol"][0] * args.steps / 1e9, 3),
+                          "frac_fp64_peak": round(chol_flops / tm["chol"][0] * args.steps / 1e9 / 78.6, 4)},
         "lattice_fp64": {"value": round(n / (ms_lat / 1e3), 3), "ms_per_step": round(ms_lat, 3),
                          "w_compact_ms": round(ms_wc, 3),
                          "speedup_vs_plain": round(ms_plain / ms_lat, 3),
@@ -287,10 +306,17 @@ def main():
         threads = len(os.sched_getaffinity(0))
         threads = min(threads, 16)
         cv, t_w, t_like = cpu_baseline(ds, args.cpu_k_sub, 2, threads)
+        k1 = max(1, args.cpu_k_sub // 10)
+        cv1, t_w1, t_like1 = cpu_baseline(ds, k1, 1, 1)
         line["cpu_baseline"] = {"value": round(cv, 6), "unit": "evals/s", "cores": threads, "kind": "port",
                                 "sample": f"C/OpenMP W-tilde on {args.cpu_k_sub} of {K} visibilities scaled x{K // args.cpu_k_sub} "
-                                          f"({t_w:.1f}s), numpy likelihood on 2 of {n} samples scaled ({t_like:.1f}s)"}
-        log(f"cpu baseline: {cv:.4g} evals/s")
+                                          f"({t_w:.1f}s), numpy likelihood on 2 of {n} samples scaled ({t_like:.1f}s); "
+                                          f"per-thread W-tilde accumulator {M * M * 8 / 1e6:.0f} MB",
+                                "value_1thread": round(cv1, 7),
+                                "sample_1thread": f"W-tilde on {k1} visibilities scaled x{K // k1} ({t_w1:.1f}s), "
+                                                  f"likelihood on 1 of {n} samples scaled ({t_like1:.1f}s)",
+                                "cpu": cpu_model()}
+        log(f"cpu baseline: {cv:.4g} evals/s ({threads} threads), {cv1:.4g} evals/s (1 thread)")
     print(json.dumps(line), flush=True)
 
 
