@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
+#include <atomic>
 #include <type_traits>
 
 #include "../../include/autojax_amd.h"

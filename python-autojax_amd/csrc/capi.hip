@@ -30,7 +30,7 @@ struct Slot {
   size_t used = 0;
 };
 std::mutex g_tm;
-int g_timing = 0;
+std::atomic<int> g_timing{0};  // read without the lock on every timed launch
 Slot g_slots[AJ_TIMER_COUNT];
 }  // namespace
 
@@ -60,7 +60,7 @@ void timer_end(int id, hipStream_t st) {
 
 extern "C" int aj_timing_enable(int on) {
   std::lock_guard<std::mutex> lk(aj::g_tm);
-  aj::g_timing = on ? 1 : 0;
+  aj::g_timing.store(on ? 1 : 0);
   for (auto& s : aj::g_slots) s.used = 0;
   return AJ_OK;
 }

@@ -1215,7 +1215,7 @@ __global__ void mirror_kernel(double* __restrict__ W, int64_t M) {
 // LDS layout variant of the integer SYRK (aj_w_tilde_set_variant; chosen by the autotuner).
 // Integer SYRK kernel variant; -1 selects the default per digit count: the 8-wave kernel (7) for 5
 // digits, whose 4-wave forms need more than 512 registers per lane, else variant 0.
-static int g_i8_layout = -1;
+static std::atomic<int> g_i8_layout{-1};  // process-wide tuning, read once per build
 constexpr int default_i8_variant(int ns) { return ns == 5 ? 7 : 0; }
 
 struct Plan {
@@ -1230,7 +1230,7 @@ struct Plan {
 // memory-side (infinity) cache and the re-reads do not go to HBM (1 GiB groups were 11% slower for
 // the 4-digit integer SYRK at M = 5000).
 constexpr size_t GROUP_BYTES_DEFAULT = size_t(256) << 20;
-static size_t g_group_bytes = GROUP_BYTES_DEFAULT;
+static std::atomic<size_t> g_group_bytes{GROUP_BYTES_DEFAULT};
 
 inline Plan make_plan(int64_t K, int64_t M, int mode) {
   Plan p{};
@@ -1241,7 +1241,7 @@ inline Plan make_plan(int64_t K, int64_t M, int mode) {
   p.Mpad = ceil_div(M, pad) * pad;
   const int64_t nch = std::max<int64_t>(1, ceil_div(K, KC));
   const double per_chunk = (double)p.ns * p.Mpad * FC * p.es;
-  int64_t g = (int64_t)((double)g_group_bytes / per_chunk);
+  int64_t g = (int64_t)((double)g_group_bytes.load(std::memory_order_relaxed) / per_chunk);
   g = std::max<int64_t>(1, std::min<int64_t>(g, nch));
   p.group = (int)g;
   p.ldx = (int64_t)p.group * FC;
@@ -1259,6 +1259,7 @@ int run(const double* noise, const double* uv, const double* grid, int64_t K, in
   double* sc = ar.take<double>(p.group);
   int* tiles = nullptr;
   int64_t nt8 = 0;
+  [[maybe_unused]] const int layout = g_i8_layout.load(std::memory_order_relaxed);
   if (is_i8(MODE)) {
     static thread_local std::vector<int> tl;
     build_i8_tiles(p.Mpad, tl);
@@ -1299,7 +1300,7 @@ int run(const double* noise, const double* uv, const double* grid, int64_t K, in
       constexpr int NSL = Mode<MODE>::NS, NGL = Mode<MODE>::NG;
       const dim3 g8((unsigned)(8 * ((nt8 + 7) / 8)));
       const int8_t* X8 = reinterpret_cast<const int8_t*>(X);
-      switch (g_i8_layout >= 0 ? g_i8_layout : default_i8_variant(NSL)) {
+      switch (layout >= 0 ? layout : default_i8_variant(NSL)) {
         case 1: hipLaunchKernelGGL((syrk_i8_kernel<NSL, NGL, 1>), g8, dim3(256), 0, st, X8, M, p.Mpad, p.ldx, ng, sc, W, nt8, tiles); break;
         case 2: hipLaunchKernelGGL((syrk_i8_kernel<NSL, NGL, 2>), g8, dim3(256), 0, st, X8, M, p.Mpad, p.ldx, ng, sc, W, nt8, tiles); break;
         case 3: hipLaunchKernelGGL((syrk_i8_kernel<NSL, NGL, 3>), g8, dim3(256), 0, st, X8, M, p.Mpad, p.ldx, ng, sc, W, nt8, tiles); break;
@@ -1397,7 +1398,7 @@ extern "C" int aj_w_tilde_curvature_interferometer_from(const double* noise_map_
 
 extern "C" int aj_w_tilde_set_variant(int variant) {
   AJ_REQUIRE(variant >= -1 && variant < wt::I8_NLAYOUT, "variant %d outside [-1, %d)", variant, wt::I8_NLAYOUT);
-  wt::g_i8_layout = variant;
+  wt::g_i8_layout.store(variant, std::memory_order_relaxed);
   return AJ_OK;
 }
 
@@ -1405,6 +1406,6 @@ extern "C" int aj_w_tilde_variant_count(void) { return wt::I8_NLAYOUT; }
 
 extern "C" int aj_w_tilde_set_group_bytes(size_t bytes) {
   AJ_REQUIRE(bytes == 0 || bytes >= (size_t(1) << 20), "group bytes %zu below 1 MiB", bytes);
-  wt::g_group_bytes = bytes ? bytes : wt::GROUP_BYTES_DEFAULT;
+  wt::g_group_bytes.store(bytes ? bytes : wt::GROUP_BYTES_DEFAULT, std::memory_order_relaxed);
   return AJ_OK;
 }

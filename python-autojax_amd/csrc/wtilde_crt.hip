@@ -815,12 +815,13 @@ inline Plan make_plan(int64_t M, int64_t K, int kc, int nm) {
 namespace wt {
 
 // Residue SYRK kernel (0 = syrk_pipe8_kernel<1>; 1 = syrk_pipe8_kernel<0>; 2 = syrk_ring8_kernel;
-// 3 = syrk_sk_kernel<1> + fixup_kernel, the default) and visibilities per chunk; process-wide tuning.
-int g_crt_variant = 3;
-int g_crt_kc = crt::KC_DEFAULT;
+// 3 = syrk_sk_kernel<1> + fixup_kernel, the default) and visibilities per chunk; process-wide tuning,
+// atomic so that a configure call may race a build on another thread (each build reads them once).
+std::atomic<int> g_crt_variant{3};
+std::atomic<int> g_crt_kc{crt::KC_DEFAULT};
 
 // Visibilities per chunk for nm moduli: the configured chunk, capped by the modulus product's range.
-inline int crt_kc(int nm) { return std::min(g_crt_kc, crt::kc_max_for(nm)); }
+inline int crt_kc(int nm) { return std::min(g_crt_kc.load(std::memory_order_relaxed), crt::kc_max_for(nm)); }
 
 size_t crt_workspace_bytes(int64_t M, int64_t K, int nm) { return crt::make_plan(M, K, crt_kc(nm), nm).bytes; }
 
@@ -832,6 +833,8 @@ int crt_accumulate(const double* noise, const double* uv, const double* grid, co
     set_error("int8crt: %d moduli (11 or 12 supported)", nm);
     return AJ_EINVAL;
   }
+  // the tuning read once: the schedule built below and the kernel launched must agree
+  const int variant = g_crt_variant.load(std::memory_order_relaxed);
   const int kc = crt_kc(nm), nkb = kc / 32;
   const Plan p = make_plan(M, K, kc, nm);
   const int64_t nch = std::max<int64_t>(1, ceil_div(K, kc));
@@ -869,7 +872,7 @@ int crt_accumulate(const double* noise, const double* uv, const double* grid, co
   int G = 0;
   SkSched* sk[2] = {nullptr, nullptr};
   static thread_local SkSched skt[2];
-  if (g_crt_variant == 3) {
+  if (variant == 3) {
     int dev = 0, cus = 0;
     AJ_CHECK_HIP(hipGetDevice(&dev));
     AJ_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -916,7 +919,7 @@ int crt_accumulate(const double* noise, const double* uv, const double* grid, co
     timer_end(AJ_TIMER_WT_FEATURES, st);
     AJ_LAUNCH_CHECK();
     timer_begin(AJ_TIMER_WT_SYRK, st);
-    switch (g_crt_variant) {
+    switch (variant) {
       case 1: hipLaunchKernelGGL(syrk_pipe8_kernel<0>, sg, dim3(512), 0, st, X, plane, nkb, nst, nt, per, tiles, cst, R); break;
       case 2: hipLaunchKernelGGL(syrk_ring8_kernel, sg, dim3(512), 0, st, X, plane, nkb, nst, nt, per, tiles, cst, R); break;
       case 3: {
@@ -961,8 +964,8 @@ extern "C" int aj_w_tilde_crt_configure(int variant, int64_t chunk_visibilities)
                   aj::crt::KC_MAX);
     return AJ_EINVAL;
   }
-  aj::wt::g_crt_variant = variant < 0 ? 3 : variant;
-  aj::wt::g_crt_kc = chunk_visibilities ? (int)chunk_visibilities : aj::crt::KC_DEFAULT;
+  aj::wt::g_crt_variant.store(variant < 0 ? 3 : variant, std::memory_order_relaxed);
+  aj::wt::g_crt_kc.store(chunk_visibilities ? (int)chunk_visibilities : aj::crt::KC_DEFAULT, std::memory_order_relaxed);
   return AJ_OK;
 }
 
