@@ -670,10 +670,10 @@ inline void build_tiles(int64_t nI, std::vector<int>& out) {
 }
 
 // Schedule of syrk_sk_kernel: the ntiles * nm units of `pairs` stage pairs each dealt to G workgroups
-// (G a multiple of 8; workgroup w runs on XCD w % 8).  XCD x owns the units [x U / 8, (x + 1) U / 8)
-// of the modulus-major order (unit g = modulus g / ntiles, tile g % ntiles; U = nm ntiles); its C = G / 8
-// workgroups take them in rounds of C consecutive units, so a round is a compact run of tiles of one
-// residue plane (two at a modulus boundary) and its units share row panels in the XCD's L2.  The
+// (G a multiple of 8; workgroup w runs on XCD w % 8).  XCD x owns the tiles [x ntiles / 8,
+// (x + 1) ntiles / 8) and their units in modulus-major order; its C = G / 8 workgroups take them in
+// rounds of C consecutive units (so the units in flight on an XCD share row panels in its L2, and all
+// XCDs read the same residue plane at a time, which the memory-side cache holds for all of them).  The
 // units left over after the last full round of every XCD are then dealt, as runs of stage pairs in
 // XCD order, so that every workgroup ends at the same stage count (to one pair): a unit covered by
 // one run is written whole, a unit cut between runs is written as partial residues to consecutive
@@ -695,19 +695,18 @@ inline void build_sk(int ntiles, int nm, int pairs, int G, SkSched& S) {
     int t, mi;
   };
   std::vector<Unit> left;
-  const int64_t U = (int64_t)nm * ntiles;
   for (int x = 0; x < 8; ++x) {
-    const int64_t g0 = x * U / 8, nx = (x + 1) * U / 8 - g0;
-    const int64_t F = nx / C;
+    const int t0 = (int)((int64_t)x * ntiles / 8), nx = (int)((int64_t)(x + 1) * ntiles / 8) - t0;
+    const int U = nm * nx, F = U / C;
     for (int j = 0; j < C; ++j) {
       const int w = j * 8 + x;
-      for (int64_t r = 0; r < F; ++r) {
-        const int64_t g = g0 + r * C + j;
-        per[w].push_back(SkSeg{(int)(g % ntiles), (int)(g / ntiles), 0, 2 * pairs, -1, {0, 0, 0}});
+      for (int r = 0; r < F; ++r) {
+        const int u = r * C + j;
+        per[w].push_back(SkSeg{t0 + u % nx, u / nx, 0, 2 * pairs, -1, {0, 0, 0}});
       }
-      base[w] = F * pairs;
+      base[w] = (int64_t)F * pairs;
     }
-    for (int64_t g = g0 + F * C; g < g0 + nx; ++g) left.push_back(Unit{(int)(g % ntiles), (int)(g / ntiles)});
+    for (int u = F * C; u < U; ++u) left.push_back(Unit{t0 + u % nx, u / nx});
   }
   // leftover pairs to each workgroup: fill every workgroup up to the common level H - 1, then one more
   // pair to as many workgroups below H as needed

@@ -205,6 +205,55 @@ def test_w_tilde_crt_persistent_schedule(gpu):
         _lib.call("aj_w_tilde_crt_configure", -1, 0)
 
 
+def test_w_tilde_concurrent_host_threads(gpu):
+    """Four host threads building W-tilde at the same time (int8crt11, int8crt, fp64, int8x4), each on
+    its own stream with its own workspace, two builds each: every build equals the one made alone, bit
+    for bit (the persistent SYRK's schedule tables and the other host-side scratch are per thread)."""
+    import threading
+
+    import torch
+
+    from autojax_amd import _lib
+    from autojax_amd import device as dv
+    from autojax_amd import evidence as E
+
+    rng = np.random.default_rng(12)
+    grid, _ = ds.grid_for_pixel_count(1500, 0.05)
+    K = 20000
+    uv = ds.uv_loguniform(rng, K, 1e3, 5e5)
+    _, noise = ds.visibilities(rng, K, 0.5, 2.0)
+    n_d, uv_d, g_d = dv.to_device(np.ascontiguousarray(noise.real)), dv.to_device(uv), dv.to_device(grid)
+    M = grid.shape[0]
+    precs = ("int8crt11", "int8crt", "fp64", "int8x4")
+    alone = {p: E.w_tilde(n_d, uv_d, g_d, p).cpu().numpy() for p in precs}
+    torch.cuda.synchronize()
+    out, errs = {}, []
+
+    def work(p):
+        try:
+            s = torch.cuda.Stream()
+            with torch.cuda.stream(s):
+                ws = dv.workspace(_lib.query("aj_w_tilde_workspace_bytes", K, M, _lib.PRECISIONS[p]))
+                Ws = [dv.empty((M, M)) for _ in range(2)]
+                for W in Ws:
+                    E.w_tilde(n_d, uv_d, g_d, p, out=W, workspace=ws)
+                s.synchronize()
+                out[p] = [W.cpu().numpy() for W in Ws]
+        except Exception as e:  # reported below, from the main thread
+            errs.append((p, repr(e)))
+
+    threads = [threading.Thread(target=work, args=(p,)) for p in precs]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errs, errs
+    for p in precs:
+        for W in out[p]:
+            print(f"{p}: max |dW| vs alone {np.abs(W - alone[p]).max():.3e}", flush=True)
+            np.testing.assert_array_equal(W, alone[p])
+
+
 def test_w_tilde_run_to_run_bitwise(gpu, cfg1_small):
     from autojax_amd import api
 
