@@ -657,52 +657,70 @@ __global__ void chol_finish_kernel(const SampleDesc* __restrict__ sd, int nmat, 
 }
 
 constexpr int SB = 64;  // row block of the triangular solves
+constexpr int SOLVE_THREADS = 1024;  // sixteen waves per sample
+// largest sample the solve kernel takes: its solution vector (FP64, dynamic LDS) beside the static
+// arrays (reduction, 64 x 65 diagonal block, 16 x 64 partial sums) in the 160 KB of a CU
+constexpr int SOLVE_S_MAX =
+    (int)((160 * 1024 - sizeof(double) * (SOLVE_THREADS + SB * (SB + 1) + (SOLVE_THREADS / 64) * SB)) / sizeof(double));
 
 // ---------------------------------------------------------------------------
 // Solves L y = D, L^T s = y and the log-evidence, one workgroup per sample.
 //   logL = -1/2 (chi.chi - s.D + log|A| - log|H| + noise_norm)   (numba.py:852-857)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) solve_loglike_kernel(const SampleDesc* __restrict__ sd,
-                                                            const double* __restrict__ mats,
-                                                            const double* __restrict__ D,
-                                                            const double* __restrict__ chi2_nn,
-                                                            const double* __restrict__ logdet,
-                                                            double* __restrict__ recon, double* __restrict__ logl) {
+__global__ void __launch_bounds__(SOLVE_THREADS) solve_loglike_kernel(const SampleDesc* __restrict__ sd,
+                                                                      const double* __restrict__ mats,
+                                                                      const double* __restrict__ D,
+                                                                      const double* __restrict__ chi2_nn,
+                                                                      const double* __restrict__ logdet,
+                                                                      double* __restrict__ recon,
+                                                                      double* __restrict__ logl) {
+  constexpr int NW = SOLVE_THREADS / 64;  // waves
   extern __shared__ double y[];
-  __shared__ double red[256];
+  __shared__ double red[SOLVE_THREADS];
   __shared__ double Ld[SB][SB + 1];
-  __shared__ double part[4][SB];
+  __shared__ double part[NW][SB];
   const SampleDesc d = sd[blockIdx.x];
   const double* L = mats + d.mat_off;
   const int64_t n = d.S;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  for (int64_t i = tid; i < n; i += 256) y[i] = D[d.s_off + i];
+  for (int64_t i = tid; i < n; i += SOLVE_THREADS) y[i] = D[d.s_off + i];
   __syncthreads();
   // Blocked substitution over 64-row blocks: the off-diagonal part of each block is a GEMV on
-  // contiguous row segments of L (all four waves), the 64 x 64 diagonal block is solved from LDS by
-  // one wave with the solved value broadcast by a lane shuffle.
+  // contiguous row segments of L (all sixteen waves, eight 512-byte row loads in flight per wave, so
+  // that one workgroup keeps enough of L in flight to stream it), the 64 x 64 diagonal block is solved
+  // from LDS by one wave with the solved value broadcast by a lane shuffle.
   // forward, L y = D: y_blk -= L[blk, 0:k0] y[0:k0]; then the diagonal block
   for (int64_t k0 = 0; k0 < n; k0 += SB) {
     const int nb = (int)std::min<int64_t>(SB, n - k0);
-    for (int e = tid; e < SB * SB; e += 256) {
+    for (int e = tid; e < SB * SB; e += SOLVE_THREADS) {
       const int r = e / SB, c = e % SB;
       Ld[r][c] = (r < nb && c <= r) ? L[(k0 + r) * n + k0 + c] : 0.0;
     }
-    // wave w: rows w, w+4, ..., four rows at a time (four independent loads in flight per lane)
-    for (int r0 = wave; r0 < nb; r0 += 16) {
-      double a[4] = {0.0, 0.0, 0.0, 0.0};
+    // wave w: rows w, w + 16, w + 32, w + 48; per row two partial sums (j and j + 64 of each 128-column
+    // step), added in that order
+    {
+      double a[4] = {0.0, 0.0, 0.0, 0.0}, b[4] = {0.0, 0.0, 0.0, 0.0};
       const double* Li[4];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) Li[q] = L + (k0 + std::min(r0 + 4 * q, nb - 1)) * n;
-      for (int64_t j = lane; j < k0; j += 64) {
+      for (int q = 0; q < 4; ++q) Li[q] = L + (k0 + std::min(wave + NW * q, nb - 1)) * n;
+      int64_t j = lane;
+      for (; j + 64 < k0; j += 128) {
+        const double yj = y[j], yk = y[j + 64];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          a[q] = fma(Li[q][j], yj, a[q]);
+          b[q] = fma(Li[q][j + 64], yk, b[q]);
+        }
+      }
+      if (j < k0) {
         const double yj = y[j];
 #pragma unroll
         for (int q = 0; q < 4; ++q) a[q] = fma(Li[q][j], yj, a[q]);
       }
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const double t = wave_sum(a[q]);
-        if (lane == 0 && r0 + 4 * q < nb) part[0][r0 + 4 * q] = t;
+        const double t = wave_sum(a[q] + b[q]);
+        if (lane == 0 && wave + NW * q < nb) part[0][wave + NW * q] = t;
       }
     }
     __syncthreads();
@@ -717,32 +735,35 @@ __global__ void __launch_bounds__(256) solve_loglike_kernel(const SampleDesc* __
     }
     __syncthreads();
   }
-  // backward, L^T s = y: y_blk -= L[k0+nb:n, blk]^T s[k0+nb:n] (four row-interleaved partial sums,
-  // added in a fixed order); then the transposed diagonal block
+  // backward, L^T s = y: y_blk -= L[k0+nb:n, blk]^T s[k0+nb:n] (per wave four row-interleaved partial
+  // sums, the sixteen waves' sums added in a fixed order); then the transposed diagonal block
   for (int64_t k0 = ((n - 1) / SB) * SB; k0 >= 0; k0 -= SB) {
     const int nb = (int)std::min<int64_t>(SB, n - k0);
-    for (int e = tid; e < SB * SB; e += 256) {
+    for (int e = tid; e < SB * SB; e += SOLVE_THREADS) {
       const int r = e / SB, c = e % SB;
       Ld[r][c] = (r < nb && c <= r) ? L[(k0 + r) * n + k0 + c] : 0.0;
     }
-    // thread (wave w, lane c): column k0 + c over rows j = k0 + nb + w, +4, ..., in four interleaved
-    // accumulators (four independent loads in flight), combined in a fixed order
+    // thread (wave w, lane c): column k0 + c over rows j = k0 + nb + w, + 16, ..., in four interleaved
+    // accumulators
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
     if (lane < nb) {
       const double* col = L + k0 + lane;
       int64_t j = k0 + nb + wave;
-      for (; j + 12 < n; j += 16) {
+      for (; j + 3 * NW < n; j += 4 * NW) {
         a0 = fma(col[j * n], y[j], a0);
-        a1 = fma(col[(j + 4) * n], y[j + 4], a1);
-        a2 = fma(col[(j + 8) * n], y[j + 8], a2);
-        a3 = fma(col[(j + 12) * n], y[j + 12], a3);
+        a1 = fma(col[(j + NW) * n], y[j + NW], a1);
+        a2 = fma(col[(j + 2 * NW) * n], y[j + 2 * NW], a2);
+        a3 = fma(col[(j + 3 * NW) * n], y[j + 3 * NW], a3);
       }
-      for (; j < n; j += 4) a0 = fma(col[j * n], y[j], a0);
+      for (; j < n; j += NW) a0 = fma(col[j * n], y[j], a0);
     }
     part[wave][lane] = (a0 + a1) + (a2 + a3);
     __syncthreads();
     if (wave == 0) {
-      double v = lane < nb ? y[k0 + lane] - (((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane]) : 0.0;
+      double t = 0.0;
+#pragma unroll
+      for (int w = 0; w < NW; ++w) t += part[w][lane];
+      double v = lane < nb ? y[k0 + lane] - t : 0.0;
       for (int j = nb - 1; j >= 0; --j) {
         if (lane == j) v = v / Ld[j][j];
         const double sj = __shfl(v, j);
@@ -753,13 +774,13 @@ __global__ void __launch_bounds__(256) solve_loglike_kernel(const SampleDesc* __
     __syncthreads();
   }
   double acc = 0.0;
-  for (int64_t i = tid; i < n; i += 256) {
+  for (int64_t i = tid; i < n; i += SOLVE_THREADS) {
     acc += y[i] * D[d.s_off + i];
     if (recon) recon[d.s_off + i] = y[i];
   }
   red[tid] = acc;
   __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
+  for (int o = SOLVE_THREADS / 2; o > 0; o >>= 1) {
     if (tid < o) red[tid] += red[tid + o];
     __syncthreads();
   }
@@ -901,6 +922,8 @@ int batch_pipeline(const WSrc w_tilde, int64_t M, const double* dirty_image, con
     ptr_off += S + 1;
     maxS = std::max(maxS, S);
   }
+  AJ_REQUIRE(maxS <= SOLVE_S_MAX, "a sample of %lld source pixels exceeds the solve kernel's LDS (%d at most)",
+             (long long)maxS, SOLVE_S_MAX);
   AJ_CHECK_HIP(hipMemcpyAsync(desc, hd.data(), sizeof(SampleDesc) * n, hipMemcpyHostToDevice, st));
   static thread_local std::vector<int> ho;
   ho.resize(n);
@@ -1000,7 +1023,9 @@ int batch_pipeline(const WSrc w_tilde, int64_t M, const double* dirty_image, con
   timer_end(AJ_TIMER_CHOL, st);
   AJ_LAUNCH_CHECK();
   timer_begin(AJ_TIMER_SOLVE, st);
-  hipLaunchKernelGGL(solve_loglike_kernel, dim3((unsigned)n), dim3(256), sizeof(double) * maxS, st, desc, mats, D,
+  AJ_CHECK_HIP(hipFuncSetAttribute((const void*)solve_loglike_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)(sizeof(double) * maxS)));
+  hipLaunchKernelGGL(solve_loglike_kernel, dim3((unsigned)n), dim3(SOLVE_THREADS), sizeof(double) * maxS, st, desc, mats, D,
                      chi2_nn, logdet, recon, logl);
   timer_end(AJ_TIMER_SOLVE, st);
   AJ_LAUNCH_CHECK();
