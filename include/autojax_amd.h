@@ -62,6 +62,13 @@ int aj_abi_version(void);
 int aj_timing_enable(int on);
 int aj_timing_read(int timer, double* total_ms, int64_t* count);
 
+/* Device characterisation (no reference counterpart): register-only MFMA loops giving the matrix
+ * units' achievable rate on this device.  kind 0: v_mfma_i32_16x16x64_i8 on 32 accumulators per wave
+ * (out: int64 per thread); kind 1: v_mfma_f64_16x16x4f64 on 16 accumulators per wave (out: double per
+ * thread).  Operands are small integers, so out for iters iterations is exactly iters times out for one.
+ * workgroups of 512 threads, iters <= 2^18; MFMAs per launch: workgroups * 8 * iters * (32 or 16). */
+int aj_mfma_peak_probe(int kind, int64_t workgroups, int64_t iters, void* out, void* stream);
+
 /* numba.py:155-207 w_tilde_curvature_interferometer_from(noise_map_real f8[K],
  * uv_wavelengths f8[K,2], grid_radians_slim f8[M,2]) -> f8[M,M].
  * Writes the full symmetric matrix w_tilde[M*M]. */

@@ -30,6 +30,7 @@ SIGNATURES = {
     "aj_abi_version": (_INT, []),
     "aj_timing_enable": (_INT, [_INT]),
     "aj_timing_read": (_INT, [_INT, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    "aj_mfma_peak_probe": (_INT, [_INT, _I, _I, _P, _P]),
     "aj_w_tilde_set_variant": (_INT, [_INT]),
     "aj_w_tilde_variant_count": (_INT, []),
     "aj_w_tilde_set_group_bytes": (_INT, [_S]),

@@ -5,6 +5,9 @@
     python -m autojax_amd.kernel_bench copy --mib 1024
     python -m autojax_amd.kernel_bench intmm --M 8192
     python -m autojax_amd.kernel_bench mapper --M 5000 --samples 64
+    python -m autojax_amd.kernel_bench mfma
+    python -m autojax_amd.kernel_bench fill --mib 1024
+    python -m autojax_amd.kernel_bench vendor --M 5120 --K 32768
 
 ``wtilde`` builds W-tilde on synthetic data of the batching workload's
 generator; ``batch`` evaluates the batched log-evidence of a sample batch on
@@ -12,8 +15,12 @@ one W-tilde; ``copy`` runs a plain device copy of known size, the reference
 for scaling the FETCH_SIZE / WRITE_SIZE counters; ``intmm`` times the vendor
 library's int8 GEMM (torch._int_mm, M x M x M) as the device's achievable
 int8 matrix rate; ``mapper`` builds the source meshes and mappings of a sample batch
-(S ~ U{300..1500} vertices each, the M image pixels as positions) on the device.  Each prints one JSON line
-with the per-call time measured with HIP events.
+(S ~ U{300..1500} vertices each, the M image pixels as positions) on the device; ``mfma`` runs the
+library's register-only MFMA loops (int8 16x16x64 and FP64 16x16x4, aj_mfma_peak_probe) as the device's
+achievable matrix rates; ``fill`` writes a buffer of known size (achievable write bandwidth); ``vendor``
+times the vendor libraries on the Gram shape C = X X^T, X of M rows and K columns: int8 (torch._int_mm),
+fp8 (torch._scaled_mm), FP64 (torch matmul) full products and rocBLAS dsyrk (the lower triangle, FP64).
+Each prints one JSON line per measurement with the per-call time measured with HIP events.
 """
 
 from __future__ import annotations
@@ -27,13 +34,16 @@ import numpy as np
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["wtilde", "batch", "copy", "intmm", "mapper"])
+    ap.add_argument("what", choices=["wtilde", "batch", "copy", "intmm", "mapper", "mfma", "fill", "vendor"])
     ap.add_argument("--M", type=int, default=5000)
     ap.add_argument("--K", type=int, default=131072)
     ap.add_argument("--precision", default="int8x4")
     ap.add_argument("--samples", type=int, default=64)
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--mib", type=int, default=1024)
+    ap.add_argument("--workgroups", type=int, default=1024, help="mfma: workgroups of 512 threads")
+    ap.add_argument("--iters", type=int, default=0, help="mfma: loop iterations (0 = 5000 int8, 2500 fp64)")
+    ap.add_argument("--kinds", default="int8,fp64", help="mfma: probes to run (int8, fp64)")
     ap.add_argument("--variant", type=int, default=None, help="integer SYRK kernel variant (LDS layout)")
     ap.add_argument("--chunk", type=int, default=0, help="int8crt: visibilities per exact chunk (0 = default)")
     ap.add_argument("--check", action="store_true", help="compare W-tilde with kernel variant 0 (bitwise)")
@@ -72,6 +82,66 @@ def main(argv=None):
         ms = timed(lambda: y.copy_(x), a.reps)
         print(json.dumps({"kernel": "copy", "bytes_read": n * 8, "bytes_written": n * 8, "ms": ms,
                           "GBps": 2 * n * 8 / ms / 1e6}), flush=True)
+        return
+    if a.what == "fill":
+        n = a.mib * (1 << 20) // 8
+        x = torch.empty(n, dtype=torch.float64, device="cuda")
+        ms = timed(lambda: x.fill_(1.0), a.reps)
+        print(json.dumps({"kernel": "fill", "bytes_written": n * 8, "ms": ms, "GBps": n * 8 / ms / 1e6}), flush=True)
+        return
+    if a.what == "mfma":
+        wg = a.workgroups
+        for kind, name, ops_per, dtype, chains in (
+                (0, "int8 v_mfma_i32_16x16x64_i8", 16 * 16 * 64 * 2, torch.int64, 32),
+                (1, "fp64 v_mfma_f64_16x16x4f64", 16 * 16 * 4 * 2, torch.float64, 16)):
+            if ("int8", "fp64")[kind] not in a.kinds.split(","):
+                continue
+            iters = a.iters or (5000 if kind == 0 else 2500)
+            one = torch.empty(wg * 512, dtype=dtype, device="cuda")
+            _lib.call("aj_mfma_peak_probe", kind, wg, 1, one.data_ptr(), dv.stream_ptr())
+            out = torch.empty(wg * 512, dtype=dtype, device="cuda")
+            ms = timed(lambda: _lib.call("aj_mfma_peak_probe", kind, wg, iters, out.data_ptr(), dv.stream_ptr()), a.reps)
+            ops = wg * 8 * iters * chains * ops_per
+            exact = bool((out == one * iters).all().item())
+            print(json.dumps({"kernel": "mfma_probe", "instruction": name, "workgroups": wg, "iters": iters, "ms": ms,
+                              "T_ops_per_s": ops / ms / 1e9, "exact": exact}), flush=True)
+        return
+    if a.what == "vendor":
+        import ctypes as C
+
+        n, k = a.M, a.K
+        xi = torch.randint(-128, 128, (n, k), dtype=torch.int8, device="cuda")
+        ms = timed(lambda: torch._int_mm(xi, xi.t()), a.reps)
+        print(json.dumps({"kernel": "torch._int_mm", "M": n, "K": k, "ms": ms, "T_ops_per_s_full": 2.0 * n * n * k / ms / 1e9}),
+              flush=True)
+        x8 = xi.to(torch.float16).to(torch.float8_e4m3fn)
+        one = torch.ones((), device="cuda")
+        ms = timed(lambda: torch._scaled_mm(x8, x8.t(), scale_a=one, scale_b=one, out_dtype=torch.bfloat16), a.reps)
+        print(json.dumps({"kernel": "torch._scaled_mm fp8", "M": n, "K": k, "ms": ms,
+                          "T_flops_per_s_full": 2.0 * n * n * k / ms / 1e9}), flush=True)
+        del xi, x8
+        xd = torch.rand((n, k), dtype=torch.float64, device="cuda")
+        ms = timed(lambda: xd @ xd.t(), a.reps)
+        print(json.dumps({"kernel": "torch matmul fp64", "M": n, "K": k, "ms": ms,
+                          "T_flops_per_s_full": 2.0 * n * n * k / ms / 1e9}), flush=True)
+        # rocBLAS dsyrk: column-major C (n x n, lower) = A^T A with A = the row-major X seen as K x n
+        rb = C.CDLL("librocblas.so")
+        handle = C.c_void_p()
+        assert rb.rocblas_create_handle(C.byref(handle)) == 0
+        assert rb.rocblas_set_stream(handle, C.c_void_p(dv.stream_ptr())) == 0
+        cd = torch.empty((n, n), dtype=torch.float64, device="cuda")
+        alpha, beta = C.c_double(1.0), C.c_double(0.0)
+        fill_lower, op_transpose = 122, 112
+
+        def syrk():
+            st = rb.rocblas_dsyrk(handle, fill_lower, op_transpose, n, k, C.byref(alpha), C.c_void_p(xd.data_ptr()), k,
+                                  C.byref(beta), C.c_void_p(cd.data_ptr()), n)
+            assert st == 0, st
+
+        ms = timed(syrk, a.reps)
+        rb.rocblas_destroy_handle(handle)
+        print(json.dumps({"kernel": "rocblas_dsyrk lower", "M": n, "K": k, "ms": ms,
+                          "T_flops_per_s_triangle": 1.0 * n * n * k / ms / 1e9}), flush=True)
         return
     if a.what == "intmm":
         n = a.M

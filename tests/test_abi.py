@@ -97,6 +97,10 @@ def test_new_entry_points_reject_bad_arguments_without_gpu():
         _lib.call("aj_delaunay_mesh_batched", 1, 1, offs.ctypes.data, 1, 32, 1, 1, 1, 1, 1, None)
     with pytest.raises(ValueError):
         _lib.call("aj_delaunay_locate_batched", 1, 1, 1, 1, 1, 1, 0, 0, 1, 1, None)
+    # MFMA probe: unknown kind, no workgroups, too many iterations
+    for args in ((2, 1, 1), (0, 0, 1), (1, 1, 1 << 19)):
+        with pytest.raises(ValueError):
+            _lib.call("aj_mfma_peak_probe", *args, 1, None)
     # CRT constants: host-only query, 11 or 12 moduli
     buf = [np.zeros(12) for _ in range(3)] + [np.zeros(4)]
     with pytest.raises(ValueError):
