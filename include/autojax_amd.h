@@ -92,6 +92,13 @@ int aj_w_tilde_set_group_bytes(size_t bytes);
  * y_i = (P/m_i) ((P/m_i)^-1 mod m_i) mod P split as y_hi[i] (a multiple of 2^53) + y_lo[i] (an integer
  * below 2^53); pq = {P_hi, P_lo, 1/P, Q} with Q the largest feature integer of the mode. */
 int aj_crt_constants(int nmod, double* moduli, double* y_hi, double* y_lo, double* pq);
+/* The persistent residue SYRK's schedule (host-only query for checks, no reference counterpart): for
+ * ntiles tiles, nm moduli, `pairs` stage pairs per (tile, modulus) unit and G workgroups (a multiple of
+ * 8, at most 256), the segments each workgroup runs (segs: 8 ints each, {tile, modulus, first stage,
+ * stages, slot or -1, 0, 0, 0}; beg: G + 1 offsets), the units split between workgroups (fix: 4 ints
+ * each, {tile, modulus, first slot, slots}) and counts = {segments, split units, slots}. */
+int aj_crt_sk_schedule(int ntiles, int nm, int pairs, int G, int* segs, int64_t seg_cap, int* beg, int* fix,
+                       int64_t fix_cap, int64_t* counts);
 /* Tuning of AJ_PREC_INT8CRT and AJ_PREC_INT8CRT11 (process-wide): residue SYRK kernel variant 0..3
  * (-1 = default 3, the persistent form with one workgroup per CU; all compute identical residues) and visibilities per exact integer chunk (a
  * multiple of 4096 up to 32768, capped at 16384 with 11 moduli; 0 = default 16384).  The chunk size

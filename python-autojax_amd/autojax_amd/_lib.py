@@ -35,6 +35,7 @@ SIGNATURES = {
     "aj_w_tilde_variant_count": (_INT, []),
     "aj_w_tilde_set_group_bytes": (_INT, [_S]),
     "aj_crt_constants": (_INT, [_INT, _P, _P, _P, _P]),
+    "aj_crt_sk_schedule": (_INT, [_INT, _INT, _INT, _INT, _P, _I, _P, _P, _I, _P]),
     "aj_w_tilde_crt_configure": (_INT, [_INT, _I]),
     "aj_w_tilde_workspace_bytes": (_S, [_I, _I, _INT]),
     "aj_w_tilde_curvature_interferometer_from": (_INT, [_P, _P, _P, _I, _I, _INT, _P, _P, _S, _P]),

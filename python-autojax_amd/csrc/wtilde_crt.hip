@@ -675,9 +675,10 @@ inline void build_tiles(int64_t nI, std::vector<int>& out) {
 // rounds of C consecutive units (so the units in flight on an XCD share row panels in its L2, and all
 // XCDs read the same residue plane at a time, which the memory-side cache holds for all of them).  The
 // units left over after the last full round of every XCD are then dealt, as runs of stage pairs in
-// XCD order, so that every workgroup ends at the same stage count (to one pair): a unit covered by
-// one run is written whole, a unit cut between runs is written as partial residues to consecutive
-// slots and summed by fixup_kernel.
+// XCD order, raising the workgroups below a common level to it (to one pair): every workgroup ends at
+// the same stage count when the XCDs run the same number of full rounds; an XCD with one more full
+// round than the level ends up to one unit later.  A unit covered by one run is written whole, a unit
+// cut between runs is written as partial residues to consecutive slots and summed by fixup_kernel.
 constexpr int SK_WG_MAX = 256;  // workgroups of the persistent SYRK (one per CU on MI355X)
 
 struct SkSched {
@@ -986,5 +987,37 @@ extern "C" int aj_crt_constants(int nmod, double* moduli, double* y_hi, double* 
   pq[1] = c.p_lo;
   pq[2] = c.inv_p;
   pq[3] = aj::crt::qmax_for(nmod);
+  return AJ_OK;
+}
+
+// The persistent residue SYRK's schedule for ntiles tiles, nm moduli, `pairs` stage pairs per unit and
+// G workgroups, as crt_accumulate builds and uploads it (host-only query for checks).  segs: 8 ints per
+// segment {tile, modulus, first stage, stages, slot (-1 whole), 0, 0, 0}; beg: G + 1 offsets; fix: 4 ints
+// per split unit {tile, modulus, first slot, slots}; counts = {segments, split units, slots}.  Fails when
+// a table exceeds its capacity (seg_cap segments, fix_cap split units) or the kernel's table bounds.
+extern "C" int aj_crt_sk_schedule(int ntiles, int nm, int pairs, int G, int* segs, int64_t seg_cap, int* beg,
+                                  int* fix, int64_t fix_cap, int64_t* counts) {
+  using namespace aj::crt;
+  if (!segs || !beg || !fix || !counts || ntiles < 1 || ntiles > (1 << 20) || (nm != 11 && nm != 12) || pairs < 1 ||
+      pairs > (1 << 16) || G < 8 || G > SK_WG_MAX || G % 8 != 0) {
+    aj::set_error("aj_crt_sk_schedule: bad arguments ntiles=%d nm=%d pairs=%d G=%d", ntiles, nm, pairs, G);
+    return AJ_EINVAL;
+  }
+  SkSched S;
+  build_sk(ntiles, nm, pairs, G, S);
+  const int64_t units = (int64_t)nm * ntiles;
+  if ((int64_t)S.segs.size() > seg_cap || (int64_t)S.fix.size() > fix_cap ||
+      (int64_t)S.segs.size() > sk_seg_cap(units) || (int64_t)S.nslots > sk_slot_cap(units)) {
+    aj::set_error("aj_crt_sk_schedule: %zu segments, %zu split units, %d slots exceed the tables", S.segs.size(),
+                  S.fix.size(), S.nslots);
+    return AJ_EINVAL;
+  }
+  static_assert(sizeof(SkSeg) == 8 * sizeof(int) && sizeof(SkFix) == 4 * sizeof(int), "table layout");
+  std::memcpy(segs, S.segs.data(), sizeof(SkSeg) * S.segs.size());
+  std::memcpy(beg, S.beg.data(), sizeof(int) * S.beg.size());
+  if (!S.fix.empty()) std::memcpy(fix, S.fix.data(), sizeof(SkFix) * S.fix.size());
+  counts[0] = (int64_t)S.segs.size();
+  counts[1] = (int64_t)S.fix.size();
+  counts[2] = S.nslots;
   return AJ_OK;
 }

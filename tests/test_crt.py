@@ -106,3 +106,72 @@ def test_recombination_fails_with_a_wrong_basis():
     x = 123456789123456789123
     assert abs(int(_recombine(x, mi, yh, yl, pq)) - x) <= 2**20
     assert abs(int(_recombine(x, mi, yh2, yl2, pq)) - x) > 2**40
+
+
+def _sk_schedule(ntiles, nm, pairs, G):
+    from autojax_amd import _lib
+
+    units = nm * ntiles
+    seg_cap = units + 256
+    segs = np.zeros((seg_cap, 8), np.int32)
+    beg = np.zeros(G + 1, np.int32)
+    fix = np.zeros((units, 4), np.int32)
+    counts = np.zeros(3, np.int64)
+    _lib.call("aj_crt_sk_schedule", ntiles, nm, pairs, G, segs.ctypes.data, seg_cap, beg.ctypes.data,
+              fix.ctypes.data, units, counts.ctypes.data)
+    return segs[:counts[0]], beg, fix[:counts[1]], int(counts[2])
+
+
+@pytest.mark.parametrize("G", [8, 64, 256])
+@pytest.mark.parametrize("pairs", [1, 7, 256])
+@pytest.mark.parametrize("nm", [11, 12])
+@pytest.mark.parametrize("ntiles", [1, 3, 10, 55, 210, 861])
+def test_persistent_syrk_schedule(ntiles, nm, pairs, G):
+    """The persistent residue SYRK's schedule as the library builds it: every (tile, modulus) unit's
+    stage pairs covered exactly once by contiguous segments, a unit written whole or in pieces whose
+    partial-residue slots are distinct and listed for the fix-up, and the workgroups' loads within one
+    stage pair of each other (when every XCD runs the same number of full rounds)."""
+    segs, beg, fix, nslots = _sk_schedule(ntiles, nm, pairs, G)
+    t, mi, k0, ns, slot = (segs[:, c].astype(np.int64) for c in range(5))
+    assert beg[0] == 0 and beg[G] == len(segs) and np.all(np.diff(beg) >= 0)
+    assert np.all((t >= 0) & (t < ntiles) & (mi >= 0) & (mi < nm))
+    assert np.all((ns >= 2) & (ns % 2 == 0) & (k0 % 2 == 0) & (k0 + ns <= 2 * pairs))
+    whole = slot < 0
+    assert np.all(k0[whole] == 0) and np.all(ns[whole] == 2 * pairs)
+    # coverage: per unit, segments sorted by first stage start at 0, abut, and end at 2 pairs
+    unit = mi * ntiles + t
+    o = np.lexsort((k0, unit))
+    u, a, e = unit[o], k0[o], (k0 + ns)[o]
+    assert np.array_equal(np.unique(u), np.arange(nm * ntiles))
+    first = np.r_[True, u[1:] != u[:-1]]
+    last = np.r_[u[1:] != u[:-1], True]
+    assert np.all(a[first] == 0) and np.all(e[last] == 2 * pairs)
+    assert np.all(a[1:][~last[:-1]] == e[:-1][~last[:-1]])
+    # slots: the pieces' slots are 0 .. nslots-1 once each; the fix-up list names each split unit's run
+    assert np.array_equal(np.sort(slot[~whole]), np.arange(nslots))
+    unit_of_slot = np.empty(nslots, np.int64)
+    unit_of_slot[slot[~whole]] = unit[~whole]
+    assert int(fix[:, 3].sum()) == nslots and np.all(fix[:, 3] >= 2)
+    for ft, fm, s0, n in fix:
+        assert np.all(unit_of_slot[s0:s0 + n] == fm * ntiles + ft)
+    # balance: stage pairs per workgroup.  Workgroup j * 8 + x runs on XCD x, whose tiles
+    # [x ntiles / 8, (x + 1) ntiles / 8) give F_x = (their units) // (G / 8) full rounds; the leftover pairs
+    # level the rest, so the loads agree to one pair when all F_x are equal and otherwise differ by at
+    # most the difference of full rounds
+    load = np.array([ns[beg[w]:beg[w + 1]].sum() // 2 for w in range(G)])
+    assert load.sum() == nm * ntiles * pairs
+    F = [nm * ((x + 1) * ntiles // 8 - x * ntiles // 8) // (G // 8) for x in range(8)]
+    assert load.max() - load.min() <= (max(F) - min(F)) * pairs + 1
+
+
+def test_persistent_syrk_schedule_rejects_bad_arguments():
+    from autojax_amd import _lib
+
+    buf = np.zeros(4096, np.int64)
+    for ntiles, nm, pairs, G in ((10, 11, 4, 12), (10, 10, 4, 8), (0, 11, 4, 8), (10, 12, 0, 8), (10, 12, 4, 264)):
+        with pytest.raises(ValueError):
+            _lib.call("aj_crt_sk_schedule", ntiles, nm, pairs, G, buf.ctypes.data, 1, buf.ctypes.data, buf.ctypes.data,
+                      1, buf.ctypes.data)
+    with pytest.raises(ValueError, match="exceed"):  # tables too small for the schedule
+        _lib.call("aj_crt_sk_schedule", 10, 11, 4, 8, buf.ctypes.data, 1, buf.ctypes.data, buf.ctypes.data, 1,
+                  buf.ctypes.data)
