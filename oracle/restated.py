@@ -68,6 +68,36 @@ def w_tilde_curvature_interferometer_from(noise_map_real, uv_wavelengths, grid_r
     return w
 
 
+def w_tilde_rows_from(noise_map_real, uv_wavelengths, grid_radians_slim, rows, k_block=4096, threads=None):
+    """Rows `rows` of numba.py:155-207's W (the same element sum, W_ij = sum_k n_k^-2 cos(dg_ij1 u_k0 +
+    dg_ij0 u_k1), dg = 2 pi (g_i - g_j)) for sizes whose full M x M matrix the oracle cannot build in
+    seconds.  K is cut into blocks of k_block evaluated on a thread pool (numpy releases the GIL in its
+    ufuncs); the blocks' partial sums are added in block order, so the result does not depend on the
+    thread count.  Memory O(threads len(rows) M k_block)."""
+    import os
+    from concurrent.futures import ThreadPoolExecutor
+
+    if noise_map_real.shape[0] != uv_wavelengths.shape[0]:
+        raise ValueError("noise_map_real and uv_wavelengths must have the same number of visibilities")
+    g_2pi = TWO_PI * grid_radians_slim
+    rows = np.asarray(rows)
+    d0 = g_2pi[rows, None, 0] - g_2pi[None, :, 0]
+    d1 = g_2pi[rows, None, 1] - g_2pi[None, :, 1]
+    inv_n2 = np.reciprocal(np.square(noise_map_real))
+
+    def part(k0):
+        u = uv_wavelengths[k0:k0 + k_block]
+        return np.cos(d1[:, :, None] * u[None, None, :, 0] + d0[:, :, None] * u[None, None, :, 1]) @ inv_n2[k0:k0 + k_block]
+
+    starts = range(0, uv_wavelengths.shape[0], k_block)
+    nt = threads or min(16, len(os.sched_getaffinity(0)))
+    out = np.zeros((rows.size, grid_radians_slim.shape[0]))
+    with ThreadPoolExecutor(max_workers=nt) as pool:
+        for p in pool.map(part, starts):
+            out += p
+    return out
+
+
 def w_compact_curvature_interferometer_from(grid_size, noise_map_real, uv_wavelengths, pixel_scale):
     """numba.py:210-234: lattice form, N x (2N-1) unique W values."""
     N = grid_size

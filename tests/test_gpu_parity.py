@@ -113,6 +113,29 @@ def test_w_tilde_full_size_properties(gpu):
     assert torch.equal(W, W.T) and torch.equal(W, W2)
 
 
+def test_w_tilde_cfg2_rows_vs_oracle(gpu):
+    """The plain-method workload at full size (M 7,860, K 1e5, the generator of config 2): rows 0, M/2 and
+    M-1 of the FP64 W against the oracle's row sum, within 1e-10 of max|W| (the tolerance that workload
+    states), and of the mixed path (int8crt11) within its mode tolerance; both W exactly symmetric."""
+    import torch
+
+    from autojax_amd import device as dv
+    from autojax_amd import evidence as E
+
+    d = ds.config2()
+    rows = np.array([0, d.M // 2, d.M - 1])
+    want = R.w_tilde_rows_from(d.noise_map_real, d.uv_wavelengths, d.grid_radians_slim, rows, k_block=1024)
+    n_d, uv_d, g_d = dv.to_device(d.noise_map_real), dv.to_device(d.uv_wavelengths), dv.to_device(d.grid_radians_slim)
+    scale = np.abs(want).max()
+    for precision, tol in (("fp64", 1e-10), ("int8crt11", WT_TOL["int8crt11"])):
+        W = E.w_tilde(n_d, uv_d, g_d, precision)
+        torch.cuda.synchronize()
+        assert torch.equal(W, W.T)
+        err = np.abs(W[torch.from_numpy(rows).cuda()].cpu().numpy() - want).max() / scale
+        print(f"cfg2 {precision}: rows vs oracle {err:.3e} (M {d.M}, K {d.K})", flush=True)
+        assert err < tol
+
+
 def test_w_tilde_multichunk_rows(gpu):
     """K spanning several 4096-visibility chunks; oracle on a row subset at full K."""
     from autojax_amd import api

@@ -35,6 +35,21 @@ def test_w_tilde_matches_fsum_golden():
     np.testing.assert_allclose(W, g["w_tilde"], rtol=0, atol=1e-12 * np.abs(g["w_tilde"]).max())
 
 
+def test_w_tilde_rows_match_fsum_golden_and_full_oracle():
+    """The row-subset oracle against the brute-force fsum golden (every row, in blocks smaller than K)
+    and against the full-matrix oracle on the 716-pixel workload."""
+    g = np.load(GOLD / "tiny_wtilde.npz")
+    M = g["grid_radians_slim"].shape[0]
+    rows = R.w_tilde_rows_from(g["noise_map_real"], g["uv_wavelengths"], g["grid_radians_slim"], np.arange(M),
+                               k_block=5)
+    np.testing.assert_allclose(rows, g["w_tilde"], rtol=0, atol=1e-12 * np.abs(g["w_tilde"]).max())
+    d = ds.config1(K=300)
+    W = R.w_tilde_curvature_interferometer_from(d.noise_map_real, d.uv_wavelengths, d.grid_radians_slim)
+    sel = np.array([0, 5, 357, 715])
+    Wr = R.w_tilde_rows_from(d.noise_map_real, d.uv_wavelengths, d.grid_radians_slim, sel, k_block=64)
+    np.testing.assert_allclose(Wr, W[sel], rtol=0, atol=1e-12 * np.abs(W).max())
+
+
 def test_w_tilde_pairing_is_not_transposed():
     # swapping the (y, x) grid columns must change W for a generic uv set
     g = np.load(GOLD / "tiny_wtilde.npz")
