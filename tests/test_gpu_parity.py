@@ -400,6 +400,32 @@ def test_log_likelihood_cfg1_vs_oracle(gpu, cfg1_small):
     assert abs(got - want) < 1e-10 * abs(want)
 
 
+def test_log_likelihood_cfg2_vs_oracle(gpu):
+    """The plain-method workload at full size (M 7,860, K 1e5, a 1,000-vertex Delaunay source, B = 3):
+    the single-sample log-likelihood drop-in (numba.py:755-857, lambda 1) on the device against the
+    oracle's (dense T, LU solve, two slogdets) with the same FP64 W and dirty image, within 1e-10
+    relative (the workload's bound)."""
+    import torch
+
+    from autojax_amd import api
+    from autojax_amd import device as dv
+    from autojax_amd import evidence as E
+
+    d = ds.config2()
+    smp = d.samples[0]
+    n_d, uv_d, g_d = dv.to_device(d.noise_map_real), dv.to_device(d.uv_wavelengths), dv.to_device(d.grid_radians_slim)
+    W = E.w_tilde(n_d, uv_d, g_d, "fp64")
+    y = E.dirty_image(d.data, d.noise_map, d.uv_wavelengths, d.grid_radians_slim)
+    torch.cuda.synchronize()
+    args = (y.cpu().numpy(), d.data, d.noise_map, W.cpu().numpy(), smp.pix_indexes_for_sub_slim_index,
+            smp.pix_weights_for_sub_slim_index, smp.neighbors, smp.neighbors_sizes)
+    got = api.log_likelihood_function_via_w_tilde_from(*args)
+    want = R.log_likelihood_function_via_w_tilde_from(*args)
+    print(f"cfg2 logL {got:.12g} vs oracle {want:.12g} (rel {abs(got - want) / abs(want):.3e}; M {d.M}, "
+          f"S {smp.pixels})", flush=True)
+    assert abs(got - want) < 1e-10 * abs(want)
+
+
 def test_batched_log_evidence_vs_oracle(gpu, delaunay_small):
     import torch
 
