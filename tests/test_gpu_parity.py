@@ -426,6 +426,48 @@ def test_log_likelihood_cfg2_vs_oracle(gpu):
     assert abs(got - want) < 1e-10 * abs(want)
 
 
+def test_batched_log_evidence_full_size_vs_oracle(gpu):
+    """The batching workload at full size (M 5,000, K 1e6, 64 samples, S ~ U{300..1500}, lambda in
+    [0.01, 100]): the batched log-evidence from the FP64 W within 1e-9 relative, and from the mixed
+    path's W (int8crt11) within 1e-3 absolute, of the oracle's on six samples (the smallest lambda, the
+    largest and the smallest S, three more), the oracle given the FP64 W and the same dirty image."""
+    import torch
+
+    from autojax_amd import device as dv
+    from autojax_amd import evidence as E
+
+    d = ds.config5()
+    n_d, uv_d, g_d = dv.to_device(d.noise_map_real), dv.to_device(d.uv_wavelengths), dv.to_device(d.grid_radians_slim)
+    batch = E.SampleBatch(d.samples, d.M)
+    y = E.dirty_image(d.data, d.noise_map, d.uv_wavelengths, d.grid_radians_slim)
+    c2nn = E.chi2_noise_norm(d.data, d.noise_map)
+    W64 = E.w_tilde(n_d, uv_d, g_d, "fp64")
+    out = {"fp64": batch.log_evidence(W64, y, c2nn, check=True).cpu().numpy()}
+    Wm = E.w_tilde(n_d, uv_d, g_d, "int8crt11")
+    out["int8crt11"] = batch.log_evidence(Wm, y, c2nn, check=True).cpu().numpy()
+    torch.cuda.synchronize()
+    Wh, yh = W64.cpu().numpy(), y.cpu().numpy()
+    del Wm
+    lam = np.array([s.coefficient for s in d.samples])
+    S = np.array([s.pixels for s in d.samples])
+    sel = list(dict.fromkeys([int(lam.argmin()), int(S.argmax()), int(S.argmin()), 7, 30, 55]))
+    chi = d.data.view(np.float64) / d.noise_map.view(np.float64)
+    nn = R.noise_normalization_complex_from(d.noise_map)
+    for b in sel:
+        smp = d.samples[b]
+        To = R.mapping_matrix_from(smp.pix_indexes_for_sub_slim_index, smp.pix_weights_for_sub_slim_index, smp.pixels)
+        F = R.curvature_matrix_via_w_tilde_from(Wh, To)
+        D = R.data_vector_from(To, yh)
+        H = R.constant_regularization_matrix_from(smp.coefficient, smp.neighbors, smp.neighbors_sizes)
+        ldh = ev.logdet_deflated(H, smp.coefficient**2 * smp.neighbors_sizes.mean())
+        want = R.log_evidence_from_parts(F, D, chi @ chi, nn, smp.neighbors, smp.neighbors_sizes, smp.coefficient,
+                                         logdet_h=ldh)[0]
+        print(f"sample {b} S={smp.pixels} lam={smp.coefficient:.3g}: oracle {want:.12g}, fp64 "
+              f"{out['fp64'][b] - want:.2e}, int8crt11 {out['int8crt11'][b] - want:.2e}", flush=True)
+        assert abs(out["fp64"][b] - want) < 1e-9 * max(1.0, abs(want))
+        assert abs(out["int8crt11"][b] - want) < 1e-3
+
+
 def test_batched_log_evidence_vs_oracle(gpu, delaunay_small):
     import torch
 
