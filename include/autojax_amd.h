@@ -233,7 +233,7 @@ int aj_chi_squared_noise_normalization(const double* data, const double* noise_m
 /* numba.py:755-857 log_likelihood_function_via_w_tilde_from(dirty_image f8[M], data c16[K],
  * noise_map c16[K], w_tilde f8[M,M], pix_indexes i8[M,B], pix_weights f8[M,B], neighbors i8[S,P],
  * neighbors_sizes i8[S]) -> f8, coefficient 1 as at numba.py:819.  log_likelihood is a device
- * scalar; info[2] = failing pivots of A and H (0 = none). */
+ * scalar; info[2] = failing pivots of A and H (0 = none).  S at most 14272 (as the batched form). */
 size_t aj_log_likelihood_workspace_bytes(int64_t M, int64_t K, int64_t B, int64_t S, int64_t P);
 int aj_log_likelihood_function_via_w_tilde_from(const double* dirty_image, const double* data, const double* noise_map,
                                                 const double* w_tilde, const int64_t* pix_indexes_for_sub_slim_index,
@@ -254,7 +254,8 @@ int aj_log_likelihood_function_via_w_compact_from(
 
 /* Batched log-evidence: numba.py:755-857 evaluated for n samples that share one w_tilde, with an
  * explicit regularisation coefficient per sample (the reference fixes it to 1, numba.py:819).
- * Sample b has S_host[b] source pixels; mapping blocks are [n,M,B]; neighbour rows of all samples
+ * Sample b has S_host[b] source pixels (1 .. 14272: the solve holds a sample's solution in LDS; larger
+ * samples are rejected with AJ_EINVAL); mapping blocks are [n,M,B]; neighbour rows of all samples
  * are concatenated ([sum S, P], [sum S]).  chi2_noise_norm is the device pair from
  * aj_chi_squared_noise_normalization.  deflation_beta_host (may be NULL) gives per sample the
  * rank-one shift used for log|H| (0 = plain factorisation of H).  Outputs: log_evidence[n];

@@ -468,6 +468,31 @@ def test_batched_log_evidence_full_size_vs_oracle(gpu):
         assert abs(out["int8crt11"][b] - want) < 1e-3
 
 
+def test_batched_rejects_samples_beyond_the_solve_limit(gpu):
+    """A sample of 14,273 source pixels, one above what the solve kernel holds in LDS, is rejected with a
+    ValueError naming the limit (the entry point checks it before its first launch)."""
+    import torch
+
+    from autojax_amd import evidence as E
+
+    S, M = 14273, 64
+    rng = np.random.default_rng(13)
+    idx = rng.integers(0, S, (M, 1))
+    w = np.ones((M, 1))
+    nbr = np.full((S, 2), -1, np.int64)
+    nbr[1:, 0] = np.arange(S - 1)
+    nbr[:-1, 1] = np.arange(1, S)
+    nbr[0] = [1, -1]
+    sizes = np.full(S, 2, np.int64)
+    sizes[[0, -1]] = 1
+    batch = E.SampleBatch([ds.Sample(idx, w, nbr, sizes, 1.0)], M)
+    W = torch.eye(M, dtype=torch.float64, device="cuda")
+    y = torch.zeros(M, dtype=torch.float64, device="cuda")
+    c2nn = torch.zeros(2, dtype=torch.float64, device="cuda")
+    with pytest.raises(ValueError, match="14272"):
+        batch.log_evidence(W, y, c2nn)
+
+
 def test_batched_log_evidence_vs_oracle(gpu, delaunay_small):
     import torch
 
