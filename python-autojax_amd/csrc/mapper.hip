@@ -148,7 +148,16 @@ __global__ void __launch_bounds__(DT_THREADS) delaunay_kernel(const double* __re
         break;
       }
       if (nb == 0) continue;  // coincides with a vertex already in the mesh
-      if (tid == 0) {         // slot order, so that the result does not depend on the atomics' order
+      // slot order, so that the result does not depend on the atomics' order: one wave ranks the (distinct)
+      // slot numbers by lane shuffles when they fit in a wave, else one thread sorts them
+      if (nb <= 64) {
+        if (tid < 64) {
+          const int v = tid < nb ? bad[tid] : 0x7fffffff;
+          int rank = 0;
+          for (int j = 0; j < nb; ++j) rank += __shfl(v, j) < v;
+          if (tid < nb) bad[rank] = v;  // every lane has read bad[] above (one wave, program order)
+        }
+      } else if (tid == 0) {
         for (int i = 1; i < nb; ++i) {
           const int v = bad[i];
           int j = i - 1;
@@ -173,7 +182,37 @@ __global__ void __launch_bounds__(DT_THREADS) delaunay_kernel(const double* __re
         isb[i] = inner ? 0 : 1;
       }
       __syncthreads();
-      if (tid == 0) {
+      if (3 * nb <= 64) {
+        // one wave: boundary edge i (in (slot order, edge) order) takes the new triangle of its rank among
+        // the boundary edges, as the serial form below assigns them
+        if (tid < 64) {
+          const bool isbnd = tid < 3 * nb && isb[tid];
+          const unsigned long long mask = __ballot(isbnd);
+          const int m = __popcll(mask);
+          if (m != nb + 2 || s_ntri + 2 > DT_TMAX) {
+            if (tid == 0) s_err = m != nb + 2 ? ST_CAVITY_SHAPE : ST_CAVITY_SIZE;
+          } else {
+            int u = 0, v = 0, slot = 0;
+            if (isbnd) {
+              const int rank = __popcll(mask & ((1ull << tid) - 1));
+              const int t = bad[tid / 3], e = tid % 3;
+              u = tv[3 * t + e];
+              v = tv[3 * t + (e + 1) % 3];
+              slot = rank < nb ? bad[rank] : s_ntri + (rank - nb);
+            }
+            // the cavity's triangles and s_ntri are read above, before any lane writes below
+            if (isbnd) {
+              const int r0 = u == GHOST ? v : (v == GHOST ? k : u);
+              const int r1 = u == GHOST ? k : (v == GHOST ? u : v);
+              const int r2 = (u == GHOST || v == GHOST) ? GHOST : k;
+              tv[3 * slot] = r0;
+              tv[3 * slot + 1] = r1;
+              tv[3 * slot + 2] = r2;
+            }
+            if (tid == 0) s_ntri += 2;
+          }
+        }
+      } else if (tid == 0) {
         int m = 0;
         for (int i = 0; i < 3 * nb && m <= DT_CAV + 1; ++i)
           if (isb[i]) {
