@@ -26,7 +26,7 @@
 namespace aj {
 namespace mp {
 
-constexpr int DT_THREADS = 256;
+constexpr int DT_THREADS = 1024;  // sixteen waves: the only workgroup on its CU (LDS), so they hide each other's LDS latency
 constexpr int DT_NMAX = 3072;          // vertices per sample
 constexpr int DT_TMAX = 2 * DT_NMAX;   // triangles including ghosts: 2n - 2
 constexpr int DT_CAV = 512;            // conflicting triangles per insertion
@@ -148,94 +148,102 @@ __global__ void __launch_bounds__(DT_THREADS) delaunay_kernel(const double* __re
         break;
       }
       if (nb == 0) continue;  // coincides with a vertex already in the mesh
-      // slot order, so that the result does not depend on the atomics' order: one wave ranks the (distinct)
-      // slot numbers by lane shuffles when they fit in a wave, else one thread sorts them
-      if (nb <= 64) {
-        if (tid < 64) {
-          const int v = tid < nb ? bad[tid] : 0x7fffffff;
-          int rank = 0;
-          for (int j = 0; j < nb; ++j) rank += __shfl(v, j) < v;
-          if (tid < nb) bad[rank] = v;  // every lane has read bad[] above (one wave, program order)
-        }
-      } else if (tid == 0) {
-        for (int i = 1; i < nb; ++i) {
-          const int v = bad[i];
-          int j = i - 1;
-          while (j >= 0 && bad[j] > v) {
-            bad[j + 1] = bad[j];
-            --j;
-          }
-          bad[j + 1] = v;
-        }
-      }
-      __syncthreads();
-      for (int i = tid; i < 3 * nb; i += DT_THREADS) {
-        const int t = bad[i / 3], e = i % 3;
-        const int u = tv[3 * t + e], v = tv[3 * t + (e + 1) % 3];
-        bool inner = false;
-        for (int j = 0; j < nb && !inner; ++j) {
-          const int t2 = bad[j];
-          if (t2 == t) continue;
-          for (int e2 = 0; e2 < 3; ++e2)
-            if (tv[3 * t2 + e2] == v && tv[3 * t2 + (e2 + 1) % 3] == u) inner = true;
-        }
-        isb[i] = inner ? 0 : 1;
-      }
-      __syncthreads();
       if (3 * nb <= 64) {
-        // one wave: boundary edge i (in (slot order, edge) order) takes the new triangle of its rank among
-        // the boundary edges, as the serial form below assigns them
+        // One wave: the cavity's slots in slot order (so that the result does not depend on the atomics'
+        // order) by a rank sort over lane shuffles; lane i < 3 nb takes edge i % 3 of the i / 3-th
+        // triangle; an edge is on the boundary when its reverse is no other cavity edge (an edge's reverse
+        // is never in its own triangle); boundary edge of rank r among them makes the new triangle in
+        // slot r (the cavity's slots, then two appended).
         if (tid < 64) {
-          const bool isbnd = tid < 3 * nb && isb[tid];
+          const int sv = tid < nb ? bad[tid] : 0x7fffffff;
+          int rank = 0;
+          for (int j = 0; j < nb; ++j) rank += __shfl(sv, j) < sv;
+          if (tid < nb) bad[rank] = sv;  // every lane has read bad[] above (one wave, program order)
+          const bool live = tid < 3 * nb;
+          int u = -2, v = -3;  // ids that match no vertex (GHOST is -1)
+          if (live) {
+            const int t = bad[tid / 3], e = tid % 3;  // reads the sorted slots written above
+            u = tv[3 * t + e];
+            v = tv[3 * t + (e + 1) % 3];
+          }
+          bool inner = false;
+          for (int j = 0; j < 3 * nb; ++j) inner |= (__shfl(u, j) == v) & (__shfl(v, j) == u);
+          const bool isbnd = live && !inner;
           const unsigned long long mask = __ballot(isbnd);
           const int m = __popcll(mask);
           if (m != nb + 2 || s_ntri + 2 > DT_TMAX) {
             if (tid == 0) s_err = m != nb + 2 ? ST_CAVITY_SHAPE : ST_CAVITY_SIZE;
           } else {
-            int u = 0, v = 0, slot = 0;
+            int slot = 0;
             if (isbnd) {
-              const int rank = __popcll(mask & ((1ull << tid) - 1));
-              const int t = bad[tid / 3], e = tid % 3;
-              u = tv[3 * t + e];
-              v = tv[3 * t + (e + 1) % 3];
-              slot = rank < nb ? bad[rank] : s_ntri + (rank - nb);
+              const int r = __popcll(mask & ((1ull << tid) - 1));
+              slot = r < nb ? bad[r] : s_ntri + (r - nb);
             }
             // the cavity's triangles and s_ntri are read above, before any lane writes below
             if (isbnd) {
-              const int r0 = u == GHOST ? v : (v == GHOST ? k : u);
-              const int r1 = u == GHOST ? k : (v == GHOST ? u : v);
-              const int r2 = (u == GHOST || v == GHOST) ? GHOST : k;
-              tv[3 * slot] = r0;
-              tv[3 * slot + 1] = r1;
-              tv[3 * slot + 2] = r2;
+              tv[3 * slot] = u == GHOST ? v : (v == GHOST ? k : u);
+              tv[3 * slot + 1] = u == GHOST ? k : (v == GHOST ? u : v);
+              tv[3 * slot + 2] = (u == GHOST || v == GHOST) ? GHOST : k;
             }
             if (tid == 0) s_ntri += 2;
           }
         }
-      } else if (tid == 0) {
-        int m = 0;
-        for (int i = 0; i < 3 * nb && m <= DT_CAV + 1; ++i)
-          if (isb[i]) {
-            const int t = bad[i / 3], e = i % 3;
-            eu[m] = tv[3 * t + e];
-            ev[m] = tv[3 * t + (e + 1) % 3];
-            ++m;
+      } else {
+        // larger cavities: the same steps with workgroup barriers between them
+        if (nb <= 64) {
+          if (tid < 64) {
+            const int sv = tid < nb ? bad[tid] : 0x7fffffff;
+            int rank = 0;
+            for (int j = 0; j < nb; ++j) rank += __shfl(sv, j) < sv;
+            if (tid < nb) bad[rank] = sv;
           }
-        if (m != nb + 2 || s_ntri + 2 > DT_TMAX) {
-          s_err = m != nb + 2 ? ST_CAVITY_SHAPE : ST_CAVITY_SIZE;
-        } else {
-          for (int j = 0; j < m; ++j) {
-            const int slot = j < nb ? bad[j] : s_ntri + (j - nb);
-            const int u = eu[j], v = ev[j];
-            // (u, v, k), rotated so that a ghost vertex comes last
-            const int r0 = u == GHOST ? v : (v == GHOST ? k : u);
-            const int r1 = u == GHOST ? k : (v == GHOST ? u : v);
-            const int r2 = (u == GHOST || v == GHOST) ? GHOST : k;
-            tv[3 * slot] = r0;
-            tv[3 * slot + 1] = r1;
-            tv[3 * slot + 2] = r2;
+        } else if (tid == 0) {
+          for (int i = 1; i < nb; ++i) {
+            const int sv = bad[i];
+            int j = i - 1;
+            while (j >= 0 && bad[j] > sv) {
+              bad[j + 1] = bad[j];
+              --j;
+            }
+            bad[j + 1] = sv;
           }
-          s_ntri += 2;
+        }
+        __syncthreads();
+        for (int i = tid; i < 3 * nb; i += DT_THREADS) {
+          const int t = bad[i / 3], e = i % 3;
+          const int u = tv[3 * t + e], v = tv[3 * t + (e + 1) % 3];
+          bool inner = false;
+          for (int j = 0; j < nb && !inner; ++j) {
+            const int t2 = bad[j];
+            if (t2 == t) continue;
+            for (int e2 = 0; e2 < 3; ++e2)
+              if (tv[3 * t2 + e2] == v && tv[3 * t2 + (e2 + 1) % 3] == u) inner = true;
+          }
+          isb[i] = inner ? 0 : 1;
+        }
+        __syncthreads();
+        if (tid == 0) {
+          int m = 0;
+          for (int i = 0; i < 3 * nb && m <= DT_CAV + 1; ++i)
+            if (isb[i]) {
+              const int t = bad[i / 3], e = i % 3;
+              eu[m] = tv[3 * t + e];
+              ev[m] = tv[3 * t + (e + 1) % 3];
+              ++m;
+            }
+          if (m != nb + 2 || s_ntri + 2 > DT_TMAX) {
+            s_err = m != nb + 2 ? ST_CAVITY_SHAPE : ST_CAVITY_SIZE;
+          } else {
+            for (int j = 0; j < m; ++j) {
+              const int slot = j < nb ? bad[j] : s_ntri + (j - nb);
+              const int u = eu[j], v = ev[j];
+              // (u, v, k), rotated so that a ghost vertex comes last
+              tv[3 * slot] = u == GHOST ? v : (v == GHOST ? k : u);
+              tv[3 * slot + 1] = u == GHOST ? k : (v == GHOST ? u : v);
+              tv[3 * slot + 2] = (u == GHOST || v == GHOST) ? GHOST : k;
+            }
+            s_ntri += 2;
+          }
         }
       }
       __syncthreads();
