@@ -124,3 +124,21 @@ def test_batched_evidence_over_device_mapper(gpu):
         want = R.log_evidence_from_parts(F, D, chi @ chi, nn, nb, sz, lams[b], logdet_h=ldh)[0]
         print(f"sample {b}: S={len(V)} lam={lams[b]:.3g} {out[b]:.12g} vs {want:.12g}", flush=True)
         assert abs(out[b] - want) < 1e-9 * max(1.0, abs(want))
+
+
+def test_mapper_reports_degree_above_max_neighbors(gpu):
+    """A wheel (a centre vertex inside a ring of 40) gives the centre 40 neighbours: above max_neighbors = 32
+    the sample fails with a ValueError naming the limit; with max_neighbors = 48 it succeeds and the
+    centre's neighbour list is the whole ring (as scipy's triangulation gives)."""
+    from autojax_amd import mapper
+
+    th = 2.0 * np.pi * np.arange(40) / 40
+    wheel = np.concatenate([[[0.0, 0.0]], np.stack([np.sin(th), np.cos(th)], 1)]) * 1e-5
+    pos = np.zeros((10, 2))
+    with pytest.raises(ValueError, match="max_neighbors"):
+        mapper.build_mappers([wheel], pos)
+    res = mapper.build_mappers([wheel], pos, max_neighbors=48)
+    got = res.sample(0)
+    nb_ref, sz_ref = MP.neighbors_from_triangles(len(wheel), MP.delaunay_triangles(wheel))
+    assert got["neighbors_sizes"][0] == 40 == sz_ref[0]
+    np.testing.assert_array_equal(got["neighbors"][0, :40], np.arange(1, 41))
