@@ -264,7 +264,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": f"{args.precision} W-tilde MFMA + fp64 likelihood",
+        "dtype": ("i8+f64" if args.precision.startswith("int8") else "f64" if args.precision == "fp64" else
+                  args.precision + "+f64") + f" ({args.precision} W-tilde on MFMA, f64 likelihood)",
         "data": "synthetic",
         "config": {
             "workload": "batched log-evidence: M=5000 px, K=1e6 vis, 64 Delaunay samples S~U{300..1500}, B=3, "
