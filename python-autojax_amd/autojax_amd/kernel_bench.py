@@ -8,6 +8,7 @@
     python -m autojax_amd.kernel_bench mfma
     python -m autojax_amd.kernel_bench fill --mib 1024
     python -m autojax_amd.kernel_bench vendor --M 5120 --K 32768
+    python -m autojax_amd.kernel_bench grad --samples 64
 
 ``wtilde`` builds W-tilde on synthetic data of the batching workload's
 generator; ``batch`` evaluates the batched log-evidence of a sample batch on
@@ -19,7 +20,10 @@ int8 matrix rate; ``mapper`` builds the source meshes and mappings of a sample b
 library's register-only MFMA loops (int8 16x16x64 and FP64 16x16x4, aj_mfma_peak_probe) as the device's
 achievable matrix rates; ``fill`` writes a buffer of known size (achievable write bandwidth); ``vendor``
 times the vendor libraries on the Gram shape C = X X^T, X of M rows and K columns: int8 (torch._int_mm),
-fp8 (torch._scaled_mm), FP64 (torch matmul) full products and rocBLAS dsyrk (the lower triangle, FP64).
+fp8 (torch._scaled_mm), FP64 (torch matmul) full products and rocBLAS dsyrk (the lower triangle, FP64);
+``grad`` times the batched log-evidence forward and its hand-written backward (gradients with respect to
+the coefficients and the mapping weights) on the differentiation workload (--samples 1: M 7,860, K 1e5,
+S 1,000) or the batching workload's samples (--samples 64).
 Each prints one JSON line per measurement with the per-call time measured with HIP events.
 """
 
@@ -34,7 +38,7 @@ import numpy as np
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["wtilde", "batch", "copy", "intmm", "mapper", "mfma", "fill", "vendor"])
+    ap.add_argument("what", choices=["wtilde", "batch", "copy", "intmm", "mapper", "mfma", "fill", "vendor", "grad"])
     ap.add_argument("--M", type=int, default=5000)
     ap.add_argument("--K", type=int, default=131072)
     ap.add_argument("--precision", default="int8x4")
@@ -105,6 +109,47 @@ def main(argv=None):
             exact = bool((out == one * iters).all().item())
             print(json.dumps({"kernel": "mfma_probe", "instruction": name, "workgroups": wg, "iters": iters, "ms": ms,
                               "T_ops_per_s": ops / ms / 1e9, "exact": exact}), flush=True)
+        return
+    if a.what == "grad":
+        from . import torch_api as TA
+
+        d = dsm.config4() if a.samples == 1 else dsm.config5(K=a.K, n_samples=a.samples)
+        noise, uv, grid = (dv.to_device(x) for x in (d.noise_map_real, d.uv_wavelengths, d.grid_radians_slim))
+        W = E.w_tilde(noise, uv, grid, "fp64")
+        y = E.dirty_image(d.data, d.noise_map, d.uv_wavelengths, d.grid_radians_slim)
+        c2nn = E.chi2_noise_norm(d.data, d.noise_map)
+        batch = E.SampleBatch(d.samples, d.M)
+        n = len(d.samples)
+        w0 = torch.from_numpy(np.stack([s.pix_weights_for_sub_slim_index for s in d.samples])).cuda()
+        c0 = torch.tensor([s.coefficient for s in d.samples], dtype=torch.float64)
+        state = {}
+
+        def fwd():
+            c = c0.clone().requires_grad_(True)
+            w = w0.clone().requires_grad_(True)
+            state["out"] = TA.log_evidence(c, w, W, y, c2nn, batch)
+            state["cw"] = (c, w)
+
+        def bwd():
+            state["out"].sum().backward()
+
+        fwd()
+        bwd()
+        torch.cuda.synchronize()
+        s0, s1, s2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        tf = tb = 0.0
+        for _ in range(a.reps):
+            s0.record()
+            fwd()
+            s1.record()
+            bwd()
+            s2.record()
+            torch.cuda.synchronize()
+            tf += s0.elapsed_time(s1)
+            tb += s1.elapsed_time(s2)
+        print(json.dumps({"kernel": "log_evidence_grad", "M": d.M, "K": d.K, "samples": n,
+                          "sum_S": int(batch.S_host.sum()), "forward_ms": tf / a.reps, "backward_ms": tb / a.reps}),
+              flush=True)
         return
     if a.what == "vendor":
         import ctypes as C
