@@ -172,3 +172,14 @@ def test_host_validation():
     with pytest.raises(ValueError):
         api.validate_neighbors(np.array([[1], [-1]]), np.array([1, 0]))
     api.validate_neighbors(np.array([[1], [0]]), np.array([1, 1]))
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus 2 exits with status 2 and says so, before touching a device."""
+    import subprocess
+    import sys
+
+    r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
