@@ -1621,42 +1621,82 @@ namespace lk {
 
 // Column j of the inverse from the lower factor L (one wave per column):
 //   inv != nullptr: inv[:, j] = L^-T L^-1 e_j;   colnorm2[j] = ||L^-1 e_j||^2 always.
+// Columns j0 .. j0 + nc - 1 of L^-1 (the forward substitution L y = e_j) and, with inv_all, of
+// A^-1 = L^-T L^-1 (then L^T x = y), for one sample per blockIdx.y; one wave per workgroup.  The nc
+// columns share every factor element read (each is used for all of them), so the factor is streamed
+// once per nc columns; nc = INV_NC unless the columns of the largest sample do not fit in LDS.  Two
+// columns per workgroup were measured fastest (4 and 8 need more LDS, fewer workgroups fit on a CU and
+// the single-wave workgroups no longer hide the factor reads' latency).
+constexpr int INV_NC = 2;
+
 __global__ void __launch_bounds__(64) factor_inverse_kernel(const SampleDesc* __restrict__ sd,
-                                                            const double* __restrict__ mats, int which_h,
+                                                            const double* __restrict__ mats, int which_h, int nc_max,
                                                             double* __restrict__ inv_all,
                                                             double* __restrict__ colnorm2_all) {
-  extern __shared__ double y[];
+  extern __shared__ double y[];  // [nc_max][n]: column c of the block at y + c n
   const SampleDesc d = sd[blockIdx.y];
   const int64_t n = d.S;
-  const int64_t j = blockIdx.x;
-  if (j >= n) return;
+  const int64_t j0 = (int64_t)blockIdx.x * nc_max;
+  if (j0 >= n) return;
+  const int nc = (int)std::min<int64_t>(nc_max, n - j0);
   const double* L = mats + d.mat_off + (which_h ? n * n : 0);
   const int lane = threadIdx.x;
-  for (int64_t i = lane; i < n; i += 64) y[i] = 0.0;
+  for (int64_t i = lane; i < (int64_t)nc * n; i += 64) y[i] = 0.0;
   __syncthreads();
-  // forward: L y = e_j, y_i = 0 for i < j
-  for (int64_t i = j; i < n; ++i) {
-    double acc = 0.0;
-    for (int64_t k = j + lane; k < i; k += 64) acc = fma(L[i * n + k], y[k], acc);
-    acc = wave_sum(acc);
-    if (lane == 0) y[i] = ((i == j ? 1.0 : 0.0) - acc) / L[i * n + i];
+  // forward: L y_c = e_(j0 + c); y_c is 0 above row j0 + c
+  for (int64_t i = j0; i < n; ++i) {
+    double acc[INV_NC];
+#pragma unroll
+    for (int c = 0; c < INV_NC; ++c) acc[c] = 0.0;
+    for (int64_t k = j0 + lane; k < i; k += 64) {
+      const double l = L[i * n + k];
+#pragma unroll
+      for (int c = 0; c < INV_NC; ++c)
+        if (c < nc) acc[c] = fma(l, y[c * n + k], acc[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < INV_NC; ++c)
+      if (c < nc) acc[c] = wave_sum(acc[c]);
+    if (lane == 0) {
+      const double dii = L[i * n + i];
+#pragma unroll
+      for (int c = 0; c < INV_NC; ++c)
+        if (c < nc && i >= j0 + c) y[c * n + i] = ((i == j0 + c ? 1.0 : 0.0) - acc[c]) / dii;
+    }
     __syncthreads();
   }
-  double nrm = 0.0;
-  for (int64_t i = j + lane; i < n; i += 64) nrm = fma(y[i], y[i], nrm);
-  nrm = wave_sum(nrm);
-  if (lane == 0) colnorm2_all[d.s_off + j] = nrm;
+  for (int c = 0; c < nc; ++c) {
+    double nrm = 0.0;
+    for (int64_t i = j0 + c + lane; i < n; i += 64) nrm = fma(y[c * n + i], y[c * n + i], nrm);
+    nrm = wave_sum(nrm);
+    if (lane == 0) colnorm2_all[d.s_off + j0 + c] = nrm;
+  }
   if (!inv_all) return;
-  // backward: L^T x = y (x overwrites y), row i of L^T is column i of L
+  // backward: L^T x_c = y_c (x overwrites y), row i of L^T is column i of L
   for (int64_t i = n - 1; i >= 0; --i) {
-    double acc = 0.0;
-    for (int64_t k = i + 1 + lane; k < n; k += 64) acc = fma(L[k * n + i], y[k], acc);
-    acc = wave_sum(acc);
-    if (lane == 0) y[i] = (y[i] - acc) / L[i * n + i];
+    double acc[INV_NC];
+#pragma unroll
+    for (int c = 0; c < INV_NC; ++c) acc[c] = 0.0;
+    for (int64_t k = i + 1 + lane; k < n; k += 64) {
+      const double l = L[k * n + i];
+#pragma unroll
+      for (int c = 0; c < INV_NC; ++c)
+        if (c < nc) acc[c] = fma(l, y[c * n + k], acc[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < INV_NC; ++c)
+      if (c < nc) acc[c] = wave_sum(acc[c]);
+    if (lane == 0) {
+      const double dii = L[i * n + i];
+#pragma unroll
+      for (int c = 0; c < INV_NC; ++c)
+        if (c < nc) y[c * n + i] = (y[c * n + i] - acc[c]) / dii;
+    }
     __syncthreads();
   }
   double* inv = inv_all + d.mat_off / 2;  // one S*S block per sample
-  for (int64_t i = lane; i < n; i += 64) inv[i * n + j] = y[i];
+  for (int c = 0; c < nc; ++c)
+    for (int64_t i = lane; i < n; i += 64) inv[i * n + j0 + c] = y[c * n + i];
 }
 
 // Per-sample scalar gradient: one workgroup per sample.
@@ -1935,10 +1975,16 @@ extern "C" int aj_log_evidence_batched_backward(const double* w_tilde, int64_t M
   int64_t maxS = 0;
   for (int64_t b = 0; b < n_samples; ++b) maxS = std::max(maxS, S_host[b]);
   // inverse of A (mat_off counts both A and H blocks, so inv uses mat_off/2 per sample)
-  hipLaunchKernelGGL(factor_inverse_kernel, dim3((unsigned)maxS, (unsigned)n_samples), dim3(64),
-                     sizeof(double) * maxS, st, desc, mats, 0, inv, an2);
-  hipLaunchKernelGGL(factor_inverse_kernel, dim3((unsigned)maxS, (unsigned)n_samples), dim3(64),
-                     sizeof(double) * maxS, st, desc, mats, 1, nullptr, hn2);
+  // columns per workgroup: as many as fit in LDS for the largest sample (at least one: maxS <= SOLVE_S_MAX)
+  const int nc = (int)std::max<int64_t>(1, std::min<int64_t>(INV_NC, (160 * 1024) / ((int64_t)sizeof(double) * maxS)));
+  const size_t inv_lds = sizeof(double) * nc * maxS;
+  const unsigned inv_blocks = (unsigned)ceil_div(maxS, (int64_t)nc);
+  AJ_CHECK_HIP(hipFuncSetAttribute((const void*)factor_inverse_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)inv_lds));
+  hipLaunchKernelGGL(factor_inverse_kernel, dim3(inv_blocks, (unsigned)n_samples), dim3(64), inv_lds, st, desc, mats,
+                     0, nc, inv, an2);
+  hipLaunchKernelGGL(factor_inverse_kernel, dim3(inv_blocks, (unsigned)n_samples), dim3(64), inv_lds, st, desc, mats,
+                     1, nc, nullptr, hn2);
   hipLaunchKernelGGL(coef_grad_kernel, dim3((unsigned)n_samples), dim3(256), 0, st, desc, neighbors, neighbors_sizes,
                      P, reconstruction, inv, hn2, coefficients_dev, dcoefficients);
   const unsigned gm = (unsigned)std::min<int64_t>(ceil_div(M, 256), 1024);
