@@ -1622,7 +1622,9 @@ namespace lk {
 // Column j of the inverse from the lower factor L (one wave per column):
 //   inv != nullptr: inv[:, j] = L^-T L^-1 e_j;   colnorm2[j] = ||L^-1 e_j||^2 always.
 // Columns j0 .. j0 + nc - 1 of L^-1 (the forward substitution L y = e_j) and, with inv_all, of
-// A^-1 = L^-T L^-1 (then L^T x = y), for one sample per blockIdx.y; one wave per workgroup.  The nc
+// A^-1 = L^-T L^-1 (then L^T x = y), for one sample per blockIdx.y; one wave per workgroup.  Both
+// substitutions read the factor by rows (the backward one right-looking), so a wave's loads are
+// contiguous.  The nc
 // columns share every factor element read (each is used for all of them), so the factor is streamed
 // once per nc columns; nc = INV_NC unless the columns of the largest sample do not fit in LDS.  Two
 // columns per workgroup were measured fastest (4 and 8 need more LDS, fewer workgroups fit on a CU and
@@ -1672,25 +1674,24 @@ __global__ void __launch_bounds__(64) factor_inverse_kernel(const SampleDesc* __
     if (lane == 0) colnorm2_all[d.s_off + j0 + c] = nrm;
   }
   if (!inv_all) return;
-  // backward: L^T x_c = y_c (x overwrites y), row i of L^T is column i of L
+  // backward: L^T x_c = y_c (x overwrites y), right-looking so that the factor is read by rows
+  // (contiguous): once x_i = y_i / L_ii is final, row i of L updates y_k -= L_ik x_i for every k < i
   for (int64_t i = n - 1; i >= 0; --i) {
-    double acc[INV_NC];
+    const double dii = L[i * n + i];
+    double xi[INV_NC];
 #pragma unroll
-    for (int c = 0; c < INV_NC; ++c) acc[c] = 0.0;
-    for (int64_t k = i + 1 + lane; k < n; k += 64) {
-      const double l = L[k * n + i];
-#pragma unroll
-      for (int c = 0; c < INV_NC; ++c)
-        if (c < nc) acc[c] = fma(l, y[c * n + k], acc[c]);
-    }
-#pragma unroll
-    for (int c = 0; c < INV_NC; ++c)
-      if (c < nc) acc[c] = wave_sum(acc[c]);
+    for (int c = 0; c < INV_NC; ++c) xi[c] = c < nc ? y[c * n + i] / dii : 0.0;
+    __syncthreads();  // every lane has read y_i before lane 0 stores x_i below
     if (lane == 0) {
-      const double dii = L[i * n + i];
 #pragma unroll
       for (int c = 0; c < INV_NC; ++c)
-        if (c < nc) y[c * n + i] = (y[c * n + i] - acc[c]) / dii;
+        if (c < nc) y[c * n + i] = xi[c];
+    }
+    for (int64_t k = lane; k < i; k += 64) {
+      const double l = L[i * n + k];
+#pragma unroll
+      for (int c = 0; c < INV_NC; ++c)
+        if (c < nc) y[c * n + k] = fma(-l, xi[c], y[c * n + k]);
     }
     __syncthreads();
   }
