@@ -120,8 +120,8 @@ int aj_w_tilde_via_compact_from(const double* w_compact, int64_t grid_size,
                                 const int64_t* native_index_for_slim_index, int64_t M, double* w_tilde, void* stream);
 
 /* numba.py:259-291 w_tilde_via_preload_from(w_tilde_preload f8[Py,Px], native_index_for_slim_index i8[M,2])
- * -> f8[M,M], W[i, j] = preload[y_j - y_i, x_j - x_i] for j >= i with negative offsets wrapping as numpy
- * indexes, W[j, i] = W[i, j] (the caller guarantees |offset| < Py, Px). */
+ * -> f8[M,M], W[i, j] = preload[y_j - y_i, x_j - x_i] for every pair (i, j), negative offsets wrapping as
+ * numpy indexes (numba.py:282-291; the caller guarantees |offset| < Py, Px). */
 int aj_w_tilde_via_preload_from(const double* w_tilde_preload, int64_t preload_rows, int64_t preload_cols,
                                 const int64_t* native_index_for_slim_index, int64_t M, double* w_tilde, void* stream);
 
@@ -198,8 +198,9 @@ int aj_curvature_matrix_via_w_compact_sparse_mapping_matrix_from(
 
 /* original.py:600-700 curvature_matrix_via_w_tilde_curvature_preload_interferometer_from(curvature_preload
  * f8[Py,Px], pix_indexes i8[M,B], pix_size i8[M], pix_weights f8[M,B], native_index_for_slim_index i8[M,2],
- * pix_pixels) -> f8[pixels,pixels] = T^T W T with W read from the preload table as
- * aj_w_tilde_via_preload_from reads it.  The mapping is in this library's format (-1 ends a row; the
+ * pix_pixels) -> f8[pixels,pixels] = T^T W T with W[i, j] read from the preload table at the offset of the
+ * later pixel in slim order from the earlier one, as original.py:640-663 reads it (the pairs ip0 < ip1, then
+ * the transpose).  The mapping is in this library's format (-1 ends a row; the
  * Python layer turns pix_size into it).  Uses aj_curvature_sparse_workspace_bytes. */
 int aj_curvature_matrix_via_w_tilde_curvature_preload_interferometer_from(
     const double* curvature_preload, int64_t preload_rows, int64_t preload_cols,

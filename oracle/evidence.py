@@ -98,9 +98,11 @@ def longdouble_cholesky_logdet(A):
 
 def torch_log_evidence(coefficient, weights, idx, W, dirty, chi2, nn, neighbors, neighbors_sizes):
     """Dense FP64 torch restatement of numba.py:818-857 with an explicit coefficient, for autograd
-    cross-checks on the CPU (coefficient and weights may require grad).  Tiny sizes only."""
+    cross-checks (coefficient and weights may require grad).  Runs on the device of `weights` (W and
+    dirty may be numpy arrays or tensors); dense T, so sizes up to a few thousand source pixels."""
     import torch
 
+    dev = weights.device
     M, B = idx.shape
     S = neighbors_sizes.shape[0]
     live = np.ones(idx.shape, bool)
@@ -108,18 +110,20 @@ def torch_log_evidence(coefficient, weights, idx, W, dirty, chi2, nn, neighbors,
         live[:, 1:] = np.cumprod(idx[:, :-1] != -1, axis=1).astype(bool)
     live &= idx >= 0
     rows, cols = np.nonzero(live)
-    T = torch.zeros((M, S), dtype=torch.float64)
-    T = T.index_put((torch.as_tensor(rows), torch.as_tensor(idx[rows, cols])), weights[rows, cols])
-    Wt = torch.as_tensor(W)
-    F = T.T @ Wt @ T
-    D = T.T @ torch.as_tensor(dirty)
-    Lap = torch.diag(torch.as_tensor(neighbors_sizes, dtype=torch.float64))
+    T = torch.zeros((M, S), dtype=torch.float64, device=dev)
+    r_t, c_t = torch.as_tensor(rows, device=dev), torch.as_tensor(cols, device=dev)
+    T = T.index_put((r_t, torch.as_tensor(idx[rows, cols], device=dev)), weights[r_t, c_t])
+    Wt = torch.as_tensor(W, device=dev)
+    F = T.T @ (Wt @ T)
+    D = T.T @ torch.as_tensor(dirty, device=dev)
+    Lap = np.diag(np.asarray(neighbors_sizes, dtype=np.float64))
     for i in range(S):
         for k in neighbors[i]:
             if k == -1:
                 break
             Lap[i, k] -= 1.0
-    H = coefficient**2 * Lap + 1e-8 * torch.eye(S, dtype=torch.float64)
+    Lap = torch.as_tensor(Lap, device=dev)
+    H = coefficient**2 * Lap + 1e-8 * torch.eye(S, dtype=torch.float64, device=dev)
     A = F + H
     s = torch.linalg.solve(A, D)
     return -0.5 * (chi2 - s @ D + torch.linalg.slogdet(A)[1] - torch.linalg.slogdet(H)[1] + nn)

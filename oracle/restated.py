@@ -156,12 +156,12 @@ def curvature_matrix_via_w_compact_sparse_mapping_matrix_from(w_compact, native_
 
 
 def w_tilde_via_preload_from(w_tilde_preload, native_index_for_slim_index):
-    """numba.py:259-291: W[i, j] = preload[y_j - y_i, x_j - x_i] for j >= i (numpy negative indexes
-    wrap), then W[j, i] = W[i, j]."""
+    """numba.py:259-291: W[i, j] = preload[y_j - y_i, x_j - x_i] for every pair (i, j) (numba.py:282-291;
+    negative offsets wrap as numpy indexes).  original.py:431-447 reads the table for j >= i only and
+    mirrors; the two agree when the table is point-symmetric."""
     y = native_index_for_slim_index[:, 0]
     x = native_index_for_slim_index[:, 1]
-    T = w_tilde_preload[y[None, :] - y[:, None], x[None, :] - x[:, None]]
-    return np.triu(T) + np.triu(T, 1).T
+    return w_tilde_preload[y[None, :] - y[:, None], x[None, :] - x[:, None]]
 
 
 def _preload_quadrant(noise_map_real, uv_wavelengths, dy, dx):

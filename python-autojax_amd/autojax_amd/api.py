@@ -194,8 +194,8 @@ def w_tilde_via_compact_from(w_compact, native_index_for_slim_index):
 
 
 def w_tilde_via_preload_from(w_tilde_preload, native_index_for_slim_index):
-    """numba.py:259-291 -> f8[M, M]: W[i, j] = w_tilde_preload[y_j - y_i, x_j - x_i], negative offsets
-    wrapping as numpy indexes do."""
+    """numba.py:259-291 -> f8[M, M]: W[i, j] = w_tilde_preload[y_j - y_i, x_j - x_i] for every pair (i, j)
+    (numba.py:282-291), negative offsets wrapping as numpy indexes do."""
     torch_in = _any_t(w_tilde_preload, native_index_for_slim_index)
     Pm = _dev(w_tilde_preload, np.float64, "w_tilde_preload", 2)
     Py, Px = Pm.shape

@@ -114,6 +114,8 @@ class SampleBatch:
         return self
 
     def _setup(self, idx, wts, nbr, nsz, coef, deg):
+        # incremented by every evaluation; a backward checks it to know the workspace's factors are its own
+        self.evaluations = 0
         self.coef_host = coef
         self.deg_host = deg
         self.beta_host = self.coef_host**2 * self.deg_host
@@ -143,6 +145,7 @@ class SampleBatch:
         """
         if w_tilde.shape != (self.M, self.M) or w_tilde.dtype != torch.float64:
             raise ValueError("w_tilde must be float64 [M, M] on the device")
+        self.evaluations += 1
         _lib.call(
             "aj_log_evidence_batched",
             w_tilde.data_ptr(), self.M, dirty_image.data_ptr(), chi2_noise_norm.data_ptr(), self.n, self.B, self.P,
@@ -165,6 +168,7 @@ class SampleBatch:
             raise ValueError("w_compact must be float64 [N, 2N-1] on the device")
         if native_index_for_slim_index.shape != (self.M, 2) or native_index_for_slim_index.dtype != torch.int64:
             raise ValueError("native_index_for_slim_index must be int64 [M, 2] on the device")
+        self.evaluations += 1
         _lib.call(
             "aj_log_evidence_batched_compact",
             w_compact.data_ptr(), N, native_index_for_slim_index.data_ptr(), self.M, dirty_image.data_ptr(),

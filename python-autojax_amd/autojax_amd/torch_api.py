@@ -20,6 +20,17 @@ from . import _lib
 from . import device as dv
 
 
+def check_factors_current(batch, evaluation):
+    """The backward reads the Cholesky factors, Omega and reconstruction that the batch's workspace holds;
+    they belong to the batch's last evaluation only.  Raises if the batch was evaluated after the forward
+    pass being differentiated (its numbers would be silently wrong)."""
+    if batch.evaluations != evaluation:
+        raise RuntimeError(
+            f"the batch was evaluated again (evaluation {batch.evaluations}) after the forward pass being "
+            f"differentiated (evaluation {evaluation}); its factors are gone: evaluate again, or use one "
+            "SampleBatch per graph")
+
+
 class _LogEvidence(torch.autograd.Function):
     @staticmethod
     def forward(ctx, coefficients, weights, w_tilde, dirty_image, chi2_noise_norm, batch):
@@ -30,6 +41,7 @@ class _LogEvidence(torch.autograd.Function):
         batch.wts = weights.detach().contiguous()
         out = batch.log_evidence(w_tilde, dirty_image, chi2_noise_norm).clone()
         ctx.batch = batch
+        ctx.evaluation = batch.evaluations
         ctx.coef_device = coefficients.device
         ctx.save_for_backward(coef.to(weights.device), batch.wts, w_tilde, dirty_image)
         return out
@@ -38,8 +50,7 @@ class _LogEvidence(torch.autograd.Function):
     def backward(ctx, grad):
         coef_dev, wts, w_tilde, dirty = ctx.saved_tensors
         b = ctx.batch
-        if b.wts.data_ptr() != wts.data_ptr():
-            raise RuntimeError("the batch was evaluated again before backward; its factors are gone")
+        check_factors_current(b, ctx.evaluation)
         dcoef = torch.empty(b.n, dtype=torch.float64, device=wts.device)
         dw = torch.empty_like(wts)
         ws = dv.workspace(_lib.query("aj_log_evidence_backward_workspace_bytes", b.M, b.n,

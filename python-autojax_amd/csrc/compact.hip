@@ -124,16 +124,16 @@ __global__ void compact_expand_kernel(const double* __restrict__ C, int64_t N, c
   }
 }
 
-// W[i, j] = preload[y_j - y_i, x_j - x_i] for j >= i, negative offsets wrapping as numpy indexes, and
-// W[j, i] = W[i, j] (numba.py:280-290: the upper triangle from the table, then mirrored).
+// W[i, j] = preload[y_j - y_i, x_j - x_i] for every pair (i, j), negative offsets wrapping as numpy
+// indexes (numba.py:282-291: each entry is read from the table, the lower triangle included, so W is
+// symmetric only when the table is point-symmetric).
 __global__ void preload_expand_kernel(const double* __restrict__ P, int64_t Py, int64_t Px,
                                       const int64_t* __restrict__ native, int64_t M, double* __restrict__ W) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= M) return;
   const int64_t yj = native[2 * j], xj = native[2 * j + 1];
   for (int64_t i = blockIdx.y; i < M; i += gridDim.y) {
-    const int64_t yi = native[2 * i], xi = native[2 * i + 1];
-    int64_t dy = i <= j ? yj - yi : yi - yj, dx = i <= j ? xj - xi : xi - xj;
+    int64_t dy = yj - native[2 * i], dx = xj - native[2 * i + 1];
     dy += dy < 0 ? Py : 0;
     dx += dx < 0 ? Px : 0;
     W[i * M + j] = P[dy * Px + dx];

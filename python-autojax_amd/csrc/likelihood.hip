@@ -1620,7 +1620,7 @@ namespace aj {
 namespace lk {
 
 // Column j of the inverse from the lower factor L (one wave per column):
-//   inv != nullptr: inv[:, j] = L^-T L^-1 e_j;   colnorm2[j] = ||L^-1 e_j||^2 always.
+//   inv != nullptr: inv[:, j] = L^-T L^-1 e_j;   colnorm2 != nullptr: colnorm2[j] = ||L^-1 e_j||^2.
 // Columns j0 .. j0 + nc - 1 of L^-1 (the forward substitution L y = e_j) and, with inv_all, of
 // A^-1 = L^-T L^-1 (then L^T x = y), for one sample per blockIdx.y; one wave per workgroup.  Both
 // substitutions read the factor by rows (the backward one right-looking), so a wave's loads are
@@ -1667,11 +1667,13 @@ __global__ void __launch_bounds__(64) factor_inverse_kernel(const SampleDesc* __
     }
     __syncthreads();
   }
-  for (int c = 0; c < nc; ++c) {
-    double nrm = 0.0;
-    for (int64_t i = j0 + c + lane; i < n; i += 64) nrm = fma(y[c * n + i], y[c * n + i], nrm);
-    nrm = wave_sum(nrm);
-    if (lane == 0) colnorm2_all[d.s_off + j0 + c] = nrm;
+  if (colnorm2_all) {
+    for (int c = 0; c < nc; ++c) {
+      double nrm = 0.0;
+      for (int64_t i = j0 + c + lane; i < n; i += 64) nrm = fma(y[c * n + i], y[c * n + i], nrm);
+      nrm = wave_sum(nrm);
+      if (lane == 0) colnorm2_all[d.s_off + j0 + c] = nrm;
+    }
   }
   if (!inv_all) return;
   // backward: L^T x_c = y_c (x overwrites y), right-looking so that the factor is read by rows
@@ -1972,7 +1974,6 @@ extern "C" int aj_log_evidence_batched_backward(const double* w_tilde, int64_t M
   double* hn2 = ar.take<double>(p.sumS);
   double* u = ar.take<double>(M * n_samples);
   double* v = ar.take<double>(M * n_samples);
-  double* an2 = v;  // column norms of the A factor are not needed; reuse as scratch before v is written
   int64_t maxS = 0;
   for (int64_t b = 0; b < n_samples; ++b) maxS = std::max(maxS, S_host[b]);
   // inverse of A (mat_off counts both A and H blocks, so inv uses mat_off/2 per sample)
@@ -1983,7 +1984,7 @@ extern "C" int aj_log_evidence_batched_backward(const double* w_tilde, int64_t M
   AJ_CHECK_HIP(hipFuncSetAttribute((const void*)factor_inverse_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)inv_lds));
   hipLaunchKernelGGL(factor_inverse_kernel, dim3(inv_blocks, (unsigned)n_samples), dim3(64), inv_lds, st, desc, mats,
-                     0, nc, inv, an2);
+                     0, nc, inv, nullptr);
   hipLaunchKernelGGL(factor_inverse_kernel, dim3(inv_blocks, (unsigned)n_samples), dim3(64), inv_lds, st, desc, mats,
                      1, nc, nullptr, hn2);
   hipLaunchKernelGGL(coef_grad_kernel, dim3((unsigned)n_samples), dim3(256), 0, st, desc, neighbors, neighbors_sizes,

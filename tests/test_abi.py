@@ -183,3 +183,21 @@ def test_bench_refuses_more_than_one_gpu():
                        timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_backward_workspace_holds_per_sample_vectors_when_sources_outnumber_pixels():
+    """The backward workspace (host-only query) covers the inverse blocks (sum S^2), the per-source-pixel
+    column norms (sum S) and two per-image-pixel vectors per sample (2 M n), each 256-byte aligned, also
+    when S exceeds M."""
+    import ctypes
+
+    from autojax_amd import _lib
+
+    def align(x):
+        return (x + 255) // 256 * 256
+
+    for M, S in ((64, [200]), (400, [600, 30]), (5000, [300, 1500, 1041])):
+        Sa = np.array(S, np.int64)
+        got = _lib.query("aj_log_evidence_backward_workspace_bytes", M, len(S), Sa.ctypes.data_as(ctypes.c_void_p))
+        need = align(8 * int((Sa**2).sum())) + align(8 * int(Sa.sum())) + 2 * align(8 * M * len(S))
+        assert got >= need, (M, S, got, need)

@@ -105,3 +105,51 @@ def test_vertex_gradient(gpu, prob):
     err = np.abs(V.grad.cpu().numpy() - Vt.grad.numpy()).max() / np.abs(Vt.grad.numpy()).max()
     print("vertex vjp rel err", err, flush=True)
     assert err < 1e-9
+
+
+def test_grad_sample_larger_than_image(gpu):
+    """A sample with more source pixels than image pixels (M 64, S 200: F has rank <= 64, A = F + H is
+    positive definite through H): value and gradients against torch FP64 autograd of the dense form.  The
+    backward's scratch holds per-sample vectors of both lengths, so S > M must not write past it."""
+    d = ds.small_delaunay(43, 64, 300, 200, n_samples=1, lam_range=(0.5, 2.0))
+    W = R.w_tilde_curvature_interferometer_from(d.noise_map_real, d.uv_wavelengths, d.grid_radians_slim)
+    d.ensure_dirty_image()
+    chi = d.data.view(np.float64) / d.noise_map.view(np.float64)
+    nn = R.noise_normalization_complex_from(d.noise_map)
+    smp = d.samples[0]
+    assert smp.pixels > d.M
+    out, c, w = _gpu_eval(d, W, [smp.coefficient])
+    out.sum().backward()
+    ct = torch.tensor(smp.coefficient, dtype=torch.float64, requires_grad=True)
+    wt = torch.tensor(smp.pix_weights_for_sub_slim_index, requires_grad=True)
+    ref = ev.torch_log_evidence(ct, wt, smp.pix_indexes_for_sub_slim_index, W, d.dirty_image, float(chi @ chi), nn,
+                                smp.neighbors, smp.neighbors_sizes)
+    ref.backward()
+    err_w = np.abs(w.grad[0].cpu().numpy() - wt.grad.numpy()).max() / np.abs(wt.grad.numpy()).max()
+    print(f"M {d.M} S {smp.pixels}: logL {out[0].item():.10f} vs {ref.item():.10f}; dc {c.grad[0].item():.8e} vs "
+          f"{ct.grad.item():.8e}; dweights {err_w:.2e}", flush=True)
+    assert abs(out[0].item() - ref.item()) < 1e-8 * abs(ref.item())
+    assert abs(c.grad[0].item() - ct.grad.item()) < 1e-7 * abs(ct.grad.item()) + 1e-9
+    assert err_w < 1e-7
+
+
+def test_backward_after_a_second_evaluation_raises(gpu, prob):
+    """Two forward passes on one SampleBatch, then backward of the first: its factors were overwritten by the
+    second, so backward raises instead of returning the second evaluation's gradients."""
+    from autojax_amd import evidence as E
+    from autojax_amd import torch_api as TA
+
+    d, W, chi2, nn = prob
+    batch = E.SampleBatch(d.samples, d.M)
+    Wd = torch.from_numpy(W).cuda()
+    y = torch.from_numpy(d.dirty_image).cuda()
+    c2nn = E.chi2_noise_norm(d.data, d.noise_map)
+    w = torch.from_numpy(np.stack([s.pix_weights_for_sub_slim_index for s in d.samples])).cuda().requires_grad_(True)
+    c1 = torch.tensor([s.coefficient for s in d.samples], dtype=torch.float64, requires_grad=True)
+    c2 = (2.0 * c1).detach().requires_grad_(True)
+    first = TA.log_evidence(c1, w, Wd, y, c2nn, batch)
+    second = TA.log_evidence(c2, w, Wd, y, c2nn, batch)
+    with pytest.raises(RuntimeError, match="evaluated again"):
+        first.sum().backward(retain_graph=True)
+    second.sum().backward()
+    assert torch.isfinite(c2.grad).all()

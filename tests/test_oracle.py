@@ -50,6 +50,41 @@ def test_w_tilde_rows_match_fsum_golden_and_full_oracle():
     np.testing.assert_allclose(Wr, W[sel], rtol=0, atol=1e-12 * np.abs(W).max())
 
 
+@pytest.fixture(scope="module")
+def c_oracle():
+    """The C restatement (oracle/wtilde_cpu.c), built with `make -C oracle` when it is not there yet."""
+    import subprocess
+
+    from oracle import wtilde_c
+
+    if not wtilde_c.LIB_PATH.exists():
+        subprocess.run(["make", "-C", str(wtilde_c.LIB_PATH.parent), "-s"], check=True)
+    return wtilde_c
+
+
+def test_c_oracle_matches_fsum_golden_and_numpy_oracle(c_oracle):
+    """The C/OpenMP loop (full matrix, any thread count) and its row-subset form against the brute-force
+    fsum golden and the numpy restatement on the 716-pixel workload; a transposed grid pairing fails."""
+    g = np.load(GOLD / "tiny_wtilde.npz")
+    args = (g["noise_map_real"], g["uv_wavelengths"], g["grid_radians_slim"])
+    tol = 1e-12 * np.abs(g["w_tilde"]).max()
+    for threads in (1, 3):
+        np.testing.assert_allclose(c_oracle.w_tilde(*args, threads=threads), g["w_tilde"], rtol=0, atol=tol)
+    M = g["grid_radians_slim"].shape[0]
+    np.testing.assert_allclose(c_oracle.w_tilde_rows(*args, np.arange(M), threads=2), g["w_tilde"], rtol=0, atol=tol)
+    swapped = c_oracle.w_tilde(args[0], args[1], args[2][:, ::-1].copy(), threads=2)
+    assert np.abs(swapped - g["w_tilde"]).max() > 1e-3 * np.abs(g["w_tilde"]).max()
+    d = ds.config1(K=300)
+    W = R.w_tilde_curvature_interferometer_from(d.noise_map_real, d.uv_wavelengths, d.grid_radians_slim)
+    Wc = c_oracle.w_tilde(d.noise_map_real, d.uv_wavelengths, d.grid_radians_slim, threads=4)
+    np.testing.assert_allclose(Wc, W, rtol=0, atol=1e-12 * np.abs(W).max())
+    sel = np.array([0, 5, 300, 357, 715])
+    Wr = c_oracle.w_tilde_rows(d.noise_map_real, d.uv_wavelengths, d.grid_radians_slim, sel, threads=4)
+    np.testing.assert_allclose(Wr, W[sel], rtol=0, atol=1e-12 * np.abs(W).max())
+    with pytest.raises(ValueError):
+        c_oracle.w_tilde_rows(d.noise_map_real, d.uv_wavelengths, d.grid_radians_slim, [716])
+
+
 def test_w_tilde_pairing_is_not_transposed():
     # swapping the (y, x) grid columns must change W for a generic uv set
     g = np.load(GOLD / "tiny_wtilde.npz")
@@ -329,19 +364,24 @@ def test_preload_curvature_matches_dense_route():
     np.testing.assert_allclose(F, T.T @ W @ T, rtol=0, atol=1e-10 * np.abs(F).max())
 
 
-def test_preload_gather_mirrors_the_upper_triangle():
-    # numba.py:280-290 reads the table for j >= i only and mirrors: for any table (not only a
-    # symmetric one) W is symmetric and its upper triangle is the plain wrapped gather
+def test_preload_gather_reads_every_pair():
+    # numba.py:282-291 reads the table for every (i, j), with numpy's wrapping of negative offsets: against
+    # the reference's double loop written out with Python indexing on a table that is not point-symmetric
+    # (so the lower triangle is not the mirror of the upper one); a mirrored gather (original.py:431-447)
+    # would fail this
     d = ds.reference_convention_dataset(seed=23, K=8, S=10)
     N = d.n_side
     nat = d.native_index_for_slim_index[:50]
     pre = np.random.default_rng(0).standard_normal((2 * N, 2 * N))
     W = R.w_tilde_via_preload_from(pre, nat)
-    np.testing.assert_array_equal(W, W.T)
-    y, x = nat[:, 0], nat[:, 1]
-    G = pre[y[None, :] - y[:, None], x[None, :] - x[:, None]]
-    np.testing.assert_array_equal(np.triu(W), np.triu(G))
-    assert not np.array_equal(np.tril(W, -1), np.tril(G, -1))
+    loop = np.empty_like(W)
+    for i in range(nat.shape[0]):
+        for j in range(nat.shape[0]):
+            loop[i, j] = pre[int(nat[j, 0] - nat[i, 0]), int(nat[j, 1] - nat[i, 1])]
+    np.testing.assert_array_equal(W, loop)
+    assert not np.array_equal(W, W.T)
+    mirrored = np.triu(W) + np.triu(W, 1).T
+    assert not np.array_equal(mirrored, loop)
 
 
 # ------------------------------------------------------- source-mesh mapper (upstream of the reference)
