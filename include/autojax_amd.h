@@ -215,9 +215,10 @@ int aj_constant_regularization_matrix_from(double coefficient, const int64_t* ne
                                            double* regularization_matrix, void* stream);
 
 /* numba.py:698-731 reconstruction_positive_negative_from(data_vector f8[S], curvature_reg_matrix f8[S,S])
- * -> f8[S].  FP64 Cholesky (lower triangle of the input is used) and two triangular solves.
- * logdet (may be NULL) receives log|A|; info[0] = 0 or the 1-based failing pivot (AJ_ENOTPD is
- * reported by the caller after reading info). */
+ * -> f8[S], as its np.linalg.solve (numba.py:731): LU with partial pivoting of the whole matrix (symmetric or
+ * not, definite or not), S <= 16384.  logdet (may be NULL) receives log|det A| (np.linalg.slogdet(A)[1],
+ * numba.py:849); reconstruction may be NULL (then data_vector may be NULL too: log-determinant only);
+ * info[0] = 0 or the 1-based column of an exact zero pivot (A singular: numpy raises LinAlgError). */
 size_t aj_solve_workspace_bytes(int64_t S);
 int aj_reconstruction_positive_negative_from(const double* data_vector, const double* curvature_reg_matrix, int64_t S,
                                              double* reconstruction, double* logdet, int64_t* info, void* workspace,

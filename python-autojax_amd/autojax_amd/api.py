@@ -66,26 +66,68 @@ def _view_f8(c: torch.Tensor) -> torch.Tensor:
     return torch.view_as_real(c).reshape(-1)
 
 
-def validate_mapping(pix_indexes_for_sub_slim_index, pixels, unique=True):
-    """Indices must lie in [-1, pixels); with ``unique`` no index repeats within a row.
+def _live(idx):
+    """Entries before a row's first -1 (numba.py:372: a -1 ends the row)."""
+    live = np.ones(idx.shape, bool)
+    if idx.shape[1] > 1:
+        live[:, 1:] = np.cumprod(idx[:, :-1] != -1, axis=1).astype(bool)
+    return live & (idx != -1)
 
-    The reference writes out of bounds for other values (numba.py:371-375);
-    repeated indices are assigned in the dense form and summed in the sparse
-    forms of the reference, so they are rejected here.
-    """
+
+def validate_mapping(pix_indexes_for_sub_slim_index, pixels, unique=False):
+    """Indices must lie in [-1, pixels) (the reference writes out of bounds for other values,
+    numba.py:371-375).  Repeated indices within a row are accepted: the dense mapping assigns them (the
+    last one wins, numba.py:375) and the sparse forms sum them (numba.py:585-586); ``unique=True`` rejects
+    them instead."""
     idx = _host(pix_indexes_for_sub_slim_index)
     if idx.size and (idx.min() < -1 or idx.max() >= pixels):
         raise ValueError(f"pix_indexes_for_sub_slim_index out of range [-1, {pixels})")
-    if unique and idx.shape[1] > 1:
-        live = np.ones(idx.shape, bool)
-        live[:, 1:] = np.cumprod(idx[:, :-1] != -1, axis=1).astype(bool)
-        s = np.where(live & (idx >= 0), idx, -2 - np.arange(idx.shape[1])[None, :])
-        ss = np.sort(s, axis=1)
-        if np.any((ss[:, 1:] == ss[:, :-1]) & (ss[:, 1:] >= 0)):
-            raise ValueError("pix_indexes_for_sub_slim_index repeats a source pixel within a row")
+    if unique and has_repeated_indices(idx):
+        raise ValueError("pix_indexes_for_sub_slim_index repeats a source pixel within a row")
 
 
-def validate_neighbors(neighbors, neighbors_sizes, symmetric=True):
+def has_repeated_indices(pix_indexes_for_sub_slim_index) -> bool:
+    """True when a row names one source pixel twice before its first -1."""
+    idx = _host(pix_indexes_for_sub_slim_index)
+    if idx.ndim != 2 or idx.shape[1] < 2:
+        return False
+    s = np.where(_live(idx), idx, -2 - np.arange(idx.shape[1])[None, :])
+    ss = np.sort(s, axis=1)
+    return bool(np.any((ss[:, 1:] == ss[:, :-1]) & (ss[:, 1:] >= 0)))
+
+
+def assignment_weights(pix_indexes_for_sub_slim_index, pix_weights_for_sub_slim_index):
+    """Weights for which summing the mapping's entries gives the dense mapping matrix of numba.py:368-375,
+    which assigns: where a row names a source pixel more than once, every entry but the last gets weight 0.
+    Returns the input unchanged when no row repeats an index."""
+    idx = _host(pix_indexes_for_sub_slim_index)
+    if not has_repeated_indices(idx):
+        return pix_weights_for_sub_slim_index
+    w = np.array(_host(pix_weights_for_sub_slim_index), dtype=np.float64, copy=True)
+    live = _live(idx)
+    B = idx.shape[1]
+    for b in range(B - 1):
+        later = (idx[:, b + 1:] == idx[:, b:b + 1]) & live[:, b + 1:]
+        w[later.any(axis=1) & live[:, b], b] = 0.0
+    if _is_t(pix_weights_for_sub_slim_index):
+        return torch.from_numpy(w).to(pix_weights_for_sub_slim_index.device)
+    return w
+
+
+def neighbors_symmetric(neighbors, neighbors_sizes) -> bool:
+    """True when k lists i whenever i lists k (before the rows' first -1): then the regularisation matrix of
+    numba.py:686-694 is symmetric."""
+    nb = _host(neighbors)
+    S = _host(neighbors_sizes).size
+    live = _live(nb) & (nb >= 0)
+    i = np.repeat(np.arange(S), nb.shape[1])[live.ravel()]
+    k = nb.ravel()[live.ravel()]
+    a = set(zip(i.tolist(), k.tolist()))
+    return all((kk, ii) in a for ii, kk in a)
+
+
+def validate_neighbors(neighbors, neighbors_sizes, symmetric=False):
+    """Shapes and index range ([-1, S)); with ``symmetric`` also that the lists form a symmetric relation."""
     nb = _host(neighbors)
     sz = _host(neighbors_sizes)
     S = sz.size
@@ -93,15 +135,8 @@ def validate_neighbors(neighbors, neighbors_sizes, symmetric=True):
         raise ValueError(f"neighbors has {nb.shape[0]} rows, neighbors_sizes {S}")
     if nb.size and (nb.min() < -1 or nb.max() >= S):
         raise ValueError(f"neighbors out of range [-1, {S})")
-    if symmetric:
-        live = np.ones(nb.shape, bool)
-        live[:, 1:] = np.cumprod(nb[:, :-1] != -1, axis=1).astype(bool)
-        live &= nb >= 0
-        i = np.repeat(np.arange(S), nb.shape[1])[live.ravel()]
-        k = nb.ravel()[live.ravel()]
-        a = set(zip(i.tolist(), k.tolist()))
-        if any((kk, ii) not in a for ii, kk in a):
-            raise ValueError("neighbors is not a symmetric relation: the regularisation matrix would not be symmetric")
+    if symmetric and not neighbors_symmetric(nb, sz):
+        raise ValueError("neighbors is not a symmetric relation: the regularisation matrix would not be symmetric")
 
 
 # ---------------------------------------------------------------------------
@@ -445,10 +480,25 @@ def constant_regularization_matrix_from(coefficient, neighbors, neighbors_sizes)
     return _out(out, torch_in)
 
 
-def reconstruction_positive_negative_from(data_vector, curvature_reg_matrix):
-    """numba.py:698-731 -> f8[S] solving A s = D (FP64 Cholesky; A symmetric positive definite).
+def _lu_solve(A, D=None):
+    """(x, log|det A|, info) on the device: LU with partial pivoting of the general matrix A (np.linalg.solve /
+    slogdet); x is None when D is None (log-determinant only); info 0 or the 1-based zero pivot."""
+    S = A.shape[0]
+    x = dv.empty((S,)) if D is not None else None
+    ld = dv.empty((1,))
+    info = torch.zeros(1, dtype=torch.int64, device=A.device)
+    ws = dv.workspace(_lib.query("aj_solve_workspace_bytes", S))
+    _lib.call("aj_reconstruction_positive_negative_from", None if D is None else D.data_ptr(), A.data_ptr(), S,
+              None if x is None else x.data_ptr(), ld.data_ptr(), info.data_ptr(), ws.data_ptr(), ws.numel(),
+              dv.stream_ptr())
+    return x, ld, info
 
-    Raises numpy.linalg.LinAlgError when A is not positive definite.
+
+def reconstruction_positive_negative_from(data_vector, curvature_reg_matrix):
+    """numba.py:698-731 -> f8[S] solving A s = D as np.linalg.solve does: LU with partial pivoting of the
+    whole matrix (symmetric or not, definite or not).
+
+    Raises numpy.linalg.LinAlgError when A is singular (an exact zero pivot), as numpy does.
     """
     torch_in = _any_t(data_vector, curvature_reg_matrix)
     D = _dev(data_vector, np.float64, "data_vector", 1)
@@ -456,14 +506,10 @@ def reconstruction_positive_negative_from(data_vector, curvature_reg_matrix):
     S = D.shape[0]
     if A.shape != (S, S):
         raise ValueError("curvature_reg_matrix must be [S, S]")
-    out = dv.empty((S,))
-    info = torch.zeros(1, dtype=torch.int64, device=out.device)
-    ws = dv.workspace(_lib.query("aj_solve_workspace_bytes", S))
-    _lib.call("aj_reconstruction_positive_negative_from", D.data_ptr(), A.data_ptr(), S, out.data_ptr(), None,
-              info.data_ptr(), ws.data_ptr(), ws.numel(), dv.stream_ptr())
+    out, _, info = _lu_solve(A, D)
     bad = int(info.item())
     if bad:
-        raise np.linalg.LinAlgError(f"curvature_reg_matrix is not positive definite (pivot {bad})")
+        raise np.linalg.LinAlgError(f"Singular matrix (zero pivot in column {bad})")
     return _out(out, torch_in)
 
 
@@ -475,47 +521,74 @@ def noise_normalization_complex_from(noise_map):
     return float(out.item())
 
 
+def _loglike_composed(y, d, n, F, idx, w_assign, nb, sz, S):
+    """numba.py:818-857 stage by stage on the device, for the inputs the Cholesky pipeline does not take
+    (neighbour lists that are not symmetric, or curvature_reg_matrix / regularization_matrix not positive
+    definite): H (numba.py:686-694), A = F + H, D = T^T y with the dense mapping's assignment, s and both
+    log-determinants by LU with partial pivoting (np.linalg.solve / slogdet), chi^2 and the noise
+    normalisation.  Raises LinAlgError when A or H is singular."""
+    H = constant_regularization_matrix_from(1.0, nb, sz)
+    A = F + H
+    D = sparse_mapping_matrix_transpose_matmul(y.reshape(-1, 1), idx, w_assign, S).reshape(-1)
+    s, ld_a, inf_a = _lu_solve(A, D)
+    _, ld_h, inf_h = _lu_solve(H)
+    c2nn = dv.empty((2,))
+    _lib.call("aj_chi_squared_noise_normalization", d.data_ptr(), n.data_ptr(), d.numel() // 2, c2nn.data_ptr(),
+              dv.stream_ptr())
+    if int(inf_a.item()) or int(inf_h.item()):
+        which = "curvature_reg_matrix" if int(inf_a.item()) else "regularization_matrix"
+        raise np.linalg.LinAlgError(f"Singular matrix ({which})")
+    return float((-0.5 * (c2nn[0] - s @ D + ld_a[0] - ld_h[0] + c2nn[1])).item())
+
+
 def log_likelihood_function_via_w_tilde_from(dirty_image, data, noise_map, w_tilde, pix_indexes_for_sub_slim_index,
                                              pix_weights_for_sub_slim_index, neighbors, neighbors_sizes):
     """numba.py:755-857 -> float (coefficient 1, numba.py:819).
 
-    Sparse mapping path: F = T^T W T by source-pixel gathers, FP64 Cholesky of
-    A = F + H and of H (the reference's two LU slogdets and solve).
+    Symmetric neighbour lists (every mesh's): F = T^T W T by source-pixel gathers, FP64 Cholesky of
+    A = F + H and of H on the device (the reference's LU solve and two slogdets, in the form that holds for
+    symmetric positive definite matrices).  Otherwise, or when A or H is not positive definite, the
+    reference's own steps (LU solve and LU log-determinants) on the device.  Rows that name a source pixel
+    twice follow the dense mapping's assignment (numba.py:375: the last entry wins).
     """
     S = int(_host(neighbors_sizes).size)
     validate_mapping(pix_indexes_for_sub_slim_index, S)
     validate_neighbors(neighbors, neighbors_sizes)
+    w_assign = assignment_weights(pix_indexes_for_sub_slim_index, pix_weights_for_sub_slim_index)
     y = _dev(dirty_image, np.float64, "dirty_image", 1)
     d = _view_f8(_dev(data, np.complex128, "data", 1))
     n = _view_f8(_dev(noise_map, np.complex128, "noise_map", 1))
     W = _dev(w_tilde, np.float64, "w_tilde", 2)
     idx = _dev(pix_indexes_for_sub_slim_index, np.int64, "pix_indexes_for_sub_slim_index", 2)
-    w = _dev(pix_weights_for_sub_slim_index, np.float64, "pix_weights_for_sub_slim_index", 2)
+    w = _dev(w_assign, np.float64, "pix_weights_for_sub_slim_index", 2)
     nb = _dev(neighbors, np.int64, "neighbors", 2)
     sz = _dev(neighbors_sizes, np.int64, "neighbors_sizes", 1)
     M, B = idx.shape
     K = d.numel() // 2
     P = nb.shape[1]
-    if W.shape != (M, M) or y.shape[0] != M or n.numel() != 2 * K:
+    if W.shape != (M, M) or y.shape[0] != M or n.numel() != 2 * K or w.shape != (M, B):
         raise ValueError("shape mismatch between dirty_image, w_tilde, mapping, data and noise_map")
-    out = dv.empty((1,))
-    info = torch.zeros(2, dtype=torch.int64, device=out.device)
-    ws = dv.workspace(_lib.query("aj_log_likelihood_workspace_bytes", M, K, B, S, P))
-    _lib.call("aj_log_likelihood_function_via_w_tilde_from", y.data_ptr(), d.data_ptr(), n.data_ptr(), W.data_ptr(),
-              idx.data_ptr(), w.data_ptr(), nb.data_ptr(), sz.data_ptr(), M, K, B, S, P, out.data_ptr(),
-              info.data_ptr(), ws.data_ptr(), ws.numel(), dv.stream_ptr())
-    inf = info.tolist()
-    if inf[0] or inf[1]:
-        which = "curvature_reg_matrix" if inf[0] else "regularization_matrix"
-        raise np.linalg.LinAlgError(f"{which} is not positive definite (pivot {inf[0] or inf[1]})")
-    return float(out.item())
+    if neighbors_symmetric(neighbors, neighbors_sizes):
+        out = dv.empty((1,))
+        info = torch.zeros(2, dtype=torch.int64, device=out.device)
+        ws = dv.workspace(_lib.query("aj_log_likelihood_workspace_bytes", M, K, B, S, P))
+        _lib.call("aj_log_likelihood_function_via_w_tilde_from", y.data_ptr(), d.data_ptr(), n.data_ptr(),
+                  W.data_ptr(), idx.data_ptr(), w.data_ptr(), nb.data_ptr(), sz.data_ptr(), M, K, B, S, P,
+                  out.data_ptr(), info.data_ptr(), ws.data_ptr(), ws.numel(), dv.stream_ptr())
+        if not any(info.tolist()):
+            return float(out.item())
+    F = curvature_matrix_via_w_wilde_sparse_mapping_matrix_from(W, idx, w, S)
+    return _loglike_composed(y, d, n, F, idx, w, nb, sz, S)
 
 
 def log_likelihood_function_via_w_compact_from(dirty_image, data, noise_map, w_compact, native_index_for_slim_index,
                                                pix_indexes_for_sub_slim_index, pix_weights_for_sub_slim_index,
                                                neighbors, neighbors_sizes):
     """numba.py:860-967 -> float (coefficient 1, numba.py:923): as
-    log_likelihood_function_via_w_tilde_from with W-tilde read from the lattice table."""
+    log_likelihood_function_via_w_tilde_from with W-tilde read from the lattice table.  The reference forms
+    F by the sparse products (repeated indices within a row summed, numba.py:930-936) and D from the dense
+    mapping (assigned, numba.py:947-953); inputs with such rows, neighbour lists that are not symmetric, or
+    a matrix that is not positive definite take the stage-by-stage route with exactly those semantics."""
     S = int(_host(neighbors_sizes).size)
     validate_mapping(pix_indexes_for_sub_slim_index, S)
     validate_neighbors(neighbors, neighbors_sizes)
@@ -535,16 +608,19 @@ def log_likelihood_function_via_w_compact_from(dirty_image, data, noise_map, w_c
     M, B = idx.shape
     K = d.numel() // 2
     P = nb.shape[1]
-    if ni.shape[0] != M or y.shape[0] != M or n.numel() != 2 * K:
+    if ni.shape[0] != M or y.shape[0] != M or n.numel() != 2 * K or w.shape != (M, B):
         raise ValueError("shape mismatch between dirty_image, native indexes, mapping, data and noise_map")
-    out = dv.empty((1,))
-    info = torch.zeros(2, dtype=torch.int64, device=out.device)
-    ws = dv.workspace(_lib.query("aj_log_likelihood_workspace_bytes", M, K, B, S, P))
-    _lib.call("aj_log_likelihood_function_via_w_compact_from", y.data_ptr(), d.data_ptr(), n.data_ptr(), C.data_ptr(),
-              N, ni.data_ptr(), idx.data_ptr(), w.data_ptr(), nb.data_ptr(), sz.data_ptr(), M, K, B, S, P,
-              out.data_ptr(), info.data_ptr(), ws.data_ptr(), ws.numel(), dv.stream_ptr())
-    inf = info.tolist()
-    if inf[0] or inf[1]:
-        which = "curvature_reg_matrix" if inf[0] else "regularization_matrix"
-        raise np.linalg.LinAlgError(f"{which} is not positive definite (pivot {inf[0] or inf[1]})")
-    return float(out.item())
+    repeated = has_repeated_indices(pix_indexes_for_sub_slim_index)
+    if not repeated and neighbors_symmetric(neighbors, neighbors_sizes):
+        out = dv.empty((1,))
+        info = torch.zeros(2, dtype=torch.int64, device=out.device)
+        ws = dv.workspace(_lib.query("aj_log_likelihood_workspace_bytes", M, K, B, S, P))
+        _lib.call("aj_log_likelihood_function_via_w_compact_from", y.data_ptr(), d.data_ptr(), n.data_ptr(),
+                  C.data_ptr(), N, ni.data_ptr(), idx.data_ptr(), w.data_ptr(), nb.data_ptr(), sz.data_ptr(), M, K,
+                  B, S, P, out.data_ptr(), info.data_ptr(), ws.data_ptr(), ws.numel(), dv.stream_ptr())
+        if not any(info.tolist()):
+            return float(out.item())
+    F = curvature_matrix_via_w_compact_sparse_mapping_matrix_from(C, ni, idx, w, S)
+    w_assign = _dev(assignment_weights(pix_indexes_for_sub_slim_index, pix_weights_for_sub_slim_index), np.float64,
+                    "pix_weights_for_sub_slim_index", 2)
+    return _loglike_composed(y, d, n, F, idx, w_assign, nb, sz, S)

@@ -24,7 +24,7 @@ import torch
 
 from . import _lib
 from . import device as dv
-from .api import validate_mapping, validate_neighbors
+from .api import assignment_weights, validate_mapping, validate_neighbors
 
 
 def deflation_beta(neighbors, neighbors_sizes, coefficient) -> float:
@@ -77,10 +77,12 @@ class SampleBatch:
             if s.pix_indexes_for_sub_slim_index.shape != (M, B):
                 raise ValueError(f"sample {b}: mapping shape {s.pix_indexes_for_sub_slim_index.shape} != {(M, B)}")
             if validate:
+                # the batched Cholesky factors the lower triangles: the regularisation matrix must be symmetric
                 validate_mapping(s.pix_indexes_for_sub_slim_index, s.pixels)
-                validate_neighbors(s.neighbors, s.neighbors_sizes)
+                validate_neighbors(s.neighbors, s.neighbors_sizes, symmetric=True)
             idx[b] = s.pix_indexes_for_sub_slim_index
-            wts[b] = s.pix_weights_for_sub_slim_index
+            # a row naming a source pixel twice: the dense mapping's assignment (numba.py:375, the last wins)
+            wts[b] = assignment_weights(s.pix_indexes_for_sub_slim_index, s.pix_weights_for_sub_slim_index)
             S = s.pixels
             nbr[off : off + S, : s.neighbors.shape[1]] = s.neighbors
             nsz[off : off + S] = s.neighbors_sizes

@@ -329,134 +329,8 @@ __global__ void regularize_kernel(const SampleDesc* __restrict__ sd, const int64
   }
 }
 
-// ---------------------------------------------------------------------------
-// FP64 Cholesky of one matrix by one workgroup (256 threads), in place on the
-// lower triangle (the single-matrix solve entry point).  Right-looking, 32-wide
-// panels; the trailing update runs on v_mfma_f64_16x16x4, each wave on its own
-// 32x32 blocks.  *info_out = 0 or the 1-based column of the first non-positive
-// pivot; *logdet_out = 2 sum log L_jj.
-// ---------------------------------------------------------------------------
+// Column-block width of the blocked Cholesky factorisations.
 constexpr int NB = 32;
-
-__device__ void chol_one(double* __restrict__ A, int64_t n, double* logdet_out, int64_t* info_out) {
-  __shared__ double Lkk[NB][NB + 1];
-  __shared__ double Xs[256][NB + 1];
-  __shared__ int fail;
-  __shared__ double ld_acc;
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
-  if (tid == 0) {
-    fail = 0;
-    ld_acc = 0.0;
-  }
-  __syncthreads();
-  for (int64_t k0 = 0; k0 < n; k0 += NB) {
-    const int nb = (int)std::min<int64_t>(NB, n - k0);
-    // (a) factor the diagonal block in LDS
-    for (int e = tid; e < NB * NB; e += 256) {
-      int i = e / NB, j = e % NB;
-      Lkk[i][j] = (i < nb && j <= i) ? A[(k0 + i) * n + k0 + j] : 0.0;
-    }
-    __syncthreads();
-    for (int j = 0; j < nb; ++j) {
-      if (tid == 0) {
-        double dj = Lkk[j][j];
-        if (!(dj > 0.0)) {
-          if (!fail) fail = (int)(k0 + j + 1);
-          dj = 1.0;  // keep the arithmetic finite; the result is flagged
-        }
-        dj = sqrt(dj);
-        Lkk[j][j] = dj;
-        ld_acc += log(dj);
-      }
-      __syncthreads();
-      if (tid > j && tid < nb) Lkk[tid][j] /= Lkk[j][j];
-      __syncthreads();
-      for (int e = tid; e < NB * NB; e += 256) {
-        int i = e / NB, l = e % NB;
-        if (i > j && l > j && l <= i && i < nb) Lkk[i][l] -= Lkk[i][j] * Lkk[l][j];
-      }
-      __syncthreads();
-    }
-    for (int e = tid; e < NB * NB; e += 256) {
-      int i = e / NB, j = e % NB;
-      if (i < nb && j <= i) A[(k0 + i) * n + k0 + j] = Lkk[i][j];
-    }
-    // (b) panel rows below: x L_kk^T = a; each thread solves its own row,
-    // kept in its own LDS row (no cross-thread traffic, no barriers).
-    const int64_t r0 = k0 + nb;
-    for (int64_t base = r0; base < n; base += 256) {
-      const int64_t i = base + tid;
-      if (i < n) {
-        double* row = A + i * n + k0;
-        for (int j = 0; j < nb; ++j) Xs[tid][j] = row[j];
-        for (int j = 0; j < nb; ++j) {
-          double v = Xs[tid][j];
-          for (int l = 0; l < j; ++l) v -= Xs[tid][l] * Lkk[j][l];
-          Xs[tid][j] = v / Lkk[j][j];
-        }
-        for (int j = 0; j < nb; ++j) row[j] = Xs[tid][j];
-      }
-    }
-    __syncthreads();
-    // (c) trailing update A[r0:, r0:] -= P P^T on the lower triangle, P = A[r0:, k0:k0+nb].
-    // Each wave owns whole 32x32 blocks (2x2 MFMA 16x16 blocks) and reads the panel rows
-    // straight from global memory, so the four waves work on different blocks without barriers.
-    {
-      const int64_t m = n - r0;
-      const int64_t nbk = (m + 31) / 32;
-      const int64_t nblk = nbk * (nbk + 1) / 2;
-      const int lr = lane & 15, lk = lane >> 4;
-      for (int64_t blk = wave; blk < nblk; blk += 4) {
-        int64_t bi = (int64_t)((sqrt(8.0 * (double)blk + 1.0) - 1.0) * 0.5);
-        while (bi * (bi + 1) / 2 > blk) --bi;
-        while ((bi + 1) * (bi + 2) / 2 <= blk) ++bi;
-        const int64_t bj = blk - bi * (bi + 1) / 2;
-        const int64_t R0 = r0 + bi * 32, C0 = r0 + bj * 32;
-        d4 acc[2][2];
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-          for (int b = 0; b < 2; ++b)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-              const int64_t gi = R0 + a * 16 + lk + 4 * r, gj = C0 + b * 16 + lr;
-              acc[a][b][r] = (gi < n && gj <= gi) ? A[gi * n + gj] : 0.0;
-            }
-        const int64_t ia0 = R0 + lr, ia1 = R0 + 16 + lr, jb0 = C0 + lr, jb1 = C0 + 16 + lr;
-#pragma unroll
-        for (int ks = 0; ks < NB / 4; ++ks) {
-          const int p = ks * 4 + lk;
-          if (ks * 4 >= nb) break;
-          const bool pv = p < nb;
-          const double fa0 = (pv && ia0 < n) ? -A[ia0 * n + k0 + p] : 0.0;
-          const double fa1 = (pv && ia1 < n) ? -A[ia1 * n + k0 + p] : 0.0;
-          const double fb0 = (pv && jb0 < n) ? A[jb0 * n + k0 + p] : 0.0;
-          const double fb1 = (pv && jb1 < n) ? A[jb1 * n + k0 + p] : 0.0;
-          acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa0, fb0, acc[0][0], 0, 0, 0);
-          acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa0, fb1, acc[0][1], 0, 0, 0);
-          acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa1, fb0, acc[1][0], 0, 0, 0);
-          acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa1, fb1, acc[1][1], 0, 0, 0);
-        }
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-          for (int b = 0; b < 2; ++b)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-              const int64_t gi = R0 + a * 16 + lk + 4 * r, gj = C0 + b * 16 + lr;
-              if (gi < n && gj <= gi) A[gi * n + gj] = acc[a][b][r];
-            }
-      }
-      __syncthreads();
-    }
-  }
-  __syncthreads();
-  if (tid == 0) {
-    *logdet_out = 2.0 * ld_acc;
-    *info_out = fail;
-  }
-}
 
 
 
@@ -1261,42 +1135,6 @@ __global__ void reg_standalone_kernel(double coef2, const int64_t* __restrict__ 
   }
 }
 
-__global__ void copy_lower_kernel(const double* __restrict__ src, int64_t n, double* __restrict__ dst) {
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n * n; e += (int64_t)gridDim.x * blockDim.x)
-    dst[e] = src[e];
-}
-
-__global__ void __launch_bounds__(256) chol_solve_kernel(double* __restrict__ A, int64_t n,
-                                                         const double* __restrict__ D, double* __restrict__ s,
-                                                         double* __restrict__ logdet, int64_t* __restrict__ info) {
-  extern __shared__ double y[];
-  double ld;
-  int64_t inf;
-  chol_one(A, n, &ld, &inf);
-  const int tid = threadIdx.x;
-  for (int64_t i = tid; i < n; i += 256) y[i] = D[i];
-  __syncthreads();
-  for (int64_t j = 0; j < n; ++j) {
-    const double yj = y[j] / A[j * n + j];
-    __syncthreads();
-    if (tid == 0) y[j] = yj;
-    for (int64_t i = j + 1 + tid; i < n; i += 256) y[i] -= A[i * n + j] * yj;
-    __syncthreads();
-  }
-  for (int64_t j = n - 1; j >= 0; --j) {
-    const double sj = y[j] / A[j * n + j];
-    __syncthreads();
-    if (tid == 0) y[j] = sj;
-    for (int64_t i = tid; i < j; i += 256) y[i] -= A[j * n + i] * sj;
-    __syncthreads();
-  }
-  for (int64_t i = tid; i < n; i += 256) s[i] = y[i];
-  if (tid == 0) {
-    if (logdet) *logdet = ld;
-    *info = inf;
-  }
-}
-
 // Data-side sums over visibilities, per image pixel, absolute phases:
 //   out_i = sum_k (a_k cos 2pi r_ik + b_k sin 2pi r_ik),  r_ik = g_i1 u_k0 + g_i0 u_k1.
 // Partial sums per K-split, reduced in a fixed order (deterministic).
@@ -1551,24 +1389,6 @@ extern "C" int aj_constant_regularization_matrix_from(double coefficient, const 
   AJ_CHECK_HIP(hipMemsetAsync(regularization_matrix, 0, sizeof(double) * S * S, st));
   hipLaunchKernelGGL(reg_standalone_kernel, dim3((unsigned)ceil_div(S, 256)), dim3(256), 0, st,
                      coefficient * coefficient, neighbors, neighbors_sizes, S, P, regularization_matrix);
-  AJ_LAUNCH_CHECK();
-  return AJ_OK;
-}
-
-extern "C" size_t aj_solve_workspace_bytes(int64_t S) { return S > 0 ? sizeof(double) * S * S : 0; }
-
-extern "C" int aj_reconstruction_positive_negative_from(const double* data_vector, const double* curvature_reg_matrix,
-                                                        int64_t S, double* reconstruction, double* logdet,
-                                                        int64_t* info, void* workspace, size_t workspace_bytes,
-                                                        void* stream) {
-  AJ_REQUIRE(S > 0 && data_vector && curvature_reg_matrix && reconstruction && info, "bad arguments");
-  AJ_REQUIRE(workspace && workspace_bytes >= aj_solve_workspace_bytes(S), "workspace too small");
-  hipStream_t st = as_stream(stream);
-  double* L = static_cast<double*>(workspace);
-  hipLaunchKernelGGL(copy_lower_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(S * S, 256), 4096)), dim3(256), 0,
-                     st, curvature_reg_matrix, S, L);
-  hipLaunchKernelGGL(chol_solve_kernel, dim3(1), dim3(256), sizeof(double) * S, st, L, S, data_vector,
-                     reconstruction, logdet, info);
   AJ_LAUNCH_CHECK();
   return AJ_OK;
 }

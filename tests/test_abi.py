@@ -57,7 +57,8 @@ def test_workspace_queries_without_gpu():
 
     assert _lib.query("aj_w_tilde_workspace_bytes", 2000, 716, 0) > 716 * 8192 * 8
     assert _lib.query("aj_w_tilde_workspace_bytes", 2000, 716, 99) == 0
-    assert _lib.query("aj_solve_workspace_bytes", 100) == 800 * 100
+    # LU workspace: the factors (S^2 doubles) and the pivots (S int64), each 256-byte aligned
+    assert _lib.query("aj_solve_workspace_bytes", 100) >= 800 * 100 + 8 * 100
     assert _lib.load().aj_abi_version() == 1
 
 
@@ -166,12 +167,16 @@ def test_host_validation():
 
     with pytest.raises(ValueError):
         api.validate_mapping(np.array([[0, 5]]), 5)
+    api.validate_mapping(np.array([[1, 1]]), 5)  # repeats are accepted (assigned or summed as the reference)
     with pytest.raises(ValueError):
-        api.validate_mapping(np.array([[1, 1]]), 5)
-    api.validate_mapping(np.array([[1, -1, 1]]), 5)  # repeat after the -1 is ignored
+        api.validate_mapping(np.array([[1, 1]]), 5, unique=True)
+    api.validate_mapping(np.array([[1, -1, 1]]), 5, unique=True)  # repeat after the -1 is ignored
     with pytest.raises(ValueError):
-        api.validate_neighbors(np.array([[1], [-1]]), np.array([1, 0]))
-    api.validate_neighbors(np.array([[1], [0]]), np.array([1, 1]))
+        api.validate_neighbors(np.array([[1], [-1]]), np.array([1, 0]), symmetric=True)
+    api.validate_neighbors(np.array([[1], [-1]]), np.array([1, 0]))  # one-sided lists are the reference's input too
+    api.validate_neighbors(np.array([[1], [0]]), np.array([1, 1]), symmetric=True)
+    with pytest.raises(ValueError):
+        api.validate_neighbors(np.array([[2], [0]]), np.array([1, 1]))  # out of range
 
 
 def test_bench_refuses_more_than_one_gpu():
@@ -201,3 +206,41 @@ def test_backward_workspace_holds_per_sample_vectors_when_sources_outnumber_pixe
         got = _lib.query("aj_log_evidence_backward_workspace_bytes", M, len(S), Sa.ctypes.data_as(ctypes.c_void_p))
         need = align(8 * int((Sa**2).sum())) + align(8 * int(Sa.sum())) + 2 * align(8 * M * len(S))
         assert got >= need, (M, S, got, need)
+
+
+def test_assignment_weights_and_neighbour_symmetry():
+    """Host logic of the drop-in likelihood: a row naming a source pixel twice keeps only the last entry's
+    weight (numba.py:375 assigns), entries after a row's -1 are ignored; symmetric and one-sided neighbour
+    lists are told apart."""
+    from autojax_amd import api
+
+    idx = np.array([[3, 1, 3], [2, 2, 2], [0, -1, 0], [4, 5, 6]], np.int64)
+    w = np.array([[0.1, 0.2, 0.3], [0.4, 0.5, 0.6], [0.7, 0.8, 0.9], [1.0, 1.1, 1.2]])
+    assert api.has_repeated_indices(idx)
+    got = api.assignment_weights(idx, w)
+    np.testing.assert_array_equal(got, [[0.0, 0.2, 0.3], [0.0, 0.0, 0.6], [0.7, 0.8, 0.9], [1.0, 1.1, 1.2]])
+    # summing the kept weights reproduces the dense assignment of every row
+    dense = np.zeros((4, 7))
+    for m in range(4):
+        for b in range(3):
+            if idx[m, b] == -1:
+                break
+            dense[m, idx[m, b]] = w[m, b]
+    summed = np.zeros((4, 7))
+    for m in range(4):
+        for b in range(3):
+            if idx[m, b] == -1:
+                break
+            summed[m, idx[m, b]] += got[m, b]
+    np.testing.assert_array_equal(summed, dense)
+    plain = np.array([[0, 1], [2, -1]], np.int64)
+    assert not api.has_repeated_indices(plain)
+    assert api.assignment_weights(plain, w[:2, :2]) is w[:2, :2] or np.array_equal(api.assignment_weights(plain, w[:2, :2]), w[:2, :2])
+    nb = np.array([[1, 2], [0, -1], [0, -1]], np.int64)
+    sz = np.array([2, 1, 1], np.int64)
+    assert api.neighbors_symmetric(nb, sz)
+    one_sided = np.array([[1, 2], [0, -1], [-1, -1]], np.int64)
+    assert not api.neighbors_symmetric(one_sided, np.array([2, 1, 0], np.int64))
+    with pytest.raises(ValueError, match="symmetric"):
+        api.validate_neighbors(one_sided, np.array([2, 1, 0], np.int64), symmetric=True)
+    api.validate_neighbors(one_sided, np.array([2, 1, 0], np.int64))
