@@ -369,8 +369,13 @@ def test_regularization_solve_noise(gpu, delaunay_small):
     assert _rel(s, so) < 1e-9
     nn = api.noise_normalization_complex_from(d.noise_map)
     assert abs(nn - R.noise_normalization_complex_from(d.noise_map)) < 1e-12 * abs(nn)
+    # negative definite: solved, as np.linalg.solve does (numba.py:731), giving -s
+    np.testing.assert_allclose(api.reconstruction_positive_negative_from(Dv, -A), -so, rtol=0,
+                               atol=1e-9 * np.abs(so).max())
+    singular = A.copy()
+    singular[:, 3] = 0.0
     with pytest.raises(np.linalg.LinAlgError):
-        api.reconstruction_positive_negative_from(Dv, -A)
+        api.reconstruction_positive_negative_from(Dv, singular)
 
 
 def test_log_likelihood_golden_evidence(gpu):
