@@ -224,6 +224,16 @@ int aj_reconstruction_positive_negative_from(const double* data_vector, const do
                                              double* reconstruction, double* logdet, int64_t* info, void* workspace,
                                              size_t workspace_bytes, void* stream);
 
+/* aj_reconstruction_positive_negative_from without a workspace argument and without any allocation: no
+ * counterpart in the reference; for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Its scratch is a file-scope device array
+ * of 33,571,072 bytes (the workspace of S = 2048), one per GPU, held until the process ends; S above 2048
+ * is rejected with AJ_EINVAL.  Every call on a GPU shares that GPU's scratch: calls that overlap on one GPU
+ * (other streams or host threads) race and must be serialised by the caller. */
+int aj_reconstruction_positive_negative_from_noalloc(const double* data_vector, const double* curvature_reg_matrix,
+                                                     int64_t S, double* reconstruction, double* logdet, int64_t* info,
+                                                     void* stream);
+
 /* numba.py:734-752 noise_normalization_complex_from(noise_map c16[K]) -> f8 (device scalar). */
 int aj_noise_normalization_complex_from(const double* noise_map, int64_t K, double* out, void* stream);
 

@@ -63,6 +63,7 @@ SIGNATURES = {
     "aj_constant_regularization_matrix_from": (_INT, [_D, _P, _P, _I, _I, _P, _P]),
     "aj_solve_workspace_bytes": (_S, [_I]),
     "aj_reconstruction_positive_negative_from": (_INT, [_P, _P, _I, _P, _P, _P, _P, _S, _P]),
+    "aj_reconstruction_positive_negative_from_noalloc": (_INT, [_P, _P, _I, _P, _P, _P, _P]),
     "aj_noise_normalization_complex_from": (_INT, [_P, _I, _P, _P]),
     "aj_chi_squared_noise_normalization": (_INT, [_P, _P, _I, _P, _P]),
     "aj_log_likelihood_workspace_bytes": (_S, [_I, _I, _I, _I, _I]),

@@ -18,6 +18,7 @@
 #include "aj_common.h"
 
 #include <cmath>
+#include <mutex>
 
 namespace aj {
 namespace lu {
@@ -326,6 +327,34 @@ int lu_solve(const double* data_vector, const double* curvature_reg_matrix, int6
   return AJ_OK;
 }
 
+// Scratch of the allocation-free variant: the workspace of the largest S it takes, one copy per device
+// (a file-scope device array: each GPU that loads the library holds its own until the process ends).
+constexpr int64_t NOALLOC_S_MAX = 2048;
+constexpr size_t NOALLOC_BYTES = ((sizeof(double) * NOALLOC_S_MAX * NOALLOC_S_MAX + 255) / 256) * 256 +
+                                 ((sizeof(int64_t) * NOALLOC_S_MAX + 255) / 256) * 256 + 256;
+__device__ __align__(256) char scratch[NOALLOC_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mutex;
+void* scratch_ptr[MAX_DEVICES] = {};
+
+// The device address of `scratch` on the current device, looked up once per device (the array's name in
+// host code is only a handle, never a device address).
+int scratch_on_current_device(void** out) {
+  int dev = 0;
+  AJ_CHECK_HIP(hipGetDevice(&dev));
+  AJ_REQUIRE(dev >= 0 && dev < MAX_DEVICES, "device %d beyond the %d devices this library tracks", dev, MAX_DEVICES);
+  std::lock_guard<std::mutex> lock(scratch_mutex);
+  if (!scratch_ptr[dev]) {
+    void* p = nullptr;
+    AJ_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    AJ_REQUIRE(p != nullptr, "hipGetSymbolAddress returned no address for the solve scratch");
+    scratch_ptr[dev] = p;
+  }
+  *out = scratch_ptr[dev];
+  return AJ_OK;
+}
+
 }  // namespace
 
 extern "C" int aj_reconstruction_positive_negative_from(const double* data_vector, const double* curvature_reg_matrix,
@@ -337,5 +366,21 @@ extern "C" int aj_reconstruction_positive_negative_from(const double* data_vecto
   AJ_REQUIRE(S <= SOLVE_MAX, "S = %lld exceeds the solve's limit of %lld", (long long)S, (long long)SOLVE_MAX);
   AJ_REQUIRE(workspace && workspace_bytes >= aj_solve_workspace_bytes(S), "workspace too small");
   return lu_solve(data_vector, curvature_reg_matrix, S, reconstruction, logdet, info, workspace, workspace_bytes,
+                  as_stream(stream));
+}
+
+extern "C" int aj_reconstruction_positive_negative_from_noalloc(const double* data_vector,
+                                                                const double* curvature_reg_matrix, int64_t S,
+                                                                double* reconstruction, double* logdet,
+                                                                int64_t* info, void* stream) {
+  AJ_REQUIRE(S > 0 && curvature_reg_matrix && info && (reconstruction == nullptr || data_vector != nullptr),
+             "bad arguments");
+  AJ_REQUIRE(S <= NOALLOC_S_MAX, "S = %lld exceeds the allocation-free solve's limit of %lld (its scratch is %zu "
+             "bytes); use aj_reconstruction_positive_negative_from with a workspace", (long long)S,
+             (long long)NOALLOC_S_MAX, NOALLOC_BYTES);
+  void* ws = nullptr;
+  const int rc = scratch_on_current_device(&ws);
+  if (rc != AJ_OK) return rc;
+  return lu_solve(data_vector, curvature_reg_matrix, S, reconstruction, logdet, info, ws, NOALLOC_BYTES,
                   as_stream(stream));
 }
