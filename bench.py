@@ -179,18 +179,33 @@ def main():
     torch.cuda.synchronize()
     log("inputs resident")
     tuned = {}
-    if not args.no_autotune:
-        from autojax_amd import autotune
+    from autojax_amd import autotune
 
+    if not args.no_autotune:
         tuned = autotune.tune_w_tilde(ds.grid_radians_slim, args.precision)
         if tuned:
             log("autotune: " + ", ".join(f"v{v}={t:.2f}ms" for v, t in sorted(tuned.items())))
 
+    # Cholesky schedule of the batched evaluation, tuned on the 64-sample batch and, for the plain path's
+    # one-sample evaluations, on its median-size sample (each path then runs with its own)
+    sched = {"batch": 3, "single": 3}
+    tuned_batch, tuned_single = {}, {}
+    if not args.no_autotune:
+        E.w_tilde(noise, uv, grid, "fp64", out=W_plain, workspace=ws_plain)
+        tuned_batch = autotune.tune_batch(batch, W_plain, y, c2nn)
+        mid = singles[int(np.argsort(batch.S_host)[len(singles) // 2])]
+        tuned_single = autotune.tune_batch(mid, W_plain, y, c2nn)
+        sched = {"batch": min(tuned_batch, key=tuned_batch.get), "single": min(tuned_single, key=tuned_single.get)}
+        log("autotune batch schedule: " + ", ".join(f"{m}={t:.2f}ms" for m, t in sorted(tuned_batch.items())) +
+            "; single: " + ", ".join(f"{m}={t:.2f}ms" for m, t in sorted(tuned_single.items())))
+
     def mixed_step():
+        _lib.call("aj_log_evidence_configure", sched["batch"])
         E.w_tilde(noise, uv, grid, args.precision, out=W_mixed, workspace=ws_mixed)
         return batch.log_evidence(W_mixed, y, c2nn)
 
     def plain_step():
+        _lib.call("aj_log_evidence_configure", sched["single"])
         E.w_tilde(noise, uv, grid, "fp64", out=W_plain, workspace=ws_plain)
         return torch.cat([s.log_evidence(W_plain, y, c2nn).clone() for s in singles])
 
@@ -220,6 +235,7 @@ def main():
     ws_wc = dv.workspace(_lib.query("aj_w_compact_workspace_bytes", K, N))
 
     def lattice_step():
+        _lib.call("aj_log_evidence_configure", sched["batch"])
         _lib.call("aj_w_compact_curvature_interferometer_from", noise.data_ptr(), uv.data_ptr(), K, N,
                   float(ds.pixel_scale), wc.data_ptr(), ws_wc.data_ptr(), ws_wc.numel(), dv.stream_ptr())
         return batch.log_evidence_compact(wc, native, y, c2nn)
@@ -300,12 +316,17 @@ def main():
                      "raw_mfma_frac": round(achieved * RAW_FACTOR[args.precision] / peak, 5),
                      "avg_launch_ms": round(syrk_ms / max(syrk_launches, 1), 4), "launches_per_step": syrk_launches // args.steps,
                      "traffic": counter_traffic(args.precision, M, kernel)},
-        "autotune_ms": {str(v): round(t, 3) for v, t in sorted(tuned.items())},
+        "autotune_ms": {"w_tilde_syrk_variant": {str(v): round(t, 3) for v, t in sorted(tuned.items())},
+                        "cholesky_schedule_batch": {str(m): round(t, 3) for m, t in sorted(tuned_batch.items())},
+                        "cholesky_schedule_single": {str(m): round(t, 3) for m, t in sorted(tuned_single.items())},
+                        "selected": sched},
     }
     if not args.no_cpu_baseline:
         ds.dirty_image = y.cpu().numpy()
-        threads = len(os.sched_getaffinity(0))
-        threads = min(threads, 16)
+        affinity = len(os.sched_getaffinity(0))
+        # a GPU box allots 16 host threads to its one GPU (OMP_NUM_THREADS there); each W-tilde thread also
+        # holds an M x M accumulator (200 MB at M = 5,000)
+        threads = min(affinity, 16)
         cv, t_w, t_like = cpu_baseline(ds, args.cpu_k_sub, 2, threads)
         k1 = max(1, args.cpu_k_sub // 10)
         cv1, t_w1, t_like1 = cpu_baseline(ds, k1, 1, 1)
@@ -316,7 +337,8 @@ def main():
                                 "value_1thread": round(cv1, 7),
                                 "sample_1thread": f"W-tilde on {k1} visibilities scaled x{K // k1} ({t_w1:.1f}s), "
                                                   f"likelihood on 1 of {n} samples scaled ({t_like1:.1f}s)",
-                                "cpu": cpu_model()}
+                                "cpu": cpu_model(), "affinity_cores": affinity,
+                                "cores_note": "16 host threads per GPU on the box (its CPU share for one GPU)"}
         log(f"cpu baseline: {cv:.4g} evals/s ({threads} threads), {cv1:.4g} evals/s (1 thread)")
     print(json.dumps(line), flush=True)
 

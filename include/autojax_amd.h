@@ -281,6 +281,14 @@ int aj_log_evidence_batched(const double* w_tilde, int64_t M, const double* dirt
                             double* reconstruction, double* logdets, int64_t* info, void* workspace,
                             size_t workspace_bytes, void* stream);
 
+/* How the batched log-evidence factors its matrices (process-wide tuning, no reference counterpart):
+ * 0 = three launches per 32-column step (diagonal blocks, panels, trailing tiles); 1 = the same steps in one
+ * cooperative launch with a grid-wide barrier between phases; 2 = 1 with lookahead (the next diagonal block
+ * factored by the workgroup that updates it); 3 (default, also -1) = one launch per step with lookahead (the
+ * next step's diagonal block and panel updated, factored and solved beside the step's trailing update).
+ * All give bitwise the same factors. */
+int aj_log_evidence_configure(int chol_mode);
+
 /* aj_log_evidence_batched with W-tilde given by its lattice table (numba.py:860-967 per sample):
  * same arguments and workspace (aj_log_evidence_batched_workspace_bytes) with w_tilde replaced by
  * w_compact f8[N,2N-1], grid_size N and native_index_for_slim_index i8[M,2].  No reference counterpart

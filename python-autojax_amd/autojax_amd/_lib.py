@@ -72,6 +72,7 @@ SIGNATURES = {
     "aj_log_likelihood_function_via_w_compact_from": (
         _INT, [_P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _P, _P, _P, _S, _P]),
     "aj_log_evidence_batched_workspace_bytes": (_S, [_I, _I, _I, _P]),
+    "aj_log_evidence_configure": (_INT, [_INT]),
     "aj_log_evidence_batched_compact": (
         _INT, [_P, _I, _P, _I, _P, _P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _S, _P]),
     "aj_log_evidence_batched": (

@@ -1,4 +1,6 @@
-"""Autotuner for the W-tilde SYRK: picks the integer kernel variant for an image size and precision.
+"""Autotuners: the W-tilde SYRK's integer kernel variant for an image size and precision
+(``tune_w_tilde``), and the batched log-evidence's Cholesky schedule for a batch's sizes
+(``tune_batch``).
 
 The variants (LDS layout of the digit slices, register or direct-to-LDS staging, 16x16x64 or
 32x32x32 MFMA, 4 or 8 waves per 64x128 tile) compute bitwise-identical sums; only their speed
@@ -70,4 +72,43 @@ def tune_w_tilde(grid_radians_slim, precision: str, chunks: int = 3, reps: int =
     best = min(times, key=times.get)
     _lib.call("aj_w_tilde_set_variant", best)
     _CACHE[key] = (best, times)
+    return times
+
+
+CHOL_SCHEDULES = {0: "three launches per step", 1: "cooperative", 2: "cooperative, lookahead",
+                  3: "one launch per step, lookahead"}
+_BATCH_CACHE: dict = {}
+
+
+def tune_batch(batch, w_tilde, dirty_image, chi2_noise_norm, reps: int = 3, verbose: bool = False):
+    """Time every Cholesky schedule of the batched log-evidence (``aj_log_evidence_configure``: the same
+    factors bit for bit, in 3 launches per 32-column step, one cooperative launch without or with lookahead,
+    or one launch per step with lookahead) on this batch, whose matrix sizes decide which is fastest (how
+    many steps, how much trailing work per step); select the fastest process-wide and return
+    {schedule: ms per evaluation}.  Cached per (M, sorted source-pixel counts) for the life of the process.
+    """
+    import torch
+
+    key = (batch.M, tuple(sorted(int(s) for s in batch.S_host)))
+    if key in _BATCH_CACHE:
+        _lib.call("aj_log_evidence_configure", _BATCH_CACHE[key][0])
+        return _BATCH_CACHE[key][1]
+    times = {}
+    for mode in CHOL_SCHEDULES:
+        _lib.call("aj_log_evidence_configure", mode)
+        batch.log_evidence(w_tilde, dirty_image, chi2_noise_norm)
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            batch.log_evidence(w_tilde, dirty_image, chi2_noise_norm)
+        b.record()
+        torch.cuda.synchronize()
+        times[mode] = a.elapsed_time(b) / reps
+        if verbose:
+            print(f"autotune batch n={batch.n}: schedule {mode} ({CHOL_SCHEDULES[mode]}) {times[mode]:.3f} ms",
+                  flush=True)
+    best = min(times, key=times.get)
+    _lib.call("aj_log_evidence_configure", best)
+    _BATCH_CACHE[key] = (best, times)
     return times

@@ -52,6 +52,7 @@ def main(argv=None):
     ap.add_argument("--chunk", type=int, default=0, help="int8crt: visibilities per exact chunk (0 = default)")
     ap.add_argument("--check", action="store_true", help="compare W-tilde with kernel variant 0 (bitwise)")
     ap.add_argument("--group-mib", type=int, default=0, help="cap on the features of one SYRK launch (0 = default)")
+    ap.add_argument("--modes", default="0,1,2,3", help="batch: Cholesky schedules to time (aj_log_evidence_configure)")
     ap.add_argument("--against", default=None, help="also build W-tilde at this precision and report the "
                                                     "normwise difference max|dW| / max|W|")
     a = ap.parse_args(argv)
@@ -109,6 +110,25 @@ def main(argv=None):
             exact = bool((out == one * iters).all().item())
             print(json.dumps({"kernel": "mfma_probe", "instruction": name, "workgroups": wg, "iters": iters, "ms": ms,
                               "T_ops_per_s": ops / ms / 1e9, "exact": exact}), flush=True)
+        return
+    if a.what == "batch":
+        d = dsm.config5(K=a.K, n_samples=a.samples)
+        noise, uv, grid = (dv.to_device(x) for x in (d.noise_map_real, d.uv_wavelengths, d.grid_radians_slim))
+        W = E.w_tilde(noise, uv, grid, "fp64")
+        y = E.dirty_image(d.data, d.noise_map, d.uv_wavelengths, d.grid_radians_slim)
+        c2nn = E.chi2_noise_norm(d.data, d.noise_map)
+        batch = E.SampleBatch(d.samples, d.M)
+        for mode in (int(x) for x in a.modes.split(",")):
+            _lib.call("aj_log_evidence_configure", mode)
+            ms = timed(lambda: batch.log_evidence(W, y, c2nn), a.reps)
+            _lib.timing_enable(True)
+            for _ in range(a.reps):
+                batch.log_evidence(W, y, c2nn)
+            stages = {k: round(_lib.timing_read(k)[0] / a.reps, 4) for k in ("omega", "fgather", "chol", "solve")}
+            _lib.timing_enable(False)
+            print(json.dumps({"kernel": "batched_log_evidence", "chol_mode": mode, "M": d.M, "samples": a.samples,
+                              "sum_S": int(batch.S_host.sum()), "ms": ms, "stages_ms": stages}), flush=True)
+        _lib.call("aj_log_evidence_configure", -1)
         return
     if a.what == "grad":
         from . import torch_api as TA

@@ -17,7 +17,10 @@
 #include "aj_common.h"
 
 #include <climits>
+#include <mutex>
 #include <vector>
+
+#include <hip/hip_cooperative_groups.h>
 
 namespace aj {
 namespace lk {
@@ -145,6 +148,105 @@ __global__ void __launch_bounds__(256) csc_sort_kernel(const SampleDesc* __restr
       }
       for (int i = lo; i < hi; ++i) k[i] = (int)(k[i] / B);
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The same mapping inversion and data vector for one sample in one workgroup (the batch's first
+// launch): bucket counts in LDS, their prefix sum (the pointers), the entries placed by LDS cursors, each
+// bucket ordered by (m, b) by one wave (a rank sort up to 64 entries), then D of the sample's buckets.
+// ---------------------------------------------------------------------------
+constexpr int PREP_THREADS = 1024;
+
+__global__ void __launch_bounds__(PREP_THREADS) prep_kernel(const SampleDesc* __restrict__ sd,
+                                                            const int64_t* __restrict__ idx,
+                                                            const double* __restrict__ wts, int64_t M, int64_t B,
+                                                            const double* __restrict__ dirty,
+                                                            int* __restrict__ ptrs, int* __restrict__ ent_key,
+                                                            double* __restrict__ ent_t, double* __restrict__ D) {
+  extern __shared__ int cnt[];  // S + 1 counters, then cursors
+  __shared__ int part[PREP_THREADS];
+  const SampleDesc d = sd[blockIdx.x];
+  const int64_t S = d.S;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t* ix = idx + d.map_off;
+  const double* wt = wts + d.map_off;
+  int* p = ptrs + d.csc_ptr;
+  int* k = ent_key + d.csc_ent;
+  double* t = ent_t + d.csc_ent;
+  for (int64_t i = tid; i <= S; i += PREP_THREADS) cnt[i] = 0;
+  __syncthreads();
+  for (int64_t m = tid; m < M; m += PREP_THREADS)
+    for (int64_t b = 0; b < B; ++b) {
+      const int64_t s = ix[m * B + b];
+      if (s == -1) break;
+      if (s >= 0 && s < S) atomicAdd(&cnt[s + 1], 1);
+    }
+  __syncthreads();
+  // prefix sum of cnt[0..S] (cnt[0] = 0): thread tid owns a contiguous run of `per` entries
+  const int64_t per = (S + 1 + PREP_THREADS - 1) / PREP_THREADS;
+  const int64_t a0 = tid * per, a1 = std::min<int64_t>(S + 1, a0 + per);
+  int run = 0;
+  for (int64_t i = a0; i < a1; ++i) run += cnt[i];
+  part[tid] = run;
+  __syncthreads();
+  for (int o = 1; o < PREP_THREADS; o <<= 1) {
+    const int v = tid >= o ? part[tid - o] : 0;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  int acc = part[tid] - run;  // exclusive prefix of the runs before this one
+  for (int64_t i = a0; i < a1; ++i) {
+    acc += cnt[i];
+    cnt[i] = acc;  // inclusive: pointer to the end of bucket i - 1, the start of bucket i
+    p[i] = acc;
+  }
+  __syncthreads();
+  for (int64_t m = tid; m < M; m += PREP_THREADS)
+    for (int64_t b = 0; b < B; ++b) {
+      const int64_t s = ix[m * B + b];
+      if (s == -1) break;
+      if (s >= 0 && s < S) {
+        const int pos = atomicAdd(&cnt[s], 1);
+        k[pos] = (int)(m * B + b);
+        t[pos] = wt[m * B + b];
+      }
+    }
+  __syncthreads();
+  // order each bucket by key m*B + b (keys are distinct), then keep m
+  for (int64_t s = wave; s < S; s += PREP_THREADS / 64) {
+    const int lo = p[s], hi = p[s + 1], len = hi - lo;
+    if (len <= 64) {
+      const int key = lane < len ? k[lo + lane] : INT_MAX;
+      const double tv = lane < len ? t[lo + lane] : 0.0;
+      int rank = 0;
+      for (int j = 0; j < 64; ++j) rank += __shfl(key, j, 64) < key ? 1 : 0;
+      if (lane < len) {
+        k[lo + rank] = (int)(key / B);
+        t[lo + rank] = tv;
+      }
+    } else if (lane == 0) {
+      for (int i = lo + 1; i < hi; ++i) {
+        const int kk = k[i];
+        const double tt = t[i];
+        int j = i - 1;
+        while (j >= lo && k[j] > kk) {
+          k[j + 1] = k[j];
+          t[j + 1] = t[j];
+          --j;
+        }
+        k[j + 1] = kk;
+        t[j + 1] = tt;
+      }
+      for (int i = lo; i < hi; ++i) k[i] = (int)(k[i] / B);
+    }
+  }
+  __syncthreads();
+  for (int64_t s = tid; s < S; s += PREP_THREADS) {
+    double v = 0.0;
+    for (int e = p[s]; e < p[s + 1]; ++e) v = fma(t[e], dirty[k[e]], v);
+    D[d.s_off + s] = v;
   }
 }
 
@@ -294,6 +396,43 @@ __global__ void fgather_kernel(const SampleDesc* __restrict__ sd, const int* __r
 }
 
 // ---------------------------------------------------------------------------
+// Row s1 of A = F + H and of H, lower triangle, in one pass (the batch's third launch):
+//   F[s1, s2] = sum over bucket s1 of t Omega[m, s2],
+//   H[s1, s2] = (beta / S, the rank-one deflation of log|H|, or 0) + [s1 = s2] (1e-8 + c^2 size_s1)
+//               - c^2 (the number of times s2 appears in s1's neighbour list before -1)   (numba.py:686-694).
+// ---------------------------------------------------------------------------
+__global__ void assemble_kernel(const SampleDesc* __restrict__ sd, const int* __restrict__ ptrs,
+                                const int* __restrict__ ent_m, const double* __restrict__ ent_t,
+                                const double* __restrict__ omega, const int64_t* __restrict__ nbr,
+                                const int64_t* __restrict__ nsz, int64_t P, double* __restrict__ mats) {
+  const SampleDesc d = sd[blockIdx.y];
+  const int64_t s1 = blockIdx.x;
+  if (s1 >= d.S) return;
+  const int* p = ptrs + d.csc_ptr;
+  const int lo = p[s1], hi = p[s1 + 1];
+  double* A = mats + d.mat_off + s1 * d.S;
+  double* H = A + d.S * d.S;
+  const double* om = omega + d.omega_off;
+  const int64_t* row = nbr + (d.s_off + s1) * P;
+  const double hc = d.beta > 0.0 ? d.beta / (double)d.S : 0.0;
+  const double diag = 1e-8 + d.coef2 * (double)nsz[d.s_off + s1];
+  for (int64_t s2 = threadIdx.x; s2 <= s1; s2 += blockDim.x) {
+    double acc = 0.0;
+    for (int e = lo; e < hi; ++e) acc = fma(ent_t[d.csc_ent + e], om[(int64_t)ent_m[d.csc_ent + e] * d.S + s2], acc);
+    int times = 0;
+    for (int64_t q = 0; q < P; ++q) {
+      const int64_t kq = row[q];
+      if (kq == -1) break;
+      times += kq == s2 ? 1 : 0;
+    }
+    double r = s2 == s1 ? diag : 0.0;
+    for (int q = 0; q < times; ++q) r -= d.coef2;
+    A[s2] = acc + r;
+    H[s2] = hc + r;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Regularisation (numba.py:686-694), added into A and written into H:
 //   H = diag(1e-8 + c^2 sizes);  H[i, k] -= c^2 for k in neighbours[i] until -1.
 // With beta > 0, H receives + beta/S everywhere (rank-one deflation of the
@@ -375,22 +514,14 @@ __device__ inline double readlane_d(double v, int l) {
   return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
 }
 
-// One wave per matrix: lane l < 32 holds row l of the 32 x 32 diagonal block in registers; for each
-// column the pivot and the column entries are broadcast with readlane (no barriers, no LDS).
-__global__ void __launch_bounds__(64) chol_diag_kernel(const SampleDesc* __restrict__ sd,
-                                                       const StepEntry* __restrict__ e, int64_t k0, int kstep,
-                                                       int maxsteps, double* __restrict__ mats,
-                                                       double* __restrict__ ldp, int64_t* __restrict__ failp) {
-  const int j = e[blockIdx.x].mat;
-  int64_t n;
-  double* A = mat_ptr(sd, mats, j, n);
-  const int nb = (int)std::min<int64_t>(NB, n - k0);
-  const int lane = threadIdx.x;
-  double r[NB];
-#pragma unroll
-  for (int c = 0; c < NB; ++c) r[c] = (lane < nb && c <= lane) ? A[(k0 + lane) * n + k0 + c] : 0.0;
-  double ld = 0.0;
-  int64_t fail = 0;
+// Cholesky of an nb x nb block held by one wave, lane l < nb holding row l in r[0..l]; for each column the
+// pivot and the column entries are broadcast with readlane (no barriers, no LDS).  ld receives
+// sum_c log L_cc, taken as one log of the product of the pivots' mantissas plus their exponents;
+// fail the first non-positive pivot (1-based, offset by k0) or 0.
+__device__ inline void chol_factor_regs(double (&r)[NB], int nb, int64_t k0, int lane, double& ld, int64_t& fail) {
+  double mant = 1.0;
+  int ex = 0;
+  fail = 0;
 #pragma unroll
   for (int c = 0; c < NB; ++c) {
     if (c < nb) {
@@ -400,7 +531,9 @@ __global__ void __launch_bounds__(64) chol_diag_kernel(const SampleDesc* __restr
         d = 1.0;  // keep the arithmetic finite; the result is flagged
       }
       d = sqrt(d);
-      ld += log(d);
+      int e;
+      mant *= frexp(d, &e);  // in [0.5, 1): 32 factors stay above 2^-32
+      ex += e;
       if (lane == c) r[c] = d;
       if (lane > c) r[c] /= d;
 #pragma unroll
@@ -410,6 +543,23 @@ __global__ void __launch_bounds__(64) chol_diag_kernel(const SampleDesc* __restr
       }
     }
   }
+  ld = log(mant) + (double)ex * 0.69314718055994530942;
+}
+
+// Diagonal block k0 of matrix j, one wave: load, factor, store, record the block's log-determinant part and
+// first failing pivot for step kstep.
+__device__ inline void chol_diag_dev(const SampleDesc* __restrict__ sd, int j, int64_t k0, int kstep, int maxsteps,
+                                     double* __restrict__ mats, double* __restrict__ ldp,
+                                     int64_t* __restrict__ failp, int lane) {
+  int64_t n;
+  double* A = mat_ptr(sd, mats, j, n);
+  const int nb = (int)std::min<int64_t>(NB, n - k0);
+  double r[NB];
+#pragma unroll
+  for (int c = 0; c < NB; ++c) r[c] = (lane < nb && c <= lane) ? A[(k0 + lane) * n + k0 + c] : 0.0;
+  double ld;
+  int64_t fail;
+  chol_factor_regs(r, nb, k0, lane, ld, fail);
 #pragma unroll
   for (int c = 0; c < NB; ++c)
     if (lane < nb && c <= lane) A[(k0 + lane) * n + k0 + c] = r[c];
@@ -419,14 +569,10 @@ __global__ void __launch_bounds__(64) chol_diag_kernel(const SampleDesc* __restr
   }
 }
 
-__global__ void __launch_bounds__(256) chol_panel_kernel(const SampleDesc* __restrict__ sd,
-                                                         const StepEntry* __restrict__ e, int ne, int64_t k0,
-                                                         double* __restrict__ mats) {
-  __shared__ double Lkk[NB][NB + 1];
-  __shared__ double Xs[256][NB + 1];
-  const int ei = find_entry(e, ne, (int)blockIdx.x, false);
-  const int j = e[ei].mat;
-  const int rb = (int)blockIdx.x - e[ei].pfx_p;
+// Panel: rows [k0 + nb + 256 rb, +256) of matrix j solve X L_kk^T = A[rows, k0:k0+nb], one thread per row
+// kept in its own LDS row (no cross-thread traffic, no barriers after the L_kk load).
+__device__ inline void chol_panel_dev(const SampleDesc* __restrict__ sd, int j, int rb, int64_t k0,
+                                      double* __restrict__ mats, double (*Lkk)[NB + 1], double (*Xs)[NB + 1]) {
   int64_t n;
   double* A = mat_ptr(sd, mats, j, n);
   const int nb = (int)std::min<int64_t>(NB, n - k0);
@@ -450,19 +596,16 @@ __global__ void __launch_bounds__(256) chol_panel_kernel(const SampleDesc* __res
 }
 
 constexpr int TT = 64;  // trailing tile
+static_assert(TT >= NB, "the first trailing tile must hold the next diagonal block");
 
-__global__ void __launch_bounds__(256) chol_trail_kernel(const SampleDesc* __restrict__ sd,
-                                                         const StepEntry* __restrict__ e, int ne, int64_t k0,
-                                                         double* __restrict__ mats) {
-  __shared__ double PI[TT][NB + 1];
-  __shared__ double PJ[TT][NB + 1];
-  const int ei = find_entry(e, ne, (int)blockIdx.x, true);
-  const int j = e[ei].mat;
-  const int64_t t = (int64_t)blockIdx.x - e[ei].pfx_t;
+// Trailing tile t (lower-triangle order, tiles from row and column r0) of matrix j: A22 -= P P^T with P the
+// panel of step k0 on v_mfma_f64_16x16x4, the two panel row blocks staged in LDS; wave w owns the 32 x 32 quarter (w >> 1, w & 1) of the 64 x 64 tile, so wave 0
+// holds the tile's first 32 x 32 diagonal block (the next step's diagonal block when t = 0).
+__device__ inline void chol_trail_dev(const SampleDesc* __restrict__ sd, int j, int64_t t, int64_t k0, int64_t r0,
+                                      double* __restrict__ mats, double (*PI)[NB + 1], double (*PJ)[NB + 1]) {
   int64_t n;
   double* A = mat_ptr(sd, mats, j, n);
   const int nb = (int)std::min<int64_t>(NB, n - k0);
-  const int64_t r0 = k0 + nb;
   int64_t bi = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
   while (bi * (bi + 1) / 2 > t) --bi;
   while ((bi + 1) * (bi + 2) / 2 <= t) ++bi;
@@ -510,12 +653,203 @@ __global__ void __launch_bounds__(256) chol_trail_kernel(const SampleDesc* __res
       }
 }
 
+__global__ void __launch_bounds__(64) chol_diag_kernel(const SampleDesc* __restrict__ sd,
+                                                       const StepEntry* __restrict__ e, int64_t k0, int kstep,
+                                                       int maxsteps, double* __restrict__ mats,
+                                                       double* __restrict__ ldp, int64_t* __restrict__ failp) {
+  chol_diag_dev(sd, e[blockIdx.x].mat, k0, kstep, maxsteps, mats, ldp, failp, threadIdx.x);
+}
+
+__global__ void __launch_bounds__(256) chol_panel_kernel(const SampleDesc* __restrict__ sd,
+                                                         const StepEntry* __restrict__ e, int ne, int64_t k0,
+                                                         double* __restrict__ mats) {
+  __shared__ double Lkk[NB][NB + 1];
+  __shared__ double Xs[256][NB + 1];
+  const int ei = find_entry(e, ne, (int)blockIdx.x, false);
+  chol_panel_dev(sd, e[ei].mat, (int)blockIdx.x - e[ei].pfx_p, k0, mats, Lkk, Xs);
+}
+
+__global__ void __launch_bounds__(256) chol_trail_kernel(const SampleDesc* __restrict__ sd,
+                                                         const StepEntry* __restrict__ e, int ne, int64_t k0,
+                                                         double* __restrict__ mats) {
+  __shared__ double PI[TT][NB + 1];
+  __shared__ double PJ[TT][NB + 1];
+  const int ei = find_entry(e, ne, (int)blockIdx.x, true);
+  int64_t n;
+  mat_ptr(sd, mats, e[ei].mat, n);
+  chol_trail_dev(sd, e[ei].mat, (int64_t)blockIdx.x - e[ei].pfx_t, k0, k0 + std::min<int64_t>(NB, n - k0), mats, PI,
+                 PJ);
+}
+
+// One launch per step with lookahead (mode 3).  Launch k assumes the panel of step k (columns k0 .. k0 + 31,
+// every row below) final and everything right of it updated by the steps before k; it runs two task kinds
+// side by side:
+//   next   (one per STRIP rows below the next diagonal block, at least one per matrix with a step k + 1):
+//          the step-k update of the diagonal block k + 1 and of its STRIP rows' columns k0 + 32 .. k0 + 63,
+//          the block's factorisation (every task of the matrix factors it; the first stores it), and the
+//          panel solve of its rows: the panel of step k + 1, final when the launch ends;
+//   trail  the step-k update of the rest, rows and columns from k0 + 64, in 64 x 64 tiles.
+// Each element sees the same operations in the same order as in the step-synchronous form (same MFMA
+// operands and accumulation order, same substitutions), so the factors are bitwise the same, in
+// maxsteps + 2 launches instead of 3 maxsteps + 1.
+constexpr int STRIP = 128;
+
+__device__ inline void chol_next_dev(const SampleDesc* __restrict__ sd, int j, int rb, int64_t k0, int kstep,
+                                     int maxsteps, double* __restrict__ mats, double* __restrict__ ldp,
+                                     int64_t* __restrict__ failp, double* __restrict__ stage, double (*Pd)[NB + 1],
+                                     double (*Pr)[NB + 1], double (*Lkk)[NB + 1]) {
+  int64_t n;
+  double* A = mat_ptr(sd, mats, j, n);
+  const int nb = (int)std::min<int64_t>(NB, n - k0);
+  const int64_t k1 = k0 + nb;  // the next diagonal block
+  const int nb1 = (int)std::min<int64_t>(NB, n - k1);
+  const int64_t s0 = k1 + nb1 + (int64_t)rb * STRIP;  // first strip row
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  double* stg = stage + (int64_t)j * NB * NB;
+  // The factored block k + 1 cannot go into A during this launch: the matrix's other tasks read the block's
+  // values from before the update.  The first task keeps it in the matrix's stage slot and, here, moves
+  // the block staged by the previous launch (block k) into place; no task of this launch reads block k.
+  if (rb == 0 && kstep > 0)
+    for (int x = tid; x < NB * NB; x += 256) {
+      const int i = x / NB, c = x % NB;
+      if (i < nb && c <= i) A[(k0 + i) * n + k0 + c] = stg[x];
+    }
+  for (int x = tid; x < NB * NB; x += 256) {
+    const int i = x / NB, p = x % NB;
+    Pd[i][p] = (i < nb1 && p < nb) ? A[(k1 + i) * n + k0 + p] : 0.0;
+  }
+  for (int x = tid; x < STRIP * NB; x += 256) {
+    const int i = x / NB, p = x % NB;
+    Pr[i][p] = (s0 + i < n && p < nb) ? A[(s0 + i) * n + k0 + p] : 0.0;
+  }
+  __syncthreads();
+  if (wave == 0) {
+    // the diagonal block's update, as wave 0 of the step-k trailing tile that holds it
+    d4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int64_t gi = k1 + a * 16 + lk + 4 * r, gj = k1 + b * 16 + lr;
+          acc[a][b][r] = (gi < n && gj <= gi) ? A[gi * n + gj] : 0.0;
+        }
+#pragma unroll
+    for (int ks = 0; ks < NB / 4; ++ks) {
+      const int p = ks * 4 + lk;
+      const double fa0 = -Pd[lr][p], fa1 = -Pd[16 + lr][p];
+      const double fb0 = Pd[lr][p], fb1 = Pd[16 + lr][p];
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa0, fb0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa0, fb1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa1, fb0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa1, fb1, acc[1][1], 0, 0, 0);
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = a * 16 + lk + 4 * r, c = b * 16 + lr;
+          Lkk[i][c] = (c <= i && k1 + i < n) ? acc[a][b][r] : 0.0;
+        }
+  }
+  __syncthreads();
+  if (wave == 0) {
+    double r[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) r[c] = (lane < nb1 && c <= lane) ? Lkk[lane][c] : 0.0;
+    double ld;
+    int64_t fail;
+    chol_factor_regs(r, nb1, k1, lane, ld, fail);
+#pragma unroll
+    for (int c = 0; c < NB; ++c)
+      if (lane < NB) Lkk[lane][c] = (lane < nb1 && c <= lane) ? r[c] : 0.0;
+    if (rb == 0) {
+#pragma unroll
+      for (int c = 0; c < NB; ++c)
+        if (lane < NB) stg[lane * NB + c] = r[c];
+      if (lane == 0) {
+        ldp[(int64_t)j * maxsteps + kstep + 1] = ld;
+        failp[(int64_t)j * maxsteps + kstep + 1] = fail;
+      }
+    }
+  }
+  // the strip's step-k update, wave w on rows 32 w .. 32 w + 31 (as the trailing tiles' quarters that hold
+  // them compute it), into Pr's rows (each wave overwrites only the rows it has read)
+  {
+    const int w0 = wave * 32;
+    d4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int64_t gi = s0 + w0 + a * 16 + lk + 4 * r, gj = k1 + b * 16 + lr;
+          acc[a][b][r] = (gi < n && b * 16 + lr < nb1) ? A[gi * n + gj] : 0.0;
+        }
+#pragma unroll
+    for (int ks = 0; ks < NB / 4; ++ks) {
+      const int p = ks * 4 + lk;
+      const double fa0 = -Pr[w0 + lr][p], fa1 = -Pr[w0 + 16 + lr][p];
+      const double fb0 = Pd[lr][p], fb1 = Pd[16 + lr][p];
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa0, fb0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa0, fb1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa1, fb0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa1, fb1, acc[1][1], 0, 0, 0);
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Pr[w0 + a * 16 + lk + 4 * r][b * 16 + lr] = acc[a][b][r];
+  }
+  __syncthreads();
+  // the panel solve of step k + 1 for the strip's rows: X L_k+1^T = strip, one thread per row
+  if (tid < STRIP && s0 + tid < n) {
+    for (int c = 0; c < nb1; ++c) {
+      double v = Pr[tid][c];
+      for (int l = 0; l < c; ++l) v -= Pr[tid][l] * Lkk[c][l];
+      Pr[tid][c] = v / Lkk[c][c];
+    }
+    double* row = A + (s0 + tid) * n + k1;
+    for (int c = 0; c < nb1; ++c) row[c] = Pr[tid][c];
+  }
+}
+
+__global__ void __launch_bounds__(256) chol_step_kernel(const SampleDesc* __restrict__ sd,
+                                                        const StepEntry* __restrict__ e, int ne, int n_next,
+                                                        int64_t k0, int kstep, int maxsteps,
+                                                        double* __restrict__ mats, double* __restrict__ ldp,
+                                                        int64_t* __restrict__ failp, double* __restrict__ stage) {
+  __shared__ double smem[2 * NB * (NB + 1) + STRIP * (NB + 1)];
+  double(*Pd)[NB + 1] = reinterpret_cast<double(*)[NB + 1]>(smem);
+  double(*Lkk)[NB + 1] = reinterpret_cast<double(*)[NB + 1]>(smem + NB * (NB + 1));
+  double(*Pr)[NB + 1] = reinterpret_cast<double(*)[NB + 1]>(smem + 2 * NB * (NB + 1));
+  const int t = (int)blockIdx.x;
+  if (t < n_next) {
+    const int ei = find_entry(e, ne, t, false);
+    chol_next_dev(sd, e[ei].mat, t - e[ei].pfx_p, k0, kstep, maxsteps, mats, ldp, failp, stage, Pd, Pr, Lkk);
+  } else {
+    const int ei = find_entry(e, ne, t - n_next, true);
+    const int j = e[ei].mat;
+    int64_t n;
+    mat_ptr(sd, mats, j, n);
+    chol_trail_dev(sd, j, (int64_t)(t - n_next) - e[ei].pfx_t, k0, k0 + 2 * NB, mats,
+                   reinterpret_cast<double(*)[NB + 1]>(smem), reinterpret_cast<double(*)[NB + 1]>(smem + TT * (NB + 1)));
+  }
+}
+static_assert(2 * NB * (NB + 1) + STRIP * (NB + 1) >= 2 * TT * (NB + 1), "trailing panels must fit the step kernel");
+static_assert(STRIP <= 256 && STRIP == 4 * 32, "four waves of 32 strip rows");
+
 // logdet[j] = 2 sum_k ldp[j][k] (+ the deflation correction of H), info[j] = first failing column.
-__global__ void chol_finish_kernel(const SampleDesc* __restrict__ sd, int nmat, int maxsteps,
-                                   const double* __restrict__ ldp, const int64_t* __restrict__ failp,
-                                   double* __restrict__ logdet, int64_t* __restrict__ info) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= nmat) return;
+__device__ inline void chol_finish_dev(const SampleDesc* __restrict__ sd, int j, int maxsteps,
+                                       const double* __restrict__ ldp, const int64_t* __restrict__ failp,
+                                       double* __restrict__ logdet, int64_t* __restrict__ info) {
   const SampleDesc d = sd[j >> 1];
   const int steps = (int)((d.S + NB - 1) / NB);
   double s = 0.0;
@@ -528,6 +862,87 @@ __global__ void chol_finish_kernel(const SampleDesc* __restrict__ sd, int nmat, 
   if ((j & 1) && d.beta > 0.0) s += log(1e-8) - log(1e-8 + d.beta);
   logdet[j] = s;
   info[j] = f;
+}
+
+__global__ void chol_finish_kernel(const SampleDesc* __restrict__ sd, int nmat, int maxsteps,
+                                   const double* __restrict__ ldp, const int64_t* __restrict__ failp,
+                                   double* __restrict__ logdet, int64_t* __restrict__ info) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < nmat) chol_finish_dev(sd, j, maxsteps, ldp, failp, logdet, info);
+}
+
+// After the lookahead launches: every matrix's last diagonal block (from its second block on, each is
+// staged by the launch that factors it) moved from its stage slot into place.  One workgroup per matrix.
+__global__ void chol_unstage_kernel(const SampleDesc* __restrict__ sd, double* __restrict__ mats,
+                                    const double* __restrict__ stage) {
+  const int j = blockIdx.x;
+  int64_t n;
+  double* A = mat_ptr(sd, mats, j, n);
+  const int64_t steps = (n + NB - 1) / NB;
+  if (steps < 2) return;
+  const int64_t kl = (steps - 1) * NB;
+  const int nb = (int)(n - kl);
+  const double* stg = stage + (int64_t)j * NB * NB;
+  for (int x = threadIdx.x; x < NB * NB; x += blockDim.x) {
+    const int i = x / NB, c = x % NB;
+    if (i < nb && c <= i) A[(kl + i) * n + kl + c] = stg[x];
+  }
+}
+
+// The whole blocked Cholesky of every matrix of the batch in one cooperative launch: the workgroups stay
+// resident and walk the steps together, a grid-wide barrier between the phases of a step (the launch
+// boundaries of the step-synchronous form).  Within a phase, task t goes to workgroup t mod the grid, so
+// each task's arithmetic, and the factor, are those of the step-synchronous form.  With lookahead, the
+// workgroup that updates a matrix's first trailing tile of step k factors the diagonal block of step k + 1
+// right after it (the block lies in wave 0's quarter of that tile, which wave 0 itself wrote), which saves
+// the diagonal phase and its barrier at every step but the first.
+constexpr int COOP_LDS_DOUBLES = NB * (NB + 1) + 256 * (NB + 1);  // the panel phase's two arrays (the largest)
+static_assert(COOP_LDS_DOUBLES >= 2 * TT * (NB + 1), "the trailing phase's panels must fit the shared buffer");
+
+__global__ void __launch_bounds__(256) chol_coop_kernel(const SampleDesc* __restrict__ sd,
+                                                        const StepEntry* __restrict__ steps,
+                                                        const int* __restrict__ step_tab, int maxsteps, int nmat,
+                                                        int lookahead, double* __restrict__ mats,
+                                                        double* __restrict__ ldp, int64_t* __restrict__ failp,
+                                                        double* __restrict__ logdet, int64_t* __restrict__ info) {
+  __shared__ double smem[COOP_LDS_DOUBLES];
+  auto grid = cooperative_groups::this_grid();
+  double(*Lkk)[NB + 1] = reinterpret_cast<double(*)[NB + 1]>(smem);
+  double(*Xs)[NB + 1] = reinterpret_cast<double(*)[NB + 1]>(smem + NB * (NB + 1));
+  double(*PI)[NB + 1] = reinterpret_cast<double(*)[NB + 1]>(smem);
+  double(*PJ)[NB + 1] = reinterpret_cast<double(*)[NB + 1]>(smem + TT * (NB + 1));
+  const int G = (int)gridDim.x, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // step_tab: per step k, {entry offset, entries, panel tasks, trailing tasks}
+  for (int k = 0; k < maxsteps; ++k) {
+    const int4 tk = reinterpret_cast<const int4*>(step_tab)[k];
+    const StepEntry* e = steps + tk.x;
+    const int ne = tk.y;
+    const int64_t k0 = (int64_t)k * NB;
+    if (k == 0 || !lookahead) {
+      if (wave == 0)
+        for (int t = blockIdx.x; t < ne; t += G) chol_diag_dev(sd, e[t].mat, k0, k, maxsteps, mats, ldp, failp, lane);
+      grid.sync();
+    }
+    for (int t = blockIdx.x; t < tk.z; t += G) {
+      const int ei = find_entry(e, ne, t, false);
+      chol_panel_dev(sd, e[ei].mat, t - e[ei].pfx_p, k0, mats, Lkk, Xs);
+      __syncthreads();  // the next task reloads the shared arrays
+    }
+    grid.sync();
+    for (int t = blockIdx.x; t < tk.w; t += G) {
+      const int ei = find_entry(e, ne, t, true);
+      const int j = e[ei].mat;
+      const int64_t tt = (int64_t)t - e[ei].pfx_t;
+      int64_t n;
+      mat_ptr(sd, mats, j, n);
+      chol_trail_dev(sd, j, tt, k0, k0 + std::min<int64_t>(NB, n - k0), mats, PI, PJ);
+      if (lookahead && tt == 0 && wave == 0) chol_diag_dev(sd, j, k0 + NB, k + 1, maxsteps, mats, ldp, failp, lane);
+      __syncthreads();
+    }
+    grid.sync();
+  }
+  for (int j = blockIdx.x * 256 + threadIdx.x; j < nmat; j += G * 256)
+    chol_finish_dev(sd, j, maxsteps, ldp, failp, logdet, info);
 }
 
 constexpr int SB = 64;  // row block of the triangular solves
@@ -710,7 +1125,7 @@ __global__ void noise_norm_kernel(const double* __restrict__ noise, int64_t K2, 
 struct BatchPlan {
   int64_t n, M, B, sumS, sumS2, omega_elems, csc_ptr_elems, csc_ent_elems, maxsteps, n_entries;
   size_t off_desc, off_ptr, off_cur, off_key, off_t, off_omega, off_mats, off_D, off_logdet, off_info,
-      off_steps, off_ldp, off_failp, bytes;
+      off_steps, off_tab, off_ldp, off_failp, off_stage, bytes;
 };
 
 inline BatchPlan plan_batch(int64_t M, int64_t B, int64_t n, const int64_t* S_host) {
@@ -744,9 +1159,11 @@ inline BatchPlan plan_batch(int64_t M, int64_t B, int64_t n, const int64_t* S_ho
   p.off_D = put(sizeof(double) * p.sumS);
   p.off_logdet = put(sizeof(double) * 2 * n);
   p.off_info = put(sizeof(int64_t) * 2 * n);
-  p.off_steps = put(sizeof(StepEntry) * p.n_entries);
+  p.off_steps = put(sizeof(StepEntry) * 2 * p.n_entries);  // the step tables of two schedules
+  p.off_tab = put(sizeof(int) * 4 * p.maxsteps);
   p.off_ldp = put(sizeof(double) * 2 * n * p.maxsteps);
   p.off_failp = put(sizeof(int64_t) * 2 * n * p.maxsteps);
+  p.off_stage = put(sizeof(double) * 2 * n * NB * NB);
   p.bytes = align_up(o, 256);
   return p;
 }
@@ -758,6 +1175,35 @@ using namespace aj;
 using namespace aj::lk;
 
 namespace {
+
+// How the batched Cholesky runs (process-wide, aj_log_evidence_configure): three launches per 32-column
+// step; one cooperative launch (without or with lookahead); one launch per step with lookahead (the
+// default).  All give the same factor bit for bit.
+constexpr int CHOL_STEPS = 0, CHOL_COOP = 1, CHOL_COOP_LOOKAHEAD = 2, CHOL_STEPS_LOOKAHEAD = 3;
+std::atomic<int> g_chol_mode{CHOL_STEPS_LOOKAHEAD};
+int chol_mode() { return g_chol_mode.load(std::memory_order_relaxed); }
+
+// Workgroups of the cooperative Cholesky: every one resident at once (occupancy x compute units), per
+// device; 0 when the device cannot launch cooperatively.
+int coop_grid() {
+  static std::mutex mu;
+  static int cache[64];
+  static bool have[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!have[dev]) {
+    int coop = 0, cus = 0, per = 0;
+    int g = 0;
+    if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) == hipSuccess && coop &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)chol_coop_kernel, 256, 0) == hipSuccess)
+      g = per * cus;
+    cache[dev] = g;
+    have[dev] = true;
+  }
+  return cache[dev];
+}
 
 // Runs the whole per-sample pipeline for a batch.  Returns the workspace
 // views through `p`; outputs per-sample log-evidence into logl.
@@ -803,23 +1249,14 @@ int batch_pipeline(const WSrc w_tilde, int64_t M, const double* dirty_image, con
   ho.resize(n);
   for (int64_t b = 0; b < n; ++b) ho[b] = (int)b;
   std::stable_sort(ho.begin(), ho.end(), [&](int x, int y) { return S_host[x] > S_host[y]; });
-  AJ_CHECK_HIP(hipMemsetAsync(ptrs, 0, sizeof(int) * p.csc_ptr_elems, st));
-
-  const unsigned gm = (unsigned)std::min<int64_t>(ceil_div(M, 256), 1024);
-  hipLaunchKernelGGL(csc_count_kernel, dim3(gm, (unsigned)n), dim3(256), 0, st, desc, pix_indexes, M, B, ptrs);
+  (void)cur;
+  // 1: mapping inversion and data vector, one workgroup per sample
+  AJ_CHECK_HIP(hipFuncSetAttribute((const void*)prep_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)(sizeof(int) * (maxS + 1))));
+  hipLaunchKernelGGL(prep_kernel, dim3((unsigned)n), dim3(PREP_THREADS), sizeof(int) * (maxS + 1), st, desc,
+                     pix_indexes, pix_weights, M, B, dirty_image, ptrs, ekey, et, D);
   AJ_LAUNCH_CHECK();
-  hipLaunchKernelGGL(csc_scan_kernel, dim3((unsigned)n), dim3(1024), 0, st, desc, ptrs, cur);
-  AJ_LAUNCH_CHECK();
-  hipLaunchKernelGGL(csc_fill_kernel, dim3(gm, (unsigned)n), dim3(256), 0, st, desc, pix_indexes, pix_weights, M, B,
-                     cur, ekey, et);
-  AJ_LAUNCH_CHECK();
-  const unsigned gs = (unsigned)std::min<int64_t>(ceil_div(maxS, 256), 1024);
-  hipLaunchKernelGGL(csc_sort_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(maxS, 4), 65535), (unsigned)n),
-                     dim3(256), 0, st, desc, ptrs, B, ekey, et);
-  AJ_LAUNCH_CHECK();
-  hipLaunchKernelGGL(data_vector_kernel, dim3(gs, (unsigned)n), dim3(256), 0, st, desc, ptrs, ekey, et, dirty_image,
-                     D);
-  AJ_LAUNCH_CHECK();
+  // 2: Omega = W T, W read once for the batch
   const int R = (int)std::min<int64_t>(8, (int64_t)(OMEGA_LDS_BUDGET / (sizeof(double) * M)));
   timer_begin(AJ_TIMER_OMEGA, st);
   if (R >= 1) {
@@ -833,29 +1270,24 @@ int batch_pipeline(const WSrc w_tilde, int64_t M, const double* dirty_image, con
   }
   timer_end(AJ_TIMER_OMEGA, st);
   AJ_LAUNCH_CHECK();
+  // 3: lower triangles of A = F + H and of H
   timer_begin(AJ_TIMER_FGATHER, st);
-  hipLaunchKernelGGL(fgather_kernel, dim3((unsigned)maxS, (unsigned)n), dim3(256), 0, st, desc, ptrs, ekey, et, omega,
-                     mats, 1);
+  hipLaunchKernelGGL(assemble_kernel, dim3((unsigned)maxS, (unsigned)n), dim3(256), 0, st, desc, ptrs, ekey, et, omega,
+                     neighbors, neighbors_sizes, P, mats);
   timer_end(AJ_TIMER_FGATHER, st);
   AJ_LAUNCH_CHECK();
-  // H block: constant fill (0 or beta/S), then the regulariser into A and H.
-  hipLaunchKernelGGL(fill_kernel, dim3(256, (unsigned)n), dim3(256), 0, st, desc, mats, 1);
-  AJ_LAUNCH_CHECK();
-  hipLaunchKernelGGL(regularize_kernel, dim3(gs, (unsigned)n), dim3(256), 0, st, desc, neighbors, neighbors_sizes, P,
-                     mats, 1, 1, 1);
-  AJ_LAUNCH_CHECK();
-  // Step tables of the blocked Cholesky: matrices in decreasing size, per step the ones with
-  // columns left, with prefix counts of panel workgroups and trailing tiles.
+  // 4: blocked Cholesky of every A and H.  Step tables: matrices in decreasing size, per step the ones
+  // with columns left, with prefix counts of panel workgroups and trailing tiles.
   static thread_local std::vector<StepEntry> he;
-  static thread_local std::vector<int64_t> step_off, step_cnt, grid_p, grid_t;
+  static thread_local std::vector<int> tab;
+  static thread_local std::vector<int64_t> grid_p, grid_t;
   he.clear();
-  step_off.assign(p.maxsteps, 0);
-  step_cnt.assign(p.maxsteps, 0);
+  tab.assign(4 * p.maxsteps, 0);
   grid_p.assign(p.maxsteps, 0);
   grid_t.assign(p.maxsteps, 0);
   for (int64_t k = 0; k < p.maxsteps; ++k) {
     const int64_t k0 = k * NB;
-    step_off[k] = (int64_t)he.size();
+    tab[4 * k] = (int)he.size();
     for (int64_t q = 0; q < 2 * n; ++q) {
       const int j = 2 * ho[q >> 1] + (int)(q & 1);
       const int64_t S = S_host[j >> 1];
@@ -868,32 +1300,106 @@ int batch_pipeline(const WSrc w_tilde, int64_t M, const double* dirty_image, con
       grid_p[k] += ceil_div(m, (int64_t)256);
       grid_t[k] += tb * (tb + 1) / 2;
     }
-    step_cnt[k] = (int64_t)he.size() - step_off[k];
+    tab[4 * k + 1] = (int)he.size() - tab[4 * k];
+    tab[4 * k + 2] = (int)grid_p[k];
+    tab[4 * k + 3] = (int)grid_t[k];
   }
   AJ_REQUIRE((int64_t)he.size() == p.n_entries, "internal: step table size %zu != %lld", he.size(),
              (long long)p.n_entries);
   StepEntry* steps = reinterpret_cast<StepEntry*>(ws + p.off_steps);
+  int* step_tab = reinterpret_cast<int*>(ws + p.off_tab);
   double* ldp = reinterpret_cast<double*>(ws + p.off_ldp);
   int64_t* failp = reinterpret_cast<int64_t*>(ws + p.off_failp);
+  double* stage = reinterpret_cast<double*>(ws + p.off_stage);
   AJ_CHECK_HIP(hipMemcpyAsync(steps, he.data(), sizeof(StepEntry) * he.size(), hipMemcpyHostToDevice, st));
+  AJ_CHECK_HIP(hipMemcpyAsync(step_tab, tab.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice, st));
+  const int mode = chol_mode();
   timer_begin(AJ_TIMER_CHOL, st);
-  for (int64_t k = 0; k < p.maxsteps; ++k) {
-    const StepEntry* e = steps + step_off[k];
-    const int ne = (int)step_cnt[k];
-    hipLaunchKernelGGL(chol_diag_kernel, dim3((unsigned)ne), dim3(64), 0, st, desc, e, k * NB, (int)k,
-                       (int)p.maxsteps, mats, ldp, failp);
-    AJ_LAUNCH_CHECK();
-    if (grid_p[k] > 0) {
-      hipLaunchKernelGGL(chol_panel_kernel, dim3((unsigned)grid_p[k]), dim3(256), 0, st, desc, e, ne, k * NB, mats);
-      AJ_LAUNCH_CHECK();
-    }
-    if (grid_t[k] > 0) {
-      hipLaunchKernelGGL(chol_trail_kernel, dim3((unsigned)grid_t[k]), dim3(256), 0, st, desc, e, ne, k * NB, mats);
-      AJ_LAUNCH_CHECK();
+  bool done = false;
+  if (mode == CHOL_COOP || mode == CHOL_COOP_LOOKAHEAD) {
+    const int G = coop_grid();
+    if (G > 0) {
+      int maxsteps_i = (int)p.maxsteps, nmat = (int)(2 * n), look = mode == CHOL_COOP_LOOKAHEAD ? 1 : 0;
+      const SampleDesc* desc_c = desc;
+      const StepEntry* steps_c = steps;
+      const int* tab_c = step_tab;
+      void* args[] = {(void*)&desc_c, (void*)&steps_c, (void*)&tab_c, (void*)&maxsteps_i, (void*)&nmat, (void*)&look,
+                      (void*)&mats, (void*)&ldp, (void*)&failp, (void*)&logdet, (void*)&info};
+      AJ_CHECK_HIP(hipLaunchCooperativeKernel((const void*)chol_coop_kernel, dim3((unsigned)G), dim3(256), args, 0, st));
+      done = true;
     }
   }
-  hipLaunchKernelGGL(chol_finish_kernel, dim3((unsigned)ceil_div(2 * n, (int64_t)256)), dim3(256), 0, st, desc,
-                     (int)(2 * n), (int)p.maxsteps, ldp, failp, logdet, info);
+  if (!done && mode == CHOL_STEPS_LOOKAHEAD) {
+    // prologue: diagonal blocks and panels of step 0; then one launch per step
+    static thread_local std::vector<StepEntry> h3;
+    static thread_local std::vector<int64_t> off3, cnt3, na3, nt3;
+    h3.clear();
+    off3.assign(p.maxsteps, 0);
+    cnt3.assign(p.maxsteps, 0);
+    na3.assign(p.maxsteps, 0);
+    nt3.assign(p.maxsteps, 0);
+    for (int64_t k = 0; k + 1 < p.maxsteps; ++k) {
+      const int64_t k0 = k * NB;
+      off3[k] = (int64_t)h3.size();
+      for (int64_t q = 0; q < 2 * n; ++q) {
+        const int j = 2 * ho[q >> 1] + (int)(q & 1);
+        const int64_t S = S_host[j >> 1];
+        if (S <= k0 + NB) continue;  // no step k + 1
+        const int64_t below = S - (k0 + 2 * NB);  // rows below the next diagonal block
+        const int64_t na = std::max<int64_t>(1, ceil_div(std::max<int64_t>(below, 0), (int64_t)STRIP));
+        const int64_t tb = below > 0 ? ceil_div(below, (int64_t)TT) : 0;
+        AJ_REQUIRE(na3[k] + na < (1ll << 30) && nt3[k] + tb * (tb + 1) / 2 < (1ll << 30),
+                   "batch too large for one Cholesky step grid");
+        h3.push_back(StepEntry{j, (int)na3[k], (int)nt3[k]});
+        na3[k] += na;
+        nt3[k] += tb * (tb + 1) / 2;
+      }
+      cnt3[k] = (int64_t)h3.size() - off3[k];
+    }
+    AJ_REQUIRE((int64_t)h3.size() <= p.n_entries, "internal: lookahead step table size %zu", h3.size());
+    StepEntry* steps3 = steps + p.n_entries;
+    if (!h3.empty())
+      AJ_CHECK_HIP(hipMemcpyAsync(steps3, h3.data(), sizeof(StepEntry) * h3.size(), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(chol_diag_kernel, dim3((unsigned)tab[1]), dim3(64), 0, st, desc, steps, (int64_t)0, 0,
+                       (int)p.maxsteps, mats, ldp, failp);
+    AJ_LAUNCH_CHECK();
+    if (grid_p[0] > 0) {
+      hipLaunchKernelGGL(chol_panel_kernel, dim3((unsigned)grid_p[0]), dim3(256), 0, st, desc, steps, tab[1],
+                         (int64_t)0, mats);
+      AJ_LAUNCH_CHECK();
+    }
+    for (int64_t k = 0; k + 1 < p.maxsteps; ++k) {
+      if (cnt3[k] == 0) continue;
+      hipLaunchKernelGGL(chol_step_kernel, dim3((unsigned)(na3[k] + nt3[k])), dim3(256), 0, st, desc,
+                         steps3 + off3[k], (int)cnt3[k], (int)na3[k], k * NB, (int)k, (int)p.maxsteps, mats, ldp,
+                         failp, stage);
+      AJ_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(chol_unstage_kernel, dim3((unsigned)(2 * n)), dim3(256), 0, st, desc, mats, stage);
+    AJ_LAUNCH_CHECK();
+    hipLaunchKernelGGL(chol_finish_kernel, dim3((unsigned)ceil_div(2 * n, (int64_t)256)), dim3(256), 0, st, desc,
+                       (int)(2 * n), (int)p.maxsteps, ldp, failp, logdet, info);
+    done = true;
+  }
+  if (!done) {
+    for (int64_t k = 0; k < p.maxsteps; ++k) {
+      const StepEntry* e = steps + tab[4 * k];
+      const int ne = tab[4 * k + 1];
+      hipLaunchKernelGGL(chol_diag_kernel, dim3((unsigned)ne), dim3(64), 0, st, desc, e, k * NB, (int)k,
+                         (int)p.maxsteps, mats, ldp, failp);
+      AJ_LAUNCH_CHECK();
+      if (grid_p[k] > 0) {
+        hipLaunchKernelGGL(chol_panel_kernel, dim3((unsigned)grid_p[k]), dim3(256), 0, st, desc, e, ne, k * NB, mats);
+        AJ_LAUNCH_CHECK();
+      }
+      if (grid_t[k] > 0) {
+        hipLaunchKernelGGL(chol_trail_kernel, dim3((unsigned)grid_t[k]), dim3(256), 0, st, desc, e, ne, k * NB, mats);
+        AJ_LAUNCH_CHECK();
+      }
+    }
+    hipLaunchKernelGGL(chol_finish_kernel, dim3((unsigned)ceil_div(2 * n, (int64_t)256)), dim3(256), 0, st, desc,
+                       (int)(2 * n), (int)p.maxsteps, ldp, failp, logdet, info);
+  }
   timer_end(AJ_TIMER_CHOL, st);
   AJ_LAUNCH_CHECK();
   timer_begin(AJ_TIMER_SOLVE, st);
@@ -1854,5 +2360,12 @@ extern "C" int aj_barycentric_vjp(const double* vertices, const double* position
   hipLaunchKernelGGL(vertex_reduce_kernel, dim3((unsigned)ceil_div(S, 256)), dim3(256), 0, st, ptrs, ekey, contrib, S,
                      dvertices);
   AJ_LAUNCH_CHECK();
+  return AJ_OK;
+}
+
+extern "C" int aj_log_evidence_configure(int chol_mode_in) {
+  AJ_REQUIRE(chol_mode_in >= -1 && chol_mode_in <= CHOL_STEPS_LOOKAHEAD, "chol_mode %d: -1 (default) or 0 .. 3",
+             chol_mode_in);
+  g_chol_mode.store(chol_mode_in < 0 ? CHOL_STEPS_LOOKAHEAD : chol_mode_in, std::memory_order_relaxed);
   return AJ_OK;
 }
