@@ -1945,87 +1945,214 @@ extern "C" int aj_w_tilde_data_interferometer_from(const double* visibilities_re
 namespace aj {
 namespace lk {
 
-// Column j of the inverse from the lower factor L (one wave per column):
-//   inv != nullptr: inv[:, j] = L^-T L^-1 e_j;   colnorm2 != nullptr: colnorm2[j] = ||L^-1 e_j||^2.
-// Columns j0 .. j0 + nc - 1 of L^-1 (the forward substitution L y = e_j) and, with inv_all, of
-// A^-1 = L^-T L^-1 (then L^T x = y), for one sample per blockIdx.y; one wave per workgroup.  Both
-// substitutions read the factor by rows (the backward one right-looking), so a wave's loads are
-// contiguous.  The nc
-// columns share every factor element read (each is used for all of them), so the factor is streamed
-// once per nc columns; nc = INV_NC unless the columns of the largest sample do not fit in LDS.  Two
-// columns per workgroup were measured fastest (4 and 8 need more LDS, fewer workgroups fit on a CU and
-// the single-wave workgroups no longer hide the factor reads' latency).
-constexpr int INV_NC = 2;
+// Triangular inverses X = L^-1 of every factor of the batch (A and H), in place in a copy of the factors, by
+// recursive halving: the 32 x 32 diagonal blocks are inverted one per wave; then at each level the sibling
+// blocks (a, a + h) of size h (the second h2 <= h rows) are joined,
+//   X21 = -X22 T,   T = L21 X11,
+// two MFMA products (T kept transposed in the mirror of X21's block above the diagonal, which X does not
+// use).  ceil(log2(S / 32)) levels of two launches, all matrices' blocks of a level in one grid.
+constexpr int IB = 32;  // diagonal block of the recursion
+struct InvTask {
+  int mat;  // matrix: sample mat >> 1, A (even) or H (odd)
+  int a;    // first row and column of the pair
+  int ti;   // 64-row tile of the h2 rows below the pair's first half
+  int tj;   // 64-column tile of the h columns of the first half
+};
 
-__global__ void __launch_bounds__(64) factor_inverse_kernel(const SampleDesc* __restrict__ sd,
-                                                            const double* __restrict__ mats, int which_h, int nc_max,
-                                                            double* __restrict__ inv_all,
-                                                            double* __restrict__ colnorm2_all) {
-  extern __shared__ double y[];  // [nc_max][n]: column c of the block at y + c n
+// One wave per (matrix, diagonal block): lane l holds row l of the block's L (r) and builds row l of its
+// inverse (x), row k final when lane k divides by L_kk, broadcast by readlane to the rows below.
+__global__ void __launch_bounds__(256) trtri_diag_kernel(const SampleDesc* __restrict__ sd,
+                                                         const int2* __restrict__ blocks, int nblocks,
+                                                         double* __restrict__ X) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int t = (int)blockIdx.x * 4 + wave;
+  if (t >= nblocks) return;
+  const int j = blocks[t].x;
+  const int64_t k0 = (int64_t)blocks[t].y * IB;
+  int64_t n;
+  double* A = mat_ptr(sd, X, j, n);
+  const int nb = (int)std::min<int64_t>(IB, n - k0);
+  double r[IB], x[IB];
+#pragma unroll
+  for (int c = 0; c < IB; ++c) {
+    r[c] = (lane < nb && c <= lane) ? A[(k0 + lane) * n + k0 + c] : 0.0;
+    x[c] = 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < IB; ++k) {
+    if (k < nb) {
+      if (lane == k) {
+#pragma unroll
+        for (int c = 0; c <= k; ++c) x[c] = ((c == k ? 1.0 : 0.0) - x[c]) / r[k];
+      }
+#pragma unroll
+      for (int c = 0; c <= k; ++c) {
+        const double xkc = readlane_d(x[c], k);
+        if (lane > k) x[c] = fma(r[k], xkc, x[c]);
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < IB; ++c)
+    if (lane < nb && c <= lane) A[(k0 + lane) * n + k0 + c] = x[c];
+}
+
+// One 64 x 64 output tile of a level's product, K in chunks of 32 staged in LDS, four waves of 32 x 32.
+//   step 1: T[i][c] = sum_{k >= c} L21[i][k] X11[k][c], i < h2, c < h; stored as T^T above the diagonal.
+//   step 2: X21[i][c] = -sum_{k <= i} X22[i][k] T[k][c]; stored in X21's place.
+template <int STEP>
+__global__ void __launch_bounds__(256) trtri_level_kernel(const SampleDesc* __restrict__ sd,
+                                                          const InvTask* __restrict__ tasks, int64_t h,
+                                                          double* __restrict__ X) {
+  __shared__ double As[64][33];
+  __shared__ double Bs[32][65];
+  const InvTask tk = tasks[blockIdx.x];
+  int64_t n;
+  double* Xm = mat_ptr(sd, X, tk.mat, n);
+  const int64_t a = tk.a, h2 = std::min<int64_t>(h, n - a - h);
+  const int64_t i0 = (int64_t)tk.ti * 64, c0 = (int64_t)tk.tj * 64;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32, lr = lane & 15, lk = lane >> 4;
+  // k range: step 1 from c0 (X11 is lower triangular) to h; step 2 from 0 to the tile's last row (X22 lower)
+  const int64_t kbeg = STEP == 1 ? c0 : 0;
+  const int64_t kend = STEP == 1 ? h : std::min<int64_t>(h2, i0 + 64);
+  d4 acc[2][2];
+#pragma unroll
+  for (int p = 0; p < 2; ++p)
+#pragma unroll
+    for (int q = 0; q < 2; ++q) acc[p][q] = d4{0.0, 0.0, 0.0, 0.0};
+  for (int64_t kc = kbeg; kc < kend; kc += 32) {
+    for (int e = tid; e < 64 * 32; e += 256) {
+      const int i = e >> 5, k = e & 31;  // A operand (64 rows x 32 k), k fastest: coalesced in k
+      const int64_t gi = i0 + i, gk = kc + k;
+      double v = 0.0;
+      if (gi < h2 && gk < kend) {
+        if (STEP == 1) v = Xm[(a + h + gi) * n + a + gk];                          // L21
+        else if (gk <= gi) v = Xm[(a + h + gi) * n + a + h + gk];                  // X22, lower part
+      }
+      As[i][k] = v;
+    }
+    for (int e = tid; e < 32 * 64; e += 256) {
+      double v = 0.0;
+      if (STEP == 1) {
+        const int k = e >> 6, c = e & 63;  // X11[k][c]: a row of X11, coalesced in c
+        const int64_t gk = kc + k, gc = c0 + c;
+        if (gk < kend && gc < h && gc <= gk) v = Xm[(a + gk) * n + a + gc];
+        Bs[k][c] = v;
+      } else {
+        const int c = e >> 5, k = e & 31;  // T[k][c] = T^T[c][k]: a row of the stored T^T, coalesced in k
+        const int64_t gk = kc + k, gc = c0 + c;
+        if (gk < kend && gc < h) v = Xm[(a + gc) * n + a + h + gk];
+        Bs[k][c] = v;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < 8; ++ks) {
+      const int k = ks * 4 + lk;
+      const double fa0 = As[wr + lr][k], fa1 = As[wr + 16 + lr][k];
+      const double fb0 = Bs[k][wc + lr], fb1 = Bs[k][wc + 16 + lr];
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa0, fb0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa0, fb1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa1, fb0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa1, fb1, acc[1][1], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int p = 0; p < 2; ++p)
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t gi = i0 + wr + p * 16 + lk + 4 * r, gc = c0 + wc + q * 16 + lr;
+        if (gi < h2 && gc < h) {
+          if (STEP == 1) Xm[(a + gc) * n + a + h + gi] = acc[p][q][r];
+          else Xm[(a + h + gi) * n + a + gc] = -acc[p][q][r];
+        }
+      }
+}
+
+// A^-1 = X^T X for the A factor of every sample (X = L_A^-1 lower triangular), both triangles, into the
+// sample's S x S block of inv: tile (I, J), I >= J, sums X[k][I0 + i] X[k][J0 + j] over k >= I0.
+struct SymTask {
+  int smp;
+  int ti;
+  int tj;
+};
+
+__global__ void __launch_bounds__(256) inverse_gram_kernel(const SampleDesc* __restrict__ sd,
+                                                           const SymTask* __restrict__ tasks,
+                                                           const double* __restrict__ X, double* __restrict__ inv) {
+  __shared__ double As[32][65];
+  __shared__ double Bs[32][65];
+  const SymTask tk = tasks[blockIdx.x];
+  const SampleDesc d = sd[tk.smp];
+  const int64_t n = d.S;
+  const double* Xm = X + d.mat_off;
+  double* Ai = inv + d.mat_off / 2;
+  const int64_t I0 = (int64_t)tk.ti * 64, J0 = (int64_t)tk.tj * 64;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32, lr = lane & 15, lk = lane >> 4;
+  d4 acc[2][2];
+#pragma unroll
+  for (int p = 0; p < 2; ++p)
+#pragma unroll
+    for (int q = 0; q < 2; ++q) acc[p][q] = d4{0.0, 0.0, 0.0, 0.0};
+  for (int64_t kc = I0; kc < n; kc += 32) {
+    for (int e = tid; e < 32 * 64; e += 256) {
+      const int k = e >> 6, c = e & 63;  // rows k of X, columns of tile I (As) and J (Bs): coalesced in c
+      const int64_t gk = kc + k;
+      As[k][c] = (gk < n && I0 + c < n && I0 + c <= gk) ? Xm[gk * n + I0 + c] : 0.0;
+      Bs[k][c] = (gk < n && J0 + c < n && J0 + c <= gk) ? Xm[gk * n + J0 + c] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < 8; ++ks) {
+      const int k = ks * 4 + lk;
+      const double fa0 = As[k][wr + lr], fa1 = As[k][wr + 16 + lr];
+      const double fb0 = Bs[k][wc + lr], fb1 = Bs[k][wc + 16 + lr];
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa0, fb0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa0, fb1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa1, fb0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa1, fb1, acc[1][1], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int p = 0; p < 2; ++p)
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t gi = I0 + wr + p * 16 + lk + 4 * r, gj = J0 + wc + q * 16 + lr;
+        if (gi < n && gj < n) {
+          Ai[gi * n + gj] = acc[p][q][r];
+          if (tk.ti != tk.tj) Ai[gj * n + gi] = acc[p][q][r];
+        }
+      }
+}
+
+// hn2[row] = sum_c X_H[row][c]^2 (c <= row): squared row norms of L_H^-1, whose sum is tr(H^-1).  One wave
+// per row, four rows per workgroup, blockIdx.y the sample.
+__global__ void __launch_bounds__(256) inverse_rownorm_kernel(const SampleDesc* __restrict__ sd,
+                                                              const double* __restrict__ X,
+                                                              double* __restrict__ hn2) {
   const SampleDesc d = sd[blockIdx.y];
   const int64_t n = d.S;
-  const int64_t j0 = (int64_t)blockIdx.x * nc_max;
-  if (j0 >= n) return;
-  const int nc = (int)std::min<int64_t>(nc_max, n - j0);
-  const double* L = mats + d.mat_off + (which_h ? n * n : 0);
-  const int lane = threadIdx.x;
-  for (int64_t i = lane; i < (int64_t)nc * n; i += 64) y[i] = 0.0;
-  __syncthreads();
-  // forward: L y_c = e_(j0 + c); y_c is 0 above row j0 + c
-  for (int64_t i = j0; i < n; ++i) {
-    double acc[INV_NC];
-#pragma unroll
-    for (int c = 0; c < INV_NC; ++c) acc[c] = 0.0;
-    for (int64_t k = j0 + lane; k < i; k += 64) {
-      const double l = L[i * n + k];
-#pragma unroll
-      for (int c = 0; c < INV_NC; ++c)
-        if (c < nc) acc[c] = fma(l, y[c * n + k], acc[c]);
-    }
-#pragma unroll
-    for (int c = 0; c < INV_NC; ++c)
-      if (c < nc) acc[c] = wave_sum(acc[c]);
-    if (lane == 0) {
-      const double dii = L[i * n + i];
-#pragma unroll
-      for (int c = 0; c < INV_NC; ++c)
-        if (c < nc && i >= j0 + c) y[c * n + i] = ((i == j0 + c ? 1.0 : 0.0) - acc[c]) / dii;
-    }
-    __syncthreads();
-  }
-  if (colnorm2_all) {
-    for (int c = 0; c < nc; ++c) {
-      double nrm = 0.0;
-      for (int64_t i = j0 + c + lane; i < n; i += 64) nrm = fma(y[c * n + i], y[c * n + i], nrm);
-      nrm = wave_sum(nrm);
-      if (lane == 0) colnorm2_all[d.s_off + j0 + c] = nrm;
-    }
-  }
-  if (!inv_all) return;
-  // backward: L^T x_c = y_c (x overwrites y), right-looking so that the factor is read by rows
-  // (contiguous): once x_i = y_i / L_ii is final, row i of L updates y_k -= L_ik x_i for every k < i
-  for (int64_t i = n - 1; i >= 0; --i) {
-    const double dii = L[i * n + i];
-    double xi[INV_NC];
-#pragma unroll
-    for (int c = 0; c < INV_NC; ++c) xi[c] = c < nc ? y[c * n + i] / dii : 0.0;
-    __syncthreads();  // every lane has read y_i before lane 0 stores x_i below
-    if (lane == 0) {
-#pragma unroll
-      for (int c = 0; c < INV_NC; ++c)
-        if (c < nc) y[c * n + i] = xi[c];
-    }
-    for (int64_t k = lane; k < i; k += 64) {
-      const double l = L[i * n + k];
-#pragma unroll
-      for (int c = 0; c < INV_NC; ++c)
-        if (c < nc) y[c * n + k] = fma(-l, xi[c], y[c * n + k]);
-    }
-    __syncthreads();
-  }
-  double* inv = inv_all + d.mat_off / 2;  // one S*S block per sample
-  for (int c = 0; c < nc; ++c)
-    for (int64_t i = lane; i < n; i += 64) inv[i * n + j0 + c] = y[c * n + i];
+  const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (i >= n) return;
+  const double* row = X + d.mat_off + n * n + i * n;
+  double acc = 0.0;
+  for (int64_t c = lane; c <= i; c += 64) acc = fma(row[c], row[c], acc);
+  acc = wave_sum(acc);
+  if (lane == 0) hn2[d.s_off + i] = acc;
+}
+
+__global__ void copy_doubles_kernel(const double* __restrict__ src, int64_t count, double* __restrict__ dst) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < count; e += (int64_t)gridDim.x * blockDim.x)
+    dst[e] = src[e];
 }
 
 // Per-sample scalar gradient: one workgroup per sample.
@@ -2262,15 +2389,85 @@ __global__ void scan_small_kernel(int* __restrict__ p, int64_t S, int* __restric
 }  // namespace lk
 }  // namespace aj
 
+namespace {
+
+// Task tables of the backward's inverses: diagonal blocks of every factor, the tiles of every recursion level
+// (one list per level, the same tiles for both products of the level), the tiles of every A^-1 = X^T X.
+struct BackwardTasks {
+  std::vector<int2> diag;
+  std::vector<InvTask> level;
+  std::vector<int64_t> level_off, level_h;  // level l: tasks [level_off[l], level_off[l + 1]), block size h
+  std::vector<SymTask> gram;
+};
+
+void backward_tasks(int64_t n, const int64_t* S_host, BackwardTasks& t) {
+  t.diag.clear();
+  t.level.clear();
+  t.level_off.assign(1, 0);
+  t.level_h.clear();
+  t.gram.clear();
+  int64_t maxS = 0;
+  for (int64_t b = 0; b < n; ++b) maxS = std::max(maxS, S_host[b]);
+  for (int64_t j = 0; j < 2 * n; ++j)
+    for (int64_t k = 0; k < ceil_div(S_host[j >> 1], (int64_t)IB); ++k) t.diag.push_back(int2{(int)j, (int)k});
+  for (int64_t h = IB; h < maxS; h *= 2) {
+    for (int64_t j = 0; j < 2 * n; ++j) {
+      const int64_t S = S_host[j >> 1];
+      for (int64_t a = 0; a + h < S; a += 2 * h) {
+        const int64_t h2 = std::min<int64_t>(h, S - a - h);
+        for (int64_t ti = 0; ti < ceil_div(h2, (int64_t)64); ++ti)
+          for (int64_t tj = 0; tj < ceil_div(h, (int64_t)64); ++tj)
+            t.level.push_back(InvTask{(int)j, (int)a, (int)ti, (int)tj});
+      }
+    }
+    t.level_off.push_back((int64_t)t.level.size());
+    t.level_h.push_back(h);
+  }
+  for (int64_t b = 0; b < n; ++b) {
+    const int64_t nt = ceil_div(S_host[b], (int64_t)64);
+    for (int64_t ti = 0; ti < nt; ++ti)
+      for (int64_t tj = 0; tj <= ti; ++tj) t.gram.push_back(SymTask{(int)b, (int)ti, (int)tj});
+  }
+}
+
+struct BackwardLayout {
+  size_t off_x, off_inv, off_hn2, off_u, off_v, off_diag, off_level, off_gram, bytes;
+};
+
+BackwardLayout backward_layout(int64_t M, int64_t n, int64_t sumS, int64_t sumS2, const BackwardTasks& t) {
+  BackwardLayout L{};
+  size_t o = 0;
+  auto put = [&](size_t bytes) {
+    o = align_up(o, 256);
+    const size_t r = o;
+    o += bytes;
+    return r;
+  };
+  L.off_x = put(sizeof(double) * 2 * sumS2);
+  L.off_inv = put(sizeof(double) * sumS2);
+  L.off_hn2 = put(sizeof(double) * sumS);
+  L.off_u = put(sizeof(double) * M * n);
+  L.off_v = put(sizeof(double) * M * n);
+  L.off_diag = put(sizeof(int2) * t.diag.size());
+  L.off_level = put(sizeof(InvTask) * t.level.size());
+  L.off_gram = put(sizeof(SymTask) * t.gram.size());
+  L.bytes = align_up(o, 256);
+  return L;
+}
+
+}  // namespace
+
 extern "C" size_t aj_log_evidence_backward_workspace_bytes(int64_t M, int64_t n_samples, const int64_t* S_host) {
   if (M <= 0 || n_samples <= 0 || !S_host) return 0;
   int64_t sumS = 0, sumS2 = 0;
   for (int64_t b = 0; b < n_samples; ++b) {
+    if (S_host[b] <= 0) return 0;
     sumS += S_host[b];
     sumS2 += S_host[b] * S_host[b];
   }
-  return align_up(sizeof(double) * sumS2, 256) + align_up(sizeof(double) * sumS, 256) +
-         2 * align_up(sizeof(double) * M * n_samples, 256) + 1024;
+  static thread_local BackwardTasks t;
+  backward_tasks(n_samples, S_host, t);
+  return backward_layout(M, n_samples, sumS, sumS2, t).bytes;
 }
 
 extern "C" int aj_log_evidence_batched_backward(const double* w_tilde, int64_t M, const double* dirty_image,
@@ -2285,34 +2482,56 @@ extern "C" int aj_log_evidence_batched_backward(const double* w_tilde, int64_t M
   AJ_REQUIRE(w_tilde && dirty_image && pix_indexes && pix_weights && neighbors && neighbors_sizes &&
                  coefficients_dev && reconstruction && forward_workspace && dcoefficients && dweights,
              "null pointer");
+  for (int64_t b = 0; b < n_samples; ++b) AJ_REQUIRE(S_host[b] > 0, "sample %lld: no source pixels", (long long)b);
   BatchPlan p = plan_batch(M, B, n_samples, S_host);
   AJ_REQUIRE(forward_workspace_bytes >= p.bytes, "forward workspace does not match this batch");
-  const size_t need = aj_log_evidence_backward_workspace_bytes(M, n_samples, S_host);
-  AJ_REQUIRE(workspace && workspace_bytes >= need, "workspace too small: need %zu bytes, got %zu", need,
+  static thread_local BackwardTasks t;
+  backward_tasks(n_samples, S_host, t);
+  const BackwardLayout L = backward_layout(M, n_samples, p.sumS, p.sumS2, t);
+  AJ_REQUIRE(workspace && workspace_bytes >= L.bytes, "workspace too small: need %zu bytes, got %zu", L.bytes,
              workspace_bytes);
+  AJ_REQUIRE(t.diag.size() < (1u << 31) && t.level.size() < (1u << 31) && t.gram.size() < (1u << 31),
+             "batch too large for the backward's task grids");
   hipStream_t st = as_stream(stream);
   const char* fw = static_cast<const char*>(forward_workspace);
   const SampleDesc* desc = reinterpret_cast<const SampleDesc*>(fw + p.off_desc);
   const double* omega = reinterpret_cast<const double*>(fw + p.off_omega);
   const double* mats = reinterpret_cast<const double*>(fw + p.off_mats);
-  Arena ar(workspace, workspace_bytes);
-  double* inv = ar.take<double>(p.sumS2);
-  double* hn2 = ar.take<double>(p.sumS);
-  double* u = ar.take<double>(M * n_samples);
-  double* v = ar.take<double>(M * n_samples);
+  char* ws = static_cast<char*>(workspace);
+  double* X = reinterpret_cast<double*>(ws + L.off_x);
+  double* inv = reinterpret_cast<double*>(ws + L.off_inv);
+  double* hn2 = reinterpret_cast<double*>(ws + L.off_hn2);
+  double* u = reinterpret_cast<double*>(ws + L.off_u);
+  double* v = reinterpret_cast<double*>(ws + L.off_v);
+  int2* diag = reinterpret_cast<int2*>(ws + L.off_diag);
+  InvTask* level = reinterpret_cast<InvTask*>(ws + L.off_level);
+  SymTask* gram = reinterpret_cast<SymTask*>(ws + L.off_gram);
   int64_t maxS = 0;
   for (int64_t b = 0; b < n_samples; ++b) maxS = std::max(maxS, S_host[b]);
-  // inverse of A (mat_off counts both A and H blocks, so inv uses mat_off/2 per sample)
-  // columns per workgroup: as many as fit in LDS for the largest sample (at least one: maxS <= SOLVE_S_MAX)
-  const int nc = (int)std::max<int64_t>(1, std::min<int64_t>(INV_NC, (160 * 1024) / ((int64_t)sizeof(double) * maxS)));
-  const size_t inv_lds = sizeof(double) * nc * maxS;
-  const unsigned inv_blocks = (unsigned)ceil_div(maxS, (int64_t)nc);
-  AJ_CHECK_HIP(hipFuncSetAttribute((const void*)factor_inverse_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)inv_lds));
-  hipLaunchKernelGGL(factor_inverse_kernel, dim3(inv_blocks, (unsigned)n_samples), dim3(64), inv_lds, st, desc, mats,
-                     0, nc, inv, nullptr);
-  hipLaunchKernelGGL(factor_inverse_kernel, dim3(inv_blocks, (unsigned)n_samples), dim3(64), inv_lds, st, desc, mats,
-                     1, nc, nullptr, hn2);
+  AJ_CHECK_HIP(hipMemcpyAsync(diag, t.diag.data(), sizeof(int2) * t.diag.size(), hipMemcpyHostToDevice, st));
+  if (!t.level.empty())
+    AJ_CHECK_HIP(hipMemcpyAsync(level, t.level.data(), sizeof(InvTask) * t.level.size(), hipMemcpyHostToDevice, st));
+  AJ_CHECK_HIP(hipMemcpyAsync(gram, t.gram.data(), sizeof(SymTask) * t.gram.size(), hipMemcpyHostToDevice, st));
+  // X = L^-1 of every factor (the forward's factors are left as they are)
+  hipLaunchKernelGGL(copy_doubles_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(2 * p.sumS2, 256), 8192)),
+                     dim3(256), 0, st, mats, 2 * p.sumS2, X);
+  hipLaunchKernelGGL(trtri_diag_kernel, dim3((unsigned)ceil_div((int64_t)t.diag.size(), (int64_t)4)), dim3(256), 0,
+                     st, desc, diag, (int)t.diag.size(), X);
+  AJ_LAUNCH_CHECK();
+  for (size_t l = 0; l < t.level_h.size(); ++l) {
+    const int64_t cnt = t.level_off[l + 1] - t.level_off[l];
+    if (cnt == 0) continue;
+    hipLaunchKernelGGL(trtri_level_kernel<1>, dim3((unsigned)cnt), dim3(256), 0, st, desc, level + t.level_off[l],
+                       t.level_h[l], X);
+    hipLaunchKernelGGL(trtri_level_kernel<2>, dim3((unsigned)cnt), dim3(256), 0, st, desc, level + t.level_off[l],
+                       t.level_h[l], X);
+    AJ_LAUNCH_CHECK();
+  }
+  // A^-1 = X_A^T X_A (both triangles) and tr(H^-1) by rows of X_H
+  hipLaunchKernelGGL(inverse_gram_kernel, dim3((unsigned)t.gram.size()), dim3(256), 0, st, desc, gram, X, inv);
+  hipLaunchKernelGGL(inverse_rownorm_kernel, dim3((unsigned)ceil_div(maxS, (int64_t)4), (unsigned)n_samples),
+                     dim3(256), 0, st, desc, X, hn2);
+  AJ_LAUNCH_CHECK();
   hipLaunchKernelGGL(coef_grad_kernel, dim3((unsigned)n_samples), dim3(256), 0, st, desc, neighbors, neighbors_sizes,
                      P, reconstruction, inv, hn2, coefficients_dev, dcoefficients);
   const unsigned gm = (unsigned)std::min<int64_t>(ceil_div(M, 256), 1024);
