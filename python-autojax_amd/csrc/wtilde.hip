@@ -428,6 +428,127 @@ __global__ void __launch_bounds__(256) syrk_kernel(const void* __restrict__ Xv, 
 }
 
 // ---------------------------------------------------------------------------
+// FP64 SYRK (the plain path): 128 x 128 lower-triangle tiles, eight waves as 4 x 2, each a 32 x 64 block
+// (2 x 4 v_mfma_f64_16x16x4 blocks).  Each launch's K (its chunks' features) is cut into F64_SPLIT parts;
+// unit u is part u / ntiles of tile u % ntiles, so there are F64_SPLIT times as many units as tiles (a
+// last, partial round costs a smaller share of the launch) and the units in flight read the same part
+// of K (shared row panels in L2).  Part 0 accumulates into W, the other parts into their own partial
+// tiles P (tile-major), added to W once after the last launch.  Per 16-double stage the next stage's
+// panels are loaded into registers while the MFMAs read the current one from LDS: one barrier per stage.
+// Two workgroups fit a CU (74 KB LDS, 127 VGPRs each).  At M = 5,000 (820 tiles) the SYRK ran at 49, 57,
+// 60, 63, 60, 61 and 63 TFLOP/s with 1, 2, 3, 4, 5, 6 and 8 parts; 4 is the default (compile-time
+// AJ_F64_SPLIT).
+// ---------------------------------------------------------------------------
+#ifndef AJ_F64_SPLIT
+#define AJ_F64_SPLIT 4
+#endif
+constexpr int F64_TILE = 128;
+constexpr int F64_KS = 16;             // doubles of a row per stage
+constexpr int F64_PITCH = F64_KS + 2;  // LDS row pitch in doubles
+constexpr int F64_SPLIT = AJ_F64_SPLIT;
+static_assert(F64_SPLIT >= 1 && FC % F64_KS == 0 && FC / F64_KS >= F64_SPLIT, "every part of K holds a stage");
+
+__global__ void __launch_bounds__(512) syrk_f64_kernel(const double* __restrict__ X, int64_t M, int64_t ldx,
+                                                       int64_t kfeat, int ntiles, double* __restrict__ W,
+                                                       double* __restrict__ P, int first) {
+  __shared__ __attribute__((aligned(16))) double lds[2][2][F64_TILE * F64_PITCH];
+  const int h = (int)blockIdx.x / ntiles, t = (int)blockIdx.x % ntiles;
+  int64_t TI, TJ;
+  tile_of(t, TI, TJ);
+  const int64_t I0 = TI * F64_TILE, J0 = TJ * F64_TILE;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 64, lr = lane & 15, lk = lane >> 4;
+  const int64_t nst = kfeat / F64_KS;
+  const int64_t s_beg = h * nst / F64_SPLIT, s_end = (h + 1) * nst / F64_SPLIT;
+  // staging: per operand 128 rows x 8 16-byte segments; thread tid loads (row, seg) = (tid >> 3, tid & 7) and
+  // (row + 64, seg) of both operands (a diagonal tile loads its one panel twice: no branch in the loop).
+  // The next stage's loads are issued before this stage's MFMAs and land in registers while they run; the
+  // clamp reloads the last stage instead of branching.
+  static_assert(F64_KS == 16, "the staging below moves 8 segments of 16 bytes per row");
+  const int row0 = tid >> 3, sg = tid & 7;
+  const double* pa0 = X + (I0 + row0) * ldx + 2 * sg;
+  const double* pa1 = pa0 + 64 * ldx;
+  const double* pb0 = X + (J0 + row0) * ldx + 2 * sg;
+  const double* pb1 = pb0 + 64 * ldx;
+  const int o0 = row0 * F64_PITCH + 2 * sg, o1 = o0 + 64 * F64_PITCH;
+  d4 acc[2][4];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+  int64_t k = s_beg * F64_KS;
+  double2 ra0 = *reinterpret_cast<const double2*>(pa0 + k), ra1 = *reinterpret_cast<const double2*>(pa1 + k);
+  double2 rb0 = *reinterpret_cast<const double2*>(pb0 + k), rb1 = *reinterpret_cast<const double2*>(pb1 + k);
+  *reinterpret_cast<double2*>(lds[0][0] + o0) = ra0;
+  *reinterpret_cast<double2*>(lds[0][0] + o1) = ra1;
+  *reinterpret_cast<double2*>(lds[0][1] + o0) = rb0;
+  *reinterpret_cast<double2*>(lds[0][1] + o1) = rb1;
+  __syncthreads();
+  int buf = 0;
+  for (int64_t s = s_beg; s < s_end; ++s) {
+    // issue the next stage's loads first; they are written to the other buffer after this stage's MFMAs
+    // (that buffer was last read before the previous barrier)
+    k = std::min<int64_t>(s + 1, s_end - 1) * F64_KS;
+    ra0 = *reinterpret_cast<const double2*>(pa0 + k);
+    ra1 = *reinterpret_cast<const double2*>(pa1 + k);
+    rb0 = *reinterpret_cast<const double2*>(pb0 + k);
+    rb1 = *reinterpret_cast<const double2*>(pb1 + k);
+    const double* La = lds[buf][0];
+    const double* Lb = lds[buf][1];
+#pragma unroll
+    for (int kk = 0; kk < F64_KS / 4; ++kk) {
+      double fa[2], fb[4];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) fa[a] = La[(wr + a * 16 + lr) * F64_PITCH + kk * 4 + lk];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) fb[b] = Lb[(wc + b * 16 + lr) * F64_PITCH + kk * 4 + lk];
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[a], fb[b], acc[a][b], 0, 0, 0);
+    }
+    buf ^= 1;
+    *reinterpret_cast<double2*>(lds[buf][0] + o0) = ra0;
+    *reinterpret_cast<double2*>(lds[buf][0] + o1) = ra1;
+    *reinterpret_cast<double2*>(lds[buf][1] + o0) = rb0;
+    *reinterpret_cast<double2*>(lds[buf][1] + o1) = rb1;
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int lrow = wr + a * 16 + lk + 4 * r, lcol = wc + b * 16 + lr;
+        if (h == 0) {
+          const int64_t row = I0 + lrow, col = J0 + lcol;
+          if (row < M && col < M) {
+            double* q = W + row * M + col;
+            *q = first ? acc[a][b][r] : *q + acc[a][b][r];
+          }
+        } else {
+          double* q = P + ((int64_t)(h - 1) * ntiles + t) * (F64_TILE * F64_TILE) + lrow * F64_TILE + lcol;
+          *q = first ? acc[a][b][r] : *q + acc[a][b][r];
+        }
+      }
+}
+
+// W += the partial tiles of parts 1 .. F64_SPLIT - 1 (in part order), over the tiles' elements inside W.
+__global__ void syrk_f64_partials_kernel(const double* __restrict__ P, int ntiles, int64_t M, double* __restrict__ W) {
+  const int t = blockIdx.y;
+  int64_t TI, TJ;
+  tile_of(t, TI, TJ);
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < F64_TILE * F64_TILE; e += gridDim.x * blockDim.x) {
+    const int64_t row = TI * F64_TILE + e / F64_TILE, col = TJ * F64_TILE + e % F64_TILE;
+    if (row >= M || col >= M) continue;
+    double v = W[row * M + col];
+    for (int h = 1; h < F64_SPLIT; ++h) v += P[((int64_t)(h - 1) * ntiles + t) * (F64_TILE * F64_TILE) + e];
+    W[row * M + col] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Integer-digit SYRK: W += sum_c scale_c^2/Q^2 * sum_{p,q} 256^(2NS-2-p-q) A_p A_q^T.
 // Workgroup tile 64 rows x 128 columns (4 waves as 2x2, each 32x64 = 2x4
 // MFMA 16x16 blocks); one int32 accumulator set per digit order p+q, exact
@@ -1237,7 +1358,7 @@ inline Plan make_plan(int64_t K, int64_t M, int mode) {
   p.mode = mode;
   p.ns = slices_of(mode);
   p.es = esize_of(mode);
-  const int64_t pad = is_i8(mode) ? 2 * TILE : TILE;
+  const int64_t pad = (is_i8(mode) || mode == AJ_PREC_FP64) ? 2 * TILE : TILE;
   p.Mpad = ceil_div(M, pad) * pad;
   const int64_t nch = std::max<int64_t>(1, ceil_div(K, KC));
   const double per_chunk = (double)p.ns * p.Mpad * FC * p.es;
@@ -1248,6 +1369,10 @@ inline Plan make_plan(int64_t K, int64_t M, int mode) {
   p.feat_bytes = (size_t)p.ns * p.Mpad * p.ldx * p.es;
   p.bytes = align_up(p.feat_bytes, 256) + 256 /*centre*/ + align_up(sizeof(double) * p.group, 256) + 256;
   p.bytes += align_up(sizeof(int) * 2 * (size_t)i8_tiles(ceil_div(M, 2 * TILE) * 2 * TILE), 256) + 256;
+  if (mode == AJ_PREC_FP64) {
+    const int64_t nt = p.Mpad / F64_TILE;
+    p.bytes += align_up(sizeof(double) * (F64_SPLIT - 1) * (size_t)(nt * (nt + 1) / 2) * F64_TILE * F64_TILE, 256) + 256;
+  }
   return p;
 }
 
@@ -1257,6 +1382,11 @@ int run(const double* noise, const double* uv, const double* grid, int64_t K, in
   void* X = ar.take<char>(p.feat_bytes);
   double* centre = ar.take<double>(2);
   double* sc = ar.take<double>(p.group);
+  double* partials = nullptr;
+  if constexpr (MODE == AJ_PREC_FP64) {
+    const int64_t n128 = p.Mpad / F64_TILE;
+    partials = ar.take<double>((F64_SPLIT - 1) * (size_t)(n128 * (n128 + 1) / 2) * F64_TILE * F64_TILE);
+  }
   int* tiles = nullptr;
   int64_t nt8 = 0;
   [[maybe_unused]] const int layout = g_i8_layout.load(std::memory_order_relaxed);
@@ -1318,6 +1448,10 @@ int run(const double* noise, const double* uv, const double* grid, int64_t K, in
         case 5: hipLaunchKernelGGL((syrk_i8_dma_kernel<NSL, NGL>), g8, dim3(256), 0, st, X8, M, p.Mpad, p.ldx, ng, sc, W, nt8, tiles); break;
         default: hipLaunchKernelGGL((syrk_i8_kernel<NSL, NGL, 0>), g8, dim3(256), 0, st, X8, M, p.Mpad, p.ldx, ng, sc, W, nt8, tiles); break;
       }
+    } else if constexpr (MODE == AJ_PREC_FP64) {
+      const int64_t n128 = p.Mpad / F64_TILE, t128 = n128 * (n128 + 1) / 2;
+      hipLaunchKernelGGL(syrk_f64_kernel, dim3((unsigned)(F64_SPLIT * t128)), dim3(512), 0, st,
+                         static_cast<const double*>(X), M, p.ldx, (int64_t)ng * FC, (int)t128, W, partials, first);
     } else {
       hipLaunchKernelGGL(syrk_kernel<MODE>, dim3((unsigned)ntiles), dim3(256), 0, st, X, M, p.Mpad, p.ldx, ng, sc, W,
                          first);
@@ -1325,6 +1459,14 @@ int run(const double* noise, const double* uv, const double* grid, int64_t K, in
     timer_end(AJ_TIMER_WT_SYRK, st);
     AJ_LAUNCH_CHECK();
     first = 0;
+  }
+  if constexpr (MODE == AJ_PREC_FP64) {
+    if (!first) {
+      const int64_t n128 = p.Mpad / F64_TILE, t128 = n128 * (n128 + 1) / 2;
+      hipLaunchKernelGGL(syrk_f64_partials_kernel, dim3(16, (unsigned)t128), dim3(256), 0, st, partials, (int)t128, M,
+                         W);
+      AJ_LAUNCH_CHECK();
+    }
   }
   dim3 mg((unsigned)ceil_div(M, 32), (unsigned)ceil_div(M, 32));
   hipLaunchKernelGGL(mirror_kernel, mg, dim3(256), 0, st, W, M);
