@@ -188,7 +188,7 @@ def main():
 
     # Cholesky schedule of the batched evaluation, tuned on the 64-sample batch and, for the plain path's
     # one-sample evaluations, on its median-size sample (each path then runs with its own)
-    sched = {"batch": 3, "single": 3}
+    sched = {"batch": 4, "single": 4}
     tuned_batch, tuned_single = {}, {}
     if not args.no_autotune:
         E.w_tilde(noise, uv, grid, "fp64", out=W_plain, workspace=ws_plain)

@@ -284,8 +284,11 @@ int aj_log_evidence_batched(const double* w_tilde, int64_t M, const double* dirt
 /* How the batched log-evidence factors its matrices (process-wide tuning, no reference counterpart):
  * 0 = three launches per 32-column step (diagonal blocks, panels, trailing tiles); 1 = the same steps in one
  * cooperative launch with a grid-wide barrier between phases; 2 = 1 with lookahead (the next diagonal block
- * factored by the workgroup that updates it); 3 (default, also -1) = one launch per step with lookahead (the
- * next step's diagonal block and panel updated, factored and solved beside the step's trailing update).
+ * factored by the workgroup that updates it); 3 = one launch per step with lookahead (the next step's
+ * diagonal block and panel updated, factored and solved beside the step's trailing update); 4 (default,
+ * also -1) = left-looking, one launch per 32-column block after one that factors every first diagonal
+ * block (each strip of the block updated from all earlier blocks in registers and solved, the next
+ * diagonal block formed and factored by the block's first task; no trailing matrix written between steps).
  * All give bitwise the same factors. */
 int aj_log_evidence_configure(int chol_mode);
 

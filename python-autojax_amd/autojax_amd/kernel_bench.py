@@ -52,7 +52,7 @@ def main(argv=None):
     ap.add_argument("--chunk", type=int, default=0, help="int8crt: visibilities per exact chunk (0 = default)")
     ap.add_argument("--check", action="store_true", help="compare W-tilde with kernel variant 0 (bitwise)")
     ap.add_argument("--group-mib", type=int, default=0, help="cap on the features of one SYRK launch (0 = default)")
-    ap.add_argument("--modes", default="0,1,2,3", help="batch: Cholesky schedules to time (aj_log_evidence_configure)")
+    ap.add_argument("--modes", default="0,1,2,3,4", help="batch: Cholesky schedules to time (aj_log_evidence_configure)")
     ap.add_argument("--against", default=None, help="also build W-tilde at this precision and report the "
                                                     "normwise difference max|dW| / max|W|")
     a = ap.parse_args(argv)

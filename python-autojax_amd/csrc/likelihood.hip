@@ -871,15 +871,16 @@ __global__ void chol_finish_kernel(const SampleDesc* __restrict__ sd, int nmat, 
   if (j < nmat) chol_finish_dev(sd, j, maxsteps, ldp, failp, logdet, info);
 }
 
-// After the lookahead launches: every matrix's last diagonal block (from its second block on, each is
-// staged by the launch that factors it) moved from its stage slot into place.  One workgroup per matrix.
+// After the lookahead (or left-looking) launches: every matrix's last diagonal block (from block
+// first_staged on, each is staged by the launch that factors it) moved from its stage slot into place.
+// One workgroup per matrix.
 __global__ void chol_unstage_kernel(const SampleDesc* __restrict__ sd, double* __restrict__ mats,
-                                    const double* __restrict__ stage) {
+                                    const double* __restrict__ stage, int first_staged) {
   const int j = blockIdx.x;
   int64_t n;
   double* A = mat_ptr(sd, mats, j, n);
   const int64_t steps = (n + NB - 1) / NB;
-  if (steps < 2) return;
+  if (steps < 1 + first_staged) return;
   const int64_t kl = (steps - 1) * NB;
   const int nb = (int)(n - kl);
   const double* stg = stage + (int64_t)j * NB * NB;
@@ -888,6 +889,194 @@ __global__ void chol_unstage_kernel(const SampleDesc* __restrict__ sd, double* _
     if (i < nb && c <= i) A[(kl + i) * n + kl + c] = stg[x];
   }
 }
+
+// Left-looking form (mode 4): one launch per 32-column block k after a launch that factors every
+// diagonal block 0.  A task takes strip rows of block k below its diagonal block and applies to them the
+// updates of every earlier block p < k (A[rows, k] -= L[rows, p] L[kblock, p]^T, accumulated in
+// registers over p; the panels of each p staged in LDS while the next p's loads are in flight), then
+// solves them against L_kk, one row per lane with the row in registers.  The trailing matrix is never
+// written between steps: each element of the factor is written once.  The first task of a matrix
+// (96 strip rows; the others take 128) also forms the next diagonal block k + 1 (its updates by the blocks
+// p < k from the same staged panels, the one by block k from its own solved rows, which are that block's
+// rows) and factors it, for the next launch: it goes to a stage slot (parity k + 1), the next launch's
+// tasks read it from there and that launch's first task moves it into place (no task of that launch
+// reads the block in place).  Each element sees the same MFMA operands, in the same order, and the same
+// substitutions as in the step-synchronous form, so the factor is bitwise the same.
+constexpr int LL_STRIP = 128;            // strip rows of a task: four waves of 32 rows
+constexpr int LL_STRIP0 = 96;            // of the first task, whose fourth wave forms the next diagonal block
+constexpr int LL_ROWS = LL_STRIP + NB;   // staged rows: the strip and the diagonal block's rows
+constexpr int LL_PITCH = NB + 2;         // LDS row pitch (doubles): the fragment reads are free of conflicts
+constexpr int LL_THREADS = 256;
+constexpr int LL_LOADS = LL_ROWS * NB / LL_THREADS;  // doubles per thread per staged block
+static_assert(LL_LOADS * LL_THREADS == LL_ROWS * NB && LL_THREADS / NB == 8, "eight rows per load pass");
+
+__device__ inline const double* ll_row(const double* A, int64_t n, int64_t s0, int64_t k0, int r) {
+  return A + (r < LL_STRIP ? s0 + r : k0 + (r - LL_STRIP)) * n;
+}
+
+__global__ void __launch_bounds__(LL_THREADS) chol_left_kernel(const SampleDesc* __restrict__ sd,
+                                                              const StepEntry* __restrict__ e, int ne, int64_t k0,
+                                                              int kstep, int maxsteps, double* __restrict__ mats,
+                                                              double* __restrict__ ldp, int64_t* __restrict__ failp,
+                                                              double* __restrict__ stage) {
+  __shared__ double Ls[LL_ROWS * LL_PITCH];
+  __shared__ double Lkk[NB][NB + 1];
+  const int ei = find_entry(e, ne, (int)blockIdx.x, false);
+  const int j = e[ei].mat, rb = (int)blockIdx.x - e[ei].pfx_p;
+  int64_t n;
+  double* A = mat_ptr(sd, mats, j, n);
+  const int nb = (int)std::min<int64_t>(NB, n - k0);
+  const bool next = rb == 0 && k0 + NB < n;  // this task forms diagonal block k + 1
+  const int nb1 = next ? (int)std::min<int64_t>(NB, n - k0 - NB) : 0;
+  const int64_t s0 = k0 + nb + (rb == 0 ? 0 : LL_STRIP0 + (int64_t)(rb - 1) * LL_STRIP);  // first strip row
+  const int rows = (int)std::max<int64_t>(0, std::min<int64_t>(rb == 0 ? LL_STRIP0 : LL_STRIP, n - s0));
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const double* stg_in = stage + ((int64_t)2 * j + (kstep & 1)) * NB * NB;
+  double* stg_out = stage + ((int64_t)2 * j + ((kstep + 1) & 1)) * NB * NB;
+  // L_kk: factored in place by the first launch (step 0) or staged by the previous launch
+  for (int x = tid; x < NB * NB; x += LL_THREADS) {
+    const int i = x / NB, c = x % NB;
+    double v = 0.0;
+    if (i < nb && c <= i) v = kstep == 0 ? A[(k0 + i) * n + k0 + c] : stg_in[x];
+    Lkk[i][c] = v;
+    if (rb == 0 && kstep > 0 && i < nb && c <= i) A[(k0 + i) * n + k0 + c] = v;
+  }
+  // wave w < 3 (and wave 3 when the task does not form the next block) owns strip rows 32 w .. 32 w + 31;
+  // wave 3 of a task that does forms block k + 1 from strip rows 0 .. 31 (its rows k0 + 32 ..)
+  const bool dwave = next && wave == 3;
+  const int w0 = wave * 32;
+  const bool active = dwave || w0 < rows;
+  d4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = a * 16 + lk + 4 * r, c = b * 16 + lr;
+        double v = 0.0;
+        if (dwave) {
+          if (i < nb1 && c <= i) v = A[(k0 + NB + i) * n + k0 + NB + c];
+        } else if (w0 + i < rows && c < nb) {
+          v = A[(s0 + w0 + i) * n + k0 + c];
+        }
+        acc[a][b][r] = v;
+      }
+  if (kstep > 0) {
+    // staging: thread tid loads column lc of LDS rows lr0 + 8 q: rows < LL_STRIP are strip rows s0 + r (zero
+    // past the task's rows), the last NB the diagonal block's rows k0 + i (zero past nb); block p + 1's
+    // loads are in flight during block p's MFMAs.
+    const int lc = tid & 31, lr0 = tid >> 5;
+    unsigned okm = 0;
+#pragma unroll
+    for (int q = 0; q < LL_LOADS; ++q) {
+      const int r = lr0 + 8 * q;
+      if (r < LL_STRIP ? r < rows || (next && r < nb1) : r - LL_STRIP < nb) okm |= 1u << q;
+    }
+    double pre[LL_LOADS];
+#pragma unroll
+    for (int q = 0; q < LL_LOADS; ++q) pre[q] = (okm >> q) & 1u ? ll_row(A, n, s0, k0, lr0 + 8 * q)[lc] : 0.0;
+    for (int p = 0; p < kstep; ++p) {
+      __syncthreads();  // every wave is past its fragment reads of block p - 1
+#pragma unroll
+      for (int q = 0; q < LL_LOADS; ++q) Ls[(lr0 + 8 * q) * LL_PITCH + lc] = pre[q];
+      __syncthreads();
+      if (p + 1 < kstep) {
+        const int64_t off = (int64_t)(p + 1) * NB + lc;
+#pragma unroll
+        for (int q = 0; q < LL_LOADS; ++q) pre[q] = (okm >> q) & 1u ? ll_row(A, n, s0, k0, lr0 + 8 * q)[off] : 0.0;
+      }
+      if (active) {
+        // strip waves: rows w0 .. w0 + 31 against the diagonal block's rows; the block-(k + 1) wave: strip
+        // rows 0 .. 31 against themselves (its block (0, 1), above the diagonal, is not used)
+        const int ra = dwave ? 0 : w0, rbb = dwave ? 0 : LL_STRIP;
+#pragma unroll
+        for (int ks = 0; ks < NB / 4; ++ks) {
+          const int pc = ks * 4 + lk;
+          const double fa0 = -Ls[(ra + lr) * LL_PITCH + pc], fa1 = -Ls[(ra + 16 + lr) * LL_PITCH + pc];
+          const double fb0 = Ls[(rbb + lr) * LL_PITCH + pc], fb1 = Ls[(rbb + 16 + lr) * LL_PITCH + pc];
+          acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa0, fb0, acc[0][0], 0, 0, 0);
+          acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa0, fb1, acc[0][1], 0, 0, 0);
+          acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa1, fb0, acc[1][0], 0, 0, 0);
+          acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa1, fb1, acc[1][1], 0, 0, 0);
+        }
+      }
+    }
+  }
+  // the strip rows into Xs (the staging buffer, free after this barrier)
+  double* Xs = Ls;
+  __syncthreads();
+  if (!dwave) {
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Xs[(w0 + a * 16 + lk + 4 * r) * LL_PITCH + b * 16 + lr] = acc[a][b][r];
+  }
+  __syncthreads();
+  // X L_kk^T = (updated rows): lanes 0 .. 31 of wave w, row w0 + lane, the row in registers
+  if (!dwave && lane < 32 && w0 + lane < rows) {
+    double* xr = Xs + (w0 + lane) * LL_PITCH;
+    double x[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) x[c] = xr[c];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+      if (c < nb) {
+        double v = x[c];
+#pragma unroll
+        for (int l = 0; l < c; ++l) v -= x[l] * Lkk[c][l];
+        x[c] = v / Lkk[c][c];
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int c = 0; c < NB; ++c) xr[c] = x[c];
+  }
+  __syncthreads();
+  for (int x = tid; x < rows * NB; x += LL_THREADS) {
+    const int i = x / NB, c = x % NB;
+    if (c < nb) A[(s0 + i) * n + k0 + c] = Xs[i * LL_PITCH + c];
+  }
+  if (!dwave) return;
+  // block k + 1: the update by block k (its rows' solved values: strip rows 0 .. nb1 - 1), then the
+  // factorisation, into the stage slot of step k + 1
+#pragma unroll
+  for (int ks = 0; ks < NB / 4; ++ks) {
+    const int pc = ks * 4 + lk;
+    const double fa0 = -Xs[lr * LL_PITCH + pc], fa1 = -Xs[(16 + lr) * LL_PITCH + pc];
+    const double fb0 = Xs[lr * LL_PITCH + pc], fb1 = Xs[(16 + lr) * LL_PITCH + pc];
+    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa0, fb0, acc[0][0], 0, 0, 0);
+    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa1, fb0, acc[1][0], 0, 0, 0);
+    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa1, fb1, acc[1][1], 0, 0, 0);
+  }
+  // Lkk is free: the other waves are past the substitution (the barrier above)
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = a * 16 + lk + 4 * r, c = b * 16 + lr;
+        Lkk[i][c] = (b <= a && c <= i && i < nb1) ? acc[a][b][r] : 0.0;
+      }
+  double r[NB];
+#pragma unroll
+  for (int c = 0; c < NB; ++c) r[c] = (lane < nb1 && c <= lane) ? Lkk[lane][c] : 0.0;
+  double ld;
+  int64_t fail;
+  chol_factor_regs(r, nb1, k0 + NB, lane, ld, fail);
+#pragma unroll
+  for (int c = 0; c < NB; ++c)
+    if (lane < NB) stg_out[lane * NB + c] = r[c];
+  if (lane == 0) {
+    ldp[(int64_t)j * maxsteps + kstep + 1] = ld;
+    failp[(int64_t)j * maxsteps + kstep + 1] = fail;
+  }
+}
+static_assert(sizeof(double) * (LL_ROWS * LL_PITCH + NB * (NB + 1)) <= 54 * 1024, "three workgroups per CU");
 
 // The whole blocked Cholesky of every matrix of the batch in one cooperative launch: the workgroups stay
 // resident and walk the steps together, a grid-wide barrier between the phases of a step (the launch
@@ -1163,7 +1352,7 @@ inline BatchPlan plan_batch(int64_t M, int64_t B, int64_t n, const int64_t* S_ho
   p.off_tab = put(sizeof(int) * 4 * p.maxsteps);
   p.off_ldp = put(sizeof(double) * 2 * n * p.maxsteps);
   p.off_failp = put(sizeof(int64_t) * 2 * n * p.maxsteps);
-  p.off_stage = put(sizeof(double) * 2 * n * NB * NB);
+  p.off_stage = put(sizeof(double) * 4 * n * NB * NB);  // two slots per matrix
   p.bytes = align_up(o, 256);
   return p;
 }
@@ -1177,10 +1366,11 @@ using namespace aj::lk;
 namespace {
 
 // How the batched Cholesky runs (process-wide, aj_log_evidence_configure): three launches per 32-column
-// step; one cooperative launch (without or with lookahead); one launch per step with lookahead (the
-// default).  All give the same factor bit for bit.
-constexpr int CHOL_STEPS = 0, CHOL_COOP = 1, CHOL_COOP_LOOKAHEAD = 2, CHOL_STEPS_LOOKAHEAD = 3;
-std::atomic<int> g_chol_mode{CHOL_STEPS_LOOKAHEAD};
+// step; one cooperative launch (without or with lookahead); one launch per step with lookahead; one
+// left-looking launch per step (the default).  All give the same factor bit for bit.
+constexpr int CHOL_STEPS = 0, CHOL_COOP = 1, CHOL_COOP_LOOKAHEAD = 2, CHOL_STEPS_LOOKAHEAD = 3, CHOL_LEFT = 4;
+constexpr int CHOL_MODES = 5;
+std::atomic<int> g_chol_mode{CHOL_LEFT};
 int chol_mode() { return g_chol_mode.load(std::memory_order_relaxed); }
 
 // Workgroups of the cooperative Cholesky: every one resident at once (occupancy x compute units), per
@@ -1329,6 +1519,45 @@ int batch_pipeline(const WSrc w_tilde, int64_t M, const double* dirty_image, con
       done = true;
     }
   }
+  if (!done && mode == CHOL_LEFT) {
+    // per step, the matrices with columns left, each with its strip tasks (the first: LL_STRIP0 rows)
+    static thread_local std::vector<StepEntry> h4;
+    static thread_local std::vector<int64_t> off4, cnt4, nt4;
+    h4.clear();
+    off4.assign(p.maxsteps, 0);
+    cnt4.assign(p.maxsteps, 0);
+    nt4.assign(p.maxsteps, 0);
+    for (int64_t k = 0; k < p.maxsteps; ++k) {
+      const int64_t k0 = k * NB;
+      off4[k] = (int64_t)h4.size();
+      for (int64_t q = 0; q < 2 * n; ++q) {
+        const int j = 2 * ho[q >> 1] + (int)(q & 1);
+        const int64_t S = S_host[j >> 1];
+        if (S <= k0) continue;
+        const int64_t below = S - std::min<int64_t>(S, k0 + NB);
+        const int64_t nt = 1 + ceil_div(std::max<int64_t>(0, below - LL_STRIP0), (int64_t)LL_STRIP);
+        AJ_REQUIRE(nt4[k] + nt < (1ll << 30), "batch too large for one Cholesky step grid");
+        h4.push_back(StepEntry{j, (int)nt4[k], 0});
+        nt4[k] += nt;
+      }
+      cnt4[k] = (int64_t)h4.size() - off4[k];
+    }
+    AJ_REQUIRE((int64_t)h4.size() == p.n_entries, "internal: left-looking step table size %zu", h4.size());
+    StepEntry* steps4 = steps + p.n_entries;
+    AJ_CHECK_HIP(hipMemcpyAsync(steps4, h4.data(), sizeof(StepEntry) * h4.size(), hipMemcpyHostToDevice, st));
+    // every matrix's diagonal block 0, factored in place; then one launch per block column
+    hipLaunchKernelGGL(chol_diag_kernel, dim3((unsigned)tab[1]), dim3(64), 0, st, desc, steps, (int64_t)0, 0,
+                       (int)p.maxsteps, mats, ldp, failp);
+    AJ_LAUNCH_CHECK();
+    for (int64_t k = 0; k < p.maxsteps; ++k) {
+      hipLaunchKernelGGL(chol_left_kernel, dim3((unsigned)nt4[k]), dim3(LL_THREADS), 0, st, desc, steps4 + off4[k],
+                         (int)cnt4[k], k * NB, (int)k, (int)p.maxsteps, mats, ldp, failp, stage);
+      AJ_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(chol_finish_kernel, dim3((unsigned)ceil_div(2 * n, (int64_t)256)), dim3(256), 0, st, desc,
+                       (int)(2 * n), (int)p.maxsteps, ldp, failp, logdet, info);
+    done = true;
+  }
   if (!done && mode == CHOL_STEPS_LOOKAHEAD) {
     // prologue: diagonal blocks and panels of step 0; then one launch per step
     static thread_local std::vector<StepEntry> h3;
@@ -1375,7 +1604,7 @@ int batch_pipeline(const WSrc w_tilde, int64_t M, const double* dirty_image, con
                          failp, stage);
       AJ_LAUNCH_CHECK();
     }
-    hipLaunchKernelGGL(chol_unstage_kernel, dim3((unsigned)(2 * n)), dim3(256), 0, st, desc, mats, stage);
+    hipLaunchKernelGGL(chol_unstage_kernel, dim3((unsigned)(2 * n)), dim3(256), 0, st, desc, mats, stage, 1);
     AJ_LAUNCH_CHECK();
     hipLaunchKernelGGL(chol_finish_kernel, dim3((unsigned)ceil_div(2 * n, (int64_t)256)), dim3(256), 0, st, desc,
                        (int)(2 * n), (int)p.maxsteps, ldp, failp, logdet, info);
@@ -2583,8 +2812,8 @@ extern "C" int aj_barycentric_vjp(const double* vertices, const double* position
 }
 
 extern "C" int aj_log_evidence_configure(int chol_mode_in) {
-  AJ_REQUIRE(chol_mode_in >= -1 && chol_mode_in <= CHOL_STEPS_LOOKAHEAD, "chol_mode %d: -1 (default) or 0 .. 3",
-             chol_mode_in);
-  g_chol_mode.store(chol_mode_in < 0 ? CHOL_STEPS_LOOKAHEAD : chol_mode_in, std::memory_order_relaxed);
+  AJ_REQUIRE(chol_mode_in >= -1 && chol_mode_in < CHOL_MODES, "chol_mode %d: -1 (default) or 0 .. %d",
+             chol_mode_in, CHOL_MODES - 1);
+  g_chol_mode.store(chol_mode_in < 0 ? CHOL_LEFT : chol_mode_in, std::memory_order_relaxed);
   return AJ_OK;
 }

@@ -528,9 +528,10 @@ def test_batched_log_evidence_vs_oracle(gpu, delaunay_small):
 
 
 def test_batched_cholesky_modes_bitwise_equal(gpu, delaunay_small):
-    """The four ways the batch factors its matrices (three launches per step; one cooperative launch,
-    without and with lookahead; one launch per step with lookahead) give bitwise the same log-evidence,
-    log-determinants and reconstructions, also with samples ending inside a step's block (S 97, 31, 300)."""
+    """The five ways the batch factors its matrices (three launches per step; one cooperative launch,
+    without and with lookahead; one launch per step with lookahead; left-looking, one launch per step) give
+    bitwise the same log-evidence, log-determinants and reconstructions, also with samples ending inside a
+    step's block (S 97, 31, 300) and one whose block columns take three 224-row strips (S 700)."""
     import torch
 
     from autojax_amd import _lib
@@ -540,17 +541,18 @@ def test_batched_cholesky_modes_bitwise_equal(gpu, delaunay_small):
     Wd = torch.from_numpy(W).cuda()
     y = torch.from_numpy(d.dirty_image).cuda()
     c2nn = E.chi2_noise_norm(d.data, d.noise_map)
-    extra = [ds.small_delaunay(80 + i, d.M, 10, S, lam_range=(0.1, 10.0)).samples[0] for i, S in enumerate((97, 31, 300))]
+    extra = [ds.small_delaunay(80 + i, d.M, 10, S, lam_range=(0.1, 10.0)).samples[0]
+             for i, S in enumerate((97, 31, 300, 700))]
     batch = E.SampleBatch(d.samples + extra, d.M)
     out = {}
     try:
-        for mode in (0, 1, 2, 3):
+        for mode in (0, 1, 2, 3, 4):
             _lib.call("aj_log_evidence_configure", mode)
             le = batch.log_evidence(Wd, y, c2nn, check=True).clone()
             out[mode] = (le.cpu().numpy(), batch.logdets.cpu().numpy().copy(), batch.recon.cpu().numpy().copy())
     finally:
         _lib.call("aj_log_evidence_configure", -1)
-    for mode in (1, 2, 3):
+    for mode in (1, 2, 3, 4):
         for a, b in zip(out[mode], out[0]):
             np.testing.assert_array_equal(a, b)
     print("cholesky modes bitwise equal:", out[0][0][:4], flush=True)
