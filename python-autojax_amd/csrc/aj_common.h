@@ -78,4 +78,29 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
+// Direct global -> LDS loads (raw buffer addressing): lane L of the wave writes its `size` bytes at
+// LDS address lds_ptr + size * L (lds_ptr wave-uniform); offsets at or past the resource's byte count
+// read as zeros.
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+
+__device__ void raw_buffer_load_lds(i32x4 rsrc, __attribute__((address_space(3))) uint32_t* lds_ptr, int size,
+                                    int voffset, int soffset, int offset, int aux)
+    __asm("llvm.amdgcn.raw.buffer.load.lds");
+
+// Buffer resource for byte offsets below 2^32 from base (stride 0, raw addressing).
+__device__ inline i32x4 make_rsrc(const void* base, uint32_t bytes) {
+  const uint64_t a = reinterpret_cast<uint64_t>(base);
+  i32x4 r;
+  r[0] = (int)(uint32_t)a;
+  r[1] = (int)(uint32_t)(a >> 32) & 0xFFFF;
+  r[2] = (int)bytes;
+  r[3] = 0x00020000;
+  return r;
+}
+
+// s_waitcnt immediate that waits for vmcnt <= n and leaves expcnt and lgkmcnt unconstrained
+// (gfx9 encoding: vmcnt[3:0] and [15:14], expcnt [6:4], lgkmcnt [11:8]).
+constexpr unsigned vmcnt_imm(unsigned n) { return (n & 0xF) | ((n >> 4) << 14) | (0x7u << 4) | (0xFu << 8); }
+
 }  // namespace aj

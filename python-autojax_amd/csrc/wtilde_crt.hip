@@ -386,10 +386,6 @@ __global__ void __launch_bounds__(512, 1) syrk_ring8_kernel(const int8_t* __rest
   store_residues(acc, c, mi, ntiles, t, (wave >> 2) * 8, (wave & 3) * 4, lane, R);
 }
 
-// s_waitcnt immediate that waits for vmcnt <= n and leaves expcnt and lgkmcnt unconstrained
-// (gfx9 encoding: vmcnt[3:0] and [15:14], expcnt [6:4], lgkmcnt [11:8]).
-constexpr unsigned vmcnt_imm(unsigned n) { return (n & 0xF) | ((n >> 4) << 14) | (0x7u << 4) | (0xFu << 8); }
-
 // ---------------------------------------------------------------------------
 // Eight waves with the fragment reads software-pipelined across stages: after the barrier of
 // stage s every wave issues the loads of stage s + 3 (raw buffer loads straight to LDS, into the
@@ -397,27 +393,10 @@ constexpr unsigned vmcnt_imm(unsigned n) { return (n & 0xF) | ((n >> 4) << 14) |
 // the fragments it read during stage s - 1.  The barrier of stage s follows each wave's wait for
 // its loads of stage s + 1 (the loads of stage s + 2 may still be in flight).
 // ---------------------------------------------------------------------------
-typedef int i32x4 __attribute__((ext_vector_type(4)));
-
-__device__ void raw_buffer_load_lds(i32x4 rsrc, __attribute__((address_space(3))) uint32_t* lds_ptr, int size,
-                                    int voffset, int soffset, int offset, int aux)
-    __asm("llvm.amdgcn.raw.buffer.load.lds");
-
-// Buffer resource for byte offsets below 2^32 from base (stride 0, raw addressing).
-__device__ inline i32x4 make_rsrc(const void* base, uint32_t bytes) {
-  const uint64_t a = reinterpret_cast<uint64_t>(base);
-  i32x4 r;
-  r[0] = (int)(uint32_t)a;
-  r[1] = (int)(uint32_t)(a >> 32) & 0xFFFF;
-  r[2] = (int)bytes;
-  r[3] = 0x00020000;
-  return r;
-}
 
 // Pipeline steps shared by the residue SYRK kernels.  They use the kernel's locals: lds (the ring of
 // four 32 KB stage buffers), rsrc (buffer resource of the residue plane), oA / oB (this wave's byte
 // offsets of its A and B blocks at stage 0), jstr, wave, ablk, bblk, roff, acc and the SPLIT parameter.
-typedef __attribute__((address_space(3))) uint32_t lds_u32;
 #define AJ_PIPE_ISSUE(ST)                                                                                \
   {                                                                                                      \
     const uint32_t ko_ = (uint32_t)(ST) * 1024u;                                                         \
