@@ -546,6 +546,40 @@ __device__ inline void chol_factor_regs(double (&r)[NB], int nb, int64_t k0, int
   ld = log(mant) + (double)ex * 0.69314718055994530942;
 }
 
+// The same factorisation with each column broadcast through LDS (col: NB doubles of LDS owned by the wave)
+// instead of one lane read per element: the same operations on the same values, so the same bits.
+__device__ inline void chol_factor_lds(double (&r)[NB], int nb, int64_t k0, int lane, double& ld, int64_t& fail,
+                                       double* col) {
+  double mant = 1.0;
+  int ex = 0;
+  fail = 0;
+#pragma unroll
+  for (int c = 0; c < NB; ++c) {
+    if (c < nb) {
+      double d = readlane_d(r[c], c);
+      if (!(d > 0.0)) {
+        if (!fail) fail = k0 + c + 1;
+        d = 1.0;  // keep the arithmetic finite; the result is flagged
+      }
+      d = sqrt(d);
+      int e;
+      mant *= frexp(d, &e);
+      ex += e;
+      if (lane == c) r[c] = d;
+      if (lane > c) r[c] /= d;
+      if (lane < NB) col[lane] = r[c];  // column c: L[q][c] in lane q
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int q = c + 1; q < NB; ++q) {
+        const double lqc = col[q];
+        if (lane >= q) r[q] = fma(-r[c], lqc, r[q]);
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+  ld = log(mant) + (double)ex * 0.69314718055994530942;
+}
+
 // Diagonal block k0 of matrix j, one wave: load, factor, store, record the block's log-determinant part and
 // first failing pivot for step kstep.
 __device__ inline void chol_diag_dev(const SampleDesc* __restrict__ sd, int j, int64_t k0, int kstep, int maxsteps,
@@ -910,6 +944,10 @@ constexpr int LL_THREADS = 256;
 constexpr int LL_LOADS = LL_ROWS * NB / LL_THREADS;  // doubles per thread per staged block
 static_assert(LL_LOADS * LL_THREADS == LL_ROWS * NB && LL_THREADS / NB == 8, "eight rows per load pass");
 
+// Index (doubles) of element (r, c) in a staged block of NB-double rows whose 16-byte segments are stored
+// XOR (r mod 16), so that the 16 rows of a fragment read fall in 16 different segments.
+__device__ inline int ll_at(int r, int c) { return r * NB + ((((c >> 1) ^ (r & 15)) << 1) | (c & 1)); }
+
 __device__ inline const double* ll_row(const double* A, int64_t n, int64_t s0, int64_t k0, int r) {
   return A + (r < LL_STRIP ? s0 + r : k0 + (r - LL_STRIP)) * n;
 }
@@ -1067,7 +1105,8 @@ __global__ void __launch_bounds__(LL_THREADS) chol_left_kernel(const SampleDesc*
   for (int c = 0; c < NB; ++c) r[c] = (lane < nb1 && c <= lane) ? Lkk[lane][c] : 0.0;
   double ld;
   int64_t fail;
-  chol_factor_regs(r, nb1, k0 + NB, lane, ld, fail);
+  __shared__ double colb[NB];
+  chol_factor_lds(r, nb1, k0 + NB, lane, ld, fail, colb);
 #pragma unroll
   for (int c = 0; c < NB; ++c)
     if (lane < NB) stg_out[lane * NB + c] = r[c];
@@ -1077,6 +1116,193 @@ __global__ void __launch_bounds__(LL_THREADS) chol_left_kernel(const SampleDesc*
   }
 }
 static_assert(sizeof(double) * (LL_ROWS * LL_PITCH + NB * (NB + 1)) <= 54 * 1024, "three workgroups per CU");
+
+// The same step for the last steps, when a matrix has few rows left below its diagonal block and a task's
+// time is its chain of blocks p < k: 32-row tasks, each 16 x 16 block of the strip (and of the next
+// diagonal block, in the first task) on its own wave, so that a block p costs 8 MFMAs per wave; the
+// staged rows (32 strip rows, then the diagonal block's 32) arrive by direct global -> LDS loads five blocks
+// ahead of their use.  Same operations per element as chol_left_kernel, so the same bits.
+constexpr int LS_ROWS = 2 * NB;          // staged rows per block: the strip's 32, the diagonal block's 32
+constexpr int LS_STAGES = 6;             // ring of staged blocks: the one read and five in flight
+constexpr int LS_THREADS = 512;          // waves 0-3: the strip's four 16 x 16 blocks; 4-7: the next diagonal block's
+constexpr int LS_LOADS = LS_ROWS / 4 / (LS_THREADS / 64);  // 1 KB loads per wave per block
+static_assert(LS_LOADS == 2 && LS_STAGES * LS_ROWS * NB * 8 + NB * (NB + 1) * 8 * 2 <= 160 * 1024, "small-task LDS");
+
+__global__ void __launch_bounds__(LS_THREADS) chol_left_small_kernel(const SampleDesc* __restrict__ sd,
+                                                                    const StepEntry* __restrict__ e, int ne,
+                                                                    int64_t k0, int kstep, int maxsteps,
+                                                                    double* __restrict__ mats, double* __restrict__ ldp,
+                                                                    int64_t* __restrict__ failp,
+                                                                    double* __restrict__ stage) {
+  __shared__ __attribute__((aligned(1024))) double Ls[LS_STAGES][LS_ROWS * NB];
+  __shared__ double Lkk[NB][NB + 1];
+  __shared__ double Dn[NB][NB + 1];
+  __shared__ double colb[NB];
+  const int ei = find_entry(e, ne, (int)blockIdx.x, false);
+  const int j = e[ei].mat, rb = (int)blockIdx.x - e[ei].pfx_p;
+  int64_t n;
+  double* A = mat_ptr(sd, mats, j, n);
+  const int nb = (int)std::min<int64_t>(NB, n - k0);
+  const bool next = rb == 0 && k0 + NB < n;
+  const int nb1 = next ? (int)std::min<int64_t>(NB, n - k0 - NB) : 0;
+  const int64_t s0 = k0 + nb + (int64_t)rb * NB;  // first strip row
+  const int rows = (int)std::max<int64_t>(0, std::min<int64_t>(NB, n - s0));
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const double* stg_in = stage + ((int64_t)2 * j + (kstep & 1)) * NB * NB;
+  double* stg_out = stage + ((int64_t)2 * j + ((kstep + 1) & 1)) * NB * NB;
+  for (int x = tid; x < NB * NB; x += LS_THREADS) {
+    const int i = x / NB, c = x % NB;
+    double v = 0.0;
+    if (i < nb && c <= i) v = kstep == 0 ? A[(k0 + i) * n + k0 + c] : stg_in[x];
+    Lkk[i][c] = v;
+    if (rb == 0 && kstep > 0 && i < nb && c <= i) A[(k0 + i) * n + k0 + c] = v;
+  }
+  // wave w < 4: strip block (w >> 1, w & 1); wave w >= 4: block ((w - 4) >> 1, (w - 4) & 1) of the next
+  // diagonal block (its rows k0 + 32 .. are the strip's rows 0 .. 31; block (0, 1) is above the diagonal
+  // and not used)
+  const bool dwave = wave >= 4;
+  const int ba = (wave >> 1) & 1, bb = wave & 1;
+  d4 acc;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int i = ba * 16 + lk + 4 * r, c = bb * 16 + lr;
+    double v = 0.0;
+    if (dwave) {
+      if (i < nb1 && c <= i) v = A[(k0 + NB + i) * n + k0 + NB + c];
+    } else if (i < rows && c < nb) {
+      v = A[(s0 + i) * n + k0 + c];
+    }
+    acc[r] = v;
+  }
+  if (kstep > 0) {
+    // wave w issues LDS rows 4 (2 w + jj) .. + 3 of each block (rows < 32: strip rows s0 + r, the others the
+    // diagonal block's rows k0 + r - 32); rows outside read as zeros
+    const i32x4 rsrc = make_rsrc(A, (uint32_t)(n * n * (int64_t)sizeof(double)));
+    int voff[LS_LOADS];
+#pragma unroll
+    for (int jj = 0; jj < LS_LOADS; ++jj) {
+      const int r = 4 * (LS_LOADS * wave + jj) + (lane >> 4);
+      const bool ok = r < NB ? r < rows || (next && r < nb1) : r - NB < nb;
+      const int64_t grow = r < NB ? s0 + r : k0 + (r - NB);
+      const int g = (lane & 15) ^ (r & 15);
+      voff[jj] = ok ? (int)((grow * n + 2 * g) * (int64_t)sizeof(double)) : -1;
+    }
+    constexpr int LS_OOB = 0x7ffff000;  // past any matrix's bytes (n <= SOLVE_S_MAX)
+#define AJ_LS_ISSUE(P)                                                                                     \
+  {                                                                                                        \
+    double* lb_ = Ls[(P) % LS_STAGES];                                                                     \
+    _Pragma("unroll") for (int jj = 0; jj < LS_LOADS; ++jj) raw_buffer_load_lds(                          \
+        rsrc, (lds_u32*)(lb_ + (LS_LOADS * wave + jj) * 128), 16,                                          \
+        (P) < kstep && voff[jj] >= 0 ? voff[jj] + (P) * NB * (int)sizeof(double) : LS_OOB, 0, 0, 0);      \
+  }
+    // branch-free: blocks past the last are issued as zeros, so each wait counts the same loads
+#pragma unroll
+    for (int q = 0; q < LS_STAGES - 1; ++q) AJ_LS_ISSUE(q)
+    const int ra = 16 * ba, rbb = dwave ? 16 * bb : NB + 16 * bb;
+    for (int p = 0; p < kstep; ++p) {
+      // block p has landed (blocks p + 1 .. p + 4 may be in flight); after the barrier every wave is past
+      // its reads of block p - 1, whose buffer takes block p + 5
+      __builtin_amdgcn_s_waitcnt(vmcnt_imm(LS_LOADS * (LS_STAGES - 2)));
+      asm volatile("s_barrier" ::: "memory");
+      AJ_LS_ISSUE(p + LS_STAGES - 1)
+      // the block's fragments as LDS loads in assembly, one group waiting for itself (the compiler would
+      // wait for every direct-to-LDS load in flight before any LDS read)
+      const uint32_t lb = (uint32_t)(uintptr_t)(lds_u32*)Ls[p % LS_STAGES];
+      uint32_t aa[NB / 4], ab[NB / 4];
+#pragma unroll
+      for (int ks = 0; ks < NB / 4; ++ks) {
+        aa[ks] = lb + 8u * (uint32_t)ll_at(ra + lr, ks * 4 + lk);
+        ab[ks] = lb + 8u * (uint32_t)ll_at(rbb + lr, ks * 4 + lk);
+      }
+      double fa[NB / 4], fb[NB / 4];
+      asm volatile(
+          "ds_read_b64 %0, %16\n\tds_read_b64 %1, %17\n\tds_read_b64 %2, %18\n\tds_read_b64 %3, %19\n\t"
+          "ds_read_b64 %4, %20\n\tds_read_b64 %5, %21\n\tds_read_b64 %6, %22\n\tds_read_b64 %7, %23\n\t"
+          "ds_read_b64 %8, %24\n\tds_read_b64 %9, %25\n\tds_read_b64 %10, %26\n\tds_read_b64 %11, %27\n\t"
+          "ds_read_b64 %12, %28\n\tds_read_b64 %13, %29\n\tds_read_b64 %14, %30\n\tds_read_b64 %15, %31\n\t"
+          "s_waitcnt lgkmcnt(0)"
+          : "=&v"(fa[0]), "=&v"(fa[1]), "=&v"(fa[2]), "=&v"(fa[3]), "=&v"(fa[4]), "=&v"(fa[5]), "=&v"(fa[6]),
+            "=&v"(fa[7]), "=&v"(fb[0]), "=&v"(fb[1]), "=&v"(fb[2]), "=&v"(fb[3]), "=&v"(fb[4]), "=&v"(fb[5]),
+            "=&v"(fb[6]), "=&v"(fb[7])
+          : "v"(aa[0]), "v"(aa[1]), "v"(aa[2]), "v"(aa[3]), "v"(aa[4]), "v"(aa[5]), "v"(aa[6]), "v"(aa[7]),
+            "v"(ab[0]), "v"(ab[1]), "v"(ab[2]), "v"(ab[3]), "v"(ab[4]), "v"(ab[5]), "v"(ab[6]), "v"(ab[7])
+          : "memory");
+#pragma unroll
+      for (int ks = 0; ks < NB / 4; ++ks) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(-fa[ks], fb[ks], acc, 0, 0, 0);
+    }
+#undef AJ_LS_ISSUE
+    // the loads issued past the last block land before the buffers are reused
+    __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
+  }
+  // the strip's rows into Xs, the next diagonal block into Dn
+  double* Xs = Ls[0];
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int i = ba * 16 + lk + 4 * r, c = bb * 16 + lr;
+    if (dwave)
+      Dn[i][c] = acc[r];
+    else
+      Xs[i * LL_PITCH + c] = acc[r];
+  }
+  __syncthreads();
+  // X L_kk^T = (updated rows): lanes 0 .. 31 of wave 0, one row each, the row in registers
+  if (wave == 0 && lane < rows) {
+    double* xr = Xs + lane * LL_PITCH;
+    double x[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) x[c] = xr[c];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+      if (c < nb) {
+        double v = x[c];
+#pragma unroll
+        for (int l = 0; l < c; ++l) v -= x[l] * Lkk[c][l];
+        x[c] = v / Lkk[c][c];
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int c = 0; c < NB; ++c) xr[c] = x[c];
+  }
+  __syncthreads();
+  for (int x = tid; x < rows * NB; x += LS_THREADS) {
+    const int i = x / NB, c = x % NB;
+    if (c < nb) A[(s0 + i) * n + k0 + c] = Xs[i * LL_PITCH + c];
+  }
+  if (!next) return;
+  // the next diagonal block: the update by block k (the strip's solved rows), then its factorisation
+  if (dwave) {
+#pragma unroll
+    for (int ks = 0; ks < NB / 4; ++ks) {
+      const int pc = ks * 4 + lk;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(-Xs[(16 * ba + lr) * LL_PITCH + pc], Xs[(16 * bb + lr) * LL_PITCH + pc],
+                                                 acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = ba * 16 + lk + 4 * r, c = bb * 16 + lr;
+      Dn[i][c] = (bb <= ba && c <= i && i < nb1) ? acc[r] : 0.0;
+    }
+  }
+  __syncthreads();
+  if (wave == 4) {
+    double r[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) r[c] = (lane < nb1 && c <= lane) ? Dn[lane][c] : 0.0;
+    double ld;
+    int64_t fail;
+    chol_factor_lds(r, nb1, k0 + NB, lane, ld, fail, colb);
+#pragma unroll
+    for (int c = 0; c < NB; ++c)
+      if (lane < NB) stg_out[lane * NB + c] = r[c];
+    if (lane == 0) {
+      ldp[(int64_t)j * maxsteps + kstep + 1] = ld;
+      failp[(int64_t)j * maxsteps + kstep + 1] = fail;
+    }
+  }
+}
 
 // The whole blocked Cholesky of every matrix of the batch in one cooperative launch: the workgroups stay
 // resident and walk the steps together, a grid-wide barrier between the phases of a step (the launch
@@ -1520,22 +1746,41 @@ int batch_pipeline(const WSrc w_tilde, int64_t M, const double* dirty_image, con
     }
   }
   if (!done && mode == CHOL_LEFT) {
-    // per step, the matrices with columns left, each with its strip tasks (the first: LL_STRIP0 rows)
+    // per step, the matrices with columns left, each with its strip tasks: 128-row tasks (the first
+    // LL_STRIP0 rows), or, once the 32-row tasks of a step fit the compute units one workgroup each,
+    // 32-row tasks (a step's time is then its tasks' chains of blocks, which they run four times faster)
+    static int cus = 0;
+    if (cus == 0) {
+      int dev = 0, c = 0;
+      if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess)
+        cus = c;
+      if (cus <= 0) cus = 256;
+    }
     static thread_local std::vector<StepEntry> h4;
     static thread_local std::vector<int64_t> off4, cnt4, nt4;
+    static thread_local std::vector<char> small4;
     h4.clear();
     off4.assign(p.maxsteps, 0);
     cnt4.assign(p.maxsteps, 0);
     nt4.assign(p.maxsteps, 0);
+    small4.assign(p.maxsteps, 0);
     for (int64_t k = 0; k < p.maxsteps; ++k) {
       const int64_t k0 = k * NB;
+      int64_t nsmall = 0;
+      for (int64_t b = 0; b < n; ++b) {
+        const int64_t S = S_host[b];
+        if (S > k0) nsmall += 2 * std::max<int64_t>(1, ceil_div(S - std::min<int64_t>(S, k0 + NB), (int64_t)NB));
+      }
+      const bool small = nsmall <= cus;
+      small4[k] = small;
       off4[k] = (int64_t)h4.size();
       for (int64_t q = 0; q < 2 * n; ++q) {
         const int j = 2 * ho[q >> 1] + (int)(q & 1);
         const int64_t S = S_host[j >> 1];
         if (S <= k0) continue;
         const int64_t below = S - std::min<int64_t>(S, k0 + NB);
-        const int64_t nt = 1 + ceil_div(std::max<int64_t>(0, below - LL_STRIP0), (int64_t)LL_STRIP);
+        const int64_t nt = small ? std::max<int64_t>(1, ceil_div(below, (int64_t)NB))
+                                 : 1 + ceil_div(std::max<int64_t>(0, below - LL_STRIP0), (int64_t)LL_STRIP);
         AJ_REQUIRE(nt4[k] + nt < (1ll << 30), "batch too large for one Cholesky step grid");
         h4.push_back(StepEntry{j, (int)nt4[k], 0});
         nt4[k] += nt;
@@ -1550,8 +1795,12 @@ int batch_pipeline(const WSrc w_tilde, int64_t M, const double* dirty_image, con
                        (int)p.maxsteps, mats, ldp, failp);
     AJ_LAUNCH_CHECK();
     for (int64_t k = 0; k < p.maxsteps; ++k) {
-      hipLaunchKernelGGL(chol_left_kernel, dim3((unsigned)nt4[k]), dim3(LL_THREADS), 0, st, desc, steps4 + off4[k],
-                         (int)cnt4[k], k * NB, (int)k, (int)p.maxsteps, mats, ldp, failp, stage);
+      if (small4[k])
+        hipLaunchKernelGGL(chol_left_small_kernel, dim3((unsigned)nt4[k]), dim3(LS_THREADS), 0, st, desc,
+                           steps4 + off4[k], (int)cnt4[k], k * NB, (int)k, (int)p.maxsteps, mats, ldp, failp, stage);
+      else
+        hipLaunchKernelGGL(chol_left_kernel, dim3((unsigned)nt4[k]), dim3(LL_THREADS), 0, st, desc, steps4 + off4[k],
+                           (int)cnt4[k], k * NB, (int)k, (int)p.maxsteps, mats, ldp, failp, stage);
       AJ_LAUNCH_CHECK();
     }
     hipLaunchKernelGGL(chol_finish_kernel, dim3((unsigned)ceil_div(2 * n, (int64_t)256)), dim3(256), 0, st, desc,

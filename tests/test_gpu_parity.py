@@ -435,7 +435,8 @@ def test_batched_log_evidence_full_size_vs_oracle(gpu):
     """The batching workload at full size (M 5,000, K 1e6, 64 samples, S ~ U{300..1500}, lambda in
     [0.01, 100]): the batched log-evidence from the FP64 W within 1e-9 relative, and from the mixed
     path's W (int8crt11) within 1e-3 absolute, of the oracle's on six samples (the smallest lambda, the
-    largest and the smallest S, three more), the oracle given the FP64 W and the same dirty image."""
+    largest and the smallest S, three more), the oracle given the FP64 W and the same dirty image; and the
+    default factorisation schedule bitwise equal to schedules 0 and 3 at this size."""
     import torch
 
     from autojax_amd import device as dv
@@ -448,6 +449,19 @@ def test_batched_log_evidence_full_size_vs_oracle(gpu):
     c2nn = E.chi2_noise_norm(d.data, d.noise_map)
     W64 = E.w_tilde(n_d, uv_d, g_d, "fp64")
     out = {"fp64": batch.log_evidence(W64, y, c2nn, check=True).cpu().numpy()}
+    # the factorisation schedules at this size (the left-looking one switches from 128-row to 32-row tasks
+    # in its last steps) give the same bits as the three-launches-per-step one
+    from autojax_amd import _lib
+
+    ref = (out["fp64"].copy(), batch.logdets.cpu().numpy().copy(), batch.recon.cpu().numpy().copy())
+    try:
+        for mode in (0, 3):
+            _lib.call("aj_log_evidence_configure", mode)
+            le = batch.log_evidence(W64, y, c2nn, check=True).cpu().numpy()
+            for a, b in zip((le, batch.logdets.cpu().numpy(), batch.recon.cpu().numpy()), ref):
+                np.testing.assert_array_equal(a, b)
+    finally:
+        _lib.call("aj_log_evidence_configure", -1)
     Wm = E.w_tilde(n_d, uv_d, g_d, "int8crt11")
     out["int8crt11"] = batch.log_evidence(Wm, y, c2nn, check=True).cpu().numpy()
     torch.cuda.synchronize()
