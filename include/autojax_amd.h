@@ -58,7 +58,8 @@ int aj_abi_version(void);
 #define AJ_TIMER_CHOL 4        /* batched Cholesky of A and H */
 #define AJ_TIMER_SOLVE 5       /* triangular solves + log-evidence */
 #define AJ_TIMER_WT_COMBINE 6  /* W-tilde residue recombination (AJ_PREC_INT8CRT) */
-#define AJ_TIMER_COUNT 7
+#define AJ_TIMER_BATCH 7       /* a whole batched evaluation, first to last launch */
+#define AJ_TIMER_COUNT 8
 int aj_timing_enable(int on);
 int aj_timing_read(int timer, double* total_ms, int64_t* count);
 

@@ -130,7 +130,8 @@ def call(name: str, *args):
     check(getattr(load(), name)(*args), name)
 
 
-TIMERS = {"wt_syrk": 0, "wt_features": 1, "omega": 2, "fgather": 3, "chol": 4, "solve": 5, "wt_combine": 6}
+TIMERS = {"wt_syrk": 0, "wt_features": 1, "omega": 2, "fgather": 3, "chol": 4, "solve": 5, "wt_combine": 6,
+          "batch": 7}
 
 
 def timing_enable(on: bool = True) -> None:

@@ -1660,6 +1660,7 @@ int batch_pipeline(const WSrc w_tilde, int64_t M, const double* dirty_image, con
   }
   AJ_REQUIRE(maxS <= SOLVE_S_MAX, "a sample of %lld source pixels exceeds the solve kernel's LDS (%d at most)",
              (long long)maxS, SOLVE_S_MAX);
+  timer_begin(AJ_TIMER_BATCH, st);
   AJ_CHECK_HIP(hipMemcpyAsync(desc, hd.data(), sizeof(SampleDesc) * n, hipMemcpyHostToDevice, st));
   static thread_local std::vector<int> ho;
   ho.resize(n);
@@ -1886,6 +1887,7 @@ int batch_pipeline(const WSrc w_tilde, int64_t M, const double* dirty_image, con
   hipLaunchKernelGGL(solve_loglike_kernel, dim3((unsigned)n), dim3(SOLVE_THREADS), sizeof(double) * maxS, st, desc, mats, D,
                      chi2_nn, logdet, recon, logl);
   timer_end(AJ_TIMER_SOLVE, st);
+  timer_end(AJ_TIMER_BATCH, st);
   AJ_LAUNCH_CHECK();
   return AJ_OK;
 }
