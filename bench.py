@@ -63,9 +63,15 @@ I8_KERNELS = {0: "syrk_i8_kernel<{ns},{ng},0>", 1: "syrk_i8_kernel<{ns},{ng},1>"
 I8_DIGITS = {"int8x3": (3, 5), "int8x4": (4, 6), "int8x5": (5, 7)}
 
 
+CRT_KERNELS = {0: "crt::syrk_pipe8_kernel<1>", 1: "crt::syrk_pipe8_kernel<0>", 2: "crt::syrk_ring8_kernel",
+               3: "crt::syrk_sk_kernel<1>"}
+
+
 def syrk_kernel_name(precision, variant):
     if precision.startswith("int8crt"):
-        return "crt::syrk_sk_kernel<1>"
+        # variant: the tuner's "v<variant>/c<chunk>" key, or None (the default, persistent form)
+        v = int(variant.split("/")[0][1:]) if variant else 3
+        return CRT_KERNELS[v]
     if precision not in I8_DIGITS:
         return f"syrk_kernel<{precision}>"
     ns, ng = I8_DIGITS[precision]
@@ -74,19 +80,22 @@ def syrk_kernel_name(precision, variant):
     return I8_KERNELS[variant].format(ns=ns, ng=ng)
 
 
-def counter_traffic(precision, M, kernel):
+def counter_traffic(precision, M, kernel, chunk=None):
     """HBM bytes per SYRK launch from the committed counter summary for this precision and size
-    (profiles/counters_w_tilde_syrk_<precision>_M<M>.md, lines 'kernel: <name>' and
-    'traffic_bytes_per_launch: <float>'), if it was measured on the same kernel; else None."""
+    (profiles/counters_w_tilde_syrk_<precision>_M<M>.md, lines 'kernel: <name>',
+    'traffic_bytes_per_launch: <float>' and, for the residue modes, 'chunk_visibilities: <int>'), if it
+    was measured on the same kernel (and chunk size); else None."""
     p = ROOT / "profiles" / f"counters_w_tilde_syrk_{precision}_M{M}.md"
     if not p.exists():
         return None
     fields = {}
     for line in p.read_text().splitlines():
         key, sep, val = line.strip().strip("`").partition(":")
-        if sep and key in ("kernel", "traffic_bytes_per_launch"):
+        if sep and key in ("kernel", "traffic_bytes_per_launch", "chunk_visibilities"):
             fields[key] = val.strip().strip("`")
     if fields.get("kernel") != kernel or "traffic_bytes_per_launch" not in fields:
+        return None
+    if chunk is not None and fields.get("chunk_visibilities") != str(chunk):
         return None
     return float(fields["traffic_bytes_per_launch"])
 
@@ -174,17 +183,19 @@ def main():
     M, K = ds.M, ds.K
     W_mixed = dv.empty((M, M))
     W_plain = dv.empty((M, M))
-    ws_mixed = dv.workspace(_lib.query("aj_w_tilde_workspace_bytes", K, M, _lib.PRECISIONS[args.precision]))
-    ws_plain = dv.workspace(_lib.query("aj_w_tilde_workspace_bytes", K, M, 0))
     torch.cuda.synchronize()
     log("inputs resident")
     tuned = {}
     from autojax_amd import autotune
 
+    # the mixed path's SYRK kernel (and, for the residue modes, its chunk size), tuned at this M
     if not args.no_autotune:
         tuned = autotune.tune_w_tilde(ds.grid_radians_slim, args.precision)
         if tuned:
-            log("autotune: " + ", ".join(f"v{v}={t:.2f}ms" for v, t in sorted(tuned.items())))
+            log("autotune W-tilde: " + ", ".join(f"{v}={t:.2f}ms" for v, t in sorted(tuned.items(), key=str)))
+    # (the residue modes' workspace depends on the tuned chunk size)
+    ws_mixed = dv.workspace(_lib.query("aj_w_tilde_workspace_bytes", K, M, _lib.PRECISIONS[args.precision]))
+    ws_plain = dv.workspace(_lib.query("aj_w_tilde_workspace_bytes", K, M, 0))
 
     # Cholesky schedule of the batched evaluation, tuned on the 64-sample batch and, for the plain path's
     # one-sample evaluations, on its median-size sample (each path then runs with its own)
@@ -264,7 +275,9 @@ def main():
     algo_flops_step = 2.0 * M * M * K  # lower-triangle SYRK count, cos and sin terms
     achieved = algo_flops_step * args.steps / (syrk_ms / 1e3) / 1e12
     peak = PEAKS_TFLOPS[args.precision]
-    kernel = syrk_kernel_name(args.precision, min(tuned, key=tuned.get) if tuned else None)
+    best_variant = min(tuned, key=tuned.get) if tuned else None
+    kernel = syrk_kernel_name(args.precision, best_variant)
+    crt_chunk = (int(best_variant.split("/c")[1]) if best_variant else 16384) if args.precision.startswith("int8crt") else None
     # the batched evaluation's own span (HIP events around its launches); the rest of the step is W-tilde
     batch_ms = tm["batch"][0] / args.steps
     wt_ms = ms_mixed - batch_ms
@@ -317,8 +330,9 @@ def main():
                      "peak": peak, "unit": "TFLOP/s", "frac": round(achieved / peak, 5),
                      "raw_mfma_frac": round(achieved * RAW_FACTOR[args.precision] / peak, 5),
                      "avg_launch_ms": round(syrk_ms / max(syrk_launches, 1), 4), "launches_per_step": syrk_launches // args.steps,
-                     "traffic": counter_traffic(args.precision, M, kernel)},
+                     "traffic": counter_traffic(args.precision, M, kernel, crt_chunk)},
         "autotune_ms": {"w_tilde_syrk_variant": {str(v): round(t, 3) for v, t in sorted(tuned.items())},
+                        "w_tilde_selected": str(best_variant),
                         "cholesky_schedule_batch": {str(m): round(t, 3) for m, t in sorted(tuned_batch.items())},
                         "cholesky_schedule_single": {str(m): round(t, 3) for m, t in sorted(tuned_single.items())},
                         "selected": sched},

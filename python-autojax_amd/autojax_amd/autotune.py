@@ -1,13 +1,15 @@
-"""Autotuners: the W-tilde SYRK's integer kernel variant for an image size and precision
-(``tune_w_tilde``), and the batched log-evidence's Cholesky schedule for a batch's sizes
-(``tune_batch``).
+"""Autotuners: the W-tilde SYRK's kernel for an image size and precision (``tune_w_tilde``), and the
+batched log-evidence's Cholesky schedule for a batch's sizes (``tune_batch``).
 
-The variants (LDS layout of the digit slices, register or direct-to-LDS staging, 16x16x64 or
-32x32x32 MFMA, 4 or 8 waves per 64x128 tile) compute bitwise-identical sums; only their speed
-differs with the digit count and M.  ``tune_w_tilde`` times each one on a short synthetic build
-(a few 4096-visibility chunks at the caller's M, inputs resident on the device), sets the
-fastest process-wide through ``aj_w_tilde_set_variant`` and returns the timings.  Results are
-cached per (M, precision) for the life of the process.
+Digit modes (int8x3/4/5): the variants (LDS layout of the digit slices, register or direct-to-LDS
+staging, 16x16x64 or 32x32x32 MFMA, 4 or 8 waves per 64x128 tile) compute bitwise-identical sums.
+Residue modes (int8crt, int8crt11): the residue SYRK's variant (0 one workgroup per unit, 1 the same
+without the MFMA split around the barrier, 2 two stages per barrier, 3 persistent) and the visibilities
+per exact chunk (8,192 / 12,288 / 16,384), which give the same W-tilde per chunk size (the chunk size
+changes the fixed-point scale of each chunk's features).  ``tune_w_tilde`` times each choice on a short
+synthetic build at the caller's M (inputs resident on the device), sets the fastest process-wide
+(``aj_w_tilde_set_variant`` / ``aj_w_tilde_crt_configure``) and returns the timings.  Results are cached
+per (M, precision) for the life of the process.
 """
 
 from __future__ import annotations
@@ -19,17 +21,23 @@ from . import device as dv
 
 _CACHE: dict = {}
 INTEGER_MODES = ("int8x3", "int8x4", "int8x5")
+RESIDUE_MODES = ("int8crt", "int8crt11")
+CRT_VARIANTS = (0, 1, 2, 3)
+CRT_CHUNKS = (8192, 12288, 16384)
 
 
 def tune_w_tilde(grid_radians_slim, precision: str, chunks: int = 3, reps: int = 2, verbose: bool = False):
     """Time every integer SYRK variant for this grid and precision; select the fastest.
 
     Returns {variant: ms per build of ``chunks`` x 4096 visibilities} (variants that do not fit
-    on the device for this digit count are left out).  Non-integer precisions have one kernel and
-    return {} without timing anything.
+    on the device for this digit count are left out); for the residue modes {"v<variant>/c<chunk>": ms
+    per build of 49,152 visibilities}.  FP64, FP32 and split-BF16 have one kernel and return {} without
+    timing anything.
     """
     import torch
 
+    if precision in RESIDUE_MODES:
+        return _tune_crt(grid_radians_slim, precision, reps=reps, verbose=verbose)
     if precision not in INTEGER_MODES:
         return {}
     g = dv.to_device(np.asarray(grid_radians_slim, dtype=np.float64))
@@ -111,4 +119,60 @@ def tune_batch(batch, w_tilde, dirty_image, chi2_noise_norm, reps: int = 3, verb
     best = min(times, key=times.get)
     _lib.call("aj_log_evidence_configure", best)
     _BATCH_CACHE[key] = (best, times)
+    return times
+
+
+def _tune_crt(grid_radians_slim, precision: str, reps: int = 2, verbose: bool = False):
+    """Residue modes: time each (SYRK variant, chunk size) on 49,152 visibilities (a whole number of chunks
+    of every size) at the caller's M; select the fastest through ``aj_w_tilde_crt_configure``."""
+    import torch
+
+    g = dv.to_device(np.asarray(grid_radians_slim, dtype=np.float64))
+    M = g.shape[0]
+    key = (M, precision)
+    if key in _CACHE:
+        v, c = _CACHE[key][0]
+        _lib.call("aj_w_tilde_crt_configure", v, c)
+        return _CACHE[key][1]
+    K = 49152
+    rng = np.random.default_rng(1)
+    r = 1e3 * (5e2 ** rng.random(K))
+    th = 2 * np.pi * rng.random(K)
+    uv = dv.to_device(np.stack([r * np.cos(th), r * np.sin(th)], 1))
+    noise = dv.to_device(rng.uniform(0.5, 2.0, K))
+    prec = _lib.PRECISIONS[precision]
+    out = dv.empty((M, M))
+    max_chunk = 16384 if precision == "int8crt11" else 32768
+    times = {}
+    best, best_ms = None, None
+    try:
+        for c in CRT_CHUNKS:
+            if c > max_chunk:
+                continue
+            for v in CRT_VARIANTS:
+                _lib.call("aj_w_tilde_crt_configure", v, c)
+                ws = dv.workspace(_lib.query("aj_w_tilde_workspace_bytes", K, M, prec))
+
+                def build():
+                    _lib.call("aj_w_tilde_curvature_interferometer_from", noise.data_ptr(), uv.data_ptr(), g.data_ptr(),
+                              K, M, prec, out.data_ptr(), ws.data_ptr(), ws.numel(), dv.stream_ptr())
+
+                build()
+                torch.cuda.synchronize()
+                a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                a.record()
+                for _ in range(reps):
+                    build()
+                b.record()
+                torch.cuda.synchronize()
+                ms = a.elapsed_time(b) / reps
+                times[f"v{v}/c{c}"] = ms
+                if verbose:
+                    print(f"autotune {precision} M={M}: variant {v}, chunk {c}: {ms:.3f} ms", flush=True)
+                if best_ms is None or ms < best_ms:
+                    best, best_ms = (v, c), ms
+                del ws
+    finally:
+        _lib.call("aj_w_tilde_crt_configure", *(best if best is not None else (-1, 0)))
+    _CACHE[key] = (best, times)
     return times
