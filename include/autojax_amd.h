@@ -156,6 +156,23 @@ int aj_dirty_image_from(const double* data, const double* noise_map, const doubl
                         const double* grid_radians_slim, int64_t K, int64_t M, double* dirty_image, void* workspace,
                         size_t workspace_bytes, void* stream);
 
+/* W-tilde and the data-side sums in one pass over the visibilities: no single reference counterpart; it
+ * fuses numba.py:155-207 w_tilde_curvature_interferometer_from with numba.py:95-152
+ * w_tilde_data_interferometer_from and the dirty image (the sums the reference evaluates in loops of
+ * their own), so that each cos/sin of a (pixel, visibility) phase is evaluated once.  Outputs as the three
+ * separate calls: w_tilde[M*M] as aj_w_tilde_curvature_interferometer_from at this precision;
+ * dirty_image[M] as aj_dirty_image_from with n_k = noise_map_real[k] (data: K complex visibilities,
+ * (re, im) interleaved); w_tilde_data[M] as aj_w_tilde_data_interferometer_from.  dirty_image and
+ * w_tilde_data may each be null (not computed; then data, resp. visibilities_real, may be null).  The data
+ * sums are FP64 at every precision: with AJ_PREC_FP64 and the residue modes they reuse the FP64 cos/sin
+ * of the W-tilde features; the single-precision modes run the stand-alone sums after the build. */
+size_t aj_w_tilde_with_data_workspace_bytes(int64_t K, int64_t M, int precision);
+int aj_w_tilde_with_data_interferometer_from(const double* noise_map_real, const double* uv_wavelengths,
+                                             const double* grid_radians_slim, int64_t K, int64_t M, int precision,
+                                             const double* data, const double* visibilities_real, double* w_tilde,
+                                             double* dirty_image, double* w_tilde_data, void* workspace,
+                                             size_t workspace_bytes, void* stream);
+
 /* numba.py:293-376 mapping_matrix_from(pix_indexes_for_sub_slim_index i8[M,B],
  * pix_weights_for_sub_slim_index f8[M,B], pixels) -> f8[M,pixels] (dense, assignment). */
 int aj_mapping_matrix_from(const int64_t* pix_indexes_for_sub_slim_index, const double* pix_weights_for_sub_slim_index,

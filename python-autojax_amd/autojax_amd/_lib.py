@@ -48,6 +48,8 @@ SIGNATURES = {
     "aj_visibility_sum_workspace_bytes": (_S, [_I, _I]),
     "aj_w_tilde_data_interferometer_from": (_INT, [_P, _P, _P, _P, _I, _I, _P, _P, _S, _P]),
     "aj_dirty_image_from": (_INT, [_P, _P, _P, _P, _I, _I, _P, _P, _S, _P]),
+    "aj_w_tilde_with_data_workspace_bytes": (_S, [_I, _I, _INT]),
+    "aj_w_tilde_with_data_interferometer_from": (_INT, [_P, _P, _P, _I, _I, _INT, _P, _P, _P, _P, _P, _P, _S, _P]),
     "aj_mapping_matrix_from": (_INT, [_P, _P, _I, _I, _I, _P, _P]),
     "aj_data_vector_from": (_INT, [_P, _P, _I, _I, _P, _P]),
     "aj_curvature_dense_workspace_bytes": (_S, [_I, _I]),

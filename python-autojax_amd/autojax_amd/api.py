@@ -284,6 +284,47 @@ def w_tilde_data_interferometer_from(visibilities_real, noise_map_real, uv_wavel
     return _out(out, torch_in)
 
 
+def w_tilde_with_data_interferometer_from(noise_map_real, uv_wavelengths, grid_radians_slim, *, data=None,
+                                          visibilities_real=None, precision="fp64"):
+    """W-tilde with the data-side sums of the same visibilities, in one pass.
+
+    Fuses numba.py:155-207 (W-tilde) with numba.py:95-152 (``w_tilde_data``)
+    and the dirty image, whose cos/sin of each (pixel, visibility) phase the
+    W-tilde features already hold.  Returns ``(w_tilde, dirty_image,
+    w_tilde_data)``; ``dirty_image`` is computed from the complex ``data`` with
+    n_k = ``noise_map_real[k]`` (as :func:`dirty_image_from`), ``w_tilde_data``
+    from ``visibilities_real`` (as :func:`w_tilde_data_interferometer_from`);
+    either is None when its input is.  ``precision`` as in
+    :func:`w_tilde_curvature_interferometer_from`; the data sums are FP64 at
+    every precision.
+    """
+    torch_in = _any_t(noise_map_real, uv_wavelengths, grid_radians_slim, data, visibilities_real)
+    n = _dev(noise_map_real, np.float64, "noise_map_real", 1)
+    uv = _dev(uv_wavelengths, np.float64, "uv_wavelengths", 2)
+    g = _dev(grid_radians_slim, np.float64, "grid_radians_slim", 2)
+    K, M = n.shape[0], g.shape[0]
+    if uv.shape != (K, 2) or g.shape != (M, 2):
+        raise ValueError("shape mismatch: noise_map_real [K], uv_wavelengths [K,2], grid_radians_slim [M,2]")
+    d = v = None
+    if data is not None:
+        d = _view_f8(_dev(data, np.complex128, "data", 1))
+        if d.shape[0] != 2 * K:
+            raise ValueError("data length must equal noise_map_real length")
+    if visibilities_real is not None:
+        v = _dev(visibilities_real, np.float64, "visibilities_real", 1)
+        if v.shape[0] != K:
+            raise ValueError("visibilities_real length must equal noise_map_real length")
+    prec = _lib.PRECISIONS[precision]
+    W = dv.empty((M, M))
+    y = dv.empty((M,)) if d is not None else None
+    z = dv.empty((M,)) if v is not None else None
+    ws = dv.workspace(_lib.query("aj_w_tilde_with_data_workspace_bytes", K, M, prec))
+    ptr = lambda t: None if t is None else t.data_ptr()  # noqa: E731
+    _lib.call("aj_w_tilde_with_data_interferometer_from", n.data_ptr(), uv.data_ptr(), g.data_ptr(), K, M, prec,
+              ptr(d), ptr(v), W.data_ptr(), ptr(y), ptr(z), ws.data_ptr(), ws.numel(), dv.stream_ptr())
+    return (_out(W, torch_in), None if y is None else _out(y, torch_in), None if z is None else _out(z, torch_in))
+
+
 def dirty_image_from(data, noise_map, uv_wavelengths, grid_radians_slim):
     """Dirty image consistent with W-tilde (no reference counterpart; data preparation)."""
     torch_in = _any_t(data, noise_map, uv_wavelengths, grid_radians_slim)

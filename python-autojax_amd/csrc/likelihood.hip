@@ -15,6 +15,7 @@
 // (CSC, sorted by image pixel), so every gather below is a deterministic
 // ordered sum with no floating-point atomics.
 #include "aj_common.h"
+#include "data_sums.h"
 
 #include <climits>
 #include <mutex>
@@ -2127,7 +2128,8 @@ __global__ void reg_standalone_kernel(double coef2, const int64_t* __restrict__ 
 constexpr int DSPLIT = 8192;
 __global__ void __launch_bounds__(256) vis_sum_partial_kernel(const double* __restrict__ a, int64_t a_stride,
                                                               const double* __restrict__ b, int64_t b_stride,
-                                                              const double* __restrict__ wdiv, int mode,
+                                                              const double* __restrict__ wdiv, int64_t wstride,
+                                                              int mode,
                                                               const double* __restrict__ uv,
                                                               const double* __restrict__ grid, int64_t K, int64_t M,
                                                               double* __restrict__ partial) {
@@ -2150,12 +2152,12 @@ __global__ void __launch_bounds__(256) vis_sum_partial_kernel(const double* __re
       double av, bv;
       if (mode == 0) {
         // dirty image: (Re d, Im d) / sigma^2 with sigma = Re noise
-        const double n = wdiv[2 * k];
+        const double n = wdiv[wstride * k];
         av = a[a_stride * k] / (n * n);
         bv = b[b_stride * k] / (n * n);
       } else {
         // numba.py:149: w_k = (n_k^2 / v_k)^2 on the cos term only
-        const double n = wdiv[k];
+        const double n = wdiv[wstride * k];
         const double q = n * n / a[k];
         av = q * q;
         bv = 0.0;
@@ -2383,15 +2385,15 @@ extern "C" size_t aj_visibility_sum_workspace_bytes(int64_t K, int64_t M) {
   return (M > 0 && K >= 0) ? sizeof(double) * M * std::max<int64_t>(1, ceil_div(K, DSPLIT)) : 0;
 }
 
-static int vis_sum(const double* a, int64_t as, const double* b, int64_t bs, const double* wdiv, int mode,
-                   const double* uv, const double* grid, int64_t K, int64_t M, double* out, void* ws, size_t wsb,
-                   hipStream_t st) {
+int aj::vis_sum(const double* a, int64_t as, const double* b, int64_t bs, const double* wdiv, int64_t wstride,
+                int mode, const double* uv, const double* grid, int64_t K, int64_t M, double* out, void* ws,
+                size_t wsb, hipStream_t st) {
   AJ_REQUIRE(M > 0 && K >= 0 && grid && out && (K == 0 || (a && wdiv && uv)), "bad arguments");
   AJ_REQUIRE(ws && wsb >= aj_visibility_sum_workspace_bytes(K, M), "workspace too small");
   const int64_t ns = std::max<int64_t>(1, ceil_div(K, DSPLIT));
   double* part = static_cast<double*>(ws);
   hipLaunchKernelGGL(vis_sum_partial_kernel, dim3((unsigned)ceil_div(M, 256), (unsigned)ns), dim3(256), 0, st, a, as,
-                     b, bs, wdiv, mode, uv, grid, K, M, part);
+                     b, bs, wdiv, wstride, mode, uv, grid, K, M, part);
   hipLaunchKernelGGL(vis_sum_reduce_kernel, dim3((unsigned)ceil_div(M, 256)), dim3(256), 0, st, part, ns, M, out);
   AJ_LAUNCH_CHECK();
   return AJ_OK;
@@ -2400,7 +2402,7 @@ static int vis_sum(const double* a, int64_t as, const double* b, int64_t bs, con
 extern "C" int aj_dirty_image_from(const double* data, const double* noise_map, const double* uv_wavelengths,
                                    const double* grid_radians_slim, int64_t K, int64_t M, double* dirty_image,
                                    void* workspace, size_t workspace_bytes, void* stream) {
-  return vis_sum(data, 2, data ? data + 1 : nullptr, 2, noise_map, 0, uv_wavelengths, grid_radians_slim, K, M,
+  return vis_sum(data, 2, data ? data + 1 : nullptr, 2, noise_map, 2, 0, uv_wavelengths, grid_radians_slim, K, M,
                  dirty_image, workspace, workspace_bytes, as_stream(stream));
 }
 
@@ -2408,7 +2410,7 @@ extern "C" int aj_w_tilde_data_interferometer_from(const double* visibilities_re
                                                    const double* uv_wavelengths, const double* grid_radians_slim,
                                                    int64_t K, int64_t M, double* w_tilde_data, void* workspace,
                                                    size_t workspace_bytes, void* stream) {
-  return vis_sum(visibilities_real, 1, nullptr, 0, noise_map_real, 1, uv_wavelengths, grid_radians_slim, K, M,
+  return vis_sum(visibilities_real, 1, nullptr, 0, noise_map_real, 1, 1, uv_wavelengths, grid_radians_slim, K, M,
                  w_tilde_data, workspace, workspace_bytes, as_stream(stream));
 }
 

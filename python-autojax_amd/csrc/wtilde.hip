@@ -21,6 +21,7 @@
 //             accumulation per digit-order group, FP64 across chunks.  The
 //             result is the exact Gram matrix of the quantised features.
 #include "aj_common.h"
+#include "data_sums.h"
 
 #include <vector>
 
@@ -160,16 +161,23 @@ __device__ inline double reduced_revs(double gx, double gy, double u0, double u1
   return r - rint(r);
 }
 
-template <int MODE>
+// With DS (FP64 only) the kernel also accumulates the data-side sums of its 256 visibilities and 16 rows
+// from the same cos/sin values (data_sums.h); workgroup (x, y) writes block blk0 + x of the row sums.
+// Every thread runs all 16 rows: the grid covers whole chunks (KC is a multiple of 256) and Mpad is a
+// multiple of FEAT_ROWS.
+static_assert(KC % 256 == 0 && TILE % FEAT_ROWS == 0, "feature grid must cover whole workgroups");
+
+template <int MODE, bool DS>
 __global__ void __launch_bounds__(256) features_kernel(
     const double* __restrict__ noise, const double* __restrict__ uv, const double* __restrict__ grid,
     const double* __restrict__ centre, int64_t K, int64_t M, int64_t Mpad, int64_t k_begin, int nchunks,
-    const double* __restrict__ chunk_sc, void* __restrict__ Xv, int64_t ldx) {
+    const double* __restrict__ chunk_sc, void* __restrict__ Xv, int64_t ldx, DataSums dsum, int64_t blk0) {
+  static_assert(!DS || MODE == AJ_PREC_FP64, "data-side sums need FP64 features");
   using E = typename Mode<MODE>::E;
   constexpr int NS = Mode<MODE>::NS;
   E* X = reinterpret_cast<E*>(Xv);
   const int64_t kl = (int64_t)blockIdx.x * 256 + threadIdx.x;  // local visibility index in group
-  if (kl >= (int64_t)nchunks * KC) return;
+  if (!DS && kl >= (int64_t)nchunks * KC) return;
   const int c = (int)(kl / KC);
   const int kc = (int)(kl % KC);
   const int64_t kg = k_begin + kl;
@@ -185,22 +193,42 @@ __global__ void __launch_bounds__(256) features_kernel(
   const int64_t sstride = Mpad * ldx;
   double qscale = 0.0;
   if constexpr (is_i8(MODE)) qscale = sw / chunk_sc[c] * qmax_of(NS);
+  double ca = 0.0, cb = 0.0, cc = 0.0, cd = 0.0;  // data-side coefficients of this visibility
+  if constexpr (DS) {
+    if (kg < K) {
+      ca = dsum.coef[4 * kg];
+      cb = dsum.coef[4 * kg + 1];
+      cc = dsum.coef[4 * kg + 2];
+      cd = dsum.coef[4 * kg + 3];
+    }
+  }
+  RowSum16 rs0, rs1;
+  double s0 = 0.0, s1 = 0.0;
   const int64_t i0 = (int64_t)blockIdx.y * FEAT_ROWS;
   for (int r = 0; r < FEAT_ROWS; ++r) {
     const int64_t i = i0 + r;
-    if (i >= Mpad) break;
+    if (!DS && i >= Mpad) break;
     double cv = 0.0, sv = 0.0;
     float cf = 0.f, sf = 0.f;
     const bool live = (i < M) && (kg < K);
     if constexpr (MODE == AJ_PREC_FP64) {
+      double d0 = 0.0, d1 = 0.0;
       if (live) {
         double rv = reduced_revs(grid[2 * i + 1] - cx, grid[2 * i] - cy, u0, u1);
         sincospi(2.0 * rv, &sv, &cv);
+        if constexpr (DS) {
+          d0 = fma(ca, cv, cb * sv);
+          d1 = fma(cc, cv, cd * sv);
+        }
         cv *= sw;
         sv *= sw;
       }
       X[i * ldx + col_c] = cv;
       X[i * ldx + col_s] = sv;
+      if constexpr (DS) {
+        rs0.push(r, d0, threadIdx.x & 63, s0);
+        rs1.push(r, d1, threadIdx.x & 63, s1);
+      }
     } else if constexpr (is_i8(MODE)) {
       // FP64 cos/sin, quantised to NS balanced 8-bit digits of a fixed-point value
       if (live) {
@@ -237,6 +265,7 @@ __global__ void __launch_bounds__(256) features_kernel(
       }
     }
   }
+  if constexpr (DS) block_row_sums_out(s0, s1, dsum, blk0 + blockIdx.x, i0);
 }
 
 // ---------------------------------------------------------------------------
@@ -1376,9 +1405,14 @@ inline Plan make_plan(int64_t K, int64_t M, int mode) {
   return p;
 }
 
+// With ds (FP64 only) the feature launches also accumulate the data-side sums into ds->part.
 template <int MODE>
 int run(const double* noise, const double* uv, const double* grid, int64_t K, int64_t M, double* W,
-        const Plan& p, Arena& ar, hipStream_t st) {
+        const Plan& p, Arena& ar, hipStream_t st, const DataSums* ds = nullptr) {
+  if (ds && MODE != AJ_PREC_FP64) {
+    set_error("data-side sums fuse with the FP64 features only");
+    return AJ_EINVAL;
+  }
   void* X = ar.take<char>(p.feat_bytes);
   double* centre = ar.take<double>(2);
   double* sc = ar.take<double>(p.group);
@@ -1407,6 +1441,10 @@ int run(const double* noise, const double* uv, const double* grid, int64_t K, in
   }
   hipLaunchKernelGGL(grid_mean_kernel, dim3(1), dim3(256), 0, st, grid, M, centre);
   AJ_LAUNCH_CHECK();
+  if (ds) {
+    const int rc = data_sums_coef(*ds, noise, uv, centre, K, st);
+    if (rc != AJ_OK) return rc;
+  }
   const int64_t nch = std::max<int64_t>(1, ceil_div(K, KC));
   const int64_t nt = p.Mpad / TILE;
   const int64_t ntiles = nt * (nt + 1) / 2;
@@ -1421,8 +1459,17 @@ int run(const double* noise, const double* uv, const double* grid, int64_t K, in
     }
     dim3 fg((unsigned)ceil_div((int64_t)ng * KC, 256), (unsigned)ceil_div(p.Mpad, FEAT_ROWS));
     timer_begin(AJ_TIMER_WT_FEATURES, st);
-    hipLaunchKernelGGL(features_kernel<MODE>, fg, dim3(256), 0, st, noise, uv, grid, centre, K, M, p.Mpad, k_begin,
-                       ng, sc, X, p.ldx);
+    if constexpr (MODE == AJ_PREC_FP64) {
+      if (ds)
+        hipLaunchKernelGGL((features_kernel<MODE, true>), fg, dim3(256), 0, st, noise, uv, grid, centre, K, M, p.Mpad,
+                           k_begin, ng, sc, X, p.ldx, *ds, k_begin / 256);
+      else
+        hipLaunchKernelGGL((features_kernel<MODE, false>), fg, dim3(256), 0, st, noise, uv, grid, centre, K, M, p.Mpad,
+                           k_begin, ng, sc, X, p.ldx, DataSums{}, (int64_t)0);
+    } else {
+      hipLaunchKernelGGL((features_kernel<MODE, false>), fg, dim3(256), 0, st, noise, uv, grid, centre, K, M, p.Mpad,
+                         k_begin, ng, sc, X, p.ldx, DataSums{}, (int64_t)0);
+    }
     timer_end(AJ_TIMER_WT_FEATURES, st);
     AJ_LAUNCH_CHECK();
     timer_begin(AJ_TIMER_WT_SYRK, st);
@@ -1478,13 +1525,14 @@ int run(const double* noise, const double* uv, const double* grid, int64_t K, in
 // zeroed W; centring and the mirror are shared with the modes above.
 size_t crt_workspace_bytes(int64_t M, int64_t K, int nm);
 int crt_accumulate(const double* noise, const double* uv, const double* grid, const double* centre, int64_t K,
-                   int64_t M, int nm, double* W, void* workspace, size_t workspace_bytes, hipStream_t st);
+                   int64_t M, int nm, double* W, void* workspace, size_t workspace_bytes, hipStream_t st,
+                   const DataSums* ds);
 
 inline int crt_moduli(int precision) { return precision == AJ_PREC_INT8CRT11 ? 11 : 12; }
 inline size_t crt_mode_bytes(int64_t M, int64_t K, int nm) { return 512 + crt_workspace_bytes(M, K, nm); }
 
 int run_crt_mode(const double* noise, const double* uv, const double* grid, int64_t K, int64_t M, int nm, double* W,
-                 void* workspace, size_t workspace_bytes, hipStream_t st) {
+                 void* workspace, size_t workspace_bytes, hipStream_t st, const DataSums* ds = nullptr) {
   if (workspace_bytes < crt_mode_bytes(M, K, nm)) {
     set_error("workspace too small: need %zu bytes, got %zu", crt_mode_bytes(M, K, nm), workspace_bytes);
     return AJ_ESPACE;
@@ -1492,9 +1540,13 @@ int run_crt_mode(const double* noise, const double* uv, const double* grid, int6
   double* centre = static_cast<double*>(workspace);
   hipLaunchKernelGGL(grid_mean_kernel, dim3(1), dim3(256), 0, st, grid, M, centre);
   AJ_LAUNCH_CHECK();
+  if (ds) {
+    const int rc = data_sums_coef(*ds, noise, uv, centre, K, st);
+    if (rc != AJ_OK) return rc;
+  }
   AJ_CHECK_HIP(hipMemsetAsync(W, 0, sizeof(double) * M * M, st));
   const int rc = crt_accumulate(noise, uv, grid, centre, K, M, nm, W, static_cast<char*>(workspace) + 512,
-                                workspace_bytes - 512, st);
+                                workspace_bytes - 512, st, ds);
   if (rc != AJ_OK) return rc;
   dim3 mg((unsigned)ceil_div(M, 32), (unsigned)ceil_div(M, 32));
   hipLaunchKernelGGL(mirror_kernel, mg, dim3(256), 0, st, W, M);
@@ -1502,7 +1554,115 @@ int run_crt_mode(const double* noise, const double* uv, const double* grid, int6
   return AJ_OK;
 }
 
+// Coefficients of the data-side sums per visibility, rotated by the centre's phase phi_k (data_sums.h):
+//   dirty image   a = Re d / s^2, b = Im d / s^2:  alpha = a cos + b sin,  beta = b cos - a sin
+//   w_tilde_data  w = (s^2 / v)^2 (numba.py:149):   gamma = w cos,         delta = -w sin
+// with s = noise_k and cos, sin of 2 pi phi_k.  Absent inputs give zero coefficients.
+__global__ void data_coef_kernel(const double* __restrict__ data, const double* __restrict__ vis,
+                                 const double* __restrict__ noise, const double* __restrict__ uv,
+                                 const double* __restrict__ centre, int64_t K, double* __restrict__ coef) {
+  const double cy = centre[0], cx = centre[1];
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < K; k += (int64_t)gridDim.x * blockDim.x) {
+    double sp, cp;
+    sincospi(2.0 * reduced_revs(cx, cy, uv[2 * k], uv[2 * k + 1]), &sp, &cp);
+    const double n = noise[k];
+    double al = 0.0, be = 0.0, ga = 0.0, de = 0.0;
+    if (data) {
+      const double a = data[2 * k] / (n * n), b = data[2 * k + 1] / (n * n);
+      al = fma(a, cp, b * sp);
+      be = fma(b, cp, -(a * sp));
+    }
+    if (vis) {
+      const double q = n * n / vis[k];
+      const double w = q * q;
+      ga = w * cp;
+      de = -(w * sp);
+    }
+    coef[4 * k] = al;
+    coef[4 * k + 1] = be;
+    coef[4 * k + 2] = ga;
+    coef[4 * k + 3] = de;
+  }
+}
+
+// out[o][s][i] = sum over the blocks b of segment s, in order, of in[o][b][i] (i < M); blockIdx.z = o.
+__global__ void seg_sum_kernel(const double* __restrict__ in, int64_t nb, int64_t ld, int64_t seg, int64_t nseg,
+                               int64_t M, double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= M) return;
+  const int64_t s = blockIdx.y, o = blockIdx.z;
+  const int64_t b0 = s * seg, b1 = std::min<int64_t>(nb, b0 + seg);
+  const double* src = in + o * nb * ld + i;
+  double acc = 0.0;
+  for (int64_t b = b0; b < b1; ++b) acc += src[b * ld];
+  out[(o * nseg + s) * M + i] = acc;
+}
+
+// Final pass: the segment sums, in order, into the requested outputs (null ones skipped).
+__global__ void seg_final_kernel(const double* __restrict__ part2, int64_t nseg, int64_t M, double* __restrict__ y,
+                                 double* __restrict__ z) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= M) return;
+  double a = 0.0, b = 0.0;
+  for (int64_t s = 0; s < nseg; ++s) {
+    a += part2[s * M + i];
+    b += part2[(nseg + s) * M + i];
+  }
+  if (y) y[i] = a;
+  if (z) z[i] = b;
+}
+
+// Workgroup blocks of the visibility axis whose row sums a fused build writes: FP64 features cover
+// whole KC-chunks in blocks of 256 visibilities; the residue features cover the visibilities in blocks
+// of 512.
+inline int64_t data_sum_blocks(int64_t K, int precision) {
+  if (precision == AJ_PREC_FP64) return std::max<int64_t>(1, ceil_div(K, KC)) * (KC / 256);
+  return ceil_div(K, 512);
+}
+inline bool fuses_data_sums(int precision) { return precision == AJ_PREC_FP64 || precision >= AJ_PREC_INT8CRT; }
+constexpr int64_t DS_SEGMENTS = 64;  // segments of the first reduction pass
+
+inline DataSums data_sums_layout(int64_t K, int64_t M, int precision, Arena& ar) {
+  DataSums d{};
+  d.nblk = data_sum_blocks(K, precision);
+  d.ld = ceil_div(M, 256) * 256;  // at least every feature kernel's padded row count
+  d.nseg = std::max<int64_t>(1, std::min<int64_t>(d.nblk, DS_SEGMENTS));
+  d.coef = ar.take<double>(4 * (size_t)std::max<int64_t>(K, 1));
+  d.part = ar.take<double>(2 * (size_t)std::max<int64_t>(d.nblk, 1) * d.ld);
+  d.part2 = ar.take<double>(2 * (size_t)d.nseg * M);
+  return d;
+}
+
+inline size_t data_sums_bytes(int64_t K, int64_t M, int precision) {
+  const int64_t nblk = data_sum_blocks(K, precision), ld = ceil_div(M, 256) * 256;
+  const int64_t nseg = std::max<int64_t>(1, std::min<int64_t>(nblk, DS_SEGMENTS));
+  return align_up(sizeof(double) * 4 * (size_t)std::max<int64_t>(K, 1), 256) +
+         align_up(sizeof(double) * 2 * (size_t)std::max<int64_t>(nblk, 1) * ld, 256) +
+         align_up(sizeof(double) * 2 * (size_t)nseg * M, 256) + 256;
+}
+
 }  // namespace wt
+
+int data_sums_coef(const DataSums& ds, const double* noise, const double* uv, const double* centre, int64_t K,
+                   hipStream_t st) {
+  if (K == 0) return AJ_OK;
+  hipLaunchKernelGGL(wt::data_coef_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(K, 256), 4096)), dim3(256), 0,
+                     st, ds.data, ds.vis, noise, uv, centre, K, ds.coef);
+  AJ_LAUNCH_CHECK();
+  return AJ_OK;
+}
+
+int data_sums_finish(const DataSums& ds, int64_t M, double* dirty_image, double* w_tilde_data, hipStream_t st) {
+  const int64_t seg = ceil_div(ds.nblk, ds.nseg);
+  hipLaunchKernelGGL(wt::seg_sum_kernel, dim3((unsigned)ceil_div(M, 256), (unsigned)ds.nseg, 2), dim3(256), 0, st,
+                     ds.part, ds.nblk, ds.ld, seg, ds.nseg, M, ds.part2);
+  AJ_LAUNCH_CHECK();
+  hipLaunchKernelGGL(wt::seg_final_kernel, dim3((unsigned)ceil_div(M, 256)), dim3(256), 0, st, ds.part2, ds.nseg, M,
+                     dirty_image, w_tilde_data);
+  AJ_LAUNCH_CHECK();
+  return AJ_OK;
+}
+
 }  // namespace aj
 
 using namespace aj;
@@ -1536,6 +1696,62 @@ extern "C" int aj_w_tilde_curvature_interferometer_from(const double* noise_map_
     case AJ_PREC_INT8X4: return wt::run<AJ_PREC_INT8X4>(noise_map_real, uv_wavelengths, grid_radians_slim, K, M, w_tilde, p, ar, st);
     default: return wt::run<AJ_PREC_INT8X5>(noise_map_real, uv_wavelengths, grid_radians_slim, K, M, w_tilde, p, ar, st);
   }
+}
+
+extern "C" size_t aj_w_tilde_with_data_workspace_bytes(int64_t K, int64_t M, int precision) {
+  const size_t w = aj_w_tilde_workspace_bytes(K, M, precision);
+  if (!w) return 0;
+  const size_t d = wt::fuses_data_sums(precision) ? wt::data_sums_bytes(K, M, precision)
+                                                   : aj_visibility_sum_workspace_bytes(K, M);
+  return align_up(w, 256) + d;
+}
+
+extern "C" int aj_w_tilde_with_data_interferometer_from(const double* noise_map_real, const double* uv_wavelengths,
+                                                        const double* grid_radians_slim, int64_t K, int64_t M,
+                                                        int precision, const double* data,
+                                                        const double* visibilities_real, double* w_tilde,
+                                                        double* dirty_image, double* w_tilde_data, void* workspace,
+                                                        size_t workspace_bytes, void* stream) {
+  AJ_REQUIRE(M > 0 && K >= 0, "bad sizes M=%lld K=%lld", (long long)M, (long long)K);
+  AJ_REQUIRE(precision >= AJ_PREC_FP64 && precision <= AJ_PREC_INT8CRT11, "bad precision %d", precision);
+  AJ_REQUIRE(w_tilde && grid_radians_slim && (K == 0 || (noise_map_real && uv_wavelengths)), "null pointer");
+  AJ_REQUIRE(K == 0 || !dirty_image || data, "dirty_image requested without data");
+  AJ_REQUIRE(K == 0 || !w_tilde_data || visibilities_real, "w_tilde_data requested without visibilities_real");
+  const size_t need = aj_w_tilde_with_data_workspace_bytes(K, M, precision);
+  AJ_REQUIRE(workspace && workspace_bytes >= need, "workspace too small: need %zu bytes, got %zu", need,
+             workspace_bytes);
+  const size_t wb = align_up(aj_w_tilde_workspace_bytes(K, M, precision), 256);
+  void* dws = static_cast<char*>(workspace) + wb;
+  const size_t dwb = workspace_bytes - wb;
+  hipStream_t st = as_stream(stream);
+  if (!wt::fuses_data_sums(precision)) {
+    // single-precision features: the data-side sums evaluate their own FP64 phases
+    int rc = aj_w_tilde_curvature_interferometer_from(noise_map_real, uv_wavelengths, grid_radians_slim, K, M,
+                                                      precision, w_tilde, workspace, wb, stream);
+    if (rc == AJ_OK && dirty_image)
+      rc = vis_sum(data, 2, data ? data + 1 : nullptr, 2, noise_map_real, 1, 0, uv_wavelengths, grid_radians_slim,
+                   K, M, dirty_image, dws, dwb, st);
+    if (rc == AJ_OK && w_tilde_data)
+      rc = vis_sum(visibilities_real, 1, nullptr, 0, noise_map_real, 1, 1, uv_wavelengths, grid_radians_slim, K, M,
+                   w_tilde_data, dws, dwb, st);
+    return rc;
+  }
+  Arena da(dws, dwb);
+  DataSums ds = wt::data_sums_layout(K, M, precision, da);
+  AJ_REQUIRE(da.ok(), "workspace too small for the data-side sums");
+  ds.data = dirty_image ? data : nullptr;
+  ds.vis = w_tilde_data ? visibilities_real : nullptr;
+  int rc;
+  if (precision >= AJ_PREC_INT8CRT) {
+    rc = wt::run_crt_mode(noise_map_real, uv_wavelengths, grid_radians_slim, K, M, wt::crt_moduli(precision), w_tilde,
+                          workspace, wb, st, &ds);
+  } else {
+    const wt::Plan p = wt::make_plan(K, M, AJ_PREC_FP64);
+    Arena ar(workspace, wb);
+    rc = wt::run<AJ_PREC_FP64>(noise_map_real, uv_wavelengths, grid_radians_slim, K, M, w_tilde, p, ar, st, &ds);
+  }
+  if (rc != AJ_OK) return rc;
+  return data_sums_finish(ds, M, dirty_image, w_tilde_data, st);
 }
 
 extern "C" int aj_w_tilde_set_variant(int variant) {

@@ -25,6 +25,7 @@
 // pieces' residues), combine (CRT + FP64 accumulation into the lower triangle of W).
 
 #include "aj_common.h"
+#include "data_sums.h"
 
 #include <vector>
 
@@ -204,13 +205,17 @@ __device__ inline void store_residues4(int8_t* __restrict__ X, int64_t plane, ui
 // visibilities (4 bytes per row and plane) x the 16 rows of row-block blockIdx.y.  Padded rows and
 // visibilities past K hold zeros.
 // ---------------------------------------------------------------------------
-template <int NM>
+// With DS the kernel also accumulates the data-side sums (data_sums.h) of its 512 visibilities and 16
+// rows from the FP64 cos/sin values before they are quantised; workgroup (x, y) writes block
+// blk0 + x of the row sums.
+template <int NM, bool DS>
 __global__ void __launch_bounds__(256) features_kernel(const double* __restrict__ noise,
                                                        const double* __restrict__ uv,
                                                        const double* __restrict__ grid,
                                                        const double* __restrict__ centre, int64_t K, int64_t M,
                                                        int64_t Mpad, int64_t k_begin, int kc,
-                                                       const double* __restrict__ sc, int8_t* __restrict__ X) {
+                                                       const double* __restrict__ sc, int8_t* __restrict__ X,
+                                                       DataSums dsum, int64_t blk0) {
   const int nkb = kc / 32;
   const int k0 = (blockIdx.x * 256 + threadIdx.x) * 2;  // first of 2 visibilities (chunk-local)
   const int rb = blockIdx.y;
@@ -224,6 +229,19 @@ __global__ void __launch_bounds__(256) features_kernel(const double* __restrict_
     u1[v] = ok ? uv[2 * kg + 1] : 0.0;
     qs[v] = ok ? inv_sc / fabs(noise[kg]) : 0.0;  // zero scale: features of absent visibilities are 0
   }
+  double cf[2][4] = {};  // data-side coefficients of the two visibilities (zero when absent)
+  if constexpr (DS) {
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+      const int64_t kg = k_begin + k0 + v;
+      if (kg < K) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) cf[v][j] = dsum.coef[4 * kg + j];
+      }
+    }
+  }
+  RowSum16 rs0, rs1;
+  double s0 = 0.0, s1 = 0.0;
   const double cy = centre[0], cx = centre[1];
   const int64_t plane = Mpad * 2 * (int64_t)kc;
   // byte offset within a plane (a plane holds fewer than 2^32 bytes: checked on the host)
@@ -232,6 +250,7 @@ __global__ void __launch_bounds__(256) features_kernel(const double* __restrict_
   for (int r = 0; r < 16; ++r, off += 64) {
     const int64_t i = (int64_t)rb * 16 + r;
     double qc[2], qn[2];
+    double d0 = 0.0, d1 = 0.0;
     if (i < M) {
       const double gx = grid[2 * i + 1] - cx, gy = grid[2 * i] - cy;
 #pragma unroll
@@ -242,10 +261,18 @@ __global__ void __launch_bounds__(256) features_kernel(const double* __restrict_
         sincospi_small(2.0 * rv, s, co);
         qc[v] = rint(co * qs[v]);
         qn[v] = rint(s * qs[v]);
+        if constexpr (DS) {
+          d0 = fma(cf[v][0], co, fma(cf[v][1], s, d0));
+          d1 = fma(cf[v][2], co, fma(cf[v][3], s, d1));
+        }
       }
     } else {
 #pragma unroll
       for (int v = 0; v < 2; ++v) qc[v] = qn[v] = 0.0;
+    }
+    if constexpr (DS) {
+      rs0.push(r, d0, threadIdx.x & 63, s0);
+      rs1.push(r, d1, threadIdx.x & 63, s1);
     }
     double qcm[2], qnm[2];
 #pragma unroll
@@ -255,6 +282,7 @@ __global__ void __launch_bounds__(256) features_kernel(const double* __restrict_
     }
     store_residues4<0, NM>(X, plane, off, qc, qn, qcm, qnm);
   }
+  if constexpr (DS) block_row_sums_out(s0, s1, dsum, blk0 + blockIdx.x, (int64_t)rb * 16);
 }
 
 // Residues of a wave's exact int32 Gram block (|acc| < 2^30), balanced, 4 per word.  Layout of a
@@ -806,7 +834,8 @@ size_t crt_workspace_bytes(int64_t M, int64_t K, int nm) { return crt::make_plan
 
 // Accumulate the lower triangle of W (zeroed by the caller) over all visibilities.
 int crt_accumulate(const double* noise, const double* uv, const double* grid, const double* centre, int64_t K,
-                   int64_t M, int nm, double* W, void* workspace, size_t workspace_bytes, hipStream_t st) {
+                   int64_t M, int nm, double* W, void* workspace, size_t workspace_bytes, hipStream_t st,
+                   const DataSums* ds) {
   using namespace crt;
   if (nm != 11 && nm != 12) {
     set_error("int8crt: %d moduli (11 or 12 supported)", nm);
@@ -889,12 +918,24 @@ int crt_accumulate(const double* noise, const double* uv, const double* grid, co
     const int nst = std::min<int>(nkb, (int)(2 * ceil_div(kv, 64)));
     const dim3 fgc((unsigned)std::min<int64_t>(fg.x, ceil_div((int64_t)nst * 32, 512)), fg.y);
     timer_begin(AJ_TIMER_WT_FEATURES, st);
-    if (nm == 12)
-      hipLaunchKernelGGL(features_kernel<12>, fgc, dim3(256), 0, st, noise, uv, grid, centre, K, M, p.Mpad, ch * kc,
-                         kc, sc + ch, X);
-    else
-      hipLaunchKernelGGL(features_kernel<11>, fgc, dim3(256), 0, st, noise, uv, grid, centre, K, M, p.Mpad, ch * kc,
-                         kc, sc + ch, X);
+    // (with data-side sums: workgroup x of chunk ch covers visibilities ch kc + 512 x ..; kc is a multiple
+    // of 512, so its block index is ch kc / 512 + x)
+    const int64_t blk0 = ch * (kc / 512);
+    if (nm == 12) {
+      if (ds)
+        hipLaunchKernelGGL((features_kernel<12, true>), fgc, dim3(256), 0, st, noise, uv, grid, centre, K, M, p.Mpad,
+                           ch * kc, kc, sc + ch, X, *ds, blk0);
+      else
+        hipLaunchKernelGGL((features_kernel<12, false>), fgc, dim3(256), 0, st, noise, uv, grid, centre, K, M, p.Mpad,
+                           ch * kc, kc, sc + ch, X, DataSums{}, blk0);
+    } else {
+      if (ds)
+        hipLaunchKernelGGL((features_kernel<11, true>), fgc, dim3(256), 0, st, noise, uv, grid, centre, K, M, p.Mpad,
+                           ch * kc, kc, sc + ch, X, *ds, blk0);
+      else
+        hipLaunchKernelGGL((features_kernel<11, false>), fgc, dim3(256), 0, st, noise, uv, grid, centre, K, M, p.Mpad,
+                           ch * kc, kc, sc + ch, X, DataSums{}, blk0);
+    }
     timer_end(AJ_TIMER_WT_FEATURES, st);
     AJ_LAUNCH_CHECK();
     timer_begin(AJ_TIMER_WT_SYRK, st);
