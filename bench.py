@@ -265,6 +265,35 @@ def main():
     ms_plain, out_plain, tp = timed(plain_step, args.plain_steps, 1, "plain")
     plain_ev = out_plain.cpu().numpy()
     dlogl = float(np.abs(mixed_ev - plain_ev).max())
+
+    # the data-side sums (dirty image and w_tilde_data) fused into the mixed path's W-tilde build, against
+    # the build alone and the two stand-alone sums (HIP events on the library's stream)
+    from autojax_amd import api
+
+    def event_ms(fn, reps=2):
+        fn()
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps
+
+    data_d, nmap_d = dv.to_device(ds.data), dv.to_device(ds.noise_map)
+    vis_d = dv.to_device(np.ascontiguousarray(ds.data.real) + 3.0)
+    sums = {
+        "w_tilde_with_sums": event_ms(lambda: api.w_tilde_with_data_interferometer_from(
+            noise, uv, grid, data=data_d, visibilities_real=vis_d, precision=args.precision)),
+        "w_tilde_alone": event_ms(lambda: E.w_tilde(noise, uv, grid, args.precision, out=W_mixed, workspace=ws_mixed)),
+        "dirty_image_standalone": event_ms(lambda: api.dirty_image_from(data_d, nmap_d, uv, grid)),
+        "w_tilde_data_standalone": event_ms(lambda: api.w_tilde_data_interferometer_from(vis_d, noise, uv, grid)),
+    }
+    sums = {k: round(v, 3) for k, v in sums.items()}
+    sums["fused_marginal"] = round(sums["w_tilde_with_sums"] - sums["w_tilde_alone"], 3)
+    log("data-side sums (ms): " + ", ".join(f"{k}={v}" for k, v in sums.items()))
     info_bad = max(info_bad, lat_bad)
     n = len(ds.samples)
     value = n / (ms_mixed / 1e3)
@@ -318,6 +347,7 @@ def main():
                          "w_tilde_total": round(wt_ms, 3),
                          "batch_total": round(batch_ms, 3)},
         "evals_per_s_excluding_w_tilde": round(n / (batch_ms / 1e3), 3),
+        "data_side_sums_ms": sums,
         # blocked Cholesky of every A = F + H and every H of the batch: S^3 / 3 flops each
         "factorisation": {"flops": chol_flops, "tflops": round(chol_flops / tm["chol"][0] * args.steps / 1e9, 3),
                           "frac_fp64_peak": round(chol_flops / tm["chol"][0] * args.steps / 1e9 / 78.6, 4)},

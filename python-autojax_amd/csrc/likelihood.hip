@@ -164,7 +164,8 @@ __global__ void __launch_bounds__(PREP_THREADS) prep_kernel(const SampleDesc* __
                                                             const double* __restrict__ wts, int64_t M, int64_t B,
                                                             const double* __restrict__ dirty,
                                                             int* __restrict__ ptrs, int* __restrict__ ent_key,
-                                                            double* __restrict__ ent_t, double* __restrict__ D) {
+                                                            double* __restrict__ ent_t, double* __restrict__ D,
+                                                            int* __restrict__ sl_key, double* __restrict__ sl_t) {
   extern __shared__ int cnt[];  // S + 1 counters, then cursors
   __shared__ int part[PREP_THREADS];
   const SampleDesc d = sd[blockIdx.x];
@@ -248,6 +249,29 @@ __global__ void __launch_bounds__(PREP_THREADS) prep_kernel(const SampleDesc* __
     double v = 0.0;
     for (int e = p[s]; e < p[s + 1]; ++e) v = fma(t[e], dirty[k[e]], v);
     D[d.s_off + s] = v;
+  }
+  // The buckets again, ordered by slice for the Omega kernel: slice q holds columns 64 q .. 64 q + 63 (a
+  // lane each) and occupies the slice's range of the CSC, reordered as entry 0 of every column that has one,
+  // columns in order, then entry 1, ...: a wave reading entry j of its 64 columns reads consecutive
+  // addresses.
+  int* sk = sl_key + d.csc_ent;
+  double* st = sl_t + d.csc_ent;
+  for (int64_t q = wave; q * 64 < S; q += PREP_THREADS / 64) {
+    const int64_t s = q * 64 + lane;
+    const int lo = s < S ? p[s] : 0;
+    const int len = s < S ? p[s + 1] - lo : 0;
+    const uint64_t below = (uint64_t(1) << lane) - 1;
+    int pos = p[q * 64];
+    for (int j = 0;; ++j) {
+      const uint64_t mask = __ballot(len > j);
+      if (mask == 0) break;
+      if (len > j) {
+        const int at = pos + __popcll(mask & below);
+        sk[at] = k[lo + j];
+        st[at] = t[lo + j];
+      }
+      pos += __popcll(mask);
+    }
   }
 }
 
@@ -353,6 +377,120 @@ __global__ void __launch_bounds__(OMEGA_THREADS) omega_kernel(const WSrc W, int6
 #pragma unroll
       for (int r = 0; r < 8; ++r)
         if (r < rows) om[(m0 + r) * d.S + s] = acc[r];
+    }
+  }
+}
+
+// Omega = W T over the slice-ordered buckets: R (1, 2, 4 or 8) rows of W in LDS, interleaved (wl[m R + r]
+// = W[m0 + r, m]) so that the R values one bucket entry needs are adjacent (one 8- or 16-byte read, or R / 2
+// 16-byte reads).  Each wave takes whole slices of 64 columns (slice g: sample slices[g].x, first column
+// slices[g].y), a lane per column; entry j of its 64 columns sits at consecutive addresses, the loads of
+// OMEGA_SLICE_UNROLL entries are issued ahead of their sums, and each column's sum runs over its bucket in
+// entry order (the bits of omega_kernel).
+constexpr int OMEGA_SLICE_UNROLL = 16;
+
+template <int R>
+__device__ inline void lds_rows(const double* wl, int m, double (&w)[R]) {
+  if constexpr (R == 1) {
+    w[0] = wl[m];
+  } else {
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    const d2* src = reinterpret_cast<const d2*>(wl + (int64_t)m * R);
+#pragma unroll
+    for (int h = 0; h < R / 2; ++h) {
+      const d2 v = src[h];
+      w[2 * h] = v.x;
+      w[2 * h + 1] = v.y;
+    }
+  }
+}
+
+// A wave's slice: its sample's descriptor fields, the lane's column and bucket length, the slice's first
+// entry.
+struct OmegaSlice {
+  int64_t S, csc_ent, omega_off;
+  int64_t s;  // the lane's column
+  int len;    // its bucket's length (0 past the sample's last column)
+  int pos;    // the slice's first entry in its sample's buckets
+};
+
+__device__ inline OmegaSlice omega_slice(int g, const int2* __restrict__ slices, const SampleDesc* __restrict__ sd,
+                                         const int* __restrict__ ptrs, int lane) {
+  const int2 sq = slices[g];
+  const SampleDesc d = sd[sq.x];
+  const int* p = ptrs + d.csc_ptr;
+  OmegaSlice o;
+  o.S = d.S;
+  o.csc_ent = d.csc_ent;
+  o.omega_off = d.omega_off;
+  o.s = sq.y + lane;
+  o.len = o.s < d.S ? p[o.s + 1] - p[o.s] : 0;
+  o.pos = p[sq.y];
+  return o;
+}
+
+template <int R, int U>
+__global__ void __launch_bounds__(OMEGA_THREADS) omega_slices_kernel(const WSrc W, int64_t M, int nsl,
+                                                                      const int2* __restrict__ slices,
+                                                                      const SampleDesc* __restrict__ sd,
+                                                                      const int* __restrict__ ptrs,
+                                                                      const int* __restrict__ sl_key,
+                                                                      const double* __restrict__ sl_t,
+                                                                      double* __restrict__ omega) {
+  extern __shared__ double wl[];  // M x R, interleaved
+  const int64_t m0 = (int64_t)blockIdx.x * R;
+  const int rows = (int)std::min<int64_t>(R, M - m0);
+  const bool dense = W.C == nullptr && W.Pre == nullptr;
+  for (int r = 0; r < R; ++r)
+    for (int64_t m = threadIdx.x; m < M; m += OMEGA_THREADS)
+      wl[m * R + r] = r >= rows ? 0.0 : dense ? W.W[(m0 + r) * M + m] : wsrc_at(W, M, m0 + r, m);
+  __syncthreads();
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const uint64_t below = (uint64_t(1) << lane) - 1;
+  constexpr int WAVES = OMEGA_THREADS / 64;
+  // the next slice's descriptor is loaded while the current one is summed
+  OmegaSlice nx{};
+  if (wave < nsl) nx = omega_slice(wave, slices, sd, ptrs, lane);
+  for (int g = wave; g < nsl; g += WAVES) {
+    const OmegaSlice cu = nx;
+    if (g + WAVES < nsl) nx = omega_slice(g + WAVES, slices, sd, ptrs, lane);
+    const int* key = sl_key + cu.csc_ent;
+    const double* tv = sl_t + cu.csc_ent;
+    const int len = cu.len;
+    int pos = cu.pos;
+    double acc[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] = 0.0;
+    for (int j = 0;; j += U) {
+      uint64_t mask[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) mask[u] = __ballot(len > j + u);
+      if (mask[0] == 0) break;
+      int m[U];
+      double t[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const bool act = len > j + u;
+        const int at = pos + __popcll(mask[u] & below);
+        pos += __popcll(mask[u]);
+        m[u] = act ? key[at] : 0;
+        t[u] = act ? tv[at] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (len > j + u) {
+          double w[R];
+          lds_rows<R>(wl, m[u], w);
+#pragma unroll
+          for (int r = 0; r < R; ++r) acc[r] = fma(t[u], w[r], acc[r]);
+        }
+      }
+    }
+    if (cu.s < cu.S) {
+      double* om = omega + cu.omega_off + m0 * cu.S + cu.s;
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        if (r < rows) om[r * cu.S] = acc[r];
     }
   }
 }
@@ -1539,9 +1677,9 @@ __global__ void noise_norm_kernel(const double* __restrict__ noise, int64_t K2, 
 // Host-side planning of the batch workspace.
 // ---------------------------------------------------------------------------
 struct BatchPlan {
-  int64_t n, M, B, sumS, sumS2, omega_elems, csc_ptr_elems, csc_ent_elems, maxsteps, n_entries;
+  int64_t n, M, B, sumS, sumS2, omega_elems, csc_ptr_elems, csc_ent_elems, maxsteps, n_entries, nsl;
   size_t off_desc, off_ptr, off_cur, off_key, off_t, off_omega, off_mats, off_D, off_logdet, off_info,
-      off_steps, off_tab, off_ldp, off_failp, off_stage, bytes;
+      off_steps, off_tab, off_ldp, off_failp, off_stage, off_skey, off_st, off_slices, bytes;
 };
 
 inline BatchPlan plan_batch(int64_t M, int64_t B, int64_t n, const int64_t* S_host) {
@@ -1554,6 +1692,7 @@ inline BatchPlan plan_batch(int64_t M, int64_t B, int64_t n, const int64_t* S_ho
     p.sumS2 += S_host[b] * S_host[b];
     p.maxsteps = std::max<int64_t>(p.maxsteps, ceil_div(S_host[b], (int64_t)NB));
     p.n_entries += 2 * ceil_div(S_host[b], (int64_t)NB);
+    p.nsl += ceil_div(S_host[b], (int64_t)64);
   }
   p.omega_elems = M * p.sumS;
   p.csc_ptr_elems = p.sumS + n;
@@ -1580,6 +1719,9 @@ inline BatchPlan plan_batch(int64_t M, int64_t B, int64_t n, const int64_t* S_ho
   p.off_ldp = put(sizeof(double) * 2 * n * p.maxsteps);
   p.off_failp = put(sizeof(int64_t) * 2 * n * p.maxsteps);
   p.off_stage = put(sizeof(double) * 4 * n * NB * NB);  // two slots per matrix
+  p.off_skey = put(sizeof(int) * p.csc_ent_elems);       // the buckets in slice order
+  p.off_st = put(sizeof(double) * p.csc_ent_elems);
+  p.off_slices = put(sizeof(int2) * p.nsl);
   p.bytes = align_up(o, 256);
   return p;
 }
@@ -1643,6 +1785,9 @@ int batch_pipeline(const WSrc w_tilde, int64_t M, const double* dirty_image, con
   double* D = reinterpret_cast<double*>(ws + p.off_D);
   double* logdet = logdets_out ? logdets_out : reinterpret_cast<double*>(ws + p.off_logdet);
   int64_t* info = info_out ? info_out : reinterpret_cast<int64_t*>(ws + p.off_info);
+  int* skey = reinterpret_cast<int*>(ws + p.off_skey);
+  double* st_t = reinterpret_cast<double*>(ws + p.off_st);
+  int2* slices = reinterpret_cast<int2*>(ws + p.off_slices);
 
   // Per-sample table (host -> device).
   static thread_local std::vector<SampleDesc> hd;
@@ -1661,8 +1806,15 @@ int batch_pipeline(const WSrc w_tilde, int64_t M, const double* dirty_image, con
   }
   AJ_REQUIRE(maxS <= SOLVE_S_MAX, "a sample of %lld source pixels exceeds the solve kernel's LDS (%d at most)",
              (long long)maxS, SOLVE_S_MAX);
+  // slices of 64 columns for the Omega kernel, samples in order
+  static thread_local std::vector<int2> hs;
+  hs.clear();
+  for (int64_t b = 0; b < n; ++b)
+    for (int64_t c = 0; c < S_host[b]; c += 64) hs.push_back(int2{(int)b, (int)c});
+  AJ_REQUIRE((int64_t)hs.size() == p.nsl && p.nsl < (1ll << 31), "slice table of %zu entries", hs.size());
   timer_begin(AJ_TIMER_BATCH, st);
   AJ_CHECK_HIP(hipMemcpyAsync(desc, hd.data(), sizeof(SampleDesc) * n, hipMemcpyHostToDevice, st));
+  AJ_CHECK_HIP(hipMemcpyAsync(slices, hs.data(), sizeof(int2) * hs.size(), hipMemcpyHostToDevice, st));
   static thread_local std::vector<int> ho;
   ho.resize(n);
   for (int64_t b = 0; b < n; ++b) ho[b] = (int)b;
@@ -1672,16 +1824,29 @@ int batch_pipeline(const WSrc w_tilde, int64_t M, const double* dirty_image, con
   AJ_CHECK_HIP(hipFuncSetAttribute((const void*)prep_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)(sizeof(int) * (maxS + 1))));
   hipLaunchKernelGGL(prep_kernel, dim3((unsigned)n), dim3(PREP_THREADS), sizeof(int) * (maxS + 1), st, desc,
-                     pix_indexes, pix_weights, M, B, dirty_image, ptrs, ekey, et, D);
+                     pix_indexes, pix_weights, M, B, dirty_image, ptrs, ekey, et, D, skey, st_t);
   AJ_LAUNCH_CHECK();
-  // 2: Omega = W T, W read once for the batch
-  const int R = (int)std::min<int64_t>(8, (int64_t)(OMEGA_LDS_BUDGET / (sizeof(double) * M)));
+  // 2: Omega = W T, W read once for the batch; rows of W per workgroup: the largest power of two <= 8
+  // whose rows fit in LDS
+  int R = 8;
+  while (R >= 1 && (size_t)R * M * sizeof(double) > OMEGA_LDS_BUDGET) R >>= 1;
   timer_begin(AJ_TIMER_OMEGA, st);
   if (R >= 1) {
-    AJ_CHECK_HIP(hipFuncSetAttribute((const void*)omega_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)(R * M * sizeof(double))));
-    hipLaunchKernelGGL(omega_kernel, dim3((unsigned)ceil_div(M, R)), dim3(OMEGA_THREADS),
-                       (size_t)R * M * sizeof(double), st, w_tilde, M, R, (int)n, desc, ptrs, ekey, et, omega);
+    constexpr int U = OMEGA_SLICE_UNROLL;
+    const void* fn = R == 8 ? (const void*)omega_slices_kernel<8, U>
+                   : R == 4 ? (const void*)omega_slices_kernel<4, U>
+                   : R == 2 ? (const void*)omega_slices_kernel<2, U>
+                            : (const void*)omega_slices_kernel<1, U>;
+    AJ_CHECK_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(R * M * sizeof(double))));
+    const dim3 og((unsigned)ceil_div(M, R));
+    const size_t lds = (size_t)R * M * sizeof(double);
+    const int nsl = (int)p.nsl;
+    switch (R) {
+      case 8: hipLaunchKernelGGL((omega_slices_kernel<8, U>), og, dim3(OMEGA_THREADS), lds, st, w_tilde, M, nsl, slices, desc, ptrs, skey, st_t, omega); break;
+      case 4: hipLaunchKernelGGL((omega_slices_kernel<4, U>), og, dim3(OMEGA_THREADS), lds, st, w_tilde, M, nsl, slices, desc, ptrs, skey, st_t, omega); break;
+      case 2: hipLaunchKernelGGL((omega_slices_kernel<2, U>), og, dim3(OMEGA_THREADS), lds, st, w_tilde, M, nsl, slices, desc, ptrs, skey, st_t, omega); break;
+      default: hipLaunchKernelGGL((omega_slices_kernel<1, U>), og, dim3(OMEGA_THREADS), lds, st, w_tilde, M, nsl, slices, desc, ptrs, skey, st_t, omega); break;
+    }
   } else {
     hipLaunchKernelGGL(omega_global_kernel, dim3((unsigned)M), dim3(256), 0, st, w_tilde, M, (int)n, desc, ptrs,
                        ekey, et, omega);
