@@ -75,6 +75,8 @@ SIGNATURES = {
         _INT, [_P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _P, _P, _P, _S, _P]),
     "aj_log_evidence_batched_workspace_bytes": (_S, [_I, _I, _I, _P]),
     "aj_log_evidence_configure": (_INT, [_INT]),
+    "aj_log_evidence_flow_configure": (_INT, [_INT, _INT]),
+    "aj_log_evidence_flow_trace": (_INT, [_P, _I]),
     "aj_log_evidence_batched_compact": (
         _INT, [_P, _I, _P, _I, _P, _P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _S, _P]),
     "aj_log_evidence_batched": (

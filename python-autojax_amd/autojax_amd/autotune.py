@@ -84,7 +84,8 @@ def tune_w_tilde(grid_radians_slim, precision: str, chunks: int = 3, reps: int =
 
 
 CHOL_SCHEDULES = {0: "three launches per step", 1: "cooperative", 2: "cooperative, lookahead",
-                  3: "one launch per step, lookahead", 4: "left-looking, one launch per step"}
+                  3: "one launch per step, lookahead", 4: "left-looking, one launch per step",
+                  5: "left-looking, one persistent launch"}
 _BATCH_CACHE: dict = {}
 
 

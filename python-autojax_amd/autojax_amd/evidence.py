@@ -157,9 +157,7 @@ class SampleBatch:
             self.logdets.data_ptr(), self.info.data_ptr(), self.ws.data_ptr(), self.ws.numel(), dv.stream_ptr(),
         )
         if check:
-            bad = torch.nonzero(self.info.any(dim=1)).flatten().tolist()
-            if bad:
-                raise np.linalg.LinAlgError(f"samples {bad}: matrix not positive definite, info={self.info[bad].tolist()}")
+            self._check_info()
         return self.out
 
     def log_evidence_compact(self, w_compact, native_index_for_slim_index, dirty_image, chi2_noise_norm, check=False):
@@ -181,10 +179,19 @@ class SampleBatch:
             self.ws.data_ptr(), self.ws.numel(), dv.stream_ptr(),
         )
         if check:
-            bad = torch.nonzero(self.info.any(dim=1)).flatten().tolist()
-            if bad:
-                raise np.linalg.LinAlgError(f"samples {bad}: matrix not positive definite, info={self.info[bad].tolist()}")
+            self._check_info()
         return self.out
+
+    def _check_info(self):
+        """Raise for a failed evaluation: RuntimeError when the persistent factorisation schedule gave up a
+        dependency wait (info -1), LinAlgError for a matrix that is not positive definite (its first failing
+        column, 1-based)."""
+        if bool((self.info < 0).any()):
+            raise RuntimeError("batched Cholesky (persistent schedule): a dependency wait exceeded its limit; "
+                               "the evaluation is invalid")
+        bad = torch.nonzero(self.info.any(dim=1)).flatten().tolist()
+        if bad:
+            raise np.linalg.LinAlgError(f"samples {bad}: matrix not positive definite, info={self.info[bad].tolist()}")
 
     def reconstructions(self):
         off = np.concatenate([[0], np.cumsum(self.S_host)])

@@ -36,6 +36,34 @@ import sys
 import numpy as np
 
 
+def flow_summary(rec):
+    """Summary of a persistent-schedule task trace (rows: taken, ready, done, matrix, kind, block column, task,
+    hardware id; wall clock at 100 MHz): span, busy fraction of the workgroups, time spent waiting, and per
+    kind the task count and mean run time, in microseconds."""
+    t0 = rec[:, 0].min()
+    span = (rec[:, 2].max() - t0) / 100.0
+    run = (rec[:, 2] - rec[:, 1]) / 100.0
+    wait = (rec[:, 1] - rec[:, 0]) / 100.0
+    units = len(np.unique(rec[:, 7]))
+    out = {"tasks": int(len(rec)), "span_us": round(float(span), 1), "units": units,
+           "busy_frac": round(float(run.sum() / (span * max(units, 1))), 3),
+           "wait_frac": round(float(wait.sum() / (span * max(units, 1))), 3)}
+    for kind, name in ((0, "diag0"), (1, "strip256"), (2, "strip32")):
+        m = rec[:, 4] == kind
+        if m.any():
+            out[name] = {"n": int(m.sum()), "mean_us": round(float(run[m].mean()), 2),
+                         "max_us": round(float(run[m].max()), 2), "wait_mean_us": round(float(wait[m].mean()), 2)}
+    # the last matrix to finish: its block columns' finish times
+    last = rec[np.argmax(rec[:, 2])]
+    mm = rec[rec[:, 3] == last[3]]
+    ks = np.unique(mm[:, 5])
+    out["last_matrix"] = {"matrix": int(last[3]), "columns": int(len(ks)),
+                          "first_start_us": round(float((mm[:, 1].min() - t0) / 100.0), 1),
+                          "column_end_us": [round(float((mm[mm[:, 5] == k][:, 2].max() - t0) / 100.0), 1)
+                                            for k in ks[::4]]}
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("what", choices=["wtilde", "batch", "copy", "intmm", "mapper", "mfma", "fill", "vendor", "grad"])
@@ -52,7 +80,11 @@ def main(argv=None):
     ap.add_argument("--chunk", type=int, default=0, help="int8crt: visibilities per exact chunk (0 = default)")
     ap.add_argument("--check", action="store_true", help="compare W-tilde with kernel variant 0 (bitwise)")
     ap.add_argument("--group-mib", type=int, default=0, help="cap on the features of one SYRK launch (0 = default)")
-    ap.add_argument("--modes", default="0,1,2,3,4", help="batch: Cholesky schedules to time (aj_log_evidence_configure)")
+    ap.add_argument("--modes", default="0,1,2,3,4,5", help="batch: Cholesky schedules to time (aj_log_evidence_configure)")
+    ap.add_argument("--flow-small", default="-1", help="batch: schedule 5's 32-row-task switch values "
+                                                       "(aj_log_evidence_flow_configure small_per_cu)")
+    ap.add_argument("--trace", default=None, help="batch: trace schedule 5's tasks once per switch value and save "
+                                                  "the records to this .npz path (a summary is printed)")
     ap.add_argument("--against", default=None, help="also build W-tilde at this precision and report the "
                                                     "normwise difference max|dW| / max|W|")
     a = ap.parse_args(argv)
@@ -118,17 +150,40 @@ def main(argv=None):
         y = E.dirty_image(d.data, d.noise_map, d.uv_wavelengths, d.grid_radians_slim)
         c2nn = E.chi2_noise_norm(d.data, d.noise_map)
         batch = E.SampleBatch(d.samples, d.M)
-        for mode in (int(x) for x in a.modes.split(",")):
+        traces = {}
+        runs = [(m, int(f)) for m in (int(x) for x in a.modes.split(","))
+                for f in (a.flow_small.split(",") if m == 5 else ["-1"])]
+        for mode, small in runs:
             _lib.call("aj_log_evidence_configure", mode)
+            _lib.call("aj_log_evidence_flow_configure", small, -1)
             ms = timed(lambda: batch.log_evidence(W, y, c2nn), a.reps)
             _lib.timing_enable(True)
             for _ in range(a.reps):
                 batch.log_evidence(W, y, c2nn)
             stages = {k: round(_lib.timing_read(k)[0] / a.reps, 4) for k in ("omega", "fgather", "chol", "solve")}
             _lib.timing_enable(False)
-            print(json.dumps({"kernel": "batched_log_evidence", "chol_mode": mode, "M": d.M, "samples": a.samples,
-                              "sum_S": int(batch.S_host.sum()), "ms": ms, "stages_ms": stages}), flush=True)
+            batch._check_info()
+            print(json.dumps({"kernel": "batched_log_evidence", "chol_mode": mode, "flow_small_per_cu": small,
+                              "M": d.M, "samples": a.samples, "sum_S": int(batch.S_host.sum()), "ms": ms,
+                              "stages_ms": stages}), flush=True)
+            if mode == 5 and a.trace:
+                cap = 1 << 18
+                buf = torch.full((cap, 8), -1, dtype=torch.int64, device="cuda")
+                _lib.call("aj_log_evidence_flow_trace", buf.data_ptr(), cap)
+                try:
+                    batch.log_evidence(W, y, c2nn)
+                    torch.cuda.synchronize()
+                finally:
+                    _lib.call("aj_log_evidence_flow_trace", None, 0)
+                rec = buf.cpu().numpy()
+                rec = rec[rec[:, 2] >= 0]
+                traces[f"small{small}"] = rec
+                print(json.dumps({"kernel": "flow_trace", "flow_small_per_cu": small, **flow_summary(rec)}),
+                      flush=True)
         _lib.call("aj_log_evidence_configure", -1)
+        _lib.call("aj_log_evidence_flow_configure", -1, -1)
+        if traces:
+            np.savez(a.trace, S=batch.S_host, **traces)
         return
     if a.what == "grad":
         from . import torch_api as TA

@@ -27,6 +27,21 @@ namespace aj {
 namespace lk {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) double lds_f64;  // a double in LDS
+
+// A wave-uniform value read as such (from lane 0): for values the compiler cannot prove uniform, such as
+// a device function's arguments or what it loads, so that branches, loops and buffer resources on them
+// stay scalar.
+__device__ inline int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline int64_t uni(int64_t v) {
+  const uint64_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+  const uint64_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)v >> 32));
+  return (int64_t)(lo | (hi << 32));
+}
+template <typename T>
+__device__ inline T* uni(T* p) {  // keeps the pointer's address space
+  return (T*)uni((int64_t)(uint64_t)p);
+}
 // v_mfma_f64_16x16x4: A lane l holds A[l%16][l/16], B lane l holds B[l/16][l%16], and the
 // accumulator register r of lane l holds D[4r + l/16][l%16].
 
@@ -42,6 +57,14 @@ struct SampleDesc {
   double coef2;       // regularisation coefficient squared
   double beta;        // rank-one deflation shift for log|H| (0 = none)
 };
+
+// The persistent Cholesky's synchronisation words (ints): [q] head of XCD q's task queue (q < 8), [8] error
+// flag, [9 + j] completed tasks of matrix j; cleared by the batch's first launch.
+constexpr int FLOW_QUEUES = 8;  // one per XCD of an MI355X
+constexpr int FLOW_ERR = FLOW_QUEUES;
+constexpr int FLOW_SYNC_HEAD = FLOW_QUEUES + 1;
+// s_getreg operand of the XCC_ID hardware register (register 20, bits 3:0): the XCD a wave runs on
+constexpr int FLOW_XCC_ID_REG = (3 << 11) | (0 << 6) | 20;
 
 // ---------------------------------------------------------------------------
 // Mapping inversion (CSC): bucket s holds the entries (m, t) with idx[m,b] = s
@@ -165,10 +188,15 @@ __global__ void __launch_bounds__(PREP_THREADS) prep_kernel(const SampleDesc* __
                                                             const double* __restrict__ dirty,
                                                             int* __restrict__ ptrs, int* __restrict__ ent_key,
                                                             double* __restrict__ ent_t, double* __restrict__ D,
-                                                            int* __restrict__ sl_key, double* __restrict__ sl_t) {
+                                                            int* __restrict__ sl_key, double* __restrict__ sl_t,
+                                                            int* __restrict__ flow_sync) {
   extern __shared__ int cnt[];  // S + 1 counters, then cursors
   __shared__ int part[PREP_THREADS];
   const SampleDesc d = sd[blockIdx.x];
+  if (flow_sync != nullptr && threadIdx.x < FLOW_SYNC_HEAD) {
+    if (threadIdx.x < 2) flow_sync[FLOW_SYNC_HEAD + 2 * blockIdx.x + threadIdx.x] = 0;
+    if (blockIdx.x == 0) flow_sync[threadIdx.x] = 0;
+  }
   const int64_t S = d.S;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t* ix = idx + d.map_off;
@@ -642,8 +670,8 @@ __device__ inline int find_entry(const StepEntry* __restrict__ e, int ne, int bl
 
 __device__ inline double* mat_ptr(const SampleDesc* __restrict__ sd, double* __restrict__ mats, int j, int64_t& n) {
   const SampleDesc d = sd[j >> 1];
-  n = d.S;
-  return mats + d.mat_off + ((j & 1) ? d.S * d.S : 0);
+  n = uni(d.S);
+  return mats + uni(d.mat_off + ((j & 1) ? d.S * d.S : 0));
 }
 
 __device__ inline double readlane_d(double v, int l) {
@@ -687,8 +715,9 @@ __device__ inline void chol_factor_regs(double (&r)[NB], int nb, int64_t k0, int
 
 // The same factorisation with each column broadcast through LDS (col: NB doubles of LDS owned by the wave)
 // instead of one lane read per element: the same operations on the same values, so the same bits.
+template <typename ColPtr>
 __device__ inline void chol_factor_lds(double (&r)[NB], int nb, int64_t k0, int lane, double& ld, int64_t& fail,
-                                       double* col) {
+                                       ColPtr col) {
   double mant = 1.0;
   int ex = 0;
   fail = 0;
@@ -1075,53 +1104,65 @@ __global__ void chol_unstage_kernel(const SampleDesc* __restrict__ sd, double* _
 // tasks read it from there and that launch's first task moves it into place (no task of that launch
 // reads the block in place).  Each element sees the same MFMA operands, in the same order, and the same
 // substitutions as in the step-synchronous form, so the factor is bitwise the same.
-constexpr int LL_STRIP = 128;            // strip rows of a task: four waves of 32 rows
-constexpr int LL_STRIP0 = 96;            // of the first task, whose fourth wave forms the next diagonal block
-constexpr int LL_ROWS = LL_STRIP + NB;   // staged rows: the strip and the diagonal block's rows
+constexpr int LL_STRIP = 128;            // strip rows of a task (one launch per step): four waves of 32 rows
 constexpr int LL_PITCH = NB + 2;         // LDS row pitch (doubles): the fragment reads are free of conflicts
 constexpr int LL_THREADS = 256;
-constexpr int LL_LOADS = LL_ROWS * NB / LL_THREADS;  // doubles per thread per staged block
-static_assert(LL_LOADS * LL_THREADS == LL_ROWS * NB && LL_THREADS / NB == 8, "eight rows per load pass");
+
+// A task's shape for STRIP strip rows: a wave per 32 rows; the first task of a matrix takes STRIP - 32
+// rows and its last wave forms the next diagonal block.
+template <int STRIP>
+struct LeftShape {
+  static constexpr int STRIP0 = STRIP - NB;          // strip rows of the first task
+  static constexpr int ROWS = STRIP + NB;            // staged rows: the strip and the diagonal block's rows
+  static constexpr int THREADS = 2 * STRIP;
+  static constexpr int WAVES = THREADS / 64;
+  static constexpr int PASS = THREADS / NB;          // staged rows per load pass
+  static constexpr int LOADS = ROWS * NB / THREADS;  // doubles per thread per staged block
+  static_assert(LOADS * THREADS == ROWS * NB && LOADS <= 32 && STRIP % 32 == 0, "whole load passes");
+};
+static_assert(LeftShape<LL_STRIP>::THREADS == LL_THREADS, "one-launch-per-step tasks: four waves");
 
 // Index (doubles) of element (r, c) in a staged block of NB-double rows whose 16-byte segments are stored
 // XOR (r mod 16), so that the 16 rows of a fragment read fall in 16 different segments.
 __device__ inline int ll_at(int r, int c) { return r * NB + ((((c >> 1) ^ (r & 15)) << 1) | (c & 1)); }
 
-__device__ inline const double* ll_row(const double* A, int64_t n, int64_t s0, int64_t k0, int r) {
-  return A + (r < LL_STRIP ? s0 + r : k0 + (r - LL_STRIP)) * n;
-}
+// One double by a raw buffer load: offsets at or past the resource's byte count read as zero (no branch).
+__device__ double raw_buffer_load_f64(i32x4 rsrc, int voffset, int soffset, int aux)
+    __asm("llvm.amdgcn.raw.buffer.load.f64");
+constexpr int LL_OOB = 0x7ffff000;  // past any matrix's bytes (n <= SOLVE_S_MAX), also 47 blocks on
 
-__global__ void __launch_bounds__(LL_THREADS) chol_left_kernel(const SampleDesc* __restrict__ sd,
-                                                              const StepEntry* __restrict__ e, int ne, int64_t k0,
-                                                              int kstep, int maxsteps, double* __restrict__ mats,
-                                                              double* __restrict__ ldp, int64_t* __restrict__ failp,
-                                                              double* __restrict__ stage) {
-  __shared__ double Ls[LL_ROWS * LL_PITCH];
-  __shared__ double Lkk[NB][NB + 1];
-  const int ei = find_entry(e, ne, (int)blockIdx.x, false);
-  const int j = e[ei].mat, rb = (int)blockIdx.x - e[ei].pfx_p;
+// Task rb of block column k (k0 = 32 k) of matrix j; Ls (ROWS x LL_PITCH doubles), Lkk and colb (NB doubles)
+// in the caller's LDS; threadIdx.x < THREADS.
+template <int STRIP>
+__device__ __attribute__((always_inline)) inline void chol_left_task(const SampleDesc* __restrict__ sd, int j, int rb, int64_t k0, int kstep,
+                                      int maxsteps, double* __restrict__ mats, double* __restrict__ ldp,
+                                      int64_t* __restrict__ failp, double* __restrict__ stage,
+                                      lds_f64* __restrict__ Ls, lds_f64 (*__restrict__ Lkk)[NB + 1],
+                                      lds_f64* __restrict__ colb) {
+  using SH = LeftShape<STRIP>;
   int64_t n;
   double* A = mat_ptr(sd, mats, j, n);
   const int nb = (int)std::min<int64_t>(NB, n - k0);
   const bool next = rb == 0 && k0 + NB < n;  // this task forms diagonal block k + 1
   const int nb1 = next ? (int)std::min<int64_t>(NB, n - k0 - NB) : 0;
-  const int64_t s0 = k0 + nb + (rb == 0 ? 0 : LL_STRIP0 + (int64_t)(rb - 1) * LL_STRIP);  // first strip row
-  const int rows = (int)std::max<int64_t>(0, std::min<int64_t>(rb == 0 ? LL_STRIP0 : LL_STRIP, n - s0));
+  const int64_t s0 = k0 + nb + (rb == 0 ? 0 : SH::STRIP0 + (int64_t)(rb - 1) * STRIP);  // first strip row
+  const int rows = (int)std::max<int64_t>(0, std::min<int64_t>(rb == 0 ? SH::STRIP0 : STRIP, n - s0));
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lr = lane & 15, lk = lane >> 4;
   const double* stg_in = stage + ((int64_t)2 * j + (kstep & 1)) * NB * NB;
   double* stg_out = stage + ((int64_t)2 * j + ((kstep + 1) & 1)) * NB * NB;
   // L_kk: factored in place by the first launch (step 0) or staged by the previous launch
-  for (int x = tid; x < NB * NB; x += LL_THREADS) {
+  for (int x = tid; x < NB * NB; x += SH::THREADS) {
     const int i = x / NB, c = x % NB;
     double v = 0.0;
     if (i < nb && c <= i) v = kstep == 0 ? A[(k0 + i) * n + k0 + c] : stg_in[x];
     Lkk[i][c] = v;
     if (rb == 0 && kstep > 0 && i < nb && c <= i) A[(k0 + i) * n + k0 + c] = v;
   }
-  // wave w < 3 (and wave 3 when the task does not form the next block) owns strip rows 32 w .. 32 w + 31;
-  // wave 3 of a task that does forms block k + 1 from strip rows 0 .. 31 (its rows k0 + 32 ..)
-  const bool dwave = next && wave == 3;
+  // wave w < WAVES - 1 (and the last wave when the task does not form the next block) owns strip rows
+  // 32 w .. 32 w + 31; the last wave of a task that does forms block k + 1 from strip rows 0 .. 31 (its rows
+  // k0 + 32 ..)
+  const bool dwave = next && wave == SH::WAVES - 1;
   const int w0 = wave * 32;
   const bool active = dwave || w0 < rows;
   d4 acc[2][2];
@@ -1141,33 +1182,37 @@ __global__ void __launch_bounds__(LL_THREADS) chol_left_kernel(const SampleDesc*
         acc[a][b][r] = v;
       }
   if (kstep > 0) {
-    // staging: thread tid loads column lc of LDS rows lr0 + 8 q: rows < LL_STRIP are strip rows s0 + r (zero
+    // staging: thread tid loads column lc of LDS rows lr0 + PASS q: rows < STRIP are strip rows s0 + r (zero
     // past the task's rows), the last NB the diagonal block's rows k0 + i (zero past nb); block p + 1's
     // loads are in flight during block p's MFMAs.
+    // (raw buffer loads: rows outside take an offset past the matrix and read as zeros)
     const int lc = tid & 31, lr0 = tid >> 5;
-    unsigned okm = 0;
+    const i32x4 rsrc = make_rsrc(A, (uint32_t)(n * n * (int64_t)sizeof(double)));
+    int voff[SH::LOADS];
 #pragma unroll
-    for (int q = 0; q < LL_LOADS; ++q) {
-      const int r = lr0 + 8 * q;
-      if (r < LL_STRIP ? r < rows || (next && r < nb1) : r - LL_STRIP < nb) okm |= 1u << q;
+    for (int q = 0; q < SH::LOADS; ++q) {
+      const int r = lr0 + SH::PASS * q;
+      const bool ok = r < STRIP ? r < rows || (next && r < nb1) : r - STRIP < nb;
+      const int64_t grow = r < STRIP ? s0 + r : k0 + (r - STRIP);
+      voff[q] = ok ? (int)((grow * n + lc) * (int64_t)sizeof(double)) : LL_OOB;
     }
-    double pre[LL_LOADS];
+    double pre[SH::LOADS];
 #pragma unroll
-    for (int q = 0; q < LL_LOADS; ++q) pre[q] = (okm >> q) & 1u ? ll_row(A, n, s0, k0, lr0 + 8 * q)[lc] : 0.0;
+    for (int q = 0; q < SH::LOADS; ++q) pre[q] = raw_buffer_load_f64(rsrc, voff[q], 0, 0);
     for (int p = 0; p < kstep; ++p) {
       __syncthreads();  // every wave is past its fragment reads of block p - 1
 #pragma unroll
-      for (int q = 0; q < LL_LOADS; ++q) Ls[(lr0 + 8 * q) * LL_PITCH + lc] = pre[q];
+      for (int q = 0; q < SH::LOADS; ++q) Ls[(lr0 + SH::PASS * q) * LL_PITCH + lc] = pre[q];
       __syncthreads();
       if (p + 1 < kstep) {
-        const int64_t off = (int64_t)(p + 1) * NB + lc;
+        const int off = (p + 1) * NB * (int)sizeof(double);
 #pragma unroll
-        for (int q = 0; q < LL_LOADS; ++q) pre[q] = (okm >> q) & 1u ? ll_row(A, n, s0, k0, lr0 + 8 * q)[off] : 0.0;
+        for (int q = 0; q < SH::LOADS; ++q) pre[q] = raw_buffer_load_f64(rsrc, voff[q], off, 0);
       }
       if (active) {
         // strip waves: rows w0 .. w0 + 31 against the diagonal block's rows; the block-(k + 1) wave: strip
         // rows 0 .. 31 against themselves (its block (0, 1), above the diagonal, is not used)
-        const int ra = dwave ? 0 : w0, rbb = dwave ? 0 : LL_STRIP;
+        const int ra = dwave ? 0 : w0, rbb = dwave ? 0 : STRIP;
 #pragma unroll
         for (int ks = 0; ks < NB / 4; ++ks) {
           const int pc = ks * 4 + lk;
@@ -1182,7 +1227,7 @@ __global__ void __launch_bounds__(LL_THREADS) chol_left_kernel(const SampleDesc*
     }
   }
   // the strip rows into Xs (the staging buffer, free after this barrier)
-  double* Xs = Ls;
+  lds_f64* Xs = Ls;
   __syncthreads();
   if (!dwave) {
 #pragma unroll
@@ -1195,7 +1240,7 @@ __global__ void __launch_bounds__(LL_THREADS) chol_left_kernel(const SampleDesc*
   __syncthreads();
   // X L_kk^T = (updated rows): lanes 0 .. 31 of wave w, row w0 + lane, the row in registers
   if (!dwave && lane < 32 && w0 + lane < rows) {
-    double* xr = Xs + (w0 + lane) * LL_PITCH;
+    lds_f64* xr = Xs + (w0 + lane) * LL_PITCH;
     double x[NB];
 #pragma unroll
     for (int c = 0; c < NB; ++c) x[c] = xr[c];
@@ -1213,7 +1258,7 @@ __global__ void __launch_bounds__(LL_THREADS) chol_left_kernel(const SampleDesc*
     for (int c = 0; c < NB; ++c) xr[c] = x[c];
   }
   __syncthreads();
-  for (int x = tid; x < rows * NB; x += LL_THREADS) {
+  for (int x = tid; x < rows * NB; x += SH::THREADS) {
     const int i = x / NB, c = x % NB;
     if (c < nb) A[(s0 + i) * n + k0 + c] = Xs[i * LL_PITCH + c];
   }
@@ -1244,7 +1289,6 @@ __global__ void __launch_bounds__(LL_THREADS) chol_left_kernel(const SampleDesc*
   for (int c = 0; c < NB; ++c) r[c] = (lane < nb1 && c <= lane) ? Lkk[lane][c] : 0.0;
   double ld;
   int64_t fail;
-  __shared__ double colb[NB];
   chol_factor_lds(r, nb1, k0 + NB, lane, ld, fail, colb);
 #pragma unroll
   for (int c = 0; c < NB; ++c)
@@ -1254,7 +1298,21 @@ __global__ void __launch_bounds__(LL_THREADS) chol_left_kernel(const SampleDesc*
     failp[(int64_t)j * maxsteps + kstep + 1] = fail;
   }
 }
-static_assert(sizeof(double) * (LL_ROWS * LL_PITCH + NB * (NB + 1)) <= 54 * 1024, "three workgroups per CU");
+
+__global__ void __launch_bounds__(LL_THREADS) chol_left_kernel(const SampleDesc* __restrict__ sd,
+                                                              const StepEntry* __restrict__ e, int ne, int64_t k0,
+                                                              int kstep, int maxsteps, double* __restrict__ mats,
+                                                              double* __restrict__ ldp, int64_t* __restrict__ failp,
+                                                              double* __restrict__ stage) {
+  __shared__ double Ls[LeftShape<LL_STRIP>::ROWS * LL_PITCH];
+  __shared__ double Lkk[NB][NB + 1];
+  __shared__ double colb[NB];
+  const int ei = find_entry(e, ne, (int)blockIdx.x, false);
+  chol_left_task<LL_STRIP>(sd, e[ei].mat, (int)blockIdx.x - e[ei].pfx_p, k0, kstep, maxsteps, mats, ldp, failp,
+                           stage, (lds_f64*)Ls, (lds_f64(*)[NB + 1])Lkk, (lds_f64*)colb);
+}
+static_assert(sizeof(double) * (LeftShape<LL_STRIP>::ROWS * LL_PITCH + NB * (NB + 1) + NB) <= 54 * 1024,
+              "three workgroups per CU");
 
 // The same step for the last steps, when a matrix has few rows left below its diagonal block and a task's
 // time is its chain of blocks p < k: 32-row tasks, each 16 x 16 block of the strip (and of the next
@@ -1267,18 +1325,13 @@ constexpr int LS_THREADS = 512;          // waves 0-3: the strip's four 16 x 16 
 constexpr int LS_LOADS = LS_ROWS / 4 / (LS_THREADS / 64);  // 1 KB loads per wave per block
 static_assert(LS_LOADS == 2 && LS_STAGES * LS_ROWS * NB * 8 + NB * (NB + 1) * 8 * 2 <= 160 * 1024, "small-task LDS");
 
-__global__ void __launch_bounds__(LS_THREADS) chol_left_small_kernel(const SampleDesc* __restrict__ sd,
-                                                                    const StepEntry* __restrict__ e, int ne,
-                                                                    int64_t k0, int kstep, int maxsteps,
-                                                                    double* __restrict__ mats, double* __restrict__ ldp,
-                                                                    int64_t* __restrict__ failp,
-                                                                    double* __restrict__ stage) {
-  __shared__ __attribute__((aligned(1024))) double Ls[LS_STAGES][LS_ROWS * NB];
-  __shared__ double Lkk[NB][NB + 1];
-  __shared__ double Dn[NB][NB + 1];
-  __shared__ double colb[NB];
-  const int ei = find_entry(e, ne, (int)blockIdx.x, false);
-  const int j = e[ei].mat, rb = (int)blockIdx.x - e[ei].pfx_p;
+// Task rb of block column k (k0 = 32 k) of matrix j; the ring Ls (LS_STAGES blocks of LS_ROWS x NB doubles,
+// 1 KB aligned), Lkk, Dn and colb (NB doubles) in the caller's LDS; threadIdx.x < LS_THREADS.
+__device__ __attribute__((always_inline)) inline void chol_small_task(const SampleDesc* __restrict__ sd, int j, int rb, int64_t k0, int kstep,
+                                       int maxsteps, double* __restrict__ mats, double* __restrict__ ldp,
+                                       int64_t* __restrict__ failp, double* __restrict__ stage,
+                                       lds_f64* __restrict__ Ls, lds_f64 (*__restrict__ Lkk)[NB + 1],
+                                       lds_f64 (*__restrict__ Dn)[NB + 1], lds_f64* __restrict__ colb) {
   int64_t n;
   double* A = mat_ptr(sd, mats, j, n);
   const int nb = (int)std::min<int64_t>(NB, n - k0);
@@ -1330,7 +1383,7 @@ __global__ void __launch_bounds__(LS_THREADS) chol_left_small_kernel(const Sampl
     constexpr int LS_OOB = 0x7ffff000;  // past any matrix's bytes (n <= SOLVE_S_MAX)
 #define AJ_LS_ISSUE(P)                                                                                     \
   {                                                                                                        \
-    double* lb_ = Ls[(P) % LS_STAGES];                                                                     \
+    lds_f64* lb_ = Ls + ((P) % LS_STAGES) * (LS_ROWS * NB);                                                                     \
     _Pragma("unroll") for (int jj = 0; jj < LS_LOADS; ++jj) raw_buffer_load_lds(                          \
         rsrc, (lds_u32*)(lb_ + (LS_LOADS * wave + jj) * 128), 16,                                          \
         (P) < kstep && voff[jj] >= 0 ? voff[jj] + (P) * NB * (int)sizeof(double) : LS_OOB, 0, 0, 0);      \
@@ -1347,7 +1400,7 @@ __global__ void __launch_bounds__(LS_THREADS) chol_left_small_kernel(const Sampl
       AJ_LS_ISSUE(p + LS_STAGES - 1)
       // the block's fragments as LDS loads in assembly, one group waiting for itself (the compiler would
       // wait for every direct-to-LDS load in flight before any LDS read)
-      const uint32_t lb = (uint32_t)(uintptr_t)(lds_u32*)Ls[p % LS_STAGES];
+      const uint32_t lb = (uint32_t)(uintptr_t)(lds_u32*)(Ls + (p % LS_STAGES) * (LS_ROWS * NB));
       uint32_t aa[NB / 4], ab[NB / 4];
 #pragma unroll
       for (int ks = 0; ks < NB / 4; ++ks) {
@@ -1375,7 +1428,7 @@ __global__ void __launch_bounds__(LS_THREADS) chol_left_small_kernel(const Sampl
     __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
   }
   // the strip's rows into Xs, the next diagonal block into Dn
-  double* Xs = Ls[0];
+  lds_f64* Xs = Ls;
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
@@ -1388,7 +1441,7 @@ __global__ void __launch_bounds__(LS_THREADS) chol_left_small_kernel(const Sampl
   __syncthreads();
   // X L_kk^T = (updated rows): lanes 0 .. 31 of wave 0, one row each, the row in registers
   if (wave == 0 && lane < rows) {
-    double* xr = Xs + lane * LL_PITCH;
+    lds_f64* xr = Xs + lane * LL_PITCH;
     double x[NB];
 #pragma unroll
     for (int c = 0; c < NB; ++c) x[c] = xr[c];
@@ -1440,6 +1493,189 @@ __global__ void __launch_bounds__(LS_THREADS) chol_left_small_kernel(const Sampl
       ldp[(int64_t)j * maxsteps + kstep + 1] = ld;
       failp[(int64_t)j * maxsteps + kstep + 1] = fail;
     }
+  }
+}
+
+__global__ void __launch_bounds__(LS_THREADS) chol_left_small_kernel(const SampleDesc* __restrict__ sd,
+                                                                    const StepEntry* __restrict__ e, int ne,
+                                                                    int64_t k0, int kstep, int maxsteps,
+                                                                    double* __restrict__ mats, double* __restrict__ ldp,
+                                                                    int64_t* __restrict__ failp,
+                                                                    double* __restrict__ stage) {
+  __shared__ __attribute__((aligned(1024))) double Ls[LS_STAGES * LS_ROWS * NB];
+  __shared__ double Lkk[NB][NB + 1];
+  __shared__ double Dn[NB][NB + 1];
+  __shared__ double colb[NB];
+  const int ei = find_entry(e, ne, (int)blockIdx.x, false);
+  chol_small_task(sd, e[ei].mat, (int)blockIdx.x - e[ei].pfx_p, k0, kstep, maxsteps, mats, ldp, failp, stage,
+                  (lds_f64*)Ls, (lds_f64(*)[NB + 1])Lkk, (lds_f64(*)[NB + 1])Dn, (lds_f64*)colb);
+}
+
+// Persistent form of the left-looking schedule (mode 5): the whole factorisation of every matrix in one
+// launch of one workgroup per compute unit.  Each matrix belongs to one XCD (the host balances their S^3
+// over the eight); each XCD has its own task list (its matrices' first diagonal blocks, then block column
+// 0's tasks of its matrices, column 1's, ...) and queue head (an atomic counter).  A workgroup takes the
+// next task of the queue of the XCD it runs on, waits until every task of that matrix in the earlier block
+// columns is complete (a per-matrix count of completed tasks), runs it and counts it complete.  A task
+// waits only for tasks earlier in its list, which running workgroups have taken, so every wait ends; the
+// wait is bounded all the same (a spin limit), past which the task runs, the error flag is set and the
+// solve reports the batch as failed (also when a matrix's count is short of its total, as it would be if
+// an XCD ran none of the workgroups).  The tasks are those of the one-launch-per-step form (256 strip
+// rows on eight waves, then, once an XCD's 32-row tasks of a column fit its compute units, 32-row tasks),
+// so each element sees the same operations in the same order: the factor is bitwise that of the other
+// schedules.  A matrix's data passes only through its XCD's L2: completion is published once every wave's
+// stores have completed into that L2 (before the workgroup's barrier), and observed by invalidating the
+// waiting CU's L1.
+struct FlowTask {
+  int mat;   // matrix j (kind 0: the first of `rb` consecutive matrices)
+  int kind;  // 0: diagonal block 0 of matrices mat .. mat + rb - 1; 1: a 256-row task; 2: a 32-row task
+  int k;     // block column
+  int rb;    // task within the block column (kind 0: the number of matrices, at most eight; the host
+             // gives one)
+  int need;  // completed tasks of matrix mat required first: all of its tasks in earlier block columns
+};
+constexpr int FL_STRIP = 256;
+constexpr int FLOW_THREADS = 512;
+static_assert(LeftShape<FL_STRIP>::THREADS == FLOW_THREADS && LS_THREADS == FLOW_THREADS, "eight waves per task");
+constexpr int FLOW_RING = LS_STAGES * LS_ROWS * NB > LeftShape<FL_STRIP>::ROWS * LL_PITCH
+                              ? LS_STAGES * LS_ROWS * NB
+                              : LeftShape<FL_STRIP>::ROWS * LL_PITCH;
+constexpr int FLOW_LDS_DOUBLES = FLOW_RING + 2 * NB * (NB + 1) + NB;
+static_assert(FLOW_LDS_DOUBLES * sizeof(double) <= 160 * 1024, "one workgroup per CU");
+
+// The three task kinds as separate functions, so that each keeps its own register allocation (inlined
+// together they spill).  Their global pointers arrive typed as global memory, so that the inlined task
+// bodies load and store with global (not flat) instructions.
+typedef __attribute__((address_space(1))) double g_f64;
+typedef __attribute__((address_space(1))) int64_t g_i64;
+typedef __attribute__((address_space(1))) const SampleDesc g_desc;
+
+// A function's arguments arrive in vector registers; the callers pass wave-uniform values, read back as
+// such (uni) so that the bodies branch, loop and form buffer resources on scalars.
+
+__device__ __attribute__((noinline)) void flow_diag_task(g_desc* sd, int j0, int cnt, int maxsteps, g_f64* mats,
+                                                         g_f64* ldp, g_i64* failp) {
+  const int w = threadIdx.x >> 6;
+  if (w < uni(cnt))
+    chol_diag_dev((const SampleDesc*)uni(sd), uni(j0) + w, 0, 0, uni(maxsteps), (double*)uni(mats),
+                  (double*)uni(ldp), (int64_t*)uni(failp), threadIdx.x & 63);
+}
+
+__device__ __attribute__((noinline)) void flow_left_task(g_desc* sd, int j, int rb, int k, int maxsteps, g_f64* mats,
+                                                         g_f64* ldp, g_i64* failp, g_f64* stage, lds_f64* lds) {
+  k = uni(k);
+  chol_left_task<FL_STRIP>((const SampleDesc*)uni(sd), uni(j), uni(rb), (int64_t)k * NB, k, uni(maxsteps),
+                           (double*)uni(mats), (double*)uni(ldp), (int64_t*)uni(failp), (double*)uni(stage), lds,
+                           reinterpret_cast<lds_f64(*)[NB + 1]>(lds + FLOW_RING), lds + FLOW_RING + 2 * NB * (NB + 1));
+}
+
+__device__ __attribute__((noinline)) void flow_small_task(g_desc* sd, int j, int rb, int k, int maxsteps, g_f64* mats,
+                                                          g_f64* ldp, g_i64* failp, g_f64* stage, lds_f64* lds) {
+  k = uni(k);
+  chol_small_task((const SampleDesc*)uni(sd), uni(j), uni(rb), (int64_t)k * NB, k, uni(maxsteps), (double*)uni(mats),
+                  (double*)uni(ldp), (int64_t*)uni(failp), (double*)uni(stage), lds,
+                  reinterpret_cast<lds_f64(*)[NB + 1]>(lds + FLOW_RING),
+                  reinterpret_cast<lds_f64(*)[NB + 1]>(lds + FLOW_RING + NB * (NB + 1)),
+                  lds + FLOW_RING + 2 * NB * (NB + 1));
+}
+
+// The queue's two synchronisation points, each called by every thread of the workgroup (uniform control
+// flow: the single-thread parts stay inside these functions, so the persistent loop around them has no
+// divergent branch next to its barriers).
+// flow_next: thread 0 counts the workgroup's previous task complete (mat .. mat + cnt - 1, none when cnt is
+// 0; every wave's stores completed, into this XCD's L2, before the caller's barrier) and takes the next task
+// of queue q, whose index in the queue every thread returns.
+// With a trace buffer it also records the previous task (task index prev_t): when it was taken, when its
+// wait ended, when it was counted complete (wall clock, 100 MHz), its matrix, kind, block column, task
+// within the column and the hardware id of the compute unit that ran it.
+__device__ __attribute__((noinline)) int flow_next(int* __restrict__ sync, int q, int prev_mat, int prev_cnt,
+                                                   int* cur, long long* __restrict__ trace,
+                                                   const FlowTask* __restrict__ tasks, int prev_t,
+                                                   long long prev_taken, long long prev_ready) {
+  if (threadIdx.x == 0) {
+    if (prev_cnt > 0) {
+      for (int w = 0; w < prev_cnt; ++w)
+        __hip_atomic_fetch_add(&sync[FLOW_SYNC_HEAD + prev_mat + w], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (trace != nullptr) {
+        const FlowTask tk = tasks[prev_t];
+        long long* r = trace + 8 * (int64_t)prev_t;
+        r[0] = prev_taken;
+        r[1] = prev_ready;
+        r[2] = wall_clock64();
+        r[3] = tk.mat;
+        r[4] = tk.kind;
+        r[5] = tk.k;
+        r[6] = tk.rb;
+        r[7] = __smid();
+      }
+    }
+    *cur = atomicAdd(&sync[q], 1);
+  }
+  __syncthreads();
+  return __builtin_amdgcn_readfirstlane(*cur);
+}
+
+// flow_wait: thread 0 waits (at most spin_limit polls, then sets the error flag) until matrix mat has `need`
+// completed tasks, then invalidates this CU's L1 (the producers ran on this XCD: their stores are in its
+// L2); the barrier holds the other threads until then.
+__device__ __attribute__((noinline)) void flow_wait(int* __restrict__ sync, int mat, int need, int spin_limit) {
+  if (threadIdx.x == 0 && need > 0) {
+    int* cnt = &sync[FLOW_SYNC_HEAD + mat];
+    int spins = 0;
+    while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+      if (++spins > spin_limit) {
+        atomicOr(&sync[FLOW_ERR], 1);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    asm volatile("buffer_inv sc0\n\ts_waitcnt vmcnt(0)" ::: "memory");
+  }
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(FLOW_THREADS) chol_flow_kernel(const SampleDesc* __restrict__ sd,
+                                                                 const FlowTask* __restrict__ tasks,
+                                                                 const int* __restrict__ qoff,
+                                                                 int maxsteps, double* __restrict__ mats,
+                                                                 double* __restrict__ ldp,
+                                                                 int64_t* __restrict__ failp,
+                                                                 double* __restrict__ stage, int* __restrict__ sync,
+                                                                 int spin_limit, long long* __restrict__ trace) {
+  __shared__ __attribute__((aligned(1024))) double lds[FLOW_LDS_DOUBLES];
+  __shared__ int cur;
+  g_desc* gsd = (g_desc*)sd;
+  // the queue of the XCD this workgroup runs on: a matrix's tasks all run on one XCD, so their data passes
+  // through that XCD's L2 only
+  const int q = (int)(__builtin_amdgcn_s_getreg(FLOW_XCC_ID_REG) % FLOW_QUEUES);
+  const int q0 = qoff[q], qn = qoff[q + 1] - q0;
+  int prev_mat = 0, prev_cnt = 0, prev_t = 0;
+  long long taken = 0, ready = 0;
+  for (;;) {
+    // (a call returns in a vector register: read the index as a wave-uniform value here, so that the loop's
+    // exit and the branches on the task's fields are uniform)
+    const int i = __builtin_amdgcn_readfirstlane(
+        flow_next(sync, q, prev_mat, prev_cnt, &cur, trace, tasks, prev_t, taken, ready));
+    if (i >= qn) break;
+    const int t = q0 + i;
+    if (trace != nullptr) taken = wall_clock64();
+    const FlowTask tk = tasks[t];
+    flow_wait(sync, tk.mat, tk.need, spin_limit);
+    if (trace != nullptr) ready = wall_clock64();
+    if (tk.kind == 0)
+      flow_diag_task(gsd, tk.mat, tk.rb, maxsteps, (g_f64*)mats, (g_f64*)ldp, (g_i64*)failp);
+    else if (tk.kind == 1)
+      flow_left_task(gsd, tk.mat, tk.rb, tk.k, maxsteps, (g_f64*)mats, (g_f64*)ldp, (g_i64*)failp, (g_f64*)stage,
+                     (lds_f64*)lds);
+    else
+      flow_small_task(gsd, tk.mat, tk.rb, tk.k, maxsteps, (g_f64*)mats, (g_f64*)ldp, (g_i64*)failp, (g_f64*)stage,
+                      (lds_f64*)lds);
+    // every wave's stores complete before the barrier (they are then in this XCD's L2)
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    prev_mat = tk.mat;
+    prev_cnt = tk.kind == 0 ? tk.rb : 1;
+    prev_t = t;
   }
 }
 
@@ -1510,23 +1746,42 @@ constexpr int SOLVE_S_MAX =
 // Solves L y = D, L^T s = y and the log-evidence, one workgroup per sample.
 //   logL = -1/2 (chi.chi - s.D + log|A| - log|H| + noise_norm)   (numba.py:852-857)
 // ---------------------------------------------------------------------------
+// With ldp non-null the kernel also finishes the factorisation's log-determinants and failure codes of the
+// sample's two matrices (chol_finish_dev), and marks a matrix -1 when the persistent schedule's error flag
+// is set or the matrix's count of completed tasks is not its task total (flow_sync / flow_total).
 __global__ void __launch_bounds__(SOLVE_THREADS) solve_loglike_kernel(const SampleDesc* __restrict__ sd,
                                                                       const double* __restrict__ mats,
                                                                       const double* __restrict__ D,
                                                                       const double* __restrict__ chi2_nn,
-                                                                      const double* __restrict__ logdet,
+                                                                      double* __restrict__ logdet,
                                                                       double* __restrict__ recon,
-                                                                      double* __restrict__ logl) {
+                                                                      double* __restrict__ logl,
+                                                                      const double* __restrict__ ldp,
+                                                                      const int64_t* __restrict__ failp,
+                                                                      int maxsteps, int64_t* __restrict__ info,
+                                                                      const int* __restrict__ flow_sync,
+                                                                      const int* __restrict__ flow_total) {
   constexpr int NW = SOLVE_THREADS / 64;  // waves
   extern __shared__ double y[];
   __shared__ double red[SOLVE_THREADS];
   __shared__ double Ld[SB][SB + 1];
   __shared__ double part[NW][SB];
+  __shared__ double ldv[2];
   const SampleDesc d = sd[blockIdx.x];
   const double* L = mats + d.mat_off;
   const int64_t n = d.S;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   for (int64_t i = tid; i < n; i += SOLVE_THREADS) y[i] = D[d.s_off + i];
+  if (tid < 2) {
+    const int j = 2 * (int)blockIdx.x + tid;
+    if (ldp != nullptr) {
+      chol_finish_dev(sd, j, maxsteps, ldp, failp, logdet, info);
+      if (flow_sync != nullptr &&
+          (flow_sync[FLOW_ERR] != 0 || flow_sync[FLOW_SYNC_HEAD + j] != flow_total[j]))
+        info[j] = -1;
+    }
+    ldv[tid] = logdet[j];
+  }
   __syncthreads();
   // Blocked substitution over 64-row blocks: the off-diagonal part of each block is a GEMV on
   // contiguous row segments of L (all sixteen waves, eight 512-byte row loads in flight per wave, so
@@ -1629,8 +1884,7 @@ __global__ void __launch_bounds__(SOLVE_THREADS) solve_loglike_kernel(const Samp
   }
   if (tid == 0) {
     const double sD = red[0];
-    logl[blockIdx.x] =
-        -0.5 * (chi2_nn[0] - sD + logdet[2 * blockIdx.x] - logdet[2 * blockIdx.x + 1] + chi2_nn[1]);
+    logl[blockIdx.x] = -0.5 * (chi2_nn[0] - sD + ldv[0] - ldv[1] + chi2_nn[1]);
   }
 }
 
@@ -1677,9 +1931,9 @@ __global__ void noise_norm_kernel(const double* __restrict__ noise, int64_t K2, 
 // Host-side planning of the batch workspace.
 // ---------------------------------------------------------------------------
 struct BatchPlan {
-  int64_t n, M, B, sumS, sumS2, omega_elems, csc_ptr_elems, csc_ent_elems, maxsteps, n_entries, nsl;
+  int64_t n, M, B, sumS, sumS2, omega_elems, csc_ptr_elems, csc_ent_elems, maxsteps, n_entries, nsl, max_flow;
   size_t off_desc, off_ptr, off_cur, off_key, off_t, off_omega, off_mats, off_D, off_logdet, off_info,
-      off_steps, off_tab, off_ldp, off_failp, off_stage, off_skey, off_st, off_slices, bytes;
+      off_steps, off_tab, off_ldp, off_failp, off_stage, off_skey, off_st, off_slices, off_fsync, off_fqoff, off_ftasks, bytes;
 };
 
 inline BatchPlan plan_batch(int64_t M, int64_t B, int64_t n, const int64_t* S_host) {
@@ -1693,7 +1947,11 @@ inline BatchPlan plan_batch(int64_t M, int64_t B, int64_t n, const int64_t* S_ho
     p.maxsteps = std::max<int64_t>(p.maxsteps, ceil_div(S_host[b], (int64_t)NB));
     p.n_entries += 2 * ceil_div(S_host[b], (int64_t)NB);
     p.nsl += ceil_div(S_host[b], (int64_t)64);
+    // the persistent schedule's tasks of the sample's two matrices: at most the 32-row tasks of every step
+    for (int64_t k0 = 0; k0 < S_host[b]; k0 += NB)
+      p.max_flow += 2 * std::max<int64_t>(1, ceil_div(S_host[b] - std::min<int64_t>(S_host[b], k0 + NB), (int64_t)NB));
   }
+  p.max_flow += 2 * n;  // the first diagonal blocks, one task per matrix
   p.omega_elems = M * p.sumS;
   p.csc_ptr_elems = p.sumS + n;
   p.csc_ent_elems = n * M * B;
@@ -1722,6 +1980,9 @@ inline BatchPlan plan_batch(int64_t M, int64_t B, int64_t n, const int64_t* S_ho
   p.off_skey = put(sizeof(int) * p.csc_ent_elems);       // the buckets in slice order
   p.off_st = put(sizeof(double) * p.csc_ent_elems);
   p.off_slices = put(sizeof(int2) * p.nsl);
+  p.off_fsync = put(sizeof(int) * (FLOW_SYNC_HEAD + 2 * n));
+  p.off_fqoff = put(sizeof(int) * (FLOW_QUEUES + 1 + 2 * n));  // queue offsets, then task totals per matrix
+  p.off_ftasks = put(sizeof(FlowTask) * p.max_flow);
   p.bytes = align_up(o, 256);
   return p;
 }
@@ -1736,9 +1997,11 @@ namespace {
 
 // How the batched Cholesky runs (process-wide, aj_log_evidence_configure): three launches per 32-column
 // step; one cooperative launch (without or with lookahead); one launch per step with lookahead; one
-// left-looking launch per step (the default).  All give the same factor bit for bit.
-constexpr int CHOL_STEPS = 0, CHOL_COOP = 1, CHOL_COOP_LOOKAHEAD = 2, CHOL_STEPS_LOOKAHEAD = 3, CHOL_LEFT = 4;
-constexpr int CHOL_MODES = 5;
+// left-looking launch per step (the default); the left-looking tasks in one persistent launch.  All give
+// the same factor bit for bit.
+constexpr int CHOL_STEPS = 0, CHOL_COOP = 1, CHOL_COOP_LOOKAHEAD = 2, CHOL_STEPS_LOOKAHEAD = 3, CHOL_LEFT = 4,
+              CHOL_FLOW = 5;
+constexpr int CHOL_MODES = 6;
 std::atomic<int> g_chol_mode{CHOL_LEFT};
 int chol_mode() { return g_chol_mode.load(std::memory_order_relaxed); }
 
@@ -1763,6 +2026,41 @@ int coop_grid() {
   }
   return cache[dev];
 }
+
+// Compute units of the current device (256 when the query fails).
+int cu_count() {
+  int dev = 0, c = 0;
+  if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+      c > 0)
+    return c;
+  return 256;
+}
+
+// Resident workgroups of the persistent Cholesky per compute unit (at least one).
+int flow_occupancy() {
+  static std::atomic<int> cached{0};
+  int v = cached.load(std::memory_order_relaxed);
+  if (v == 0) {
+    int per = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)chol_flow_kernel, FLOW_THREADS, 0) != hipSuccess ||
+        per < 1)
+      per = 1;
+    cached.store(per, std::memory_order_relaxed);
+    v = per;
+  }
+  return v;
+}
+
+// The persistent schedule's dependency wait gives up after this many polls (each a short sleep and an
+// L2 read, ~1 us): a quarter of a second, ~100 times the factorisation of a whole batch.
+constexpr int FLOW_SPIN_LIMIT = 1 << 18;
+std::atomic<int> g_flow_spin_limit{FLOW_SPIN_LIMIT};
+// A block column runs as 32-row tasks once its 32-row tasks number at most this many per compute unit.
+constexpr int FLOW_SMALL_PER_CU = 4;
+std::atomic<int> g_flow_small_per_cu{FLOW_SMALL_PER_CU};
+// Optional task trace of the persistent schedule (eight int64 per task, aj_log_evidence_flow_trace).
+std::atomic<long long*> g_flow_trace{nullptr};
+std::atomic<int64_t> g_flow_trace_cap{0};
 
 // Runs the whole per-sample pipeline for a batch.  Returns the workspace
 // views through `p`; outputs per-sample log-evidence into logl.
@@ -1823,8 +2121,11 @@ int batch_pipeline(const WSrc w_tilde, int64_t M, const double* dirty_image, con
   // 1: mapping inversion and data vector, one workgroup per sample
   AJ_CHECK_HIP(hipFuncSetAttribute((const void*)prep_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)(sizeof(int) * (maxS + 1))));
+  const int mode = chol_mode();
+  int* fsync = reinterpret_cast<int*>(ws + p.off_fsync);
   hipLaunchKernelGGL(prep_kernel, dim3((unsigned)n), dim3(PREP_THREADS), sizeof(int) * (maxS + 1), st, desc,
-                     pix_indexes, pix_weights, M, B, dirty_image, ptrs, ekey, et, D, skey, st_t);
+                     pix_indexes, pix_weights, M, B, dirty_image, ptrs, ekey, et, D, skey, st_t,
+                     mode == CHOL_FLOW ? fsync : nullptr);
   AJ_LAUNCH_CHECK();
   // 2: Omega = W T, W read once for the batch; rows of W per workgroup: the largest power of two <= 8
   // whose rows fit in LDS
@@ -1896,9 +2197,9 @@ int batch_pipeline(const WSrc w_tilde, int64_t M, const double* dirty_image, con
   double* stage = reinterpret_cast<double*>(ws + p.off_stage);
   AJ_CHECK_HIP(hipMemcpyAsync(steps, he.data(), sizeof(StepEntry) * he.size(), hipMemcpyHostToDevice, st));
   AJ_CHECK_HIP(hipMemcpyAsync(step_tab, tab.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice, st));
-  const int mode = chol_mode();
   timer_begin(AJ_TIMER_CHOL, st);
   bool done = false;
+  bool solve_finishes = false;
   if (mode == CHOL_COOP || mode == CHOL_COOP_LOOKAHEAD) {
     const int G = coop_grid();
     if (G > 0) {
@@ -1912,9 +2213,85 @@ int batch_pipeline(const WSrc w_tilde, int64_t M, const double* dirty_image, con
       done = true;
     }
   }
+  if (!done && mode == CHOL_FLOW) {
+    // matrices to XCDs: in decreasing size, each to the XCD with the least work so far (S^3 each), so that
+    // the largest matrices, whose chains of block columns are the longest, land on different XCDs
+    const int cus = cu_count();
+    const int64_t per_xcd = std::max(1, cus / FLOW_QUEUES);
+    static thread_local std::vector<int> xcd, need;
+    static thread_local std::vector<FlowTask> lists[FLOW_QUEUES];
+    xcd.assign(2 * n, 0);
+    need.assign(2 * n, 1);
+    double work[FLOW_QUEUES] = {};
+    for (int64_t q = 0; q < 2 * n; ++q) {
+      const int j = 2 * ho[q >> 1] + (int)(q & 1);
+      int x = 0;
+      for (int y = 1; y < FLOW_QUEUES; ++y)
+        if (work[y] < work[x]) x = y;
+      xcd[j] = x;
+      const double S = (double)S_host[j >> 1];
+      work[x] += S * S * S;
+    }
+    // each XCD's list: its matrices' first diagonal blocks, then per block column k their tasks (matrices in
+    // decreasing size): 256-row tasks, or 32-row tasks once the XCD's 32-row tasks of the column number at
+    // most small_per_cu per compute unit; a task needs all tasks of its matrix in the earlier block columns
+    for (auto& l : lists) l.clear();
+    for (int64_t q = 0; q < 2 * n; ++q) {
+      const int j = 2 * ho[q >> 1] + (int)(q & 1);
+      lists[xcd[j]].push_back(FlowTask{j, 0, 0, 1, 0});
+    }
+    const int64_t small_per = per_xcd * g_flow_small_per_cu.load(std::memory_order_relaxed);
+    for (int64_t k = 0; k < p.maxsteps; ++k) {
+      const int64_t k0 = k * NB;
+      int64_t nsmall[FLOW_QUEUES] = {};
+      for (int64_t j = 0; j < 2 * n; ++j) {
+        const int64_t S = S_host[j >> 1];
+        if (S > k0) nsmall[xcd[j]] += std::max<int64_t>(1, ceil_div(S - std::min<int64_t>(S, k0 + NB), (int64_t)NB));
+      }
+      for (int64_t q = 0; q < 2 * n; ++q) {
+        const int j = 2 * ho[q >> 1] + (int)(q & 1);
+        const int64_t S = S_host[j >> 1];
+        if (S <= k0) continue;
+        const bool small = nsmall[xcd[j]] <= small_per;
+        const int64_t below = S - std::min<int64_t>(S, k0 + NB);
+        const int64_t nt = small ? std::max<int64_t>(1, ceil_div(below, (int64_t)NB))
+                                 : 1 + ceil_div(std::max<int64_t>(0, below - LeftShape<FL_STRIP>::STRIP0),
+                                                (int64_t)FL_STRIP);
+        for (int64_t rb = 0; rb < nt; ++rb)
+          lists[xcd[j]].push_back(FlowTask{j, small ? 2 : 1, (int)k, (int)rb, need[j]});
+        need[j] += (int)nt;
+      }
+    }
+    static thread_local std::vector<FlowTask> hf;
+    static thread_local std::vector<int> hq;
+    hf.clear();
+    hq.assign(FLOW_QUEUES + 1 + 2 * n, 0);
+    for (int x = 0; x < FLOW_QUEUES; ++x) {
+      hq[x] = (int)hf.size();
+      hf.insert(hf.end(), lists[x].begin(), lists[x].end());
+    }
+    hq[FLOW_QUEUES] = (int)hf.size();
+    for (int64_t j = 0; j < 2 * n; ++j) hq[FLOW_QUEUES + 1 + j] = need[j];  // each matrix's task total
+    AJ_REQUIRE((int64_t)hf.size() <= p.max_flow && hf.size() < (1u << 30), "internal: persistent task list %zu > %lld",
+               hf.size(), (long long)p.max_flow);
+    FlowTask* ftasks = reinterpret_cast<FlowTask*>(ws + p.off_ftasks);
+    int* fqoff = reinterpret_cast<int*>(ws + p.off_fqoff);
+    AJ_CHECK_HIP(hipMemcpyAsync(ftasks, hf.data(), sizeof(FlowTask) * hf.size(), hipMemcpyHostToDevice, st));
+    AJ_CHECK_HIP(hipMemcpyAsync(fqoff, hq.data(), sizeof(int) * hq.size(), hipMemcpyHostToDevice, st));
+    const int ntasks = (int)hf.size();
+    const int grid = cus * flow_occupancy();
+    long long* trace = g_flow_trace.load(std::memory_order_relaxed);
+    if (g_flow_trace_cap.load(std::memory_order_relaxed) < ntasks) trace = nullptr;
+    hipLaunchKernelGGL(chol_flow_kernel, dim3((unsigned)grid), dim3(FLOW_THREADS), 0, st, desc, ftasks, fqoff,
+                       (int)p.maxsteps, mats, ldp, failp, stage, fsync,
+                       g_flow_spin_limit.load(std::memory_order_relaxed), trace);
+    AJ_LAUNCH_CHECK();
+    solve_finishes = true;
+    done = true;
+  }
   if (!done && mode == CHOL_LEFT) {
     // per step, the matrices with columns left, each with its strip tasks: 128-row tasks (the first
-    // LL_STRIP0 rows), or, once the 32-row tasks of a step fit the compute units one workgroup each,
+    // task 96 rows), or, once the 32-row tasks of a step fit the compute units one workgroup each,
     // 32-row tasks (a step's time is then its tasks' chains of blocks, which they run four times faster)
     static int cus = 0;
     if (cus == 0) {
@@ -1947,7 +2324,8 @@ int batch_pipeline(const WSrc w_tilde, int64_t M, const double* dirty_image, con
         if (S <= k0) continue;
         const int64_t below = S - std::min<int64_t>(S, k0 + NB);
         const int64_t nt = small ? std::max<int64_t>(1, ceil_div(below, (int64_t)NB))
-                                 : 1 + ceil_div(std::max<int64_t>(0, below - LL_STRIP0), (int64_t)LL_STRIP);
+                                 : 1 + ceil_div(std::max<int64_t>(0, below - LeftShape<LL_STRIP>::STRIP0),
+                                                (int64_t)LL_STRIP);
         AJ_REQUIRE(nt4[k] + nt < (1ll << 30), "batch too large for one Cholesky step grid");
         h4.push_back(StepEntry{j, (int)nt4[k], 0});
         nt4[k] += nt;
@@ -2051,7 +2429,9 @@ int batch_pipeline(const WSrc w_tilde, int64_t M, const double* dirty_image, con
   AJ_CHECK_HIP(hipFuncSetAttribute((const void*)solve_loglike_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)(sizeof(double) * maxS)));
   hipLaunchKernelGGL(solve_loglike_kernel, dim3((unsigned)n), dim3(SOLVE_THREADS), sizeof(double) * maxS, st, desc, mats, D,
-                     chi2_nn, logdet, recon, logl);
+                     chi2_nn, logdet, recon, logl, solve_finishes ? ldp : nullptr, failp, (int)p.maxsteps, info,
+                     solve_finishes ? fsync : nullptr,
+                     reinterpret_cast<const int*>(ws + p.off_fqoff) + FLOW_QUEUES + 1);
   timer_end(AJ_TIMER_SOLVE, st);
   timer_end(AJ_TIMER_BATCH, st);
   AJ_LAUNCH_CHECK();
@@ -3226,6 +3606,23 @@ extern "C" int aj_barycentric_vjp(const double* vertices, const double* position
   hipLaunchKernelGGL(vertex_reduce_kernel, dim3((unsigned)ceil_div(S, 256)), dim3(256), 0, st, ptrs, ekey, contrib, S,
                      dvertices);
   AJ_LAUNCH_CHECK();
+  return AJ_OK;
+}
+
+extern "C" int aj_log_evidence_flow_configure(int small_per_cu, int spin_limit) {
+  AJ_REQUIRE(small_per_cu >= -1 && small_per_cu <= (1 << 20), "small_per_cu %d: -1 (default) or 0 .. 2^20",
+             small_per_cu);
+  AJ_REQUIRE(spin_limit >= -1, "spin_limit %d: -1 (default) or >= 0", spin_limit);
+  g_flow_small_per_cu.store(small_per_cu < 0 ? FLOW_SMALL_PER_CU : small_per_cu, std::memory_order_relaxed);
+  g_flow_spin_limit.store(spin_limit < 0 ? FLOW_SPIN_LIMIT : spin_limit, std::memory_order_relaxed);
+  return AJ_OK;
+}
+
+extern "C" int aj_log_evidence_flow_trace(void* device_buffer, int64_t capacity_tasks) {
+  AJ_REQUIRE(capacity_tasks >= 0 && (capacity_tasks == 0 || device_buffer != nullptr),
+             "trace buffer: a device buffer of capacity_tasks x 8 int64, or null and 0");
+  g_flow_trace.store(capacity_tasks > 0 ? static_cast<long long*>(device_buffer) : nullptr, std::memory_order_relaxed);
+  g_flow_trace_cap.store(capacity_tasks, std::memory_order_relaxed);
   return AJ_OK;
 }
 

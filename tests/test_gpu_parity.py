@@ -455,7 +455,7 @@ def test_batched_log_evidence_full_size_vs_oracle(gpu):
 
     ref = (out["fp64"].copy(), batch.logdets.cpu().numpy().copy(), batch.recon.cpu().numpy().copy())
     try:
-        for mode in (0, 3):
+        for mode in (0, 3, 5):
             _lib.call("aj_log_evidence_configure", mode)
             le = batch.log_evidence(W64, y, c2nn, check=True).cpu().numpy()
             for a, b in zip((le, batch.logdets.cpu().numpy(), batch.recon.cpu().numpy()), ref):
@@ -542,10 +542,12 @@ def test_batched_log_evidence_vs_oracle(gpu, delaunay_small):
 
 
 def test_batched_cholesky_modes_bitwise_equal(gpu, delaunay_small):
-    """The five ways the batch factors its matrices (three launches per step; one cooperative launch,
-    without and with lookahead; one launch per step with lookahead; left-looking, one launch per step) give
-    bitwise the same log-evidence, log-determinants and reconstructions, also with samples ending inside a
-    step's block (S 97, 31, 300) and one whose block columns take three 224-row strips (S 700)."""
+    """The six ways the batch factors its matrices (three launches per step; one cooperative launch,
+    without and with lookahead; one launch per step with lookahead; left-looking, one launch per step; the
+    left-looking tasks in one persistent launch, with its default switch to 32-row tasks, with 256-row tasks
+    in every block column and with 32-row tasks in every block column) give bitwise the same log-evidence,
+    log-determinants and reconstructions, also with samples ending inside a step's block (S 97, 31, 300) and
+    one whose block columns take several strips (S 700)."""
     import torch
 
     from autojax_amd import _lib
@@ -559,14 +561,55 @@ def test_batched_cholesky_modes_bitwise_equal(gpu, delaunay_small):
              for i, S in enumerate((97, 31, 300, 700))]
     batch = E.SampleBatch(d.samples + extra, d.M)
     out = {}
+    runs = [(m, -1) for m in range(6)] + [(5, 0), (5, 1 << 20)]
     try:
-        for mode in (0, 1, 2, 3, 4):
+        for mode, small in runs:
             _lib.call("aj_log_evidence_configure", mode)
-            le = batch.log_evidence(Wd, y, c2nn, check=True).clone()
-            out[mode] = (le.cpu().numpy(), batch.logdets.cpu().numpy().copy(), batch.recon.cpu().numpy().copy())
+            _lib.call("aj_log_evidence_flow_configure", small, -1)
+            for rep in range(2 if mode == 5 else 1):
+                le = batch.log_evidence(Wd, y, c2nn, check=True).clone()
+                out[(mode, small, rep)] = (le.cpu().numpy(), batch.logdets.cpu().numpy().copy(),
+                                           batch.recon.cpu().numpy().copy())
     finally:
         _lib.call("aj_log_evidence_configure", -1)
-    for mode in (1, 2, 3, 4):
-        for a, b in zip(out[mode], out[0]):
-            np.testing.assert_array_equal(a, b)
-    print("cholesky modes bitwise equal:", out[0][0][:4], flush=True)
+        _lib.call("aj_log_evidence_flow_configure", -1, -1)
+    ref = out[(0, -1, 0)]
+    for key, got in out.items():
+        for a, b in zip(got, ref):
+            np.testing.assert_array_equal(a, b, err_msg=f"schedule {key}")
+    print("cholesky modes bitwise equal:", ref[0][:4], flush=True)
+
+
+def test_persistent_cholesky_wait_limit_reports_failure(gpu, delaunay_small):
+    """The persistent schedule's dependency wait is bounded: with a spin limit of zero polls a task that
+    finds its dependencies incomplete gives up at once, sets the error flag and the batch raises
+    RuntimeError instead of returning numbers; if no task had to wait, the results are those of the
+    default schedule.  Every kernel still exits normally (the limit is set from the host)."""
+    import torch
+
+    from autojax_amd import _lib
+    from autojax_amd import evidence as E
+
+    d, W = delaunay_small
+    Wd = torch.from_numpy(W).cuda()
+    y = torch.from_numpy(d.dirty_image).cuda()
+    c2nn = E.chi2_noise_norm(d.data, d.noise_map)
+    extra = [ds.small_delaunay(90 + i, d.M, 10, S, lam_range=(0.1, 10.0)).samples[0] for i, S in enumerate((700, 650))]
+    batch = E.SampleBatch(d.samples + extra, d.M)
+    ref = batch.log_evidence(Wd, y, c2nn, check=True).cpu().numpy()
+    try:
+        _lib.call("aj_log_evidence_configure", 5)
+        _lib.call("aj_log_evidence_flow_configure", -1, 0)
+        try:
+            got = batch.log_evidence(Wd, y, c2nn, check=True).cpu().numpy()
+        except RuntimeError as e:
+            print("wait limit reported:", e, flush=True)
+            assert bool((batch.info == -1).all())
+        else:
+            print("no task had to wait", flush=True)
+            np.testing.assert_array_equal(got, ref)
+        _lib.call("aj_log_evidence_flow_configure", -1, -1)
+        np.testing.assert_array_equal(batch.log_evidence(Wd, y, c2nn, check=True).cpu().numpy(), ref)
+    finally:
+        _lib.call("aj_log_evidence_configure", -1)
+        _lib.call("aj_log_evidence_flow_configure", -1, -1)
