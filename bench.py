@@ -199,7 +199,7 @@ def main():
 
     # Cholesky schedule of the batched evaluation, tuned on the 64-sample batch and, for the plain path's
     # one-sample evaluations, on its median-size sample (each path then runs with its own)
-    sched = {"batch": 4, "single": 4}
+    sched = {"batch": "5", "single": "5"}
     tuned_batch, tuned_single = {}, {}
     if not args.no_autotune:
         E.w_tilde(noise, uv, grid, "fp64", out=W_plain, workspace=ws_plain)
@@ -211,12 +211,12 @@ def main():
             "; single: " + ", ".join(f"{m}={t:.2f}ms" for m, t in sorted(tuned_single.items())))
 
     def mixed_step():
-        _lib.call("aj_log_evidence_configure", sched["batch"])
+        autotune.set_schedule(sched["batch"])
         E.w_tilde(noise, uv, grid, args.precision, out=W_mixed, workspace=ws_mixed)
         return batch.log_evidence(W_mixed, y, c2nn)
 
     def plain_step():
-        _lib.call("aj_log_evidence_configure", sched["single"])
+        autotune.set_schedule(sched["single"])
         E.w_tilde(noise, uv, grid, "fp64", out=W_plain, workspace=ws_plain)
         return torch.cat([s.log_evidence(W_plain, y, c2nn).clone() for s in singles])
 
@@ -246,7 +246,7 @@ def main():
     ws_wc = dv.workspace(_lib.query("aj_w_compact_workspace_bytes", K, N))
 
     def lattice_step():
-        _lib.call("aj_log_evidence_configure", sched["batch"])
+        autotune.set_schedule(sched["batch"])
         _lib.call("aj_w_compact_curvature_interferometer_from", noise.data_ptr(), uv.data_ptr(), K, N,
                   float(ds.pixel_scale), wc.data_ptr(), ws_wc.data_ptr(), ws_wc.numel(), dv.stream_ptr())
         return batch.log_evidence_compact(wc, native, y, c2nn)

@@ -303,27 +303,29 @@ int aj_log_evidence_batched(const double* w_tilde, int64_t M, const double* dirt
  * 0 = three launches per 32-column step (diagonal blocks, panels, trailing tiles); 1 = the same steps in one
  * cooperative launch with a grid-wide barrier between phases; 2 = 1 with lookahead (the next diagonal block
  * factored by the workgroup that updates it); 3 = one launch per step with lookahead (the next step's
- * diagonal block and panel updated, factored and solved beside the step's trailing update); 4 (default,
- * also -1) = left-looking, one launch per 32-column block after one that factors every first diagonal
- * block (each strip of the block updated from all earlier blocks in registers and solved, the next
- * diagonal block formed and factored by the block's first task; no trailing matrix written between steps);
- * 5 = the left-looking tasks in one persistent launch (one workgroup per compute unit taking tasks from a
- * queue, each waiting on its matrix's count of completed tasks of the earlier block columns).
- * All give bitwise the same factors. */
+ * diagonal block and panel updated, factored and solved beside the step's trailing update); 4 = left-looking,
+ * one launch per 32-column block after one that factors every first diagonal block (each strip of the block
+ * updated from all earlier blocks in registers and solved, the next diagonal block formed and factored by
+ * the block's first task; no trailing matrix written between steps); 5 (default, also -1) = the left-looking
+ * tasks in one persistent launch (a workgroup per compute unit taking tasks from its XCD's queue, each
+ * waiting on its matrix's count of completed tasks of the earlier block columns, with lookahead: a task
+ * updates its strip by all but the last earlier block before the previous column is done).  All give bitwise
+ * the same factors. */
 int aj_log_evidence_configure(int chol_mode);
 
 /* Tuning of schedule 5 (process-wide, no reference counterpart): a block column runs as 32-row tasks once
  * its 32-row tasks on an XCD number at most small_per_cu per compute unit of the XCD (256-row tasks before;
- * -1 = default 4);
- * a dependency wait gives up after spin_limit polls (-1 = default 2^18, ~0.25 s), after which the batch's
- * info codes read -1 (the evaluation is reported failed, never returned). */
+ * -1 = default 1); a dependency wait gives up after spin_limit polls (-1 = default 2^18, ~0.25 s), after
+ * which the batch's info codes read -1 (the evaluation is reported failed, never returned). */
 int aj_log_evidence_flow_configure(int small_per_cu, int spin_limit);
 
 /* Task trace of schedule 5 (process-wide, a tracing aid with no reference counterpart): while set, every
- * batch whose task list fits capacity_tasks writes per task t, at device_buffer[8 t ..], eight int64: the
- * wall-clock ticks (100 MHz) when it was taken, when its wait ended and when it was counted complete, its
- * matrix (2 sample + 0 for A, 1 for H), kind (0 first diagonal blocks, 1 a 256-row task, 2 a 32-row task),
- * block column, task within the column and the compute unit's hardware id.  Null and 0 switch it off. */
+ * batch whose task list fits capacity_tasks writes per task t, at device_buffer[16 t ..], sixteen int64: the
+ * wall-clock ticks (100 MHz) when it was taken, when its first wait ended and when it was counted complete,
+ * its matrix (2 sample + 0 for A, 1 for H), kind (0 its first diagonal block, 1 a 256-row task, 2 a 32-row
+ * task), block column, task within the column, the compute unit's hardware id; then for 256-row tasks the
+ * ticks when its first blocks' loads were issued, when its second wait began and ended, when its update
+ * ended and when its substitution ended (slots 8-12; the others are left as they were). */
 int aj_log_evidence_flow_trace(void* device_buffer, int64_t capacity_tasks);
 
 /* aj_log_evidence_batched with W-tilde given by its lattice table (numba.py:860-967 per sample):

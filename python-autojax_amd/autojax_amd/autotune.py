@@ -85,40 +85,66 @@ def tune_w_tilde(grid_radians_slim, precision: str, chunks: int = 3, reps: int =
 
 CHOL_SCHEDULES = {0: "three launches per step", 1: "cooperative", 2: "cooperative, lookahead",
                   3: "one launch per step, lookahead", 4: "left-looking, one launch per step",
-                  5: "left-looking, one persistent launch"}
+                  5: "left-looking, one persistent launch with lookahead"}
+# schedule 5's switch to 32-row tasks (aj_log_evidence_flow_configure small_per_cu) tried by the tuner
+FLOW_SMALL = (1, 2)
 _BATCH_CACHE: dict = {}
+
+
+def _set_schedule(choice):
+    mode, small = choice
+    _lib.call("aj_log_evidence_configure", mode)
+    _lib.call("aj_log_evidence_flow_configure", small, -1)
+
+
+def set_schedule(name) -> None:
+    """Select a schedule by its ``tune_batch`` key ("4", "5/s2", or an int mode) process-wide."""
+    text = str(name)
+    if "/s" in text:
+        mode, small = text.split("/s")
+        _set_schedule((int(mode), int(small)))
+    else:
+        _set_schedule((int(text), -1))
 
 
 def tune_batch(batch, w_tilde, dirty_image, chi2_noise_norm, reps: int = 3, verbose: bool = False):
     """Time every Cholesky schedule of the batched log-evidence (``aj_log_evidence_configure``: the same
     factors bit for bit, in 3 launches per 32-column step, one cooperative launch without or with lookahead,
-    or one launch per step with lookahead) on this batch, whose matrix sizes decide which is fastest (how
-    many steps, how much trailing work per step); select the fastest process-wide and return
-    {schedule: ms per evaluation}.  Cached per (M, sorted source-pixel counts) for the life of the process.
+    one launch per step with lookahead, one left-looking launch per step, or the left-looking tasks in one
+    persistent launch, the last with each switch value of ``FLOW_SMALL``) on this batch, whose matrix sizes
+    decide which is fastest; select the fastest process-wide and return {schedule: ms per evaluation} (keys
+    "4" or "5/s1").  Cached per (M, sorted source-pixel counts) for the life of the process.
     """
     import torch
 
     key = (batch.M, tuple(sorted(int(s) for s in batch.S_host)))
     if key in _BATCH_CACHE:
-        _lib.call("aj_log_evidence_configure", _BATCH_CACHE[key][0])
+        _set_schedule(_BATCH_CACHE[key][0])
         return _BATCH_CACHE[key][1]
+    choices = [(m, -1) for m in CHOL_SCHEDULES if m != 5] + [(5, s) for s in FLOW_SMALL]
     times = {}
-    for mode in CHOL_SCHEDULES:
-        _lib.call("aj_log_evidence_configure", mode)
-        batch.log_evidence(w_tilde, dirty_image, chi2_noise_norm)
-        torch.cuda.synchronize()
-        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        a.record()
-        for _ in range(reps):
-            batch.log_evidence(w_tilde, dirty_image, chi2_noise_norm)
-        b.record()
-        torch.cuda.synchronize()
-        times[mode] = a.elapsed_time(b) / reps
-        if verbose:
-            print(f"autotune batch n={batch.n}: schedule {mode} ({CHOL_SCHEDULES[mode]}) {times[mode]:.3f} ms",
-                  flush=True)
-    best = min(times, key=times.get)
-    _lib.call("aj_log_evidence_configure", best)
+    best, best_ms = None, None
+    try:
+        for choice in choices:
+            _set_schedule(choice)
+            batch.log_evidence(w_tilde, dirty_image, chi2_noise_norm, check=True)
+            torch.cuda.synchronize()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            for _ in range(reps):
+                batch.log_evidence(w_tilde, dirty_image, chi2_noise_norm)
+            b.record()
+            torch.cuda.synchronize()
+            ms = a.elapsed_time(b) / reps
+            name = str(choice[0]) if choice[0] != 5 else f"5/s{choice[1]}"
+            times[name] = ms
+            if verbose:
+                print(f"autotune batch n={batch.n}: schedule {name} ({CHOL_SCHEDULES[choice[0]]}) {ms:.3f} ms",
+                      flush=True)
+            if best_ms is None or ms < best_ms:
+                best, best_ms = choice, ms
+    finally:
+        _set_schedule(best if best is not None else (-1, -1))
     _BATCH_CACHE[key] = (best, times)
     return times
 

@@ -57,6 +57,22 @@ def flow_summary(rec):
     last = rec[np.argmax(rec[:, 2])]
     mm = rec[rec[:, 3] == last[3]]
     ks = np.unique(mm[:, 5])
+    # 256-row tasks' phases (slots 8-12): setup, blocks 0 .. k-2, second wait, last block, substitution, rest
+    m = (rec[:, 4] == 1) & (rec[:, 8] > 0) & (rec[:, 12] > 0)
+    if m.any():
+        r = rec[m]
+        k = r[:, 5]
+        two = k >= 1
+        ph = {"setup": (r[:, 8] - r[:, 1]), "solve": (r[:, 12] - r[:, 11]), "store_rest": (r[:, 2] - r[:, 12])}
+        ph["update"] = np.where(two, r[:, 9] - r[:, 8], r[:, 11] - r[:, 8])
+        ph["wait2"] = np.where(two, r[:, 10] - r[:, 9], 0)
+        ph["last_block"] = np.where(two, r[:, 11] - r[:, 10], 0)
+        first = r[:, 6] == 0
+        out["strip256_phases_us"] = {
+            name: {"others": round(float(v[~first].mean() / 100.0), 2) if (~first).any() else None,
+                   "first": round(float(v[first].mean() / 100.0), 2) if first.any() else None}
+            for name, v in ph.items()}
+        out["strip256_update_us_per_block"] = round(float((ph["update"][two & ~first] / np.maximum(k[two & ~first] - 1, 1)).mean() / 100.0), 3) if (two & ~first).any() else None
     out["last_matrix"] = {"matrix": int(last[3]), "columns": int(len(ks)),
                           "first_start_us": round(float((mm[:, 1].min() - t0) / 100.0), 1),
                           "column_end_us": [round(float((mm[mm[:, 5] == k][:, 2].max() - t0) / 100.0), 1)
@@ -168,7 +184,7 @@ def main(argv=None):
                               "stages_ms": stages}), flush=True)
             if mode == 5 and a.trace:
                 cap = 1 << 18
-                buf = torch.full((cap, 8), -1, dtype=torch.int64, device="cuda")
+                buf = torch.full((cap, 16), -1, dtype=torch.int64, device="cuda")
                 _lib.call("aj_log_evidence_flow_trace", buf.data_ptr(), cap)
                 try:
                     batch.log_evidence(W, y, c2nn)

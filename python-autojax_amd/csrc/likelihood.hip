@@ -1122,6 +1122,21 @@ struct LeftShape {
 };
 static_assert(LeftShape<LL_STRIP>::THREADS == LL_THREADS, "one-launch-per-step tasks: four waves");
 
+// x L^T = b for one row held in registers (x = b on entry), L the nb x nb lower-triangular block in LDS:
+// x_c = (b_c - sum_{l<c} x_l L_cl) / L_cc with the sum taken in l order, column by column.
+__device__ inline void solve_row_lkk(double (&x)[NB], int nb, const lds_f64 (*__restrict__ Lkk)[NB + 1]) {
+#pragma unroll
+  for (int c = 0; c < NB; ++c) {
+    if (c < nb) {
+      double v = x[c];
+#pragma unroll
+      for (int l = 0; l < c; ++l) v -= x[l] * Lkk[c][l];
+      x[c] = v / Lkk[c][c];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
 // Index (doubles) of element (r, c) in a staged block of NB-double rows whose 16-byte segments are stored
 // XOR (r mod 16), so that the 16 rows of a fragment read fall in 16 different segments.
 __device__ inline int ll_at(int r, int c) { return r * NB + ((((c >> 1) ^ (r & 15)) << 1) | (c & 1)); }
@@ -1131,14 +1146,30 @@ __device__ double raw_buffer_load_f64(i32x4 rsrc, int voffset, int soffset, int 
     __asm("llvm.amdgcn.raw.buffer.load.f64");
 constexpr int LL_OOB = 0x7ffff000;  // past any matrix's bytes (n <= SOLVE_S_MAX), also 47 blocks on
 
+// The persistent schedule's second wait of a task (lookahead): before its last block (p = k - 1) and the
+// diagonal block L_kk, both from block column k - 1, a task waits until matrix mat has `need` completed
+// tasks (all of columns <= k - 1); it starts once the columns <= k - 2 are complete.  flow_wait is defined
+// with the persistent kernel below.
+struct FlowWait {
+  int* sync;
+  int mat, need, spin_limit;
+  long long* tr;  // the task's trace record (phase times at tr[8 ..]), or null
+};
+// wave 0 records a phase time of the traced task (every lane the same value)
+#define AJ_FLOW_MARK(FW, SLOT)                                                          \
+  do {                                                                                  \
+    if ((FW).tr != nullptr && uni((int)(threadIdx.x >> 6)) == 0) (FW).tr[SLOT] = wall_clock64(); \
+  } while (0)
+__device__ void flow_wait(int* __restrict__ sync, int mat, int need, int spin_limit);
+
 // Task rb of block column k (k0 = 32 k) of matrix j; Ls (ROWS x LL_PITCH doubles), Lkk and colb (NB doubles)
-// in the caller's LDS; threadIdx.x < THREADS.
-template <int STRIP>
+// in the caller's LDS; threadIdx.x < THREADS.  LA: the persistent schedule's lookahead (fw).
+template <int STRIP, bool LA = false>
 __device__ __attribute__((always_inline)) inline void chol_left_task(const SampleDesc* __restrict__ sd, int j, int rb, int64_t k0, int kstep,
                                       int maxsteps, double* __restrict__ mats, double* __restrict__ ldp,
                                       int64_t* __restrict__ failp, double* __restrict__ stage,
                                       lds_f64* __restrict__ Ls, lds_f64 (*__restrict__ Lkk)[NB + 1],
-                                      lds_f64* __restrict__ colb) {
+                                      lds_f64* __restrict__ colb, FlowWait fw = FlowWait{}) {
   using SH = LeftShape<STRIP>;
   int64_t n;
   double* A = mat_ptr(sd, mats, j, n);
@@ -1151,14 +1182,18 @@ __device__ __attribute__((always_inline)) inline void chol_left_task(const Sampl
   const int lr = lane & 15, lk = lane >> 4;
   const double* stg_in = stage + ((int64_t)2 * j + (kstep & 1)) * NB * NB;
   double* stg_out = stage + ((int64_t)2 * j + ((kstep + 1) & 1)) * NB * NB;
-  // L_kk: factored in place by the first launch (step 0) or staged by the previous launch
-  for (int x = tid; x < NB * NB; x += SH::THREADS) {
-    const int i = x / NB, c = x % NB;
-    double v = 0.0;
-    if (i < nb && c <= i) v = kstep == 0 ? A[(k0 + i) * n + k0 + c] : stg_in[x];
-    Lkk[i][c] = v;
-    if (rb == 0 && kstep > 0 && i < nb && c <= i) A[(k0 + i) * n + k0 + c] = v;
-  }
+  // L_kk: factored in place by the first launch (step 0) or staged by the previous launch (with lookahead:
+  // read after the second wait)
+  auto load_lkk = [&]() {
+    for (int x = tid; x < NB * NB; x += SH::THREADS) {
+      const int i = x / NB, c = x % NB;
+      double v = 0.0;
+      if (i < nb && c <= i) v = kstep == 0 ? A[(k0 + i) * n + k0 + c] : stg_in[x];
+      Lkk[i][c] = v;
+      if (rb == 0 && kstep > 0 && i < nb && c <= i) A[(k0 + i) * n + k0 + c] = v;
+    }
+  };
+  if (!LA || kstep == 0) load_lkk();
   // wave w < WAVES - 1 (and the last wave when the task does not form the next block) owns strip rows
   // 32 w .. 32 w + 31; the last wave of a task that does forms block k + 1 from strip rows 0 .. 31 (its rows
   // k0 + 32 ..)
@@ -1197,14 +1232,28 @@ __device__ __attribute__((always_inline)) inline void chol_left_task(const Sampl
       voff[q] = ok ? (int)((grow * n + lc) * (int64_t)sizeof(double)) : LL_OOB;
     }
     double pre[SH::LOADS];
+    const int last = kstep - 1;
+    if (!LA || last > 0) {
 #pragma unroll
-    for (int q = 0; q < SH::LOADS; ++q) pre[q] = raw_buffer_load_f64(rsrc, voff[q], 0, 0);
+      for (int q = 0; q < SH::LOADS; ++q) pre[q] = raw_buffer_load_f64(rsrc, voff[q], 0, 0);
+    }
+    if (LA) AJ_FLOW_MARK(fw, 8);
     for (int p = 0; p < kstep; ++p) {
+      if (LA && p == last) {
+        // lookahead: block k - 1 and L_kk once block column k - 1 is complete
+        AJ_FLOW_MARK(fw, 9);
+        flow_wait(fw.sync, fw.mat, fw.need, fw.spin_limit);
+        AJ_FLOW_MARK(fw, 10);
+        load_lkk();
+        const int off = p * NB * (int)sizeof(double);
+#pragma unroll
+        for (int q = 0; q < SH::LOADS; ++q) pre[q] = raw_buffer_load_f64(rsrc, voff[q], off, 0);
+      }
       __syncthreads();  // every wave is past its fragment reads of block p - 1
 #pragma unroll
       for (int q = 0; q < SH::LOADS; ++q) Ls[(lr0 + SH::PASS * q) * LL_PITCH + lc] = pre[q];
       __syncthreads();
-      if (p + 1 < kstep) {
+      if (p + 1 < kstep && !(LA && p + 1 == last)) {
         const int off = (p + 1) * NB * (int)sizeof(double);
 #pragma unroll
         for (int q = 0; q < SH::LOADS; ++q) pre[q] = raw_buffer_load_f64(rsrc, voff[q], off, 0);
@@ -1228,6 +1277,7 @@ __device__ __attribute__((always_inline)) inline void chol_left_task(const Sampl
   }
   // the strip rows into Xs (the staging buffer, free after this barrier)
   lds_f64* Xs = Ls;
+  if (LA) AJ_FLOW_MARK(fw, 11);
   __syncthreads();
   if (!dwave) {
 #pragma unroll
@@ -1244,20 +1294,12 @@ __device__ __attribute__((always_inline)) inline void chol_left_task(const Sampl
     double x[NB];
 #pragma unroll
     for (int c = 0; c < NB; ++c) x[c] = xr[c];
-#pragma unroll
-    for (int c = 0; c < NB; ++c) {
-      if (c < nb) {
-        double v = x[c];
-#pragma unroll
-        for (int l = 0; l < c; ++l) v -= x[l] * Lkk[c][l];
-        x[c] = v / Lkk[c][c];
-      }
-      __builtin_amdgcn_sched_barrier(0);
-    }
+    solve_row_lkk(x, nb, Lkk);
 #pragma unroll
     for (int c = 0; c < NB; ++c) xr[c] = x[c];
   }
   __syncthreads();
+  if (LA) AJ_FLOW_MARK(fw, 12);
   for (int x = tid; x < rows * NB; x += SH::THREADS) {
     const int i = x / NB, c = x % NB;
     if (c < nb) A[(s0 + i) * n + k0 + c] = Xs[i * LL_PITCH + c];
@@ -1326,12 +1368,15 @@ constexpr int LS_LOADS = LS_ROWS / 4 / (LS_THREADS / 64);  // 1 KB loads per wav
 static_assert(LS_LOADS == 2 && LS_STAGES * LS_ROWS * NB * 8 + NB * (NB + 1) * 8 * 2 <= 160 * 1024, "small-task LDS");
 
 // Task rb of block column k (k0 = 32 k) of matrix j; the ring Ls (LS_STAGES blocks of LS_ROWS x NB doubles,
-// 1 KB aligned), Lkk, Dn and colb (NB doubles) in the caller's LDS; threadIdx.x < LS_THREADS.
+// 1 KB aligned), Lkk, Dn and colb (NB doubles) in the caller's LDS; threadIdx.x < LS_THREADS.  LA: the
+// persistent schedule's lookahead (fw).
+template <bool LA = false>
 __device__ __attribute__((always_inline)) inline void chol_small_task(const SampleDesc* __restrict__ sd, int j, int rb, int64_t k0, int kstep,
                                        int maxsteps, double* __restrict__ mats, double* __restrict__ ldp,
                                        int64_t* __restrict__ failp, double* __restrict__ stage,
                                        lds_f64* __restrict__ Ls, lds_f64 (*__restrict__ Lkk)[NB + 1],
-                                       lds_f64 (*__restrict__ Dn)[NB + 1], lds_f64* __restrict__ colb) {
+                                       lds_f64 (*__restrict__ Dn)[NB + 1], lds_f64* __restrict__ colb,
+                                       FlowWait fw = FlowWait{}) {
   int64_t n;
   double* A = mat_ptr(sd, mats, j, n);
   const int nb = (int)std::min<int64_t>(NB, n - k0);
@@ -1343,13 +1388,16 @@ __device__ __attribute__((always_inline)) inline void chol_small_task(const Samp
   const int lr = lane & 15, lk = lane >> 4;
   const double* stg_in = stage + ((int64_t)2 * j + (kstep & 1)) * NB * NB;
   double* stg_out = stage + ((int64_t)2 * j + ((kstep + 1) & 1)) * NB * NB;
-  for (int x = tid; x < NB * NB; x += LS_THREADS) {
-    const int i = x / NB, c = x % NB;
-    double v = 0.0;
-    if (i < nb && c <= i) v = kstep == 0 ? A[(k0 + i) * n + k0 + c] : stg_in[x];
-    Lkk[i][c] = v;
-    if (rb == 0 && kstep > 0 && i < nb && c <= i) A[(k0 + i) * n + k0 + c] = v;
-  }
+  auto load_lkk = [&]() {
+    for (int x = tid; x < NB * NB; x += LS_THREADS) {
+      const int i = x / NB, c = x % NB;
+      double v = 0.0;
+      if (i < nb && c <= i) v = kstep == 0 ? A[(k0 + i) * n + k0 + c] : stg_in[x];
+      Lkk[i][c] = v;
+      if (rb == 0 && kstep > 0 && i < nb && c <= i) A[(k0 + i) * n + k0 + c] = v;
+    }
+  };
+  if (!LA || kstep == 0) load_lkk();  // with lookahead: after the second wait
   // wave w < 4: strip block (w >> 1, w & 1); wave w >= 4: block ((w - 4) >> 1, (w - 4) & 1) of the next
   // diagonal block (its rows k0 + 32 .. are the strip's rows 0 .. 31; block (0, 1) is above the diagonal
   // and not used)
@@ -1381,18 +1429,29 @@ __device__ __attribute__((always_inline)) inline void chol_small_task(const Samp
       voff[jj] = ok ? (int)((grow * n + 2 * g) * (int64_t)sizeof(double)) : -1;
     }
     constexpr int LS_OOB = 0x7ffff000;  // past any matrix's bytes (n <= SOLVE_S_MAX)
-#define AJ_LS_ISSUE(P)                                                                                     \
+    // (with lookahead block k - 1 is issued as zeros here and loaded after the second wait)
+#define AJ_LS_ISSUE_IF(P, COND)                                                                            \
   {                                                                                                        \
-    lds_f64* lb_ = Ls + ((P) % LS_STAGES) * (LS_ROWS * NB);                                                                     \
+    lds_f64* lb_ = Ls + ((P) % LS_STAGES) * (LS_ROWS * NB);                                                \
     _Pragma("unroll") for (int jj = 0; jj < LS_LOADS; ++jj) raw_buffer_load_lds(                          \
         rsrc, (lds_u32*)(lb_ + (LS_LOADS * wave + jj) * 128), 16,                                          \
-        (P) < kstep && voff[jj] >= 0 ? voff[jj] + (P) * NB * (int)sizeof(double) : LS_OOB, 0, 0, 0);      \
+        (COND) && voff[jj] >= 0 ? voff[jj] + (P) * NB * (int)sizeof(double) : LS_OOB, 0, 0, 0);           \
   }
+#define AJ_LS_ISSUE(P) AJ_LS_ISSUE_IF(P, (P) < kstep && !(LA && (P) == kstep - 1))
     // branch-free: blocks past the last are issued as zeros, so each wait counts the same loads
 #pragma unroll
     for (int q = 0; q < LS_STAGES - 1; ++q) AJ_LS_ISSUE(q)
     const int ra = 16 * ba, rbb = dwave ? 16 * bb : NB + 16 * bb;
     for (int p = 0; p < kstep; ++p) {
+      if (LA && p == kstep - 1) {
+        // lookahead: every earlier load (with this block's zeros) landed; once block column k - 1 is
+        // complete, L_kk and this block
+        __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
+        flow_wait(fw.sync, fw.mat, fw.need, fw.spin_limit);
+        load_lkk();
+        AJ_LS_ISSUE_IF(p, true)
+        __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
+      }
       // block p has landed (blocks p + 1 .. p + 4 may be in flight); after the barrier every wave is past
       // its reads of block p - 1, whose buffer takes block p + 5
       __builtin_amdgcn_s_waitcnt(vmcnt_imm(LS_LOADS * (LS_STAGES - 2)));
@@ -1424,6 +1483,7 @@ __device__ __attribute__((always_inline)) inline void chol_small_task(const Samp
       for (int ks = 0; ks < NB / 4; ++ks) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(-fa[ks], fb[ks], acc, 0, 0, 0);
     }
 #undef AJ_LS_ISSUE
+#undef AJ_LS_ISSUE_IF
     // the loads issued past the last block land before the buffers are reused
     __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
   }
@@ -1445,16 +1505,7 @@ __device__ __attribute__((always_inline)) inline void chol_small_task(const Samp
     double x[NB];
 #pragma unroll
     for (int c = 0; c < NB; ++c) x[c] = xr[c];
-#pragma unroll
-    for (int c = 0; c < NB; ++c) {
-      if (c < nb) {
-        double v = x[c];
-#pragma unroll
-        for (int l = 0; l < c; ++l) v -= x[l] * Lkk[c][l];
-        x[c] = v / Lkk[c][c];
-      }
-      __builtin_amdgcn_sched_barrier(0);
-    }
+    solve_row_lkk(x, nb, Lkk);
 #pragma unroll
     for (int c = 0; c < NB; ++c) xr[c] = x[c];
   }
@@ -1507,32 +1558,37 @@ __global__ void __launch_bounds__(LS_THREADS) chol_left_small_kernel(const Sampl
   __shared__ double Dn[NB][NB + 1];
   __shared__ double colb[NB];
   const int ei = find_entry(e, ne, (int)blockIdx.x, false);
-  chol_small_task(sd, e[ei].mat, (int)blockIdx.x - e[ei].pfx_p, k0, kstep, maxsteps, mats, ldp, failp, stage,
+  chol_small_task<false>(sd, e[ei].mat, (int)blockIdx.x - e[ei].pfx_p, k0, kstep, maxsteps, mats, ldp, failp, stage,
                   (lds_f64*)Ls, (lds_f64(*)[NB + 1])Lkk, (lds_f64(*)[NB + 1])Dn, (lds_f64*)colb);
 }
 
 // Persistent form of the left-looking schedule (mode 5): the whole factorisation of every matrix in one
-// launch of one workgroup per compute unit.  Each matrix belongs to one XCD (the host balances their S^3
-// over the eight); each XCD has its own task list (its matrices' first diagonal blocks, then block column
-// 0's tasks of its matrices, column 1's, ...) and queue head (an atomic counter).  A workgroup takes the
-// next task of the queue of the XCD it runs on, waits until every task of that matrix in the earlier block
-// columns is complete (a per-matrix count of completed tasks), runs it and counts it complete.  A task
-// waits only for tasks earlier in its list, which running workgroups have taken, so every wait ends; the
-// wait is bounded all the same (a spin limit), past which the task runs, the error flag is set and the
-// solve reports the batch as failed (also when a matrix's count is short of its total, as it would be if
-// an XCD ran none of the workgroups).  The tasks are those of the one-launch-per-step form (256 strip
-// rows on eight waves, then, once an XCD's 32-row tasks of a column fit its compute units, 32-row tasks),
-// so each element sees the same operations in the same order: the factor is bitwise that of the other
-// schedules.  A matrix's data passes only through its XCD's L2: completion is published once every wave's
-// stores have completed into that L2 (before the workgroup's barrier), and observed by invalidating the
-// waiting CU's L1.
+// launch of one workgroup (eight waves) per compute unit.  Each matrix belongs to one XCD (the host
+// balances their S^3 over the eight); each XCD has its own task list (its matrices' first diagonal blocks,
+// then block column 0's tasks of its matrices, column 1's, ...) and queue head (an atomic counter).  A
+// workgroup takes the next task of the queue of the XCD it runs on, waits until the tasks of that matrix
+// in block columns <= k - 2 are complete (a per-matrix count of completed tasks), updates its strip by
+// blocks 0 .. k - 2, waits for column k - 1 (lookahead: a column's chain of earlier blocks overlaps the
+// previous column), applies block k - 1, solves and counts the task complete.  A task waits only for tasks
+// earlier in its list, which running workgroups have taken, so every wait ends; the wait is bounded all the
+// same (a spin limit), past which the task runs, the error flag is set and the solve reports the batch as
+// failed (also when a matrix's count is short of its total, as it would be if an XCD ran none of the
+// workgroups).  The tasks are those of the one-launch-per-step form (256 strip rows on eight waves, then,
+// once an XCD's 32-row tasks of a column fit its compute units, 32-row tasks), so each element sees the
+// same operations in the same order: the factor is bitwise that of the other schedules.  A matrix's data
+// passes only through its XCD's L2: completion is published once every wave's stores have completed into
+// that L2 (before the workgroup's barrier; no write-back beyond it), and observed with an agent-scope
+// acquire, which invalidates the waiting CU's L1.  The queue's single-thread steps run on the whole of
+// wave 0 (lane 0's value, atomics whose other lanes add 0), so the persistent loop has no divergent branch
+// next to its barriers.
 struct FlowTask {
-  int mat;   // matrix j (kind 0: the first of `rb` consecutive matrices)
-  int kind;  // 0: diagonal block 0 of matrices mat .. mat + rb - 1; 1: a 256-row task; 2: a 32-row task
-  int k;     // block column
-  int rb;    // task within the block column (kind 0: the number of matrices, at most eight; the host
-             // gives one)
-  int need;  // completed tasks of matrix mat required first: all of its tasks in earlier block columns
+  int mat;    // matrix j
+  int kind;   // 0: its first diagonal block; 1: a 256-row task of block column k; 2: a 32-row task
+  int k;      // block column
+  int rb;     // task within the block column
+  int need;   // completed tasks of matrix mat required before the task starts: its tasks in block columns
+              // <= k - 2 (its first diagonal block's task for k <= 1)
+  int need2;  // ... before its last block (lookahead): its tasks in block columns <= k - 1
 };
 constexpr int FL_STRIP = 256;
 constexpr int FLOW_THREADS = 512;
@@ -1543,6 +1599,28 @@ constexpr int FLOW_RING = LS_STAGES * LS_ROWS * NB > LeftShape<FL_STRIP>::ROWS *
 constexpr int FLOW_LDS_DOUBLES = FLOW_RING + 2 * NB * (NB + 1) + NB;
 static_assert(FLOW_LDS_DOUBLES * sizeof(double) <= 160 * 1024, "one workgroup per CU");
 
+// The waits of a task (every thread calls it; wave 0 polls, all its lanes reading the one counter): until
+// matrix mat has `need` completed tasks, at most spin_limit polls (then the error flag is set), then an
+// agent-scope acquire (the producers ran on this XCD: their stores are in its L2, and this CU's L1 may hold
+// lines read before); the barrier holds the other waves until then.
+__device__ void flow_wait(int* __restrict__ sync, int mat, int need, int spin_limit) {
+  need = uni(need);
+  if (uni((int)(threadIdx.x >> 6)) == 0 && need > 0) {
+    const int* cnt = &sync[FLOW_SYNC_HEAD + uni(mat)];
+    const int limit = uni(spin_limit);
+    int spins = 0;
+    while (uni(__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < need) {
+      if (++spins > limit) {
+        atomicOr(&sync[FLOW_ERR], (threadIdx.x & 63) == 0 ? 1 : 0);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+}
+
 // The three task kinds as separate functions, so that each keeps its own register allocation (inlined
 // together they spill).  Their global pointers arrive typed as global memory, so that the inlined task
 // bodies load and store with global (not flat) instructions.
@@ -1550,88 +1628,36 @@ typedef __attribute__((address_space(1))) double g_f64;
 typedef __attribute__((address_space(1))) int64_t g_i64;
 typedef __attribute__((address_space(1))) const SampleDesc g_desc;
 
-// A function's arguments arrive in vector registers; the callers pass wave-uniform values, read back as
-// such (uni) so that the bodies branch, loop and form buffer resources on scalars.
-
-__device__ __attribute__((noinline)) void flow_diag_task(g_desc* sd, int j0, int cnt, int maxsteps, g_f64* mats,
-                                                         g_f64* ldp, g_i64* failp) {
-  const int w = threadIdx.x >> 6;
-  if (w < uni(cnt))
-    chol_diag_dev((const SampleDesc*)uni(sd), uni(j0) + w, 0, 0, uni(maxsteps), (double*)uni(mats),
-                  (double*)uni(ldp), (int64_t*)uni(failp), threadIdx.x & 63);
+__device__ __attribute__((noinline)) void flow_diag_task(g_desc* sd, int j, int maxsteps, g_f64* mats, g_f64* ldp,
+                                                         g_i64* failp) {
+  if ((threadIdx.x >> 6) == 0)
+    chol_diag_dev((const SampleDesc*)uni(sd), uni(j), 0, 0, uni(maxsteps), (double*)uni(mats), (double*)uni(ldp),
+                  (int64_t*)uni(failp), threadIdx.x & 63);
 }
 
 __device__ __attribute__((noinline)) void flow_left_task(g_desc* sd, int j, int rb, int k, int maxsteps, g_f64* mats,
-                                                         g_f64* ldp, g_i64* failp, g_f64* stage, lds_f64* lds) {
+                                                         g_f64* ldp, g_i64* failp, g_f64* stage, lds_f64* lds,
+                                                         int* sync, int need2, int spin_limit, long long* tr) {
   k = uni(k);
-  chol_left_task<FL_STRIP>((const SampleDesc*)uni(sd), uni(j), uni(rb), (int64_t)k * NB, k, uni(maxsteps),
-                           (double*)uni(mats), (double*)uni(ldp), (int64_t*)uni(failp), (double*)uni(stage), lds,
-                           reinterpret_cast<lds_f64(*)[NB + 1]>(lds + FLOW_RING), lds + FLOW_RING + 2 * NB * (NB + 1));
+  j = uni(j);
+  chol_left_task<FL_STRIP, true>((const SampleDesc*)uni(sd), j, uni(rb), (int64_t)k * NB, k, uni(maxsteps),
+                                 (double*)uni(mats), (double*)uni(ldp), (int64_t*)uni(failp), (double*)uni(stage), lds,
+                                 reinterpret_cast<lds_f64(*)[NB + 1]>(lds + FLOW_RING),
+                                 lds + FLOW_RING + 2 * NB * (NB + 1),
+                                 FlowWait{uni(sync), j, uni(need2), uni(spin_limit), uni(tr)});
 }
 
 __device__ __attribute__((noinline)) void flow_small_task(g_desc* sd, int j, int rb, int k, int maxsteps, g_f64* mats,
-                                                          g_f64* ldp, g_i64* failp, g_f64* stage, lds_f64* lds) {
+                                                          g_f64* ldp, g_i64* failp, g_f64* stage, lds_f64* lds,
+                                                          int* sync, int need2, int spin_limit, long long* tr) {
   k = uni(k);
-  chol_small_task((const SampleDesc*)uni(sd), uni(j), uni(rb), (int64_t)k * NB, k, uni(maxsteps), (double*)uni(mats),
-                  (double*)uni(ldp), (int64_t*)uni(failp), (double*)uni(stage), lds,
-                  reinterpret_cast<lds_f64(*)[NB + 1]>(lds + FLOW_RING),
-                  reinterpret_cast<lds_f64(*)[NB + 1]>(lds + FLOW_RING + NB * (NB + 1)),
-                  lds + FLOW_RING + 2 * NB * (NB + 1));
-}
-
-// The queue's two synchronisation points, each called by every thread of the workgroup (uniform control
-// flow: the single-thread parts stay inside these functions, so the persistent loop around them has no
-// divergent branch next to its barriers).
-// flow_next: thread 0 counts the workgroup's previous task complete (mat .. mat + cnt - 1, none when cnt is
-// 0; every wave's stores completed, into this XCD's L2, before the caller's barrier) and takes the next task
-// of queue q, whose index in the queue every thread returns.
-// With a trace buffer it also records the previous task (task index prev_t): when it was taken, when its
-// wait ended, when it was counted complete (wall clock, 100 MHz), its matrix, kind, block column, task
-// within the column and the hardware id of the compute unit that ran it.
-__device__ __attribute__((noinline)) int flow_next(int* __restrict__ sync, int q, int prev_mat, int prev_cnt,
-                                                   int* cur, long long* __restrict__ trace,
-                                                   const FlowTask* __restrict__ tasks, int prev_t,
-                                                   long long prev_taken, long long prev_ready) {
-  if (threadIdx.x == 0) {
-    if (prev_cnt > 0) {
-      for (int w = 0; w < prev_cnt; ++w)
-        __hip_atomic_fetch_add(&sync[FLOW_SYNC_HEAD + prev_mat + w], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (trace != nullptr) {
-        const FlowTask tk = tasks[prev_t];
-        long long* r = trace + 8 * (int64_t)prev_t;
-        r[0] = prev_taken;
-        r[1] = prev_ready;
-        r[2] = wall_clock64();
-        r[3] = tk.mat;
-        r[4] = tk.kind;
-        r[5] = tk.k;
-        r[6] = tk.rb;
-        r[7] = __smid();
-      }
-    }
-    *cur = atomicAdd(&sync[q], 1);
-  }
-  __syncthreads();
-  return __builtin_amdgcn_readfirstlane(*cur);
-}
-
-// flow_wait: thread 0 waits (at most spin_limit polls, then sets the error flag) until matrix mat has `need`
-// completed tasks, then invalidates this CU's L1 (the producers ran on this XCD: their stores are in its
-// L2); the barrier holds the other threads until then.
-__device__ __attribute__((noinline)) void flow_wait(int* __restrict__ sync, int mat, int need, int spin_limit) {
-  if (threadIdx.x == 0 && need > 0) {
-    int* cnt = &sync[FLOW_SYNC_HEAD + mat];
-    int spins = 0;
-    while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
-      if (++spins > spin_limit) {
-        atomicOr(&sync[FLOW_ERR], 1);
-        break;
-      }
-      __builtin_amdgcn_s_sleep(1);
-    }
-    asm volatile("buffer_inv sc0\n\ts_waitcnt vmcnt(0)" ::: "memory");
-  }
-  __syncthreads();
+  j = uni(j);
+  chol_small_task<true>((const SampleDesc*)uni(sd), j, uni(rb), (int64_t)k * NB, k, uni(maxsteps), (double*)uni(mats),
+                        (double*)uni(ldp), (int64_t*)uni(failp), (double*)uni(stage), lds,
+                        reinterpret_cast<lds_f64(*)[NB + 1]>(lds + FLOW_RING),
+                        reinterpret_cast<lds_f64(*)[NB + 1]>(lds + FLOW_RING + NB * (NB + 1)),
+                        lds + FLOW_RING + 2 * NB * (NB + 1),
+                        FlowWait{uni(sync), j, uni(need2), uni(spin_limit), uni(tr)});
 }
 
 __global__ void __launch_bounds__(FLOW_THREADS) chol_flow_kernel(const SampleDesc* __restrict__ sd,
@@ -1645,17 +1671,30 @@ __global__ void __launch_bounds__(FLOW_THREADS) chol_flow_kernel(const SampleDes
   __shared__ __attribute__((aligned(1024))) double lds[FLOW_LDS_DOUBLES];
   __shared__ int cur;
   g_desc* gsd = (g_desc*)sd;
+  const int wave = uni((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
   // the queue of the XCD this workgroup runs on: a matrix's tasks all run on one XCD, so their data passes
   // through that XCD's L2 only
   const int q = (int)(__builtin_amdgcn_s_getreg(FLOW_XCC_ID_REG) % FLOW_QUEUES);
   const int q0 = qoff[q], qn = qoff[q + 1] - q0;
-  int prev_mat = 0, prev_cnt = 0, prev_t = 0;
+  int prev_mat = -1, prev_t = 0;
   long long taken = 0, ready = 0;
   for (;;) {
-    // (a call returns in a vector register: read the index as a wave-uniform value here, so that the loop's
-    // exit and the branches on the task's fields are uniform)
-    const int i = __builtin_amdgcn_readfirstlane(
-        flow_next(sync, q, prev_mat, prev_cnt, &cur, trace, tasks, prev_t, taken, ready));
+    if (wave == 0) {
+      if (prev_mat >= 0) {
+        // the previous task complete: every wave's stores completed before the barrier that ended it
+        __hip_atomic_fetch_add(&sync[FLOW_SYNC_HEAD + prev_mat], lane == 0 ? 1 : 0, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        if (trace != nullptr) {
+          const FlowTask tp = tasks[prev_t];
+          const long long v[8] = {taken, ready, (long long)wall_clock64(), tp.mat, tp.kind, tp.k, tp.rb,
+                                  (long long)__smid()};
+          if (lane < 8) trace[16 * (int64_t)prev_t + lane] = v[lane & 7];
+        }
+      }
+      cur = uni(atomicAdd(&sync[q], lane == 0 ? 1 : 0));
+    }
+    __syncthreads();
+    const int i = uni(cur);
     if (i >= qn) break;
     const int t = q0 + i;
     if (trace != nullptr) taken = wall_clock64();
@@ -1663,18 +1702,16 @@ __global__ void __launch_bounds__(FLOW_THREADS) chol_flow_kernel(const SampleDes
     flow_wait(sync, tk.mat, tk.need, spin_limit);
     if (trace != nullptr) ready = wall_clock64();
     if (tk.kind == 0)
-      flow_diag_task(gsd, tk.mat, tk.rb, maxsteps, (g_f64*)mats, (g_f64*)ldp, (g_i64*)failp);
+      flow_diag_task(gsd, tk.mat, maxsteps, (g_f64*)mats, (g_f64*)ldp, (g_i64*)failp);
     else if (tk.kind == 1)
       flow_left_task(gsd, tk.mat, tk.rb, tk.k, maxsteps, (g_f64*)mats, (g_f64*)ldp, (g_i64*)failp, (g_f64*)stage,
-                     (lds_f64*)lds);
+                     (lds_f64*)lds, sync, tk.need2, spin_limit, trace ? trace + 16 * (int64_t)t : nullptr);
     else
       flow_small_task(gsd, tk.mat, tk.rb, tk.k, maxsteps, (g_f64*)mats, (g_f64*)ldp, (g_i64*)failp, (g_f64*)stage,
-                      (lds_f64*)lds);
-    // every wave's stores complete before the barrier (they are then in this XCD's L2)
+                      (lds_f64*)lds, sync, tk.need2, spin_limit, trace ? trace + 16 * (int64_t)t : nullptr);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     prev_mat = tk.mat;
-    prev_cnt = tk.kind == 0 ? tk.rb : 1;
     prev_t = t;
   }
 }
@@ -1997,12 +2034,12 @@ namespace {
 
 // How the batched Cholesky runs (process-wide, aj_log_evidence_configure): three launches per 32-column
 // step; one cooperative launch (without or with lookahead); one launch per step with lookahead; one
-// left-looking launch per step (the default); the left-looking tasks in one persistent launch.  All give
-// the same factor bit for bit.
+// left-looking launch per step; the left-looking tasks in one persistent launch with lookahead (the
+// default).  All give the same factor bit for bit.
 constexpr int CHOL_STEPS = 0, CHOL_COOP = 1, CHOL_COOP_LOOKAHEAD = 2, CHOL_STEPS_LOOKAHEAD = 3, CHOL_LEFT = 4,
               CHOL_FLOW = 5;
 constexpr int CHOL_MODES = 6;
-std::atomic<int> g_chol_mode{CHOL_LEFT};
+std::atomic<int> g_chol_mode{CHOL_FLOW};
 int chol_mode() { return g_chol_mode.load(std::memory_order_relaxed); }
 
 // Workgroups of the cooperative Cholesky: every one resident at once (occupancy x compute units), per
@@ -2055,10 +2092,11 @@ int flow_occupancy() {
 // L2 read, ~1 us): a quarter of a second, ~100 times the factorisation of a whole batch.
 constexpr int FLOW_SPIN_LIMIT = 1 << 18;
 std::atomic<int> g_flow_spin_limit{FLOW_SPIN_LIMIT};
-// A block column runs as 32-row tasks once its 32-row tasks number at most this many per compute unit.
-constexpr int FLOW_SMALL_PER_CU = 4;
+// A block column runs as 32-row tasks once its 32-row tasks on an XCD number at most this many per compute
+// unit of the XCD.
+constexpr int FLOW_SMALL_PER_CU = 1;
 std::atomic<int> g_flow_small_per_cu{FLOW_SMALL_PER_CU};
-// Optional task trace of the persistent schedule (eight int64 per task, aj_log_evidence_flow_trace).
+// Optional task trace of the persistent schedule (sixteen int64 per task, aj_log_evidence_flow_trace).
 std::atomic<long long*> g_flow_trace{nullptr};
 std::atomic<int64_t> g_flow_trace_cap{0};
 
@@ -2218,10 +2256,11 @@ int batch_pipeline(const WSrc w_tilde, int64_t M, const double* dirty_image, con
     // the largest matrices, whose chains of block columns are the longest, land on different XCDs
     const int cus = cu_count();
     const int64_t per_xcd = std::max(1, cus / FLOW_QUEUES);
-    static thread_local std::vector<int> xcd, need;
+    static thread_local std::vector<int> xcd, need, need_prev;
     static thread_local std::vector<FlowTask> lists[FLOW_QUEUES];
     xcd.assign(2 * n, 0);
-    need.assign(2 * n, 1);
+    need.assign(2 * n, 1);       // tasks of the matrix in block columns <= k - 1 (and its first diagonal block)
+    need_prev.assign(2 * n, 1);  // ... in block columns <= k - 2
     double work[FLOW_QUEUES] = {};
     for (int64_t q = 0; q < 2 * n; ++q) {
       const int j = 2 * ho[q >> 1] + (int)(q & 1);
@@ -2234,11 +2273,12 @@ int batch_pipeline(const WSrc w_tilde, int64_t M, const double* dirty_image, con
     }
     // each XCD's list: its matrices' first diagonal blocks, then per block column k their tasks (matrices in
     // decreasing size): 256-row tasks, or 32-row tasks once the XCD's 32-row tasks of the column number at
-    // most small_per_cu per compute unit; a task needs all tasks of its matrix in the earlier block columns
+    // most small_per_cu per compute unit; a task starts once its matrix's tasks in block columns <= k - 2 are
+    // complete and takes its last block once those of column k - 1 are
     for (auto& l : lists) l.clear();
     for (int64_t q = 0; q < 2 * n; ++q) {
       const int j = 2 * ho[q >> 1] + (int)(q & 1);
-      lists[xcd[j]].push_back(FlowTask{j, 0, 0, 1, 0});
+      lists[xcd[j]].push_back(FlowTask{j, 0, 0, 0, 0, 0});
     }
     const int64_t small_per = per_xcd * g_flow_small_per_cu.load(std::memory_order_relaxed);
     for (int64_t k = 0; k < p.maxsteps; ++k) {
@@ -2258,7 +2298,8 @@ int batch_pipeline(const WSrc w_tilde, int64_t M, const double* dirty_image, con
                                  : 1 + ceil_div(std::max<int64_t>(0, below - LeftShape<FL_STRIP>::STRIP0),
                                                 (int64_t)FL_STRIP);
         for (int64_t rb = 0; rb < nt; ++rb)
-          lists[xcd[j]].push_back(FlowTask{j, small ? 2 : 1, (int)k, (int)rb, need[j]});
+          lists[xcd[j]].push_back(FlowTask{j, small ? 2 : 1, (int)k, (int)rb, need_prev[j], need[j]});
+        need_prev[j] = need[j];
         need[j] += (int)nt;
       }
     }
@@ -3620,7 +3661,7 @@ extern "C" int aj_log_evidence_flow_configure(int small_per_cu, int spin_limit) 
 
 extern "C" int aj_log_evidence_flow_trace(void* device_buffer, int64_t capacity_tasks) {
   AJ_REQUIRE(capacity_tasks >= 0 && (capacity_tasks == 0 || device_buffer != nullptr),
-             "trace buffer: a device buffer of capacity_tasks x 8 int64, or null and 0");
+             "trace buffer: a device buffer of capacity_tasks x 16 int64, or null and 0");
   g_flow_trace.store(capacity_tasks > 0 ? static_cast<long long*>(device_buffer) : nullptr, std::memory_order_relaxed);
   g_flow_trace_cap.store(capacity_tasks, std::memory_order_relaxed);
   return AJ_OK;
@@ -3629,6 +3670,6 @@ extern "C" int aj_log_evidence_flow_trace(void* device_buffer, int64_t capacity_
 extern "C" int aj_log_evidence_configure(int chol_mode_in) {
   AJ_REQUIRE(chol_mode_in >= -1 && chol_mode_in < CHOL_MODES, "chol_mode %d: -1 (default) or 0 .. %d",
              chol_mode_in, CHOL_MODES - 1);
-  g_chol_mode.store(chol_mode_in < 0 ? CHOL_LEFT : chol_mode_in, std::memory_order_relaxed);
+  g_chol_mode.store(chol_mode_in < 0 ? CHOL_FLOW : chol_mode_in, std::memory_order_relaxed);
   return AJ_OK;
 }

@@ -544,8 +544,9 @@ def test_batched_log_evidence_vs_oracle(gpu, delaunay_small):
 def test_batched_cholesky_modes_bitwise_equal(gpu, delaunay_small):
     """The six ways the batch factors its matrices (three launches per step; one cooperative launch,
     without and with lookahead; one launch per step with lookahead; left-looking, one launch per step; the
-    left-looking tasks in one persistent launch, with its default switch to 32-row tasks, with 256-row tasks
-    in every block column and with 32-row tasks in every block column) give bitwise the same log-evidence,
+    left-looking tasks in one persistent launch with lookahead, with its default switch to 32-row tasks, with
+    256-row tasks in every block column and with 32-row tasks in every block column, twice each) give bitwise
+    the same log-evidence,
     log-determinants and reconstructions, also with samples ending inside a step's block (S 97, 31, 300) and
     one whose block columns take several strips (S 700)."""
     import torch
