@@ -1999,7 +1999,12 @@ inline BatchPlan plan_batch(int64_t M, int64_t B, int64_t n, const int64_t* S_ho
     o += bytes;
     return r;
   };
+  // the tables the host writes, adjacent so that one copy uploads them: sample descriptors, Omega slices,
+  // the persistent schedule's queue offsets and task totals, its task lists
   p.off_desc = put(sizeof(SampleDesc) * n);
+  p.off_slices = put(sizeof(int2) * p.nsl);
+  p.off_fqoff = put(sizeof(int) * (FLOW_QUEUES + 1 + 2 * n));  // queue offsets, then task totals per matrix
+  p.off_ftasks = put(sizeof(FlowTask) * p.max_flow);
   p.off_ptr = put(sizeof(int) * p.csc_ptr_elems);
   p.off_cur = put(sizeof(int) * p.csc_ptr_elems);
   p.off_key = put(sizeof(int) * p.csc_ent_elems);
@@ -2016,10 +2021,7 @@ inline BatchPlan plan_batch(int64_t M, int64_t B, int64_t n, const int64_t* S_ho
   p.off_stage = put(sizeof(double) * 4 * n * NB * NB);  // two slots per matrix
   p.off_skey = put(sizeof(int) * p.csc_ent_elems);       // the buckets in slice order
   p.off_st = put(sizeof(double) * p.csc_ent_elems);
-  p.off_slices = put(sizeof(int2) * p.nsl);
   p.off_fsync = put(sizeof(int) * (FLOW_SYNC_HEAD + 2 * n));
-  p.off_fqoff = put(sizeof(int) * (FLOW_QUEUES + 1 + 2 * n));  // queue offsets, then task totals per matrix
-  p.off_ftasks = put(sizeof(FlowTask) * p.max_flow);
   p.bytes = align_up(o, 256);
   return p;
 }
@@ -2148,18 +2150,102 @@ int batch_pipeline(const WSrc w_tilde, int64_t M, const double* dirty_image, con
   for (int64_t b = 0; b < n; ++b)
     for (int64_t c = 0; c < S_host[b]; c += 64) hs.push_back(int2{(int)b, (int)c});
   AJ_REQUIRE((int64_t)hs.size() == p.nsl && p.nsl < (1ll << 31), "slice table of %zu entries", hs.size());
-  timer_begin(AJ_TIMER_BATCH, st);
-  AJ_CHECK_HIP(hipMemcpyAsync(desc, hd.data(), sizeof(SampleDesc) * n, hipMemcpyHostToDevice, st));
-  AJ_CHECK_HIP(hipMemcpyAsync(slices, hs.data(), sizeof(int2) * hs.size(), hipMemcpyHostToDevice, st));
   static thread_local std::vector<int> ho;
   ho.resize(n);
   for (int64_t b = 0; b < n; ++b) ho[b] = (int)b;
   std::stable_sort(ho.begin(), ho.end(), [&](int x, int y) { return S_host[x] > S_host[y]; });
   (void)cur;
+  const int mode = chol_mode();
+  // the persistent schedule's task lists (built here, so that they go up with the sample table)
+  int flow_ntasks = 0;
+  static thread_local std::vector<FlowTask> hf;
+  static thread_local std::vector<int> hq;
+  hf.clear();
+  hq.clear();
+  if (mode == CHOL_FLOW) {
+    // matrices to XCDs: in decreasing size, each to the XCD with the least work so far (S^3 each), so that
+    // the largest matrices, whose chains of block columns are the longest, land on different XCDs
+    const int cus = cu_count();
+    const int64_t per_xcd = std::max(1, cus / FLOW_QUEUES);
+    static thread_local std::vector<int> xcd, need, need_prev;
+    static thread_local std::vector<FlowTask> lists[FLOW_QUEUES];
+    xcd.assign(2 * n, 0);
+    need.assign(2 * n, 1);       // tasks of the matrix in block columns <= k - 1 (and its first diagonal block)
+    need_prev.assign(2 * n, 1);  // ... in block columns <= k - 2
+    double work[FLOW_QUEUES] = {};
+    for (int64_t q = 0; q < 2 * n; ++q) {
+      const int j = 2 * ho[q >> 1] + (int)(q & 1);
+      int x = 0;
+      for (int y = 1; y < FLOW_QUEUES; ++y)
+        if (work[y] < work[x]) x = y;
+      xcd[j] = x;
+      const double S = (double)S_host[j >> 1];
+      work[x] += S * S * S;
+    }
+    // each XCD's list: its matrices' first diagonal blocks, then per block column k their tasks (matrices in
+    // decreasing size): 256-row tasks, or 32-row tasks once the XCD's 32-row tasks of the column number at
+    // most small_per_cu per compute unit; a task starts once its matrix's tasks in block columns <= k - 2 are
+    // complete and takes its last block once those of column k - 1 are
+    for (auto& l : lists) l.clear();
+    for (int64_t q = 0; q < 2 * n; ++q) {
+      const int j = 2 * ho[q >> 1] + (int)(q & 1);
+      lists[xcd[j]].push_back(FlowTask{j, 0, 0, 0, 0, 0});
+    }
+    const int64_t small_per = per_xcd * g_flow_small_per_cu.load(std::memory_order_relaxed);
+    for (int64_t k = 0; k < p.maxsteps; ++k) {
+      const int64_t k0 = k * NB;
+      int64_t nsmall[FLOW_QUEUES] = {};
+      for (int64_t j = 0; j < 2 * n; ++j) {
+        const int64_t S = S_host[j >> 1];
+        if (S > k0) nsmall[xcd[j]] += std::max<int64_t>(1, ceil_div(S - std::min<int64_t>(S, k0 + NB), (int64_t)NB));
+      }
+      for (int64_t q = 0; q < 2 * n; ++q) {
+        const int j = 2 * ho[q >> 1] + (int)(q & 1);
+        const int64_t S = S_host[j >> 1];
+        if (S <= k0) continue;
+        const bool small = nsmall[xcd[j]] <= small_per;
+        const int64_t below = S - std::min<int64_t>(S, k0 + NB);
+        const int64_t nt = small ? std::max<int64_t>(1, ceil_div(below, (int64_t)NB))
+                                 : 1 + ceil_div(std::max<int64_t>(0, below - LeftShape<FL_STRIP>::STRIP0),
+                                                (int64_t)FL_STRIP);
+        for (int64_t rb = 0; rb < nt; ++rb)
+          lists[xcd[j]].push_back(FlowTask{j, small ? 2 : 1, (int)k, (int)rb, need_prev[j], need[j]});
+        need_prev[j] = need[j];
+        need[j] += (int)nt;
+      }
+    }
+    hq.assign(FLOW_QUEUES + 1 + 2 * n, 0);
+    for (int x = 0; x < FLOW_QUEUES; ++x) {
+      hq[x] = (int)hf.size();
+      hf.insert(hf.end(), lists[x].begin(), lists[x].end());
+    }
+    hq[FLOW_QUEUES] = (int)hf.size();
+    for (int64_t j = 0; j < 2 * n; ++j) hq[FLOW_QUEUES + 1 + j] = need[j];  // each matrix's task total
+    AJ_REQUIRE((int64_t)hf.size() <= p.max_flow && hf.size() < (1u << 30), "internal: persistent task list %zu > %lld",
+               hf.size(), (long long)p.max_flow);
+    flow_ntasks = (int)hf.size();
+  }
+  // one copy of the host tables
+  timer_begin(AJ_TIMER_BATCH, st);
+  {
+    static thread_local std::vector<char> up;
+    up.resize(p.off_ftasks + sizeof(FlowTask) * p.max_flow - p.off_desc);
+    auto put_up = [&](size_t off, const void* src, size_t bytes) {
+      if (bytes) std::memcpy(up.data() + (off - p.off_desc), src, bytes);
+      return off - p.off_desc + bytes;
+    };
+    size_t used = put_up(p.off_desc, hd.data(), sizeof(SampleDesc) * n);
+    used = std::max(used, put_up(p.off_slices, hs.data(), sizeof(int2) * hs.size()));
+    if (mode == CHOL_FLOW) {
+      used = std::max(used, put_up(p.off_fqoff, hq.data(), sizeof(int) * hq.size()));
+      used = std::max(used, put_up(p.off_ftasks, hf.data(), sizeof(FlowTask) * hf.size()));
+    }
+    AJ_CHECK_HIP(hipMemcpyAsync(desc, up.data(), used, hipMemcpyHostToDevice, st));
+  }
+
   // 1: mapping inversion and data vector, one workgroup per sample
   AJ_CHECK_HIP(hipFuncSetAttribute((const void*)prep_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)(sizeof(int) * (maxS + 1))));
-  const int mode = chol_mode();
   int* fsync = reinterpret_cast<int*>(ws + p.off_fsync);
   hipLaunchKernelGGL(prep_kernel, dim3((unsigned)n), dim3(PREP_THREADS), sizeof(int) * (maxS + 1), st, desc,
                      pix_indexes, pix_weights, M, B, dirty_image, ptrs, ekey, et, D, skey, st_t,
@@ -2233,8 +2319,10 @@ int batch_pipeline(const WSrc w_tilde, int64_t M, const double* dirty_image, con
   double* ldp = reinterpret_cast<double*>(ws + p.off_ldp);
   int64_t* failp = reinterpret_cast<int64_t*>(ws + p.off_failp);
   double* stage = reinterpret_cast<double*>(ws + p.off_stage);
-  AJ_CHECK_HIP(hipMemcpyAsync(steps, he.data(), sizeof(StepEntry) * he.size(), hipMemcpyHostToDevice, st));
-  AJ_CHECK_HIP(hipMemcpyAsync(step_tab, tab.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice, st));
+  if (mode != CHOL_FLOW) {  // (the persistent schedule's tables went up with the sample table)
+    AJ_CHECK_HIP(hipMemcpyAsync(steps, he.data(), sizeof(StepEntry) * he.size(), hipMemcpyHostToDevice, st));
+    AJ_CHECK_HIP(hipMemcpyAsync(step_tab, tab.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice, st));
+  }
   timer_begin(AJ_TIMER_CHOL, st);
   bool done = false;
   bool solve_finishes = false;
@@ -2252,75 +2340,11 @@ int batch_pipeline(const WSrc w_tilde, int64_t M, const double* dirty_image, con
     }
   }
   if (!done && mode == CHOL_FLOW) {
-    // matrices to XCDs: in decreasing size, each to the XCD with the least work so far (S^3 each), so that
-    // the largest matrices, whose chains of block columns are the longest, land on different XCDs
-    const int cus = cu_count();
-    const int64_t per_xcd = std::max(1, cus / FLOW_QUEUES);
-    static thread_local std::vector<int> xcd, need, need_prev;
-    static thread_local std::vector<FlowTask> lists[FLOW_QUEUES];
-    xcd.assign(2 * n, 0);
-    need.assign(2 * n, 1);       // tasks of the matrix in block columns <= k - 1 (and its first diagonal block)
-    need_prev.assign(2 * n, 1);  // ... in block columns <= k - 2
-    double work[FLOW_QUEUES] = {};
-    for (int64_t q = 0; q < 2 * n; ++q) {
-      const int j = 2 * ho[q >> 1] + (int)(q & 1);
-      int x = 0;
-      for (int y = 1; y < FLOW_QUEUES; ++y)
-        if (work[y] < work[x]) x = y;
-      xcd[j] = x;
-      const double S = (double)S_host[j >> 1];
-      work[x] += S * S * S;
-    }
-    // each XCD's list: its matrices' first diagonal blocks, then per block column k their tasks (matrices in
-    // decreasing size): 256-row tasks, or 32-row tasks once the XCD's 32-row tasks of the column number at
-    // most small_per_cu per compute unit; a task starts once its matrix's tasks in block columns <= k - 2 are
-    // complete and takes its last block once those of column k - 1 are
-    for (auto& l : lists) l.clear();
-    for (int64_t q = 0; q < 2 * n; ++q) {
-      const int j = 2 * ho[q >> 1] + (int)(q & 1);
-      lists[xcd[j]].push_back(FlowTask{j, 0, 0, 0, 0, 0});
-    }
-    const int64_t small_per = per_xcd * g_flow_small_per_cu.load(std::memory_order_relaxed);
-    for (int64_t k = 0; k < p.maxsteps; ++k) {
-      const int64_t k0 = k * NB;
-      int64_t nsmall[FLOW_QUEUES] = {};
-      for (int64_t j = 0; j < 2 * n; ++j) {
-        const int64_t S = S_host[j >> 1];
-        if (S > k0) nsmall[xcd[j]] += std::max<int64_t>(1, ceil_div(S - std::min<int64_t>(S, k0 + NB), (int64_t)NB));
-      }
-      for (int64_t q = 0; q < 2 * n; ++q) {
-        const int j = 2 * ho[q >> 1] + (int)(q & 1);
-        const int64_t S = S_host[j >> 1];
-        if (S <= k0) continue;
-        const bool small = nsmall[xcd[j]] <= small_per;
-        const int64_t below = S - std::min<int64_t>(S, k0 + NB);
-        const int64_t nt = small ? std::max<int64_t>(1, ceil_div(below, (int64_t)NB))
-                                 : 1 + ceil_div(std::max<int64_t>(0, below - LeftShape<FL_STRIP>::STRIP0),
-                                                (int64_t)FL_STRIP);
-        for (int64_t rb = 0; rb < nt; ++rb)
-          lists[xcd[j]].push_back(FlowTask{j, small ? 2 : 1, (int)k, (int)rb, need_prev[j], need[j]});
-        need_prev[j] = need[j];
-        need[j] += (int)nt;
-      }
-    }
-    static thread_local std::vector<FlowTask> hf;
-    static thread_local std::vector<int> hq;
-    hf.clear();
-    hq.assign(FLOW_QUEUES + 1 + 2 * n, 0);
-    for (int x = 0; x < FLOW_QUEUES; ++x) {
-      hq[x] = (int)hf.size();
-      hf.insert(hf.end(), lists[x].begin(), lists[x].end());
-    }
-    hq[FLOW_QUEUES] = (int)hf.size();
-    for (int64_t j = 0; j < 2 * n; ++j) hq[FLOW_QUEUES + 1 + j] = need[j];  // each matrix's task total
-    AJ_REQUIRE((int64_t)hf.size() <= p.max_flow && hf.size() < (1u << 30), "internal: persistent task list %zu > %lld",
-               hf.size(), (long long)p.max_flow);
+    // the task lists went up with the sample table
+    const int ntasks = flow_ntasks;
+    const int grid = cu_count() * flow_occupancy();
     FlowTask* ftasks = reinterpret_cast<FlowTask*>(ws + p.off_ftasks);
     int* fqoff = reinterpret_cast<int*>(ws + p.off_fqoff);
-    AJ_CHECK_HIP(hipMemcpyAsync(ftasks, hf.data(), sizeof(FlowTask) * hf.size(), hipMemcpyHostToDevice, st));
-    AJ_CHECK_HIP(hipMemcpyAsync(fqoff, hq.data(), sizeof(int) * hq.size(), hipMemcpyHostToDevice, st));
-    const int ntasks = (int)hf.size();
-    const int grid = cus * flow_occupancy();
     long long* trace = g_flow_trace.load(std::memory_order_relaxed);
     if (g_flow_trace_cap.load(std::memory_order_relaxed) < ntasks) trace = nullptr;
     hipLaunchKernelGGL(chol_flow_kernel, dim3((unsigned)grid), dim3(FLOW_THREADS), 0, st, desc, ftasks, fqoff,
