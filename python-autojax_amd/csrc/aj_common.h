@@ -88,6 +88,12 @@ __device__ void raw_buffer_load_lds(i32x4 rsrc, __attribute__((address_space(3))
                                     int voffset, int soffset, int offset, int aux)
     __asm("llvm.amdgcn.raw.buffer.load.lds");
 
+// One dword or double by a raw buffer load (vector offset + scalar offset, in bytes): offsets at or past the
+// resource's byte count read as zero (no branch).
+__device__ int raw_buffer_load_i32(i32x4 rsrc, int voffset, int soffset, int aux) __asm("llvm.amdgcn.raw.buffer.load.i32");
+__device__ double raw_buffer_load_f64(i32x4 rsrc, int voffset, int soffset, int aux)
+    __asm("llvm.amdgcn.raw.buffer.load.f64");
+
 // Buffer resource for byte offsets below 2^32 from base (stride 0, raw addressing).
 __device__ inline i32x4 make_rsrc(const void* base, uint32_t bytes) {
   const uint64_t a = reinterpret_cast<uint64_t>(base);

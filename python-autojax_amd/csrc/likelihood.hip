@@ -181,6 +181,7 @@ __global__ void __launch_bounds__(256) csc_sort_kernel(const SampleDesc* __restr
 // bucket ordered by (m, b) by one wave (a rank sort up to 64 entries), then D of the sample's buckets.
 // ---------------------------------------------------------------------------
 constexpr int PREP_THREADS = 1024;
+constexpr int PREP_SLICES_MAX = 256;  // slices of 64 columns per sample (S <= 16,384)
 
 __global__ void __launch_bounds__(PREP_THREADS) prep_kernel(const SampleDesc* __restrict__ sd,
                                                             const int64_t* __restrict__ idx,
@@ -189,6 +190,7 @@ __global__ void __launch_bounds__(PREP_THREADS) prep_kernel(const SampleDesc* __
                                                             int* __restrict__ ptrs, int* __restrict__ ent_key,
                                                             double* __restrict__ ent_t, double* __restrict__ D,
                                                             int* __restrict__ sl_key, double* __restrict__ sl_t,
+                                                            int* __restrict__ sl_col, int* __restrict__ sl_pos,
                                                             int* __restrict__ flow_sync) {
   extern __shared__ int cnt[];  // S + 1 counters, then cursors
   __shared__ int part[PREP_THREADS];
@@ -278,25 +280,78 @@ __global__ void __launch_bounds__(PREP_THREADS) prep_kernel(const SampleDesc* __
     for (int e = p[s]; e < p[s + 1]; ++e) v = fma(t[e], dirty[k[e]], v);
     D[d.s_off + s] = v;
   }
-  // The buckets again, ordered by slice for the Omega kernel: slice q holds columns 64 q .. 64 q + 63 (a
-  // lane each) and occupies the slice's range of the CSC, reordered as entry 0 of every column that has one,
-  // columns in order, then entry 1, ...: a wave reading entry j of its 64 columns reads consecutive
-  // addresses.
+  // The buckets again, ordered by slice for the Omega kernel.  The columns are grouped by bucket length
+  // (longest first; lengths from 255 on in one group) and cut into slices of 64, so that the lanes of a
+  // slice run for about the same number of entries; within a slice they are ranked by decreasing length
+  // (ties by their order in the grouping), sl_col[64 q + rank] = the column, so the columns that have an
+  // entry j are ranks 0 .. a_j - 1.  Entry j of rank r sits at sl_pos[q] + (the slice's entries before j)
+  // + r: a wave reading entry j of its 64 columns reads consecutive addresses and needs only a_j.  Which
+  // lane sums a column does not change the column's sum.
+  __shared__ int lgrp[256];  // columns per length group, then the groups' cursors
+  __shared__ int ltot[PREP_SLICES_MAX];  // entries per slice, then the slices' first entries
+  int* order = cnt;  // the columns in group order (the bucket counters are free)
+  if (tid < 256) lgrp[tid] = 0;
+  __syncthreads();
+  for (int64_t s = tid; s < S; s += PREP_THREADS) atomicAdd(&lgrp[255 - std::min(p[s + 1] - p[s], 255)], 1);
+  __syncthreads();
+  if (wave == 0) {  // exclusive prefix of the 256 groups: four per lane
+    int v[4], sum = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) sum += v[i] = lgrp[4 * lane + i];
+    int incl = sum;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += y;
+    }
+    int run = incl - sum;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      lgrp[4 * lane + i] = run;
+      run += v[i];
+    }
+  }
+  __syncthreads();
+  for (int64_t s = tid; s < S; s += PREP_THREADS) order[atomicAdd(&lgrp[255 - std::min(p[s + 1] - p[s], 255)], 1)] = (int)s;
+  __syncthreads();
+  const int nsl = (int)((S + 63) / 64);
+  for (int q = wave; q < nsl; q += PREP_THREADS / 64) {
+    const int64_t i = (int64_t)q * 64 + lane;
+    int tot = i < S ? p[order[i] + 1] - p[order[i]] : 0;
+    for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o, 64);
+    if (lane == 0) ltot[q] = tot;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int run = 0;
+    for (int q = 0; q < nsl; ++q) {
+      const int v = ltot[q];
+      ltot[q] = run;
+      run += v;
+    }
+  }
+  __syncthreads();
   int* sk = sl_key + d.csc_ent;
   double* st = sl_t + d.csc_ent;
-  for (int64_t q = wave; q * 64 < S; q += PREP_THREADS / 64) {
-    const int64_t s = q * 64 + lane;
-    const int lo = s < S ? p[s] : 0;
-    const int len = s < S ? p[s + 1] - lo : 0;
-    const uint64_t below = (uint64_t(1) << lane) - 1;
-    int pos = p[q * 64];
+  int* sc = sl_col + d.csc_ptr;
+  for (int q = wave; q < nsl; q += PREP_THREADS / 64) {
+    const int64_t i = (int64_t)q * 64 + lane;
+    const int s = i < S ? order[i] : -1;
+    const int lo = s >= 0 ? p[s] : 0;
+    const int len = s >= 0 ? p[s + 1] - lo : 0;
+    int rank = 0;  // the lanes past S (length 0, after every column of the slice) rank last
+    for (int l = 0; l < 64; ++l) {
+      const int ll = __shfl(len, l, 64);
+      rank += ll > len || (ll == len && l < lane) ? 1 : 0;
+    }
+    if (s >= 0) sc[q * 64 + rank] = s;
+    int pos = ltot[q];
+    if (lane == 0) sl_pos[d.csc_ptr + q] = pos;
     for (int j = 0;; ++j) {
       const uint64_t mask = __ballot(len > j);
       if (mask == 0) break;
       if (len > j) {
-        const int at = pos + __popcll(mask & below);
-        sk[at] = k[lo + j];
-        st[at] = t[lo + j];
+        sk[pos + rank] = k[lo + j];
+        st[pos + rank] = t[lo + j];
       }
       pos += __popcll(mask);
     }
@@ -412,10 +467,16 @@ __global__ void __launch_bounds__(OMEGA_THREADS) omega_kernel(const WSrc W, int6
 // Omega = W T over the slice-ordered buckets: R (1, 2, 4 or 8) rows of W in LDS, interleaved (wl[m R + r]
 // = W[m0 + r, m]) so that the R values one bucket entry needs are adjacent (one 8- or 16-byte read, or R / 2
 // 16-byte reads).  Each wave takes whole slices of 64 columns (slice g: sample slices[g].x, first column
-// slices[g].y), a lane per column; entry j of its 64 columns sits at consecutive addresses, the loads of
-// OMEGA_SLICE_UNROLL entries are issued ahead of their sums, and each column's sum runs over its bucket in
-// entry order (the bits of omega_kernel).
+// slices[g].y), lane r the slice's column of rank r (by decreasing bucket length; the slices group columns
+// of about the same length), so the lanes that have an entry j are lanes 0 .. a_j - 1 and read entries
+// pos_j + lane: the loop has no per-lane branch or index arithmetic, only one ballot per entry, and runs to
+// the slice's longest bucket.  The loads of OMEGA_SLICE_UNROLL entries (then 4) are issued ahead of their
+// sums; a lane past its bucket's end reads a neighbouring entry (the slice buffers end with 64 spare entries)
+// and adds it with weight +0.0 at column 0, which leaves its sum unchanged bit for bit (the sum is never
+// -0.0: it starts at +0.0 and a rounded sum is -0.0 only if both terms are).  Each column's sum runs over
+// its bucket in entry order (the bits of omega_kernel).
 constexpr int OMEGA_SLICE_UNROLL = 16;
+constexpr int OMEGA_SLICE_PAD = 64;  // spare entries after the slice buffers
 
 template <int R>
 __device__ inline void lds_rows(const double* wl, int m, double (&w)[R]) {
@@ -437,13 +498,14 @@ __device__ inline void lds_rows(const double* wl, int m, double (&w)[R]) {
 // entry.
 struct OmegaSlice {
   int64_t S, csc_ent, omega_off;
-  int64_t s;  // the lane's column
+  int64_t s;  // the lane's column (S for a lane past the sample's columns)
   int len;    // its bucket's length (0 past the sample's last column)
   int pos;    // the slice's first entry in its sample's buckets
 };
 
 __device__ inline OmegaSlice omega_slice(int g, const int2* __restrict__ slices, const SampleDesc* __restrict__ sd,
-                                         const int* __restrict__ ptrs, int lane) {
+                                         const int* __restrict__ ptrs, const int* __restrict__ sl_col,
+                                         const int* __restrict__ sl_pos, int lane) {
   const int2 sq = slices[g];
   const SampleDesc d = sd[sq.x];
   const int* p = ptrs + d.csc_ptr;
@@ -451,10 +513,42 @@ __device__ inline OmegaSlice omega_slice(int g, const int2* __restrict__ slices,
   o.S = d.S;
   o.csc_ent = d.csc_ent;
   o.omega_off = d.omega_off;
-  o.s = sq.y + lane;
+  const int64_t r = sq.y + lane;
+  o.s = r < d.S ? sl_col[d.csc_ptr + r] : d.S;
   o.len = o.s < d.S ? p[o.s + 1] - p[o.s] : 0;
-  o.pos = p[sq.y];
+  o.pos = sl_pos[d.csc_ptr + sq.y / 64];
   return o;
+}
+
+// A group of V entries of a wave's slice: their loads (entries j .. j + V - 1, from pos on; pos is advanced
+// past them), then their sums into acc.
+template <int V>
+struct OmegaGroup {
+  int m[V];
+  double t[V];
+};
+template <int V>
+__device__ __attribute__((always_inline)) inline void omega_load(OmegaGroup<V>& g, int j, int len, int& pos, i32x4 rk,
+                                                                 i32x4 rt, int vk, int vt) {
+#pragma unroll
+  for (int u = 0; u < V; ++u) {
+    g.m[u] = raw_buffer_load_i32(rk, vk, pos * 4, 0);
+    g.t[u] = raw_buffer_load_f64(rt, vt, pos * 8, 0);
+    pos += __popcll(__ballot(len > j + u));
+  }
+}
+template <int R, int V>
+__device__ __attribute__((always_inline)) inline void omega_sum(const OmegaGroup<V>& g, int j, int len,
+                                                                const double* wl, double (&acc)[R]) {
+#pragma unroll
+  for (int u = 0; u < V; ++u) {
+    const bool act = len > j + u;
+    double w[R];
+    lds_rows<R>(wl, act ? g.m[u] : 0, w);
+    const double tu = act ? g.t[u] : 0.0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] = fma(tu, w[r], acc[r]);
+  }
 }
 
 template <int R, int U>
@@ -462,10 +556,14 @@ __global__ void __launch_bounds__(OMEGA_THREADS) omega_slices_kernel(const WSrc 
                                                                       const int2* __restrict__ slices,
                                                                       const SampleDesc* __restrict__ sd,
                                                                       const int* __restrict__ ptrs,
+                                                                      const int* __restrict__ sl_col,
+                                                                      const int* __restrict__ sl_pos,
                                                                       const int* __restrict__ sl_key,
                                                                       const double* __restrict__ sl_t,
-                                                                      double* __restrict__ omega) {
-  extern __shared__ double wl[];  // M x R, interleaved
+                                                                      int64_t sl_elems, double* __restrict__ omega) {
+  // M x R, interleaved; 16-byte aligned behind the static LDS (the slice counter) so that an entry's R
+  // values are whole 16-byte reads
+  extern __shared__ __attribute__((aligned(16))) double wl[];
   const int64_t m0 = (int64_t)blockIdx.x * R;
   const int rows = (int)std::min<int64_t>(R, M - m0);
   const bool dense = W.C == nullptr && W.Pre == nullptr;
@@ -474,45 +572,59 @@ __global__ void __launch_bounds__(OMEGA_THREADS) omega_slices_kernel(const WSrc 
       wl[m * R + r] = r >= rows ? 0.0 : dense ? W.W[(m0 + r) * M + m] : wsrc_at(W, M, m0 + r, m);
   __syncthreads();
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-  const uint64_t below = (uint64_t(1) << lane) - 1;
   constexpr int WAVES = OMEGA_THREADS / 64;
-  // the next slice's descriptor is loaded while the current one is summed
+  // slices are taken from a counter in LDS (their lengths differ: a fixed share per wave would wait for
+  // the wave with the longest ones); the next slice's descriptor is loaded while the current one is summed
+  __shared__ int next_slice;
+  if (threadIdx.x == 0) next_slice = WAVES;
+  __syncthreads();
+  auto take = [&]() {
+    int v = 0;
+    if (lane == 0) v = atomicAdd(&next_slice, 1);
+    return __builtin_amdgcn_readfirstlane(v);
+  };
   OmegaSlice nx{};
-  if (wave < nsl) nx = omega_slice(wave, slices, sd, ptrs, lane);
-  for (int g = wave; g < nsl; g += WAVES) {
+  if (wave < nsl) nx = omega_slice(wave, slices, sd, ptrs, sl_col, sl_pos, lane);
+  int gn = take();
+  for (int g = wave; g < nsl;) {
     const OmegaSlice cu = nx;
-    if (g + WAVES < nsl) nx = omega_slice(g + WAVES, slices, sd, ptrs, lane);
-    const int* key = sl_key + cu.csc_ent;
-    const double* tv = sl_t + cu.csc_ent;
+    if (gn < nsl) nx = omega_slice(gn, slices, sd, ptrs, sl_col, sl_pos, lane);
+    // raw buffer loads over the slice buffers from the sample's first entry: the lane's byte offset in a
+    // vector register, the entry position as the scalar offset (no vector address arithmetic per entry)
+    const int64_t ent = uni(cu.csc_ent);
+    const int64_t rem = sl_elems - ent;  // entries from there to the buffers' end
+    const i32x4 rk = make_rsrc(sl_key + ent, (uint32_t)std::min<int64_t>(rem * 4, 0xFFFFFFFFll));
+    const i32x4 rt = make_rsrc(sl_t + ent, (uint32_t)std::min<int64_t>(rem * 8, 0xFFFFFFFFll));
+    const int vk = lane * 4, vt = lane * 8;
     const int len = cu.len;
-    int pos = cu.pos;
+    int pos = __builtin_amdgcn_readfirstlane(cu.pos);
     double acc[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) acc[r] = 0.0;
-    for (int j = 0;; j += U) {
-      uint64_t mask[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) mask[u] = __ballot(len > j + u);
-      if (mask[0] == 0) break;
-      int m[U];
-      double t[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const bool act = len > j + u;
-        const int at = pos + __popcll(mask[u] & below);
-        pos += __popcll(mask[u]);
-        m[u] = act ? key[at] : 0;
-        t[u] = act ? tv[at] : 0.0;
+    // lane 0 has the slice's longest bucket: U entries at a time, each group's loads issued before the
+    // previous group's sums, then four at a time
+    const int L = uni(len);
+    int j = 0;
+    if (U <= L) {
+      OmegaGroup<U> ga, gb;
+      omega_load<U>(ga, 0, len, pos, rk, rt, vk, vt);
+      for (;;) {
+        const bool more = j + 2 * U <= L;
+        if (more) omega_load<U>(gb, j + U, len, pos, rk, rt, vk, vt);
+        omega_sum<R, U>(ga, j, len, wl, acc);
+        j += U;
+        if (!more) break;
+        const bool more2 = j + 2 * U <= L;
+        if (more2) omega_load<U>(ga, j + U, len, pos, rk, rt, vk, vt);
+        omega_sum<R, U>(gb, j, len, wl, acc);
+        j += U;
+        if (!more2) break;
       }
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        if (len > j + u) {
-          double w[R];
-          lds_rows<R>(wl, m[u], w);
-#pragma unroll
-          for (int r = 0; r < R; ++r) acc[r] = fma(t[u], w[r], acc[r]);
-        }
-      }
+    }
+    for (; j < L; j += 4) {
+      OmegaGroup<4> gt;
+      omega_load<4>(gt, j, len, pos, rk, rt, vk, vt);
+      omega_sum<R, 4>(gt, j, len, wl, acc);
     }
     if (cu.s < cu.S) {
       double* om = omega + cu.omega_off + m0 * cu.S + cu.s;
@@ -520,6 +632,8 @@ __global__ void __launch_bounds__(OMEGA_THREADS) omega_slices_kernel(const WSrc 
       for (int r = 0; r < R; ++r)
         if (r < rows) om[r * cu.S] = acc[r];
     }
+    g = gn;
+    if (g < nsl) gn = take();
   }
 }
 
@@ -1141,9 +1255,6 @@ __device__ inline void solve_row_lkk(double (&x)[NB], int nb, const lds_f64 (*__
 // XOR (r mod 16), so that the 16 rows of a fragment read fall in 16 different segments.
 __device__ inline int ll_at(int r, int c) { return r * NB + ((((c >> 1) ^ (r & 15)) << 1) | (c & 1)); }
 
-// One double by a raw buffer load: offsets at or past the resource's byte count read as zero (no branch).
-__device__ double raw_buffer_load_f64(i32x4 rsrc, int voffset, int soffset, int aux)
-    __asm("llvm.amdgcn.raw.buffer.load.f64");
 constexpr int LL_OOB = 0x7ffff000;  // past any matrix's bytes (n <= SOLVE_S_MAX), also 47 blocks on
 
 // The persistent schedule's second wait of a task (lookahead): before its last block (p = k - 1) and the
@@ -1778,6 +1889,7 @@ constexpr int SOLVE_THREADS = 1024;  // sixteen waves per sample
 // arrays (reduction, 64 x 65 diagonal block, 16 x 64 partial sums) in the 160 KB of a CU
 constexpr int SOLVE_S_MAX =
     (int)((160 * 1024 - sizeof(double) * (SOLVE_THREADS + SB * (SB + 1) + (SOLVE_THREADS / 64) * SB)) / sizeof(double));
+static_assert(SOLVE_S_MAX <= 64 * PREP_SLICES_MAX, "the prep kernel's slice table holds every sample's slices");
 
 // ---------------------------------------------------------------------------
 // Solves L y = D, L^T s = y and the log-evidence, one workgroup per sample.
@@ -1970,7 +2082,7 @@ __global__ void noise_norm_kernel(const double* __restrict__ noise, int64_t K2, 
 struct BatchPlan {
   int64_t n, M, B, sumS, sumS2, omega_elems, csc_ptr_elems, csc_ent_elems, maxsteps, n_entries, nsl, max_flow;
   size_t off_desc, off_ptr, off_cur, off_key, off_t, off_omega, off_mats, off_D, off_logdet, off_info,
-      off_steps, off_tab, off_ldp, off_failp, off_stage, off_skey, off_st, off_slices, off_fsync, off_fqoff, off_ftasks, bytes;
+      off_steps, off_tab, off_ldp, off_failp, off_stage, off_skey, off_st, off_scol, off_spos, off_slices, off_fsync, off_fqoff, off_ftasks, bytes;
 };
 
 inline BatchPlan plan_batch(int64_t M, int64_t B, int64_t n, const int64_t* S_host) {
@@ -2005,6 +2117,7 @@ inline BatchPlan plan_batch(int64_t M, int64_t B, int64_t n, const int64_t* S_ho
   p.off_slices = put(sizeof(int2) * p.nsl);
   p.off_fqoff = put(sizeof(int) * (FLOW_QUEUES + 1 + 2 * n));  // queue offsets, then task totals per matrix
   p.off_ftasks = put(sizeof(FlowTask) * p.max_flow);
+  p.off_fsync = put(sizeof(int) * (FLOW_SYNC_HEAD + 2 * n));  // the persistent schedule's queue heads and counts
   p.off_ptr = put(sizeof(int) * p.csc_ptr_elems);
   p.off_cur = put(sizeof(int) * p.csc_ptr_elems);
   p.off_key = put(sizeof(int) * p.csc_ent_elems);
@@ -2019,9 +2132,10 @@ inline BatchPlan plan_batch(int64_t M, int64_t B, int64_t n, const int64_t* S_ho
   p.off_ldp = put(sizeof(double) * 2 * n * p.maxsteps);
   p.off_failp = put(sizeof(int64_t) * 2 * n * p.maxsteps);
   p.off_stage = put(sizeof(double) * 4 * n * NB * NB);  // two slots per matrix
-  p.off_skey = put(sizeof(int) * p.csc_ent_elems);       // the buckets in slice order
-  p.off_st = put(sizeof(double) * p.csc_ent_elems);
-  p.off_fsync = put(sizeof(int) * (FLOW_SYNC_HEAD + 2 * n));
+  p.off_skey = put(sizeof(int) * (p.csc_ent_elems + OMEGA_SLICE_PAD));  // the buckets in slice order
+  p.off_st = put(sizeof(double) * (p.csc_ent_elems + OMEGA_SLICE_PAD));
+  p.off_scol = put(sizeof(int) * p.csc_ptr_elems);  // the slices' columns by rank
+  p.off_spos = put(sizeof(int) * p.csc_ptr_elems);  // the slices' first entries
   p.bytes = align_up(o, 256);
   return p;
 }
@@ -2125,6 +2239,8 @@ int batch_pipeline(const WSrc w_tilde, int64_t M, const double* dirty_image, con
   int64_t* info = info_out ? info_out : reinterpret_cast<int64_t*>(ws + p.off_info);
   int* skey = reinterpret_cast<int*>(ws + p.off_skey);
   double* st_t = reinterpret_cast<double*>(ws + p.off_st);
+  int* scol = reinterpret_cast<int*>(ws + p.off_scol);
+  int* spos = reinterpret_cast<int*>(ws + p.off_spos);
   int2* slices = reinterpret_cast<int2*>(ws + p.off_slices);
 
   // Per-sample table (host -> device).
@@ -2142,6 +2258,8 @@ int batch_pipeline(const WSrc w_tilde, int64_t M, const double* dirty_image, con
     ptr_off += S + 1;
     maxS = std::max(maxS, S);
   }
+  AJ_REQUIRE(M * B * (int64_t)sizeof(double) + OMEGA_SLICE_PAD * 8 < (1ll << 31),
+             "%lld mapping entries per sample exceed the Omega kernel's 32-bit offsets", (long long)(M * B));
   AJ_REQUIRE(maxS <= SOLVE_S_MAX, "a sample of %lld source pixels exceeds the solve kernel's LDS (%d at most)",
              (long long)maxS, SOLVE_S_MAX);
   // slices of 64 columns for the Omega kernel, samples in order
@@ -2248,7 +2366,7 @@ int batch_pipeline(const WSrc w_tilde, int64_t M, const double* dirty_image, con
                                    (int)(sizeof(int) * (maxS + 1))));
   int* fsync = reinterpret_cast<int*>(ws + p.off_fsync);
   hipLaunchKernelGGL(prep_kernel, dim3((unsigned)n), dim3(PREP_THREADS), sizeof(int) * (maxS + 1), st, desc,
-                     pix_indexes, pix_weights, M, B, dirty_image, ptrs, ekey, et, D, skey, st_t,
+                     pix_indexes, pix_weights, M, B, dirty_image, ptrs, ekey, et, D, skey, st_t, scol, spos,
                      mode == CHOL_FLOW ? fsync : nullptr);
   AJ_LAUNCH_CHECK();
   // 2: Omega = W T, W read once for the batch; rows of W per workgroup: the largest power of two <= 8
@@ -2266,11 +2384,12 @@ int batch_pipeline(const WSrc w_tilde, int64_t M, const double* dirty_image, con
     const dim3 og((unsigned)ceil_div(M, R));
     const size_t lds = (size_t)R * M * sizeof(double);
     const int nsl = (int)p.nsl;
+    const int64_t sl_elems = p.csc_ent_elems + OMEGA_SLICE_PAD;
     switch (R) {
-      case 8: hipLaunchKernelGGL((omega_slices_kernel<8, U>), og, dim3(OMEGA_THREADS), lds, st, w_tilde, M, nsl, slices, desc, ptrs, skey, st_t, omega); break;
-      case 4: hipLaunchKernelGGL((omega_slices_kernel<4, U>), og, dim3(OMEGA_THREADS), lds, st, w_tilde, M, nsl, slices, desc, ptrs, skey, st_t, omega); break;
-      case 2: hipLaunchKernelGGL((omega_slices_kernel<2, U>), og, dim3(OMEGA_THREADS), lds, st, w_tilde, M, nsl, slices, desc, ptrs, skey, st_t, omega); break;
-      default: hipLaunchKernelGGL((omega_slices_kernel<1, U>), og, dim3(OMEGA_THREADS), lds, st, w_tilde, M, nsl, slices, desc, ptrs, skey, st_t, omega); break;
+      case 8: hipLaunchKernelGGL((omega_slices_kernel<8, U>), og, dim3(OMEGA_THREADS), lds, st, w_tilde, M, nsl, slices, desc, ptrs, scol, spos, skey, st_t, sl_elems, omega); break;
+      case 4: hipLaunchKernelGGL((omega_slices_kernel<4, U>), og, dim3(OMEGA_THREADS), lds, st, w_tilde, M, nsl, slices, desc, ptrs, scol, spos, skey, st_t, sl_elems, omega); break;
+      case 2: hipLaunchKernelGGL((omega_slices_kernel<2, U>), og, dim3(OMEGA_THREADS), lds, st, w_tilde, M, nsl, slices, desc, ptrs, scol, spos, skey, st_t, sl_elems, omega); break;
+      default: hipLaunchKernelGGL((omega_slices_kernel<1, U>), og, dim3(OMEGA_THREADS), lds, st, w_tilde, M, nsl, slices, desc, ptrs, scol, spos, skey, st_t, sl_elems, omega); break;
     }
   } else {
     hipLaunchKernelGGL(omega_global_kernel, dim3((unsigned)M), dim3(256), 0, st, w_tilde, M, (int)n, desc, ptrs,
