@@ -325,7 +325,8 @@ int aj_log_evidence_flow_configure(int small_per_cu, int spin_limit);
  * its matrix (2 sample + 0 for A, 1 for H), kind (0 its first diagonal block, 1 a 256-row task, 2 a 32-row
  * task), block column, task within the column, the compute unit's hardware id; then for 256-row tasks the
  * ticks when its first blocks' loads were issued, when its second wait began and ended, when its update
- * ended and when its substitution ended (slots 8-12; the others are left as they were). */
+ * ended and when its substitution ended (slots 8-12), and for the first 256-row task of a block column the
+ * ticks when the next diagonal block was updated, factored and stored (slots 13-15); the others are left as they were. */
 int aj_log_evidence_flow_trace(void* device_buffer, int64_t capacity_tasks);
 
 /* aj_log_evidence_batched with W-tilde given by its lattice table (numba.py:860-967 per sample):

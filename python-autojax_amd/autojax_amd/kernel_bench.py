@@ -73,6 +73,15 @@ def flow_summary(rec):
                    "first": round(float(v[first].mean() / 100.0), 2) if first.any() else None}
             for name, v in ph.items()}
         out["strip256_update_us_per_block"] = round(float((ph["update"][two & ~first] / np.maximum(k[two & ~first] - 1, 1)).mean() / 100.0), 3) if (two & ~first).any() else None
+        # the first task's next diagonal block (slots 13-15): its update by block k, factorisation, stage store
+        nx = first & (r[:, 13] > 0) & (r[:, 14] > 0) & (r[:, 15] > 0)
+        if nx.any():
+            q = r[nx]
+            out["next_block_us"] = {
+                "stores_to_update": round(float((q[:, 13] - q[:, 12]).mean() / 100.0), 2),
+                "factor": round(float((q[:, 14] - q[:, 13]).mean() / 100.0), 2),
+                "stage_store": round(float((q[:, 15] - q[:, 14]).mean() / 100.0), 2),
+                "to_complete": round(float((q[:, 2] - q[:, 15]).mean() / 100.0), 2)}
     out["last_matrix"] = {"matrix": int(last[3]), "columns": int(len(ks)),
                           "first_start_us": round(float((mm[:, 1].min() - t0) / 100.0), 1),
                           "column_end_us": [round(float((mm[mm[:, 5] == k][:, 2].max() - t0) / 100.0), 1)

@@ -795,10 +795,25 @@ __device__ inline double readlane_d(double v, int l) {
   return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
 }
 
+// 1 / sqrt(d) for d > 0: the hardware estimate (v_rsq_f64) refined by two Newton steps
+// y <- y + y (1 - d y^2) / 2, each of which about squares the relative error.
+__device__ inline double rsqrt_f64(double d) {
+  double y = __builtin_amdgcn_rsq(d);
+#pragma unroll
+  for (int it = 0; it < 2; ++it) {
+    const double e = fma(-d * y, y, 1.0);
+    y = fma(0.5 * y, e, y);
+  }
+  return y;
+}
+
 // Cholesky of an nb x nb block held by one wave, lane l < nb holding row l in r[0..l]; for each column the
 // pivot and the column entries are broadcast with readlane (no barriers, no LDS).  ld receives
 // sum_c log L_cc, taken as one log of the product of the pivots' mantissas plus their exponents;
-// fail the first non-positive pivot (1-based, offset by k0) or 0.
+// fail the first non-positive pivot (1-based, offset by k0) or 0.  A column is scaled by 1 / sqrt(pivot)
+// (rsqrt_f64: a short dependent chain, where a correctly rounded square root and a division per column
+// made the block's factorisation a chain of some 27 dependent FP64 operations per column).  Each lane's
+// updates are selects, not branches (a lane keeps r[q] where it is not updated).
 __device__ inline void chol_factor_regs(double (&r)[NB], int nb, int64_t k0, int lane, double& ld, int64_t& fail) {
   double mant = 1.0;
   int ex = 0;
@@ -811,16 +826,16 @@ __device__ inline void chol_factor_regs(double (&r)[NB], int nb, int64_t k0, int
         if (!fail) fail = k0 + c + 1;
         d = 1.0;  // keep the arithmetic finite; the result is flagged
       }
-      d = sqrt(d);
+      const double y = rsqrt_f64(d);  // the column's entries are r[c] y, its pivot d y
+      const double lcc = d * y;
       int e;
-      mant *= frexp(d, &e);  // in [0.5, 1): 32 factors stay above 2^-32
+      mant *= frexp(lcc, &e);  // in [0.5, 1): 32 factors stay above 2^-32
       ex += e;
-      if (lane == c) r[c] = d;
-      if (lane > c) r[c] /= d;
+      r[c] = lane == c ? lcc : lane > c ? r[c] * y : r[c];
 #pragma unroll
       for (int q = c + 1; q < NB; ++q) {
         const double lqc = readlane_d(r[c], q);  // L[q][c]
-        if (lane >= q) r[q] = fma(-r[c], lqc, r[q]);
+        r[q] = lane >= q ? fma(-r[c], lqc, r[q]) : r[q];
       }
     }
   }
@@ -828,7 +843,8 @@ __device__ inline void chol_factor_regs(double (&r)[NB], int nb, int64_t k0, int
 }
 
 // The same factorisation with each column broadcast through LDS (col: NB doubles of LDS owned by the wave)
-// instead of one lane read per element: the same operations on the same values, so the same bits.
+// instead of one lane read per element, the column's entries read together: the same operations on the
+// same values, so the same bits.
 template <typename ColPtr>
 __device__ inline void chol_factor_lds(double (&r)[NB], int nb, int64_t k0, int lane, double& ld, int64_t& fail,
                                        ColPtr col) {
@@ -843,20 +859,23 @@ __device__ inline void chol_factor_lds(double (&r)[NB], int nb, int64_t k0, int 
         if (!fail) fail = k0 + c + 1;
         d = 1.0;  // keep the arithmetic finite; the result is flagged
       }
-      d = sqrt(d);
+      const double y = rsqrt_f64(d);
+      const double lcc = d * y;
       int e;
-      mant *= frexp(d, &e);
+      mant *= frexp(lcc, &e);
       ex += e;
-      if (lane == c) r[c] = d;
-      if (lane > c) r[c] /= d;
-      if (lane < NB) col[lane] = r[c];  // column c: L[q][c] in lane q
-      __builtin_amdgcn_wave_barrier();
+      r[c] = lane == c ? lcc : lane > c ? r[c] * y : r[c];
+      // column c: L[q][c] in lane q (one wave: its LDS accesses complete in order, and the compiler keeps
+      // this write after the previous column's reads and before these)
+      if (lane < NB) col[lane] = r[c];
+      double lq[NB];
 #pragma unroll
-      for (int q = c + 1; q < NB; ++q) {
-        const double lqc = col[q];
-        if (lane >= q) r[q] = fma(-r[c], lqc, r[q]);
-      }
-      __builtin_amdgcn_wave_barrier();
+      for (int q = c + 2; q < NB; ++q) lq[q] = col[q];
+      // the next pivot's column first, its entry L[c+1][c] by a lane read (the same value as in LDS), so
+      // that the next column does not wait for the LDS round trip
+      if (c + 1 < NB) r[c + 1] = lane >= c + 1 ? fma(-r[c], readlane_d(r[c], c + 1), r[c + 1]) : r[c + 1];
+#pragma unroll
+      for (int q = c + 2; q < NB; ++q) r[q] = lane >= q ? fma(-r[c], lq[q], r[q]) : r[q];
     }
   }
   ld = log(mant) + (double)ex * 0.69314718055994530942;
@@ -1271,6 +1290,11 @@ struct FlowWait {
   do {                                                                                  \
     if ((FW).tr != nullptr && uni((int)(threadIdx.x >> 6)) == 0) (FW).tr[SLOT] = wall_clock64(); \
   } while (0)
+// the calling wave records a phase time of the traced task
+#define AJ_FLOW_MARK_WAVE(FW, SLOT)                                   \
+  do {                                                                \
+    if ((FW).tr != nullptr) (FW).tr[SLOT] = wall_clock64();           \
+  } while (0)
 __device__ void flow_wait(int* __restrict__ sync, int mat, int need, int spin_limit);
 
 // Task rb of block column k (k0 = 32 k) of matrix j; Ls (ROWS x LL_PITCH doubles), Lkk and colb (NB doubles)
@@ -1437,15 +1461,21 @@ __device__ __attribute__((always_inline)) inline void chol_left_task(const Sampl
         const int i = a * 16 + lk + 4 * r, c = b * 16 + lr;
         Lkk[i][c] = (b <= a && c <= i && i < nb1) ? acc[a][b][r] : 0.0;
       }
+  if (LA) AJ_FLOW_MARK_WAVE(fw, 13);
   double r[NB];
 #pragma unroll
   for (int c = 0; c < NB; ++c) r[c] = (lane < nb1 && c <= lane) ? Lkk[lane][c] : 0.0;
   double ld;
   int64_t fail;
   chol_factor_lds(r, nb1, k0 + NB, lane, ld, fail, colb);
+  if (LA) AJ_FLOW_MARK_WAVE(fw, 14);
 #pragma unroll
   for (int c = 0; c < NB; ++c)
     if (lane < NB) stg_out[lane * NB + c] = r[c];
+  if (LA && fw.tr != nullptr) {
+    __builtin_amdgcn_s_waitcnt(0);  // the stage slot's stores have landed
+    AJ_FLOW_MARK_WAVE(fw, 15);
+  }
   if (lane == 0) {
     ldp[(int64_t)j * maxsteps + kstep + 1] = ld;
     failp[(int64_t)j * maxsteps + kstep + 1] = fail;
@@ -1937,11 +1967,16 @@ __global__ void __launch_bounds__(SOLVE_THREADS) solve_loglike_kernel(const Samp
   // that one workgroup keeps enough of L in flight to stream it), the 64 x 64 diagonal block is solved
   // from LDS by one wave with the solved value broadcast by a lane shuffle.
   // forward, L y = D: y_blk -= L[blk, 0:k0] y[0:k0]; then the diagonal block
+  constexpr int LDQ = SB * SB / SOLVE_THREADS;  // diagonal-block entries per thread
+  static_assert(LDQ * SOLVE_THREADS == SB * SB, "whole passes");
   for (int64_t k0 = 0; k0 < n; k0 += SB) {
     const int nb = (int)std::min<int64_t>(SB, n - k0);
-    for (int e = tid; e < SB * SB; e += SOLVE_THREADS) {
-      const int r = e / SB, c = e % SB;
-      Ld[r][c] = (r < nb && c <= r) ? L[(k0 + r) * n + k0 + c] : 0.0;
+    // the diagonal block's loads are issued before the GEMV and stored to LDS after it
+    double ldq[LDQ];
+#pragma unroll
+    for (int q = 0; q < LDQ; ++q) {
+      const int e = tid + q * SOLVE_THREADS, r = e / SB, c = e % SB;
+      ldq[q] = (r < nb && c <= r) ? L[(k0 + r) * n + k0 + c] : 0.0;
     }
     // wave w: rows w, w + 16, w + 32, w + 48; per row two partial sums (j and j + 64 of each 128-column
     // step), added in that order
@@ -1970,13 +2005,21 @@ __global__ void __launch_bounds__(SOLVE_THREADS) solve_loglike_kernel(const Samp
         if (lane == 0 && wave + NW * q < nb) part[0][wave + NW * q] = t;
       }
     }
+#pragma unroll
+    for (int q = 0; q < LDQ; ++q) {
+      const int e = tid + q * SOLVE_THREADS;
+      Ld[e / SB][e % SB] = ldq[q];
+    }
     __syncthreads();
     if (wave == 0) {
+      // lane j's value is scaled by the reciprocal of its pivot (computed by every lane at once) and read
+      // by the others with a lane read: one multiply, one lane read and one FMA per row on the chain
       double v = lane < nb ? y[k0 + lane] - part[0][lane] : 0.0;
+      const double inv = lane < nb ? 1.0 / Ld[lane][lane] : 0.0;
       for (int j = 0; j < nb; ++j) {
-        if (lane == j) v = v / Ld[j][j];
-        const double yj = __shfl(v, j);
-        if (lane > j) v = fma(-Ld[lane][j], yj, v);
+        v = lane == j ? v * inv : v;
+        const double yj = readlane_d(v, j);
+        v = lane > j ? fma(-Ld[lane][j], yj, v) : v;
       }
       if (lane < nb) y[k0 + lane] = v;
     }
@@ -1986,9 +2029,11 @@ __global__ void __launch_bounds__(SOLVE_THREADS) solve_loglike_kernel(const Samp
   // sums, the sixteen waves' sums added in a fixed order); then the transposed diagonal block
   for (int64_t k0 = ((n - 1) / SB) * SB; k0 >= 0; k0 -= SB) {
     const int nb = (int)std::min<int64_t>(SB, n - k0);
-    for (int e = tid; e < SB * SB; e += SOLVE_THREADS) {
-      const int r = e / SB, c = e % SB;
-      Ld[r][c] = (r < nb && c <= r) ? L[(k0 + r) * n + k0 + c] : 0.0;
+    double ldq[LDQ];
+#pragma unroll
+    for (int q = 0; q < LDQ; ++q) {
+      const int e = tid + q * SOLVE_THREADS, r = e / SB, c = e % SB;
+      ldq[q] = (r < nb && c <= r) ? L[(k0 + r) * n + k0 + c] : 0.0;
     }
     // thread (wave w, lane c): column k0 + c over rows j = k0 + nb + w, + 16, ..., in four interleaved
     // accumulators
@@ -2005,16 +2050,22 @@ __global__ void __launch_bounds__(SOLVE_THREADS) solve_loglike_kernel(const Samp
       for (; j < n; j += NW) a0 = fma(col[j * n], y[j], a0);
     }
     part[wave][lane] = (a0 + a1) + (a2 + a3);
+#pragma unroll
+    for (int q = 0; q < LDQ; ++q) {
+      const int e = tid + q * SOLVE_THREADS;
+      Ld[e / SB][e % SB] = ldq[q];
+    }
     __syncthreads();
     if (wave == 0) {
       double t = 0.0;
 #pragma unroll
       for (int w = 0; w < NW; ++w) t += part[w][lane];
       double v = lane < nb ? y[k0 + lane] - t : 0.0;
+      const double inv = lane < nb ? 1.0 / Ld[lane][lane] : 0.0;
       for (int j = nb - 1; j >= 0; --j) {
-        if (lane == j) v = v / Ld[j][j];
-        const double sj = __shfl(v, j);
-        if (lane < j) v = fma(-Ld[j][lane], sj, v);
+        v = lane == j ? v * inv : v;
+        const double sj = readlane_d(v, j);
+        v = lane < j ? fma(-Ld[j][lane], sj, v) : v;
       }
       if (lane < nb) y[k0 + lane] = v;
     }
