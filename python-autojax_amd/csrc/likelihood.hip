@@ -914,7 +914,9 @@ __device__ inline void chol_panel_dev(const SampleDesc* __restrict__ sd, int j, 
   const int tid = threadIdx.x;
   for (int x = tid; x < NB * NB; x += 256) {
     const int i = x / NB, c = x % NB;
-    Lkk[i][c] = (i < nb && c <= i) ? A[(k0 + i) * n + k0 + c] : 0.0;
+    const double v = (i < nb && c <= i) ? A[(k0 + i) * n + k0 + c] : 0.0;
+    Lkk[i][c] = v;
+    if (c == i) Lkk[i][NB] = i < nb ? 1.0 / v : 0.0;  // the pivot's reciprocal (the row's padding column)
   }
   __syncthreads();
   const int64_t i = k0 + nb + (int64_t)rb * 256 + tid;
@@ -924,7 +926,7 @@ __device__ inline void chol_panel_dev(const SampleDesc* __restrict__ sd, int j, 
     for (int c = 0; c < nb; ++c) {
       double v = Xs[tid][c];
       for (int l = 0; l < c; ++l) v -= Xs[tid][l] * Lkk[c][l];
-      Xs[tid][c] = v / Lkk[c][c];
+      Xs[tid][c] = v * Lkk[c][NB];
     }
     for (int c = 0; c < nb; ++c) row[c] = Xs[tid][c];
   }
@@ -1102,6 +1104,8 @@ __device__ inline void chol_next_dev(const SampleDesc* __restrict__ sd, int j, i
 #pragma unroll
     for (int c = 0; c < NB; ++c)
       if (lane < NB) Lkk[lane][c] = (lane < nb1 && c <= lane) ? r[c] : 0.0;
+    // the pivot's reciprocal (the row's padding column), as the other schedules form it from the factor
+    if (lane < NB) Lkk[lane][NB] = lane < nb1 ? 1.0 / Lkk[lane][lane] : 0.0;
     if (rb == 0) {
 #pragma unroll
       for (int c = 0; c < NB; ++c)
@@ -1149,7 +1153,7 @@ __device__ inline void chol_next_dev(const SampleDesc* __restrict__ sd, int j, i
     for (int c = 0; c < nb1; ++c) {
       double v = Pr[tid][c];
       for (int l = 0; l < c; ++l) v -= Pr[tid][l] * Lkk[c][l];
-      Pr[tid][c] = v / Lkk[c][c];
+      Pr[tid][c] = v * Lkk[c][NB];
     }
     double* row = A + (s0 + tid) * n + k1;
     for (int c = 0; c < nb1; ++c) row[c] = Pr[tid][c];
@@ -1264,7 +1268,7 @@ __device__ inline void solve_row_lkk(double (&x)[NB], int nb, const lds_f64 (*__
       double v = x[c];
 #pragma unroll
       for (int l = 0; l < c; ++l) v -= x[l] * Lkk[c][l];
-      x[c] = v / Lkk[c][c];
+      x[c] = v * Lkk[c][NB];
     }
     __builtin_amdgcn_sched_barrier(0);
   }
@@ -1325,6 +1329,7 @@ __device__ __attribute__((always_inline)) inline void chol_left_task(const Sampl
       double v = 0.0;
       if (i < nb && c <= i) v = kstep == 0 ? A[(k0 + i) * n + k0 + c] : stg_in[x];
       Lkk[i][c] = v;
+      if (c == i) Lkk[i][NB] = i < nb ? 1.0 / v : 0.0;  // the pivot's reciprocal (the row's padding column)
       if (rb == 0 && kstep > 0 && i < nb && c <= i) A[(k0 + i) * n + k0 + c] = v;
     }
   };
@@ -1535,6 +1540,7 @@ __device__ __attribute__((always_inline)) inline void chol_small_task(const Samp
       double v = 0.0;
       if (i < nb && c <= i) v = kstep == 0 ? A[(k0 + i) * n + k0 + c] : stg_in[x];
       Lkk[i][c] = v;
+      if (c == i) Lkk[i][NB] = i < nb ? 1.0 / v : 0.0;  // the pivot's reciprocal (the row's padding column)
       if (rb == 0 && kstep > 0 && i < nb && c <= i) A[(k0 + i) * n + k0 + c] = v;
     }
   };
