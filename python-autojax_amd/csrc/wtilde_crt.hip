@@ -622,11 +622,15 @@ __global__ void __launch_bounds__(256) fixup_kernel(const SkFix* __restrict__ fi
 //   S_lo = sum_i r_i y_lo_i  (|.| < 2^64, rounding error below 2^16)
 //   n    = rint((S_hi + S_lo) / P)
 //   G    = (S_hi - n p_hi) + (S_lo - n p_lo)   (first bracket exact)
+// One launch recombines the ng chunks whose residues sit in sets 0 .. ng - 1 (set g at R + g set_bytes, scale
+// sc[g]), adding them to W in chunk order with one FMA each, as ng launches of one chunk would: W is read and
+// written once per group instead of once per chunk.
 // Thread = one packed word of one unit: 4 elements (one column, 4 rows) of a 16 x 16 block.
 // ---------------------------------------------------------------------------
+constexpr int COMBINE_GROUP = 4;  // chunks per combine launch
 template <int NM>
-__global__ void __launch_bounds__(256) combine_kernel(const int8_t* __restrict__ R, int ntiles,
-                                                      const int* __restrict__ tiles, Consts c,
+__global__ void __launch_bounds__(256) combine_kernel(const int8_t* __restrict__ R, int64_t set_bytes, int ng,
+                                                      int ntiles, const int* __restrict__ tiles, Consts c,
                                                       const double* __restrict__ sc, double* __restrict__ W,
                                                       int64_t M) {
   const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -637,26 +641,38 @@ __global__ void __launch_bounds__(256) combine_kernel(const int8_t* __restrict__
   const int64_t i0 = (int64_t)tiles[2 * t] * CT + rb * 16 + 4 * (lane >> 4);
   const int64_t j = (int64_t)tiles[2 * t + 1] * CT + cb * 16 + (lane & 15);
   if (j >= M || i0 >= M || i0 + 3 < j) return;
-  uint32_t w[NM];
-#pragma unroll
-  for (int m = 0; m < NM; ++m)
-    w[m] = reinterpret_cast<const uint32_t*>(R + ((int64_t)m * ntiles + t) * UNIT_BYTES)[rem];
-  constexpr double Q = qmax_for(NM);
-  const double s2 = sc[0] * sc[0] / (Q * Q);
+  double wv[4];
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int64_t i = i0 + r;
-    if (i >= M || i < j) continue;
-    double shi = 0.0, slo = 0.0;
+    wv[r] = (i < M && i >= j) ? W[i * M + j] : 0.0;
+  }
+  constexpr double Q = qmax_for(NM);
+  for (int s = 0; s < ng; ++s) {
+    const int8_t* Rs = R + s * set_bytes;
+    uint32_t w[NM];
 #pragma unroll
-    for (int m = 0; m < NM; ++m) {
-      const double rv = (double)(int8_t)(w[m] >> (8 * r));
-      shi = fma(rv, c.y_hi[m], shi);
-      slo = fma(rv, c.y_lo[m], slo);
+    for (int m = 0; m < NM; ++m)
+      w[m] = reinterpret_cast<const uint32_t*>(Rs + ((int64_t)m * ntiles + t) * UNIT_BYTES)[rem];
+    const double s2 = sc[s] * sc[s] / (Q * Q);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      double shi = 0.0, slo = 0.0;
+#pragma unroll
+      for (int m = 0; m < NM; ++m) {
+        const double rv = (double)(int8_t)(w[m] >> (8 * r));
+        shi = fma(rv, c.y_hi[m], shi);
+        slo = fma(rv, c.y_lo[m], slo);
+      }
+      const double n = rint((shi + slo) * c.inv_p);
+      const double G = fma(-n, c.p_hi, shi) + fma(-n, c.p_lo, slo);
+      wv[r] = fma(G, s2, wv[r]);
     }
-    const double n = rint((shi + slo) * c.inv_p);
-    const double G = fma(-n, c.p_hi, shi) + fma(-n, c.p_lo, slo);
-    W[i * M + j] += G * s2;
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int64_t i = i0 + r;
+    if (i < M && i >= j) W[i * M + j] = wv[r];
   }
 }
 
@@ -793,7 +809,8 @@ inline int64_t sk_slot_cap(int64_t units) { return std::min<int64_t>(units, 8 * 
 
 struct Plan {
   int64_t Mpad, nI, ntiles, per;
-  size_t feat_bytes, res_bytes, bytes;
+  size_t feat_bytes, set_bytes, res_bytes, bytes;
+  int nset;  // residue sets: the chunks recombined by one combine launch (at most COMBINE_GROUP)
   int64_t seg_cap, slot_cap;  // persistent SYRK schedule tables (two: full chunks, last chunk)
 };
 
@@ -804,8 +821,10 @@ inline Plan make_plan(int64_t M, int64_t K, int kc, int nm) {
   p.ntiles = p.nI * (p.nI + 1) / 2;
   p.per = ceil_div(p.ntiles, 8);
   p.feat_bytes = (size_t)nm * p.Mpad * 2 * kc;  // one chunk
-  p.res_bytes = (size_t)nm * p.ntiles * UNIT_BYTES;
   const int64_t nch = std::max<int64_t>(1, ceil_div(K, kc));
+  p.set_bytes = (size_t)nm * p.ntiles * UNIT_BYTES;
+  p.nset = (int)std::min<int64_t>(COMBINE_GROUP, nch);
+  p.res_bytes = p.set_bytes * p.nset;
   const int64_t units = (int64_t)nm * p.ntiles;
   p.seg_cap = sk_seg_cap(units);
   p.slot_cap = sk_slot_cap(units);
@@ -939,31 +958,39 @@ int crt_accumulate(const double* noise, const double* uv, const double* grid, co
     timer_end(AJ_TIMER_WT_FEATURES, st);
     AJ_LAUNCH_CHECK();
     timer_begin(AJ_TIMER_WT_SYRK, st);
+    const int set = (int)(ch % p.nset);
+    int8_t* Rc = R + set * p.set_bytes;  // this chunk's residues
     switch (variant) {
-      case 1: hipLaunchKernelGGL(syrk_pipe8_kernel<0>, sg, dim3(512), 0, st, X, plane, nkb, nst, nt, per, tiles, cst, R); break;
-      case 2: hipLaunchKernelGGL(syrk_ring8_kernel, sg, dim3(512), 0, st, X, plane, nkb, nst, nt, per, tiles, cst, R); break;
+      case 1: hipLaunchKernelGGL(syrk_pipe8_kernel<0>, sg, dim3(512), 0, st, X, plane, nkb, nst, nt, per, tiles, cst, Rc); break;
+      case 2: hipLaunchKernelGGL(syrk_ring8_kernel, sg, dim3(512), 0, st, X, plane, nkb, nst, nt, per, tiles, cst, Rc); break;
       case 3: {
         const int i = ch + 1 < nch ? 0 : 1;
         hipLaunchKernelGGL(syrk_sk_kernel<1>, dim3((unsigned)G), dim3(512), 0, st, X, plane, nkb, tiles, dsegs[i], dbeg[i],
-                           cst, nt, R, Rp);
+                           cst, nt, Rc, Rp);
         if (!sk[i]->fix.empty()) {
           AJ_LAUNCH_CHECK();
           hipLaunchKernelGGL(fixup_kernel, dim3(UNIT_BYTES / 4 / 256, (unsigned)sk[i]->fix.size()), dim3(256), 0, st,
-                             dfix[i], Rp, cst, nt, R);
+                             dfix[i], Rp, cst, nt, Rc);
         }
         break;
       }
-      default: hipLaunchKernelGGL(syrk_pipe8_kernel<1>, sg, dim3(512), 0, st, X, plane, nkb, nst, nt, per, tiles, cst, R); break;
+      default: hipLaunchKernelGGL(syrk_pipe8_kernel<1>, sg, dim3(512), 0, st, X, plane, nkb, nst, nt, per, tiles, cst, Rc); break;
     }
     timer_end(AJ_TIMER_WT_SYRK, st);
     AJ_LAUNCH_CHECK();
-    timer_begin(AJ_TIMER_WT_COMBINE, st);
-    if (nm == 12)
-      hipLaunchKernelGGL(combine_kernel<12>, cg, dim3(256), 0, st, R, nt, tiles, cst, sc + ch, W, M);
-    else
-      hipLaunchKernelGGL(combine_kernel<11>, cg, dim3(256), 0, st, R, nt, tiles, cst, sc + ch, W, M);
-    timer_end(AJ_TIMER_WT_COMBINE, st);
-    AJ_LAUNCH_CHECK();
+    // the group's chunks into W once its last chunk (or the W-tilde's last) has its residues
+    if (set + 1 == p.nset || ch + 1 == nch) {
+      const int ng = set + 1;
+      timer_begin(AJ_TIMER_WT_COMBINE, st);
+      if (nm == 12)
+        hipLaunchKernelGGL(combine_kernel<12>, cg, dim3(256), 0, st, R, (int64_t)p.set_bytes, ng, nt, tiles, cst,
+                           sc + ch + 1 - ng, W, M);
+      else
+        hipLaunchKernelGGL(combine_kernel<11>, cg, dim3(256), 0, st, R, (int64_t)p.set_bytes, ng, nt, tiles, cst,
+                           sc + ch + 1 - ng, W, M);
+      timer_end(AJ_TIMER_WT_COMBINE, st);
+      AJ_LAUNCH_CHECK();
+    }
   }
   return AJ_OK;
 }
