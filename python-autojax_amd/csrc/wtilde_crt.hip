@@ -627,7 +627,7 @@ __global__ void __launch_bounds__(256) fixup_kernel(const SkFix* __restrict__ fi
 // written once per group instead of once per chunk.
 // Thread = one packed word of one unit: 4 elements (one column, 4 rows) of a 16 x 16 block.
 // ---------------------------------------------------------------------------
-constexpr int COMBINE_GROUP = 4;  // chunks per combine launch
+constexpr int COMBINE_GROUP = 8;  // chunks per combine launch
 template <int NM>
 __global__ void __launch_bounds__(256) combine_kernel(const int8_t* __restrict__ R, int64_t set_bytes, int ng,
                                                       int ntiles, const int* __restrict__ tiles, Consts c,
