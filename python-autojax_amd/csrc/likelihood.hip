@@ -280,38 +280,33 @@ __global__ void __launch_bounds__(PREP_THREADS) prep_kernel(const SampleDesc* __
     for (int e = p[s]; e < p[s + 1]; ++e) v = fma(t[e], dirty[k[e]], v);
     D[d.s_off + s] = v;
   }
-  // The buckets again, ordered by slice for the Omega kernel.  The columns are grouped by bucket length
-  // (longest first; lengths from 255 on in one group) and cut into slices of 64, so that the lanes of a
-  // slice run for about the same number of entries; within a slice they are ranked by decreasing length
-  // (ties by their order in the grouping), sl_col[64 q + rank] = the column, so the columns that have an
-  // entry j are ranks 0 .. a_j - 1.  Entry j of rank r sits at sl_pos[q] + (the slice's entries before j)
-  // + r: a wave reading entry j of its 64 columns reads consecutive addresses and needs only a_j.  Which
-  // lane sums a column does not change the column's sum.
-  __shared__ int lgrp[256];  // columns per length group, then the groups' cursors
+  // The buckets again, ordered by slice for the Omega kernel.  Within each window of 256 columns the columns
+  // are ranked by bucket length (longest first, ties by column) and cut into slices of 64, so that the lanes
+  // of a slice run for about the same number of entries while a slice's Omega columns stay in the window's
+  // 2 KB of each Omega row (the window's slices complete those lines together in the L2); within a slice the
+  // lanes are ranked by decreasing length (sl_col[64 q + rank] = the column), so the columns that have an
+  // entry j are ranks 0 .. a_j - 1.  Entry j of rank r sits at sl_pos[q] + (the slice's entries before j) + r:
+  // a wave reading entry j of its 64 columns reads consecutive addresses and needs only a_j.  Which lane sums
+  // a column does not change the column's sum.
+  constexpr int SLW = 256;  // columns per window
+  static_assert(SLW <= PREP_THREADS, "a window's lengths fit part[]");
   __shared__ int ltot[PREP_SLICES_MAX];  // entries per slice, then the slices' first entries
-  int* order = cnt;  // the columns in group order (the bucket counters are free)
-  if (tid < 256) lgrp[tid] = 0;
-  __syncthreads();
-  for (int64_t s = tid; s < S; s += PREP_THREADS) atomicAdd(&lgrp[255 - std::min(p[s + 1] - p[s], 255)], 1);
-  __syncthreads();
-  if (wave == 0) {  // exclusive prefix of the 256 groups: four per lane
-    int v[4], sum = 0;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) sum += v[i] = lgrp[4 * lane + i];
-    int incl = sum;
-    for (int o = 1; o < 64; o <<= 1) {
-      const int y = __shfl_up(incl, o, 64);
-      if (lane >= o) incl += y;
-    }
-    int run = incl - sum;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      lgrp[4 * lane + i] = run;
-      run += v[i];
+  int* order = cnt;  // the columns in window order (the bucket counters are free)
+  for (int64_t w0 = 0; w0 < S; w0 += SLW) {
+    const int nw = (int)std::min<int64_t>(SLW, S - w0);
+    __syncthreads();
+    if (tid < nw) part[tid] = p[w0 + tid + 1] - p[w0 + tid];
+    __syncthreads();
+    if (tid < nw) {
+      const int li = part[tid];
+      int rank = 0;
+      for (int j = 0; j < nw; ++j) {
+        const int lj = part[j];
+        rank += lj > li || (lj == li && j < tid) ? 1 : 0;
+      }
+      order[w0 + rank] = (int)(w0 + tid);
     }
   }
-  __syncthreads();
-  for (int64_t s = tid; s < S; s += PREP_THREADS) order[atomicAdd(&lgrp[255 - std::min(p[s + 1] - p[s], 255)], 1)] = (int)s;
   __syncthreads();
   const int nsl = (int)((S + 63) / 64);
   for (int q = wave; q < nsl; q += PREP_THREADS / 64) {
