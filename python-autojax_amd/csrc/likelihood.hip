@@ -470,7 +470,7 @@ __global__ void __launch_bounds__(OMEGA_THREADS) omega_kernel(const WSrc W, int6
 // and adds it with weight +0.0 at column 0, which leaves its sum unchanged bit for bit (the sum is never
 // -0.0: it starts at +0.0 and a rounded sum is -0.0 only if both terms are).  Each column's sum runs over
 // its bucket in entry order (the bits of omega_kernel).
-constexpr int OMEGA_SLICE_UNROLL = 16;
+constexpr int OMEGA_SLICE_UNROLL = 8;
 constexpr int OMEGA_SLICE_PAD = 64;  // spare entries after the slice buffers
 
 template <int R>
