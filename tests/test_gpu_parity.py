@@ -541,6 +541,40 @@ def test_batched_log_evidence_vs_oracle(gpu, delaunay_small):
         assert abs(out[b] - want) < 1e-9 * max(1.0, abs(want))
 
 
+@pytest.mark.parametrize("n_pix", [12000, 21000])
+def test_batched_log_evidence_large_images_vs_oracle(gpu, n_pix):
+    """The batched evaluation where one row of W-tilde fills the LDS (12,000 pixels: one row per Omega
+    workgroup) and where it does not fit (21,000: Omega gathered from global memory), three small Delaunay
+    samples against the oracle's dense products (numba.py:552-614 and :755-857), within 1e-9 relative."""
+    import torch
+
+    from autojax_amd import device as dv
+    from autojax_amd import evidence as E
+
+    d = ds.small_delaunay(41, n_pix, 200, (60, 120), n_samples=3, lam_range=(0.1, 10.0))
+    d.ensure_dirty_image()
+    n_d, uv_d, g_d = dv.to_device(d.noise_map_real), dv.to_device(d.uv_wavelengths), dv.to_device(d.grid_radians_slim)
+    Wd = E.w_tilde(n_d, uv_d, g_d, "fp64")
+    batch = E.SampleBatch(d.samples, d.M)
+    c2nn = E.chi2_noise_norm(d.data, d.noise_map)
+    y = torch.from_numpy(d.dirty_image).cuda()
+    out = batch.log_evidence(Wd, y, c2nn, check=True).cpu().numpy()
+    W = Wd.cpu().numpy()
+    del Wd
+    chi = d.data.view(np.float64) / d.noise_map.view(np.float64)
+    nn = R.noise_normalization_complex_from(d.noise_map)
+    for b, smp in enumerate(d.samples):
+        To = R.mapping_matrix_from(smp.pix_indexes_for_sub_slim_index, smp.pix_weights_for_sub_slim_index, smp.pixels)
+        F = R.curvature_matrix_via_w_tilde_from(W, To)
+        D = R.data_vector_from(To, d.dirty_image)
+        H = R.constant_regularization_matrix_from(smp.coefficient, smp.neighbors, smp.neighbors_sizes)
+        ldh = ev.logdet_deflated(H, smp.coefficient**2 * smp.neighbors_sizes.mean())
+        want, _, _, _ = R.log_evidence_from_parts(F, D, chi @ chi, nn, smp.neighbors, smp.neighbors_sizes,
+                                                   smp.coefficient, logdet_h=ldh)
+        print(f"M {d.M} sample {b} S={smp.pixels}: {out[b]:.12g} vs {want:.12g} d={out[b] - want:.2e}", flush=True)
+        assert abs(out[b] - want) < 1e-9 * max(1.0, abs(want))
+
+
 def test_batched_cholesky_modes_bitwise_equal(gpu, delaunay_small):
     """The six ways the batch factors its matrices (three launches per step; one cooperative launch,
     without and with lookahead; one launch per step with lookahead; left-looking, one launch per step; the
